@@ -1,0 +1,284 @@
+"""Pins the oracle (CPU restatement of the reference prover) before it is trusted as a checker.
+
+Each check below is chosen so that a plausible mistake in the oracle fails it:
+reference known-answer vectors (Blake2s, channel digests, draw counting, Poseidon MDS
+matrices, circle generator order), brute-force evaluation of the FFT basis at tiny sizes
+(pyref.py, written from the mathematics), algebraic round trips, and the reference verifier
+flow accepting wide-Fibonacci proofs while rejecting tampered ones.
+"""
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pyref
+
+P = O.P
+
+
+# ---------------------------------------------------------------- hash / channel KATs ------
+def test_blake2s_single_hash_kat():
+    # crates/stwo/src/core/vcs/blake2_hash.rs:133-140
+    assert O.blake2s(b"a").hex() == "4a0d129873403037c2cd9b9048203687f6233fb6738956e0349bd4320fec3e90"
+    assert O.blake2s(b"") == hashlib.blake2s(b"", digest_size=32).digest()
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 127, 128, 129, 400, 1024, 5056])
+def test_blake2s_matches_rfc7693(n):
+    data = bytes(random.Random(n).getrandbits(8) for _ in range(n))
+    assert O.blake2s(data) == hashlib.blake2s(data, digest_size=32).digest()
+
+
+def test_channel_mix_u64_kat():
+    # crates/stwo/src/core/channel/blake2s.rs:214-232
+    import ctypes
+    out = np.zeros(32, np.uint8)
+    O.lib().oracle_channel_mix_u64(ctypes.c_uint64(0x1111222233334444), O.u8ptr(out))
+    expect = bytes([0xbc, 0x9e, 0x3f, 0xc1, 0xd2, 0x4e, 0x88, 0x97, 0x95, 0x6d, 0x33, 0x59, 0x32, 0x73,
+                    0x97, 0x24, 0x9d, 0x6b, 0xca, 0xcd, 0x22, 0x4d, 0x92, 0x74, 0x4, 0xe7, 0xba, 0x4a,
+                    0x77, 0xdc, 0x6e, 0xce])
+    assert out.tobytes() == expect
+
+
+def test_channel_mix_u32s_kat():
+    # crates/stwo/src/core/channel/blake2s.rs:235-247
+    out = np.zeros(32, np.uint8)
+    data = np.arange(1, 10, dtype=np.uint32)
+    O.lib().oracle_channel_mix_u32s(O.u32ptr(data), 9, O.u8ptr(out))
+    expect = bytes([0x70, 0x91, 0x76, 0x83, 0x57, 0xbb, 0x1b, 0xb3, 0x34, 0x6f, 0xda, 0xb6, 0xb3, 0x57,
+                    0xd7, 0xfa, 0x46, 0xb8, 0xfb, 0xe3, 0x2c, 0x2e, 0x43, 0x24, 0xa0, 0xff, 0xc2, 0x94,
+                    0xcb, 0xf9, 0xa1, 0xc7])
+    assert out.tobytes() == expect
+
+
+def test_channel_draw_counting():
+    # crates/stwo/src/core/channel/blake2s.rs:154-165: one draw_u32s + draw_secure_felts(9) -> 6 draws
+    assert O.lib().oracle_channel_draw_count(9) == 6
+
+
+def test_grind_is_smallest_nonce():
+    # simd/grind.rs: smallest (hi<<32)|lo with lo < 2^20; verify_pow_nonce is the spec.
+    digest = hashlib.blake2s(b"grind", digest_size=32).digest()
+    for bits in (1, 3, 6, 9):
+        nonce = O.grind(digest, bits)
+        assert O.verify_pow(digest, bits, nonce)
+        for smaller in range(nonce):
+            assert not O.verify_pow(digest, bits, smaller)
+
+
+# ---------------------------------------------------------------- fields / circle ----------
+def test_circle_generator_order():
+    import ctypes
+    out = np.zeros(2, np.uint32)
+    O.lib().oracle_index_to_point(ctypes.c_uint32(1 << 30), O.u32ptr(out))
+    assert tuple(out) == (P - 1, 0)  # GEN * 2^30 = (-1, 0) != identity
+    O.lib().oracle_index_to_point(ctypes.c_uint32(1), O.u32ptr(out))
+    assert tuple(out) == pyref.GEN
+    for i in (3, 12345, (1 << 31) - 5):
+        O.lib().oracle_index_to_point(ctypes.c_uint32(i), O.u32ptr(out))
+        assert tuple(out) == pyref.index_point(i)
+
+
+def test_qm31_mul_and_inverse():
+    rng = random.Random(7)
+    for _ in range(50):
+        a = np.array([rng.randrange(P) for _ in range(4)], np.uint32)
+        b = np.array([rng.randrange(P) for _ in range(4)], np.uint32)
+        out = np.zeros(4, np.uint32)
+        O.lib().oracle_qm31_mul(O.u32ptr(a), O.u32ptr(b), O.u32ptr(out))
+        assert tuple(out) == pyref.qm_mul(tuple(int(x) for x in a), tuple(int(x) for x in b))
+        O.lib().oracle_qm31_inv(O.u32ptr(a), O.u32ptr(out))
+        assert pyref.qm_mul(tuple(int(x) for x in a), tuple(int(x) for x in out)) == (1, 0, 0, 0)
+
+
+# ---------------------------------------------------------------- FFT -----------------------
+@pytest.mark.parametrize("log_n", [1, 2, 3, 4, 5, 6])
+def test_evaluate_matches_bruteforce_basis(log_n):
+    rng = random.Random(log_n)
+    coeffs = [rng.randrange(P) for _ in range(1 << log_n)]
+    for log_dom in (log_n, log_n + 1, log_n + 2):
+        got = O.evaluate(np.array(coeffs, np.uint32), log_dom)
+        assert list(map(int, got)) == pyref.evaluate_bruteforce(coeffs, log_dom)
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 3, 5, 8, 11, 14])
+def test_interpolate_inverts_evaluate(log_n):
+    rng = np.random.default_rng(log_n)
+    coeffs = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+    evals = O.evaluate(coeffs, log_n)
+    assert np.array_equal(O.interpolate(evals), coeffs)
+    ext = O.evaluate(coeffs, log_n + 2)
+    # interpolation on the larger domain recovers the zero-padded coefficients
+    back = O.interpolate(ext)
+    assert np.array_equal(back[: 1 << log_n], coeffs) and not back[1 << log_n:].any()
+
+
+def test_twiddle_tree_layout():
+    # layer j holds the x-coordinates of the first half of half_coset doubled j times, bit reversed;
+    # the buffer ends with a padding 1 (prover/backend/cpu/circle.rs:274-291)
+    log_dom = 6
+    tw, itw = O.twiddles(log_dom)
+    assert tw[-1] == 1 and len(tw) == 1 << (log_dom - 1)
+    h = log_dom - 1
+    init, step = 1 << (31 - (h + 2)), 1 << (31 - h)
+    off = 0
+    for j in range(h):
+        n = 1 << (h - 1 - j)
+        xs = [pyref.index_point((init + step * i) << j)[0] for i in range(n)]
+        xs = [xs[pyref.bitrev(i, h - 1 - j)] for i in range(n)]
+        assert list(map(int, tw[off:off + n])) == xs
+        off += n
+    assert all((int(a) * int(b)) % P == 1 for a, b in zip(tw, itw))
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 3, 6])
+def test_eval_at_point_bruteforce_and_barycentric(log_n):
+    rng = random.Random(100 + log_n)
+    coeffs = [rng.randrange(P) for _ in range(1 << log_n)]
+    # a QM31 circle point from a channel-style rational parametrisation
+    dig = np.frombuffer(hashlib.blake2s(bytes([log_n]), digest_size=32).digest(), np.uint8).copy()
+    pt = np.zeros(8, np.uint32)
+    O.lib().oracle_random_point(O.u8ptr(dig), O.u32ptr(pt))
+    px, py = tuple(map(int, pt[:4])), tuple(map(int, pt[4:]))
+    got = O.eval_at_point(np.array(coeffs, np.uint32), pt)
+    assert tuple(map(int, got)) == pyref.qm_eval_at_point(coeffs, px, py)
+    if log_n >= 1:
+        for log_dom in (max(log_n, 1), log_n + 1):
+            evals = O.evaluate(np.array(coeffs, np.uint32), log_dom)
+            assert np.array_equal(O.barycentric_eval(evals, pt), got)
+
+
+def test_offset_bit_reversed_index_is_a_domain_shift():
+    # core/utils.rs offset_bit_reversed_circle_domain_index: row j holds the eval-domain point that is
+    # the point of row i shifted by `off` steps of the trace domain CanonicCoset(dlog)
+    for dlog, elog in [(3, 4), (3, 5), (4, 6)]:
+        n = 1 << elog
+        step = pyref.index_point(1 << (31 - dlog))
+        for i in range(n):
+            p = pyref.canonic_domain_point(elog, pyref.bitrev(i, elog))
+            for off in (-1, 1, 2):
+                j = O.lib().oracle_offset_bit_reversed_index(i, dlog, elog, off)
+                q = pyref.canonic_domain_point(elog, pyref.bitrev(j, elog))
+                s = pyref.mul(step, off) if off > 0 else pyref.mul((step[0], (-step[1]) % P), -off)
+                assert q == pyref.add(p, s), (dlog, elog, i, off)
+
+
+# ---------------------------------------------------------------- Merkle / FRI -------------
+def test_merkle_layer_is_blake2s_of_node():
+    rng = np.random.default_rng(1)
+    cols = [rng.integers(0, P, 8, dtype=np.uint32) for _ in range(3)]
+    leaves = O.commit_on_layer(3, None, cols).reshape(8, 32)
+    for i in range(8):
+        msg = b"".join(int(c[i]).to_bytes(4, "little") for c in cols)
+        assert leaves[i].tobytes() == hashlib.blake2s(msg, digest_size=32).digest()
+    up = O.commit_on_layer(2, leaves.reshape(-1), [cols[0][:4]]).reshape(4, 32)
+    for i in range(4):
+        msg = leaves[2 * i].tobytes() + leaves[2 * i + 1].tobytes() + int(cols[0][i]).to_bytes(4, "little")
+        assert up[i].tobytes() == hashlib.blake2s(msg, digest_size=32).digest()
+
+
+def test_empty_merkle_root_is_empty_hash():
+    # prover/vcs/prover.rs:41-45: commit([]) = commit_on_layer(0, None, [])
+    assert O.commit_on_layer(0, None, []).tobytes() == hashlib.blake2s(b"", digest_size=32).digest()
+
+
+def test_fold_circle_identity():
+    # 2 f(p) = f0(p.x) + p.y f1(p.x): folding a constant c with alpha gives 2c (f1 = 0)
+    n = 16
+    c = 12345
+    src = [np.full(n, c, np.uint32)] + [np.zeros(n, np.uint32)] * 3
+    dst = [np.zeros(n // 2, np.uint32) for _ in range(4)]
+    out = O.fold_circle_into_line(src, dst, [3, 5, 7, 11])
+    assert (out[0] == 2 * c % P).all() and not out[1].any()
+
+
+# ---------------------------------------------------------------- Poseidon MDS ---------------
+def test_poseidon_m4_matches_matrix():
+    # crates/examples/src/poseidon/mod.rs:822-836
+    m4 = [[5, 7, 1, 3], [4, 6, 1, 1], [1, 3, 5, 7], [1, 1, 4, 6]]
+    x = np.array([0, 1, 2, 3], np.uint32)
+    O.lib().oracle_apply_m4(O.u32ptr(x))
+    assert list(map(int, x)) == [sum(m4[r][k] * k for k in range(4)) % P for r in range(4)]
+
+
+def test_poseidon_internal_matrix():
+    # crates/examples/src/poseidon/mod.rs:838-857
+    s = [(i * 3 + 187) for i in range(16)]
+    x = np.array(s, np.uint32)
+    O.lib().oracle_internal_matrix(O.u32ptr(x))
+    exp = [(sum(s) + s[i] * (1 << (i + 1))) % P for i in range(16)]
+    assert list(map(int, x)) == exp
+
+
+# ---------------------------------------------------------------- inputs --------------------
+def test_reference_inputs():
+    t = O.gen_wide_fib(4, 6, 0, 0)
+    assert (t[0] == 1).all() and list(t[1]) == list(range(16))
+    for c in range(2, 6):
+        assert np.array_equal(t[c], (t[c - 2].astype(np.uint64) ** 2 + t[c - 1].astype(np.uint64) ** 2) % P)
+
+
+def test_seeded_inputs_uniform_range():
+    t = O.gen_wide_fib(10, 3, 1, 42)
+    assert (t < P).all() and len(set(t[0].tolist())) > 1000 - 5
+
+
+# ---------------------------------------------------------------- whole proofs --------------
+WF = dict(air="wide_fib", n_cols=16, pow_bits=5, log_blowup=1, log_last_layer=0, n_queries=3, fold_step=1)
+
+
+@pytest.mark.parametrize("log_n,mode", [(3, 0), (5, 0), (5, 1), (8, 1)])
+def test_wide_fib_prove_verify(log_n, mode):
+    st, proof, dig, cs, _ = O.prove(log_n_rows=log_n, input_mode=mode, seed=9, **WF)
+    assert st == 0
+    assert O.verify(proof, log_n_rows=log_n, **WF) == ""
+    # deterministic: same inputs, same bytes
+    st2, proof2, dig2, _, _ = O.prove(log_n_rows=log_n, input_mode=mode, seed=9, **WF)
+    assert proof2 == proof and dig2 == dig
+
+
+def test_wide_fib_other_configs_verify():
+    cfg = dict(WF)
+    cfg.update(log_blowup=2, n_queries=5, log_last_layer=2, pow_bits=8)
+    st, proof, _, _, _ = O.prove(log_n_rows=6, input_mode=1, seed=3, **cfg)
+    assert st == 0 and O.verify(proof, log_n_rows=6, **cfg) == ""
+    cfg.update(fold_step=2, log_blowup=1, log_last_layer=1)
+    st, proof, _, _, _ = O.prove(log_n_rows=7, input_mode=1, seed=4, **cfg)
+    assert st == 0 and O.verify(proof, log_n_rows=7, **cfg) == ""
+
+
+def test_tampered_proofs_are_rejected():
+    st, proof, _, _, _ = O.prove(log_n_rows=5, input_mode=1, seed=1, **WF)
+    rng = random.Random(0)
+    # flip bytes in a spread of positions (skip the config header)
+    for pos in sorted(rng.sample(range(24, len(proof)), 25)):
+        bad = bytearray(proof)
+        bad[pos] ^= 0x01
+        assert O.verify(bytes(bad), log_n_rows=5, **WF) != "", pos
+
+
+def test_wrong_trace_breaks_constraints():
+    # a proof for n_cols=16 does not verify as an n_cols=17 statement
+    st, proof, _, _, _ = O.prove(log_n_rows=4, input_mode=0, **WF)
+    cfg = dict(WF)
+    cfg["n_cols"] = 17
+    assert O.verify(proof, log_n_rows=4, **cfg) != ""
+
+
+PO = dict(air="poseidon", pow_bits=5, log_blowup=1, log_last_layer=0, n_queries=3, fold_step=1)
+
+
+@pytest.mark.parametrize("ipr", [1, 2])
+def test_poseidon_prove_reports_lifted_protocol_limit(ipr):
+    # The reference disables every Poseidon prove->verify test: "AIRs with constraint degree >= 2
+    # are not supported yet in the lifted protocol" (crates/examples/src/poseidon/mod.rs:881,928).
+    # With LOG_EXPAND = 2 the split composition is committed one log size above the trace, so the
+    # verifier's lifted evaluation differs from the committed composition: prove_ex's own sanity
+    # check (prover/mod.rs:127-144) reports ConstraintsNotSatisfied (status 2) and verify rejects.
+    st, proof, dig, cs, _ = O.prove(log_n_rows=5, ipr=ipr, input_mode=1, seed=5, **PO)
+    assert st == 2
+    assert O.verify(proof, claimed_sum=cs, log_n_rows=5, ipr=ipr, **PO) == "OodsNotMatching"
+    st2, proof2, dig2, _, _ = O.prove(log_n_rows=5, ipr=ipr, input_mode=1, seed=5, **PO)
+    assert proof2 == proof and dig2 == dig
