@@ -1,0 +1,189 @@
+/*
+ * libstwo_hip: MI355X (gfx950) prover backend for Circle STARKs over M31 with the Blake2s
+ * Merkle channel. C ABI with plain pointers and sizes: the boundary a reference-side backend
+ * binds, as CudaBackend binds libstwo_cuda through crates/stwo/src/stwo_cuda/bindings.rs.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 ok, 1 invalid argument, 2 HIP runtime error,
+ *    3 unsupported size or configuration, 4 protocol failure (e.g. FRI degree check),
+ *    5 unknown component. The message of the last failure is in stwo_hip_last_error().
+ *    (The reference exits the process on any device error, CU/utils.cuh:37-40.)
+ *  - Column arguments are device pointers to u32 words; "const uint32_t* const* cols" is a
+ *    HOST array of device pointers. Every M31 word produced is canonical, in [0, 2^31 - 1).
+ *  - QM31 values are passed by value as stwo_qm31 {a, b, c, d} = (a + b i) + (c + d i) u,
+ *    as CudaSecureField (bindings.rs:9-43). Secure columns are 4 coordinate columns.
+ *  - Evaluations are in bit-reversed circle-domain order of CanonicCoset(log_size);
+ *    coefficients are in the FFT basis (crates/stwo/src/prover/poly/circle/ops.rs:125-141).
+ *  - One context owns one stream; calls on one context are serialised by the caller.
+ *    Different contexts are independent. Calls are stream-ordered unless stated otherwise.
+ */
+#ifndef STWO_HIP_H
+#define STWO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct stwo_hip_ctx stwo_hip_ctx;
+typedef struct { uint32_t a, b, c, d; } stwo_qm31;
+typedef struct { stwo_qm31 x, y; } stwo_circle_point_qm31;
+
+/* PcsConfig + FriConfig (crates/stwo/src/core/pcs/mod.rs:29-80, core/fri.rs:32-81).
+ * lifting_log_size = 0 means None. */
+typedef struct {
+  uint32_t pow_bits;
+  uint32_t log_blowup_factor;
+  uint32_t log_last_layer_degree_bound;
+  uint32_t n_queries;
+  uint32_t line_fold_step;
+  uint32_t lifting_log_size;
+} stwo_pcs_config;
+
+/* ---------------------------------------------------------------- context / memory ----- */
+/* No reference counterpart: the CUDA library uses the legacy default stream and a global
+ * pool (CU/cuda_mem_pool.cuh). */
+int stwo_hip_create(int device, stwo_hip_ctx** out);
+void stwo_hip_destroy(stwo_hip_ctx* ctx);
+int stwo_hip_synchronize(stwo_hip_ctx* ctx);
+const char* stwo_hip_last_error(stwo_hip_ctx* ctx);
+
+/* cuda_malloc_uint32_t / cuda_alloc_zeroes_uint32_t / cuda_free_memory and the copies
+ * (bindings.rs:84-131): Column::zeros / uninitialized / to_cpu / from_iter. */
+int stwo_hip_alloc(stwo_hip_ctx* ctx, size_t n_words, int zeroed, uint32_t** out);
+int stwo_hip_free(stwo_hip_ctx* ctx, uint32_t* ptr);
+int stwo_hip_copy_h2d(stwo_hip_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t n_words);
+int stwo_hip_copy_d2h(stwo_hip_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t n_words); /* synchronous */
+/* Column::batch_at_multi (crates/stwo/src/prover/backend/mod.rs:67-89; batch_gather_multi_uint32,
+ * bindings.rs:277-332): out[r * n_cols + c] = cols[c][rows[r]], row-major; synchronous. */
+int stwo_hip_batch_at_multi(stwo_hip_ctx* ctx, const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows,
+                            uint32_t n_rows, uint32_t* out_host);
+
+/* ---------------------------------------------------------------- PolyOps -------------- */
+/* precompute_twiddles (bindings.rs:165-169; prover/poly/circle/ops.rs:122): caches the twiddle
+ * tree of the half coset of CanonicCoset(log_domain) in the context. A tree serves every canonic
+ * domain up to its size. tw/itw (host, may be NULL) receive 2^(log_domain-1) words each. */
+int stwo_hip_precompute_twiddles(stwo_hip_ctx* ctx, uint32_t log_domain, uint32_t* tw_host, uint32_t* itw_host);
+/* interpolate_columns (ops.rs:23-39; ntt_b2n_column, bindings.rs:503-510): evals[i] (bit-reversed
+ * evaluations on CanonicCoset(log_size)) -> coeffs[i]; in place when the pointers are equal. */
+int stwo_hip_interpolate_columns(stwo_hip_ctx* ctx, const uint32_t* const* evals, uint32_t* const* coeffs,
+                                 uint32_t n_cols, uint32_t log_size);
+/* evaluate_polynomials (ops.rs:98-120; ntt_n2b_columns, bindings.rs:512-519): coeffs (2^log_size)
+ * -> evals on CanonicCoset(log_size + log_blowup) (2^(log_size+log_blowup) words each). */
+int stwo_hip_evaluate_polynomials(stwo_hip_ctx* ctx, const uint32_t* const* coeffs, uint32_t* const* evals,
+                                  uint32_t n_cols, uint32_t log_size, uint32_t log_blowup);
+/* batch_eval_at_point (ops.rs:41-59; batch_eval_at_points, bindings.rs:180-194): out_host[i] =
+ * coeffs[i](point), synchronous. */
+int stwo_hip_batch_eval_at_point(stwo_hip_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
+                                 stwo_circle_point_qm31 point, stwo_qm31* out_host);
+
+/* ---------------------------------------------------------------- MerkleOps ------------ */
+/* MerkleOps<Blake2sMerkleHasher>::commit_on_layer (prover/vcs/ops.rs:8-29; commit_on_first_layer /
+ * commit_on_layer_with_previous, bindings.rs:262-275): out (2^log_size nodes x 8 words) =
+ * Blake2s(prev[2i] || prev[2i+1] || cols[0][i] || ...). prev may be NULL (leaf layer). */
+int stwo_hip_commit_on_layer(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols,
+                             uint32_t n_cols, uint32_t* out);
+
+/* ---------------------------------------------------------------- FriOps --------------- */
+/* fold_circle_into_line (prover/fri.rs:46-66; bindings.rs:196-212): dst (4 coordinate columns of
+ * 2^(log_src-1)) = dst * alpha^2 + fold(src), src on CanonicCoset(log_src). Uses the cached twiddles. */
+int stwo_hip_fold_circle_into_line(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4],
+                                   uint32_t* const dst[4], stwo_qm31 alpha);
+/* fold_line (prover/fri.rs:31-44): src on the line domain of log size log_src
+ * (Coset::half_odds(log_src)), dst has 2^(log_src-1) words per coordinate. */
+int stwo_hip_fold_line(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
+                       stwo_qm31 alpha);
+
+/* ---------------------------------------------------------------- QuotientOps ---------- */
+/* One ColumnSampleBatch (core/pcs/quotients.rs:56-100): a point and, per sampled column, the
+ * column index, the sampled value and the random-coefficient power. */
+typedef struct {
+  stwo_circle_point_qm31 point;
+  uint32_t n_columns;
+  const uint32_t* column_indices;
+  const stwo_qm31* sample_values;
+  const stwo_qm31* random_coeffs;
+} stwo_sample_batch;
+/* accumulate_numerators (prover/pcs/quotient_ops.rs:19-43; accumulate_numerators_batch,
+ * bindings.rs:354-381): for batch b, out[4b..4b+4) (2^log_size each) =
+ * sum_j (c_j * cols[idx_j] - b_j), and first_linear_host[b] = sum_j a_j. */
+int stwo_hip_accumulate_numerators(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_t* const* cols, uint32_t n_cols,
+                                   const stwo_sample_batch* batches, uint32_t n_batches, uint32_t* const* out,
+                                   stwo_qm31* first_linear_host);
+/* AccumulatedNumerators (quotient_ops.rs:45-70). */
+typedef struct {
+  stwo_circle_point_qm31 sample_point;
+  const uint32_t* partial[4];
+  uint32_t log_size;
+  stwo_qm31 first_linear_term;
+} stwo_accumulated_numerators;
+/* compute_quotients_and_combine (quotient_ops.rs:34-43; compute_quotients_and_combine,
+ * bindings.rs:354-381): out (4 x 2^lifting_log_size) on CanonicCoset(lifting_log_size). */
+int stwo_hip_compute_quotients_and_combine(stwo_hip_ctx* ctx, const stwo_accumulated_numerators* accs, uint32_t n_accs,
+                                           uint32_t lifting_log_size, uint32_t* const out[4]);
+
+/* ---------------------------------------------------------------- AccumulationOps ------ */
+/* accumulate (prover/air/accumulation.rs:127-145; bindings.rs:214-221): dst += src. */
+int stwo_hip_accumulate(stwo_hip_ctx* ctx, uint32_t log_size, uint32_t* const dst[4], const uint32_t* const src[4]);
+/* lift_and_accumulate (accumulation.rs:127-145; lift_and_accumulate, bindings.rs:214-221): cols4
+ * holds 4 coordinate pointers per column, sizes ascending; out (4 x 2^max) = sum of lifts. */
+int stwo_hip_lift_and_accumulate(stwo_hip_ctx* ctx, const uint32_t* const* cols4, const uint32_t* log_sizes,
+                                 uint32_t n_cols, uint32_t* const out[4]);
+/* generate_secure_powers (accumulation.rs:127-145): out_host[i] = felt^i. */
+int stwo_hip_generate_secure_powers(stwo_hip_ctx* ctx, stwo_qm31 felt, uint32_t n, stwo_qm31* out_host);
+
+/* ---------------------------------------------------------------- GrindOps ------------- */
+/* GrindOps<Blake2sChannel>::grind (core/proof_of_work.rs:3-7; grind_blake2s, bindings.rs:615):
+ * smallest nonce (hi << 32) | lo, lo < 2^20, for the channel digest; synchronous. */
+int stwo_hip_grind(stwo_hip_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce);
+
+/* ---------------------------------------------------------------- ComponentProver ------ */
+/* evaluate_constraint_quotients_on_domain (bindings.rs:469-490; constraint-framework
+ * component_prover.rs:249-499). eval_blob = fnv1a(name) (u32 LE) || component parameters:
+ *   "fibonacci_example": nothing more (the sequence length is n_trace);
+ *   "poseidon_example":  u32 instances_per_row || stwo_qm31 z || stwo_qm31 alpha.
+ * acc (4 x 2^eval_log) += (sum_k coeff_powers[k] C_k(row)) * denom_inv[row >> trace_log].
+ * trace / interaction columns are on the evaluation domain CanonicCoset(eval_log).
+ * Returns 5 for an unknown component (the reference then falls back to the CPU). */
+int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* ctx, uint32_t* const acc[4],
+                                                     const uint32_t* const* trace, uint32_t n_trace,
+                                                     const uint32_t* const* interaction, uint32_t n_interaction,
+                                                     const stwo_qm31* coeff_powers, uint32_t n_constraints,
+                                                     const uint32_t* denom_inv_host, uint32_t trace_log,
+                                                     uint32_t eval_log, const uint8_t* eval_blob, size_t blob_len,
+                                                     stwo_qm31 cumsum_shift);
+
+/* ---------------------------------------------------------------- whole proofs --------- */
+/* prove() over the wide-Fibonacci AIR (crates/examples/src/wide_fibonacci/mod.rs:186-261) and the
+ * Poseidon2 AIR (crates/examples/src/poseidon/mod.rs:337-419), with the trace already resident:
+ * trace[i] are device columns of 2^log_n_rows words (left untouched). The proof is written in the
+ * library's canonical byte encoding. status_out: 0 ok, 2 ConstraintsNotSatisfied (prove_ex's
+ * sanity check, prover/mod.rs:127-144). Synchronous. */
+typedef struct stwo_proof stwo_proof;
+int stwo_hip_prove_wide_fibonacci(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t n_cols,
+                                  uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
+int stwo_hip_prove_poseidon(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t instances_per_row,
+                            uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
+int stwo_proof_status(const stwo_proof* p);
+size_t stwo_proof_size(const stwo_proof* p);
+const uint8_t* stwo_proof_bytes(const stwo_proof* p);
+void stwo_proof_digest(const stwo_proof* p, uint8_t out[32]);
+void stwo_proof_claimed_sum(const stwo_proof* p, stwo_qm31* out);
+/* stage wall times in ms: trace commit, interaction trace, composition, OODS, quotients, FRI, PoW,
+ * decommit */
+void stwo_proof_stage_ms(const stwo_proof* p, double out[8]);
+void stwo_proof_free(stwo_proof* p);
+
+/* Trace generation on the device (crates/examples/src/wide_fibonacci/mod.rs:61-83,
+ * poseidon/mod.rs:216-303). mode 0: the reference inputs; mode 1: uniform [0, P) from seed. */
+int stwo_hip_gen_trace_wide_fibonacci(stwo_hip_ctx* ctx, uint32_t* const* cols, uint32_t n_cols, uint32_t log_n_rows,
+                                      uint32_t mode, uint64_t seed);
+int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* ctx, uint32_t* const* cols, uint32_t instances_per_row,
+                                uint32_t log_n_rows, uint32_t mode, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,396 @@
+// Constraint evaluation on the evaluation domain (ComponentProver::
+// evaluate_constraint_quotients_on_domain) for the two benchmark AIRs, their trace generation,
+// and the Poseidon LogUp interaction trace.
+//
+//   wide Fibonacci   crates/examples/src/wide_fibonacci/mod.rs:34-83
+//   Poseidon2        crates/examples/src/poseidon/mod.rs:38-335 (16-wide state, 4+4 full rounds,
+//                    14 partial rounds, x^5, all round constants 1234)
+//   LogUp            crates/constraint-framework/src/lib.rs:149-258 (finalize_logup_in_pairs),
+//                    prover/logup.rs:21-200 (LogupTraceGenerator), simd/prefix_sum.rs
+//   domain driver    crates/constraint-framework/src/prover/component_prover.rs:32-169:
+//                    row_res = sum_k coeff_k C_k(row), accumulated as row_res * denom_inv[row >> trace_log]
+//
+// One lane evaluates one evaluation-domain row; trace columns are column-major, so the 64 lanes
+// of a wave read 256 contiguous bytes of each column. Constraint coefficients are wave-uniform
+// and load through the scalar cache.
+#include "field.h"
+#include "kernels.h"
+
+namespace sh {
+
+__device__ __forceinline__ void acc_add_qm(qacc& acc, qm31 v) {
+  acc.a += v.a; acc.b += v.b; acc.c += v.c; acc.d += v.d;
+}
+
+__global__ void __launch_bounds__(256) wide_fib_comp_kernel(uint32_t n_rows, uint32_t trace_log,
+                                                            const uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                                            const qm31* __restrict__ pows,
+                                                            const uint32_t* __restrict__ denom_inv, Cols4 acc_out) {
+  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  uint32_t a = cols[0][row], b = cols[1][row];
+  qacc acc = qacc_zero();
+  for (uint32_t k = 2; k < n_cols; ++k) {
+    uint32_t c = cols[k][row];
+    uint32_t s = m31_reduce64((uint64_t)a * a + (uint64_t)b * b);
+    qacc_mac(acc, pows[k - 2], m31_sub(c, s));
+    a = b;
+    b = c;
+  }
+  qm31 r = qm_mul_m31(qacc_reduce(acc), denom_inv[row >> trace_log]);
+  qm31 prev = {acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]};
+  r = qm_add(prev, r);
+  acc_out.c[0][row] = r.a; acc_out.c[1][row] = r.b; acc_out.c[2][row] = r.c; acc_out.c[3][row] = r.d;
+}
+
+void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* cols, uint32_t n_cols,
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream) {
+  uint32_t n = 1u << eval_log;
+  hipLaunchKernelGGL(wide_fib_comp_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, trace_log, cols, n_cols,
+                     pows, denom_inv, acc);
+}
+
+// ---- Poseidon2 -------------------------------------------------------------------------------
+constexpr uint32_t PSD_COLS_PER_REP = 158;
+constexpr uint32_t PSD_ROUND_CONST = 1234;
+
+__device__ __forceinline__ uint32_t pow5(uint32_t x) {
+  uint32_t x2 = m31_sqr(x), x4 = m31_sqr(x2);
+  return m31_mul(x4, x);
+}
+__device__ __forceinline__ void apply_m4(uint32_t& x0, uint32_t& x1, uint32_t& x2, uint32_t& x3) {
+  uint32_t t0 = m31_add(x0, x1), t02 = m31_add(t0, t0), t1 = m31_add(x2, x3), t12 = m31_add(t1, t1);
+  uint32_t t2 = m31_add(m31_add(x1, x1), t1), t3 = m31_add(m31_add(x3, x3), t0);
+  uint32_t t4 = m31_add(m31_add(t12, t12), t3), t5 = m31_add(m31_add(t02, t02), t2);
+  uint32_t t6 = m31_add(t3, t5), t7 = m31_add(t2, t4);
+  x0 = t6; x1 = t5; x2 = t7; x3 = t4;
+}
+__device__ __forceinline__ void external_matrix(uint32_t* s) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) apply_m4(s[4 * i], s[4 * i + 1], s[4 * i + 2], s[4 * i + 3]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t t = m31_add(m31_add(s[j], s[j + 4]), m31_add(s[j + 8], s[j + 12]));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[4 * i + j] = m31_add(s[4 * i + j], t);
+  }
+}
+__device__ __forceinline__ void internal_matrix(uint32_t* s) {
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) sum += s[i];
+  uint32_t sm = m31_reduce64(sum);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s[i] = m31_add(m31_mul(s[i], 1u << (i + 1)), sm);
+}
+__device__ __forceinline__ qm31 combine16(const PoseidonParams& p, const uint32_t* v) {
+  qacc acc = qacc_zero();
+#pragma unroll
+  for (int i = 0; i < 16; ++i) qacc_mac(acc, p.alpha_pow[i], v[i]);
+  return qm_sub(qacc_reduce(acc), p.z);
+}
+__device__ __forceinline__ uint32_t offset_prev_row(uint32_t row, uint32_t trace_log, uint32_t eval_log) {
+  // offset_bit_reversed_circle_domain_index(row, trace_log, eval_log, -1) (core/utils.rs)
+  uint32_t prev = bitrev(row, eval_log);
+  uint32_t half = 1u << (eval_log - 1);
+  uint32_t step = 1u << (eval_log - trace_log - 1);
+  if (prev < half) prev = (prev + half - step) & (half - 1);
+  else prev = ((prev - half + step) & (half - 1)) + half;
+  return bitrev(prev, eval_log);
+}
+
+// One Poseidon2 instance: rounds with their 142 constraints; returns the final state in s.
+__device__ __forceinline__ void poseidon_rounds_constraints(uint32_t* s, const uint32_t* const* __restrict__ cols,
+                                                            uint32_t row, const qm31* __restrict__ pows, qacc& acc) {
+  uint32_t ci = 16, k = 0;
+  for (uint32_t r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
+    external_matrix(s);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      uint32_t m = cols[ci + i][row];
+      qacc_mac(acc, pows[k + i], m31_sub(pow5(s[i]), m));
+      s[i] = m;
+    }
+    ci += 16;
+    k += 16;
+  }
+  for (uint32_t r = 0; r < 14; ++r) {
+    s[0] = m31_add(s[0], PSD_ROUND_CONST);
+    internal_matrix(s);
+    uint32_t m = cols[ci][row];
+    qacc_mac(acc, pows[k], m31_sub(pow5(s[0]), m));
+    s[0] = m;
+    ++ci;
+    ++k;
+  }
+  for (uint32_t r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
+    external_matrix(s);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      uint32_t m = cols[ci + i][row];
+      qacc_mac(acc, pows[k + i], m31_sub(pow5(s[i]), m));
+      s[i] = m;
+    }
+    ci += 16;
+    k += 16;
+  }
+}
+
+__global__ void __launch_bounds__(256) poseidon_comp_kernel(uint32_t eval_log, uint32_t trace_log,
+                                                            const uint32_t* const* __restrict__ trace_cols,
+                                                            const uint32_t* const* __restrict__ inter_cols,
+                                                            const PoseidonParams* __restrict__ params,
+                                                            const qm31* __restrict__ pows,
+                                                            const uint32_t* __restrict__ denom_inv, Cols4 acc_out) {
+  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (1u << eval_log)) return;
+  const PoseidonParams& p = *params;
+  const uint32_t ipr = p.ipr;
+  qacc acc = qacc_zero();
+  qm31 prev_col = qm_zero();
+  for (uint32_t rep = 0; rep < ipr; ++rep) {
+    const uint32_t* const* cols = trace_cols + rep * PSD_COLS_PER_REP;
+    uint32_t s[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = cols[i][row];
+    qm31 d0 = combine16(p, s);
+    poseidon_rounds_constraints(s, cols, row, pows + rep * 142, acc);
+    qm31 d1 = combine16(p, s);
+    // pair fraction 1/d0 - 1/d1 = (d1 - d0) / (d0 d1)
+    qm31 num = qm_sub(d1, d0), den = qm_mul(d0, d1);
+    const uint32_t* const* ic = inter_cols + 4 * rep;
+    qm31 cur = {ic[0][row], ic[1][row], ic[2][row], ic[3][row]};
+    qm31 diff;
+    if (rep + 1 < ipr) {
+      diff = qm_sub(cur, prev_col);
+      prev_col = cur;
+    } else {
+      uint32_t pr = offset_prev_row(row, trace_log, eval_log);
+      qm31 prow = {ic[0][pr], ic[1][pr], ic[2][pr], ic[3][pr]};
+      diff = qm_add(qm_sub(qm_sub(cur, prow), prev_col), p.shift);
+    }
+    qm31 cons = qm_sub(qm_mul(diff, den), num);
+    acc_add_qm(acc, qm_mul(pows[ipr * 142 + rep], cons));
+  }
+  qm31 r = qm_mul_m31(qacc_reduce(acc), denom_inv[row >> trace_log]);
+  qm31 prev = {acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]};
+  r = qm_add(prev, r);
+  acc_out.c[0][row] = r.a; acc_out.c[1][row] = r.b; acc_out.c[2][row] = r.c; acc_out.c[3][row] = r.d;
+}
+
+void launch_poseidon_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* trace_cols,
+                                 const uint32_t* const* inter_cols, const PoseidonParams* params, const qm31* pows,
+                                 const uint32_t* denom_inv, Cols4 acc, hipStream_t stream) {
+  uint32_t n = 1u << eval_log;
+  hipLaunchKernelGGL(poseidon_comp_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, eval_log, trace_log,
+                     trace_cols, inter_cols, params, pows, denom_inv, acc);
+}
+
+// ---- trace generation -----------------------------------------------------------------------
+// Seeded inputs: element idx of stream `seed` is the first of splitmix64 outputs
+// (seed + (4 idx + a + 1) * golden), a = 0..3, whose top 31 bits are below P.
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint32_t uniform_m31(uint64_t seed, uint64_t idx) {
+  for (uint64_t a = 0; a < 4; ++a) {
+    uint32_t v = (uint32_t)(splitmix64_at(seed, idx * 4 + a) >> 33);
+    if (v < P) return v;
+  }
+  return 0;
+}
+
+__global__ void gen_wide_fib_kernel(uint32_t n_rows, uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                    uint32_t mode, uint64_t seed) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  uint32_t a = mode == 0 ? 1u : uniform_m31(seed, 2ull * r);
+  uint32_t b = mode == 0 ? (r % P) : uniform_m31(seed, 2ull * r + 1);
+  cols[0][r] = a;
+  cols[1][r] = b;
+  for (uint32_t c = 2; c < n_cols; ++c) {
+    uint32_t x = m31_reduce64((uint64_t)a * a + (uint64_t)b * b);
+    cols[c][r] = x;
+    a = b;
+    b = x;
+  }
+}
+void launch_gen_wide_fib(uint32_t log_n, uint32_t* const* cols, uint32_t n_cols, uint32_t mode, uint64_t seed,
+                         hipStream_t stream) {
+  uint32_t n = 1u << log_n;
+  hipLaunchKernelGGL(gen_wide_fib_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, cols, n_cols, mode, seed);
+}
+
+__global__ void gen_poseidon_kernel(uint32_t n_rows, uint32_t* const* __restrict__ cols, uint32_t ipr, uint32_t mode,
+                                    uint64_t seed) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  for (uint32_t rep = 0; rep < ipr; ++rep) {
+    uint32_t* const* c = cols + rep * PSD_COLS_PER_REP;
+    uint32_t s[16];
+    for (uint32_t i = 0; i < 16; ++i)
+      s[i] = mode == 0 ? (r + i + rep) % P : uniform_m31(seed, ((uint64_t)r * ipr + rep) * 16 + i);
+    uint32_t ci = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) c[ci++][r] = s[i];
+    for (int rd = 0; rd < 4; ++rd) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
+      external_matrix(s);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) { s[i] = pow5(s[i]); c[ci++][r] = s[i]; }
+    }
+    for (int rd = 0; rd < 14; ++rd) {
+      s[0] = m31_add(s[0], PSD_ROUND_CONST);
+      internal_matrix(s);
+      s[0] = pow5(s[0]);
+      c[ci++][r] = s[0];
+    }
+    for (int rd = 0; rd < 4; ++rd) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
+      external_matrix(s);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) { s[i] = pow5(s[i]); c[ci++][r] = s[i]; }
+    }
+  }
+}
+void launch_gen_poseidon(uint32_t log_n, uint32_t* const* cols, uint32_t ipr, uint32_t mode, uint64_t seed,
+                         hipStream_t stream) {
+  uint32_t n = 1u << log_n;
+  hipLaunchKernelGGL(gen_poseidon_kernel, dim3((n + 127) / 128), dim3(128), 0, stream, n, cols, ipr, mode, seed);
+}
+
+// LogupTraceGenerator for the Poseidon relation: column rep holds sum_{r' <= rep} frac_{r'} per row;
+// the last column's coordinate sums (over all rows) are accumulated into sums[0..4].
+__global__ void __launch_bounds__(256) poseidon_logup_kernel(uint32_t n_rows,
+                                                             const uint32_t* const* __restrict__ trace_cols,
+                                                             uint32_t* const* __restrict__ inter_cols,
+                                                             const PoseidonParams* __restrict__ params,
+                                                             unsigned long long* __restrict__ sums) {
+  __shared__ unsigned long long red[4][256];
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const PoseidonParams& p = *params;
+  qm31 cum = qm_zero();
+  if (r < n_rows) {
+    for (uint32_t rep = 0; rep < p.ipr; ++rep) {
+      const uint32_t* const* c = trace_cols + rep * PSD_COLS_PER_REP;
+      uint32_t s[16];
+      for (int i = 0; i < 16; ++i) s[i] = c[i][r];
+      qm31 d0 = combine16(p, s);
+      for (int i = 0; i < 16; ++i) s[i] = c[PSD_COLS_PER_REP - 16 + i][r];
+      qm31 d1 = combine16(p, s);
+      qm31 frac = qm_mul(qm_sub(d1, d0), qm_inv(qm_mul(d0, d1)));
+      cum = qm_add(cum, frac);
+      uint32_t* const* ic = inter_cols + 4 * rep;
+      ic[0][r] = cum.a; ic[1][r] = cum.b; ic[2][r] = cum.c; ic[3][r] = cum.d;
+    }
+  }
+  red[0][threadIdx.x] = r < n_rows ? cum.a : 0;
+  red[1][threadIdx.x] = r < n_rows ? cum.b : 0;
+  red[2][threadIdx.x] = r < n_rows ? cum.c : 0;
+  red[3][threadIdx.x] = r < n_rows ? cum.d : 0;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) atomicAdd(&sums[threadIdx.x], red[threadIdx.x][0]);
+}
+void launch_poseidon_logup_fracs(uint32_t log_n, const uint32_t* const* trace_cols, uint32_t* const* inter_cols,
+                                 const PoseidonParams* params, unsigned long long* sums, hipStream_t stream) {
+  uint32_t n = 1u << log_n;
+  hipLaunchKernelGGL(poseidon_logup_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, trace_cols, inter_cols,
+                     params, sums);
+}
+
+// Inclusive prefix sum in coset order of (value - shift), for 4 coordinate columns stored in
+// bit-reversed circle-domain order (simd/prefix_sum.rs): coset index c maps to circle index
+// c/2 (c even) or n-1-(c-1)/2 (c odd), stored at bitrev(circle index).
+__device__ __forceinline__ uint32_t coset_storage(uint32_t c, uint32_t log_n) {
+  uint32_t n = 1u << log_n;
+  uint32_t circ = (c & 1) ? n - 1 - (c >> 1) : (c >> 1);
+  return bitrev(circ, log_n);
+}
+constexpr uint32_t SCAN_BLOCK = 1024;
+
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_local_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols,
+                                                               qm31 shift, uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t sm[SCAN_BLOCK];
+  uint32_t c = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  uint32_t k = blockIdx.y;
+  uint32_t sh = k == 0 ? shift.a : k == 1 ? shift.b : k == 2 ? shift.c : shift.d;
+  uint32_t idx = coset_storage(c, log_n);
+  uint32_t v = m31_sub(cols[k][idx], sh);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < SCAN_BLOCK; off <<= 1) {
+    uint32_t add = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
+    __syncthreads();
+    sm[threadIdx.x] = m31_add(sm[threadIdx.x], add);
+    __syncthreads();
+  }
+  cols[k][idx] = sm[threadIdx.x];
+  if (threadIdx.x == SCAN_BLOCK - 1) block_sums[k * gridDim.x + blockIdx.x] = sm[threadIdx.x];
+}
+
+__global__ void scan_blocks_kernel(uint32_t n_blocks, uint32_t* __restrict__ block_sums) {
+  uint32_t k = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  uint32_t run = 0;
+  for (uint32_t b = 0; b < n_blocks; ++b) {
+    uint32_t v = block_sums[k * n_blocks + b];
+    block_sums[k * n_blocks + b] = run;  // exclusive
+    run = m31_add(run, v);
+  }
+}
+
+__global__ void scan_add_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols,
+                                const uint32_t* __restrict__ block_sums) {
+  uint32_t c = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  uint32_t k = blockIdx.y;
+  uint32_t idx = coset_storage(c, log_n);
+  cols[k][idx] = m31_add(cols[k][idx], block_sums[k * gridDim.x + blockIdx.x]);
+}
+
+// Small columns (n < SCAN_BLOCK) are scanned by one workgroup with n lanes.
+__global__ void scan_small_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols, qm31 shift) {
+  __shared__ uint32_t sm[SCAN_BLOCK];
+  uint32_t n = 1u << log_n;
+  uint32_t k = blockIdx.y;
+  uint32_t sh = k == 0 ? shift.a : k == 1 ? shift.b : k == 2 ? shift.c : shift.d;
+  uint32_t c = threadIdx.x;
+  uint32_t idx = c < n ? coset_storage(c, log_n) : 0;
+  sm[c] = c < n ? m31_sub(cols[k][idx], sh) : 0;
+  __syncthreads();
+  for (uint32_t off = 1; off < n; off <<= 1) {
+    uint32_t add = (c < n && c >= off) ? sm[c - off] : 0;
+    __syncthreads();
+    if (c < n) sm[c] = m31_add(sm[c], add);
+    __syncthreads();
+  }
+  if (c < n) cols[k][idx] = sm[c];
+}
+
+void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, uint32_t* scratch,
+                             hipStream_t stream) {
+  uint32_t n = 1u << log_n;
+  if (n < SCAN_BLOCK) {
+    hipLaunchKernelGGL(scan_small_kernel, dim3(1, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, shift);
+    return;
+  }
+  uint32_t nb = n / SCAN_BLOCK;
+  hipLaunchKernelGGL(scan_local_kernel, dim3(nb, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, shift, scratch);
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(4), dim3(64), 0, stream, nb, scratch);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, scratch);
+}
+
+}  // namespace sh

@@ -1,0 +1,450 @@
+// C entry points of libstwo_hip (declared in include/stwo_hip.h).
+#include "../../include/stwo_hip.h"
+
+#include <cstring>
+
+#include "prover.h"
+
+using namespace sh;
+
+struct stwo_hip_ctx {
+  Ctx* ctx = nullptr;
+  std::string err;
+};
+struct stwo_proof {
+  ProveResult r;
+};
+
+namespace {
+
+template <class F>
+int guarded(stwo_hip_ctx* h, F&& f) {
+  if (!h || !h->ctx) return 1;
+  try {
+    f(*h->ctx);
+    return 0;
+  } catch (const HipError& e) {
+    h->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return 1;
+  }
+}
+inline qm31 q(stwo_qm31 v) { return {v.a, v.b, v.c, v.d}; }
+inline stwo_qm31 q(qm31 v) { return {v.a, v.b, v.c, v.d}; }
+inline cpqm31 cp(stwo_circle_point_qm31 p) { return {q(p.x), q(p.y)}; }
+void check_canonical(const stwo_qm31& v) {
+  if (v.a >= P || v.b >= P || v.c >= P || v.d >= P) throw HipError(1, "non-canonical QM31 argument");
+}
+PcsCfg cfg_of(const stwo_pcs_config* c) {
+  PcsCfg r;
+  if (!c) return r;
+  r.pow_bits = c->pow_bits;
+  r.log_blowup = c->log_blowup_factor;
+  r.log_last_layer = c->log_last_layer_degree_bound;
+  r.n_queries = c->n_queries;
+  r.fold_step = c->line_fold_step;
+  r.lifting_log_size = c->lifting_log_size;
+  if (r.log_blowup < 1 || r.log_blowup > 16 || r.log_last_layer > 10 || r.fold_step < 1 || r.n_queries < 1)
+    throw HipError(1, "invalid PcsConfig");
+  return r;
+}
+uint32_t fnv1a(const char* s) {
+  uint32_t h = 0x811C9DC5u;
+  for (; *s; ++s) { h ^= (uint8_t)*s; h *= 0x01000193u; }
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stwo_hip_create(int device, stwo_hip_ctx** out) {
+  if (!out) return 1;
+  auto* h = new stwo_hip_ctx();
+  try {
+    h->ctx = new Ctx(device);
+  } catch (const std::exception& e) {
+    delete h;
+    *out = nullptr;
+    return 2;
+  }
+  *out = h;
+  return 0;
+}
+void stwo_hip_destroy(stwo_hip_ctx* h) {
+  if (!h) return;
+  delete h->ctx;
+  delete h;
+}
+int stwo_hip_synchronize(stwo_hip_ctx* h) { return guarded(h, [&](Ctx& c) { c.sync(); c.reset_arena(); }); }
+const char* stwo_hip_last_error(stwo_hip_ctx* h) { return h ? h->err.c_str() : "no context"; }
+
+int stwo_hip_alloc(stwo_hip_ctx* h, size_t n, int zeroed, uint32_t** out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!out) throw HipError(1, "null out");
+    HIP_CHECK(hipMallocAsync((void**)out, n ? n * 4 : 4, c.stream));
+    if (zeroed) launch_fill_zero(*out, n, c.stream);
+  });
+}
+int stwo_hip_free(stwo_hip_ctx* h, uint32_t* p) {
+  return guarded(h, [&](Ctx& c) { if (p) HIP_CHECK(hipFreeAsync(p, c.stream)); });
+}
+int stwo_hip_copy_h2d(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_t n) {
+  return guarded(h, [&](Ctx& c) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, c.stream));
+    c.sync();
+  });
+}
+int stwo_hip_copy_d2h(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_t n) {
+  return guarded(h, [&](Ctx& c) { c.d2h(dst, src, n * 4); });
+}
+int stwo_hip_batch_at_multi(stwo_hip_ctx* h, const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows,
+                            uint32_t n_rows, uint32_t* out_host) {
+  return guarded(h, [&](Ctx& c) {
+    std::vector<const uint32_t*> bases(cols, cols + n_cols);
+    std::vector<uint32_t> bi, of;
+    for (uint32_t r = 0; r < n_rows; ++r)
+      for (uint32_t k = 0; k < n_cols; ++k) { bi.push_back(k); of.push_back(rows[r]); }
+    DevBuf o(&c, bi.size());
+    launch_gather_words((const uint32_t* const*)c.upload_vec(bases), c.upload_vec(bi), c.upload_vec(of),
+                        (uint32_t)bi.size(), o.p, c.stream);
+    c.d2h(out_host, o.p, bi.size() * 4);
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_precompute_twiddles(stwo_hip_ctx* h, uint32_t log_domain, uint32_t* tw, uint32_t* itw) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_domain < 2 || log_domain > 30) throw HipError(3, "log_domain out of range");
+    c.ensure_twiddles(log_domain);
+    size_t n = (size_t)1 << (log_domain - 1);
+    // the cached tree may be larger: a smaller domain's tree is the tail of the larger one
+    size_t off = c.tree_len() - n;
+    if (tw) {
+      std::vector<uint32_t> t(n);
+      c.d2h(t.data(), c.tw.p + off, n * 4);
+      // layers of the smaller tree are the last layers of the larger tree, followed by the pad
+      memcpy(tw, t.data(), n * 4);
+    }
+    if (itw) c.d2h(itw, c.itw.p + off, n * 4);
+  });
+}
+
+int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, uint32_t* const* coeffs,
+                                 uint32_t n_cols, uint32_t log_size) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size < 3 || log_size > 30) throw HipError(3, "log_size must be in [3, 30]");
+    c.ensure_twiddles(log_size);
+    std::vector<const uint32_t*> s(evals, evals + n_cols);
+    std::vector<uint32_t*> d(coeffs, coeffs + n_cols);
+    launch_fft(true, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
+               log_size, c.itw.p, c.tree_len(), c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs, uint32_t* const* evals,
+                                  uint32_t n_cols, uint32_t log_size, uint32_t log_blowup) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size < 3 || log_size + log_blowup > 30) throw HipError(3, "sizes out of range");
+    c.ensure_twiddles(log_size + log_blowup);
+    std::vector<const uint32_t*> s(coeffs, coeffs + n_cols);
+    std::vector<uint32_t*> d(evals, evals + n_cols);
+    launch_fft(false, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
+               log_size + log_blowup, c.tw.p, c.tree_len(), c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_batch_eval_at_point(stwo_hip_ctx* h, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
+                                 stwo_circle_point_qm31 point, stwo_qm31* out_host) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size > 30) throw HipError(3, "log_size out of range");
+    check_canonical(point.x);
+    check_canonical(point.y);
+    std::vector<OodsTask> tasks;
+    std::vector<qm31> f(32, qm_zero());
+    cpqm31 p = cp(point);
+    f[0] = p.y;
+    qm31 x = p.x;
+    for (uint32_t b = 1; b < log_size; ++b) { f[b] = x; x = qm_double_x(x); }
+    std::vector<qm31> factors;
+    for (uint32_t i = 0; i < n_cols; ++i) {
+      tasks.push_back({coeffs[i], log_size, i, {0, 0}});
+      factors.insert(factors.end(), f.begin(), f.end());
+    }
+    uint32_t max_chunks = log_size > 11 ? 1u << (log_size - 11) : 1u;
+    DevBuf partials(&c, (size_t)n_cols * max_chunks * 4), vals(&c, (size_t)n_cols * 4);
+    launch_oods(c.upload_vec(tasks), n_cols, log_size, c.upload_vec(factors), (qm31*)partials.p, (qm31*)vals.p,
+                c.stream);
+    c.d2h(out_host, vals.p, (size_t)n_cols * 16);
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_commit_on_layer(stwo_hip_ctx* h, uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols,
+                             uint32_t n_cols, uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size > 30) throw HipError(3, "log_size out of range");
+    std::vector<const uint32_t*> v(cols, cols + n_cols);
+    const uint32_t* const* d = n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
+    launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4],
+                                   uint32_t* const dst[4], stwo_qm31 alpha) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_src < 3 || log_src > 30) throw HipError(3, "log_src must be in [3, 30]");
+    check_canonical(alpha);
+    c.ensure_twiddles(log_src);
+    launch_fold_circle_into_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
+                                 q(alpha), c.itw.p, c.tree_len(), c.stream);
+    c.sync();
+  });
+}
+int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
+                       stwo_qm31 alpha) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_src < 1 || log_src > 29) throw HipError(3, "log_src must be in [1, 29]");
+    check_canonical(alpha);
+    c.ensure_twiddles(log_src + 1);
+    launch_fold_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}}, q(alpha), c.itw.p,
+                     c.tree_len(), c.stream);
+    c.sync();
+  });
+}
+
+int stwo_hip_accumulate_numerators(stwo_hip_ctx* h, uint32_t log_size, const uint32_t* const* cols, uint32_t n_cols,
+                                   const stwo_sample_batch* batches, uint32_t n_batches, uint32_t* const* out,
+                                   stwo_qm31* first_linear_host) {
+  return guarded(h, [&](Ctx& c) {
+    std::vector<NumEntry> ents;
+    std::vector<uint32_t> starts;
+    std::vector<qm31> Bs;
+    for (uint32_t b = 0; b < n_batches; ++b) {
+      const auto& bt = batches[b];
+      cpqm31 p = cp(bt.point);
+      if (qm_eq(p.y, qm_conj(p.y))) throw HipError(1, "Cannot evaluate a line with a single point");
+      starts.push_back((uint32_t)ents.size());
+      qm31 A = qm_zero(), B = qm_zero();
+      for (uint32_t j = 0; j < bt.n_columns; ++j) {
+        if (bt.column_indices[j] >= n_cols) throw HipError(1, "column index out of range");
+        qm31 v = q(bt.sample_values[j]), al = q(bt.random_coeffs[j]);
+        qm31 aa = qm_sub(qm_conj(v), v), cc = qm_sub(qm_conj(p.y), p.y);
+        qm31 bb = qm_sub(qm_mul(v, cc), qm_mul(aa, p.y));
+        NumEntry e{};
+        e.col = bt.column_indices[j];
+        e.c = qm_mul(al, cc);
+        ents.push_back(e);
+        A = qm_add(A, qm_mul(al, aa));
+        B = qm_add(B, qm_mul(al, bb));
+      }
+      Bs.push_back(B);
+      if (first_linear_host) first_linear_host[b] = q(A);
+    }
+    starts.push_back((uint32_t)ents.size());
+    std::vector<const uint32_t*> cv(cols, cols + n_cols);
+    std::vector<uint32_t*> ov(out, out + 4 * (size_t)n_batches);
+    launch_accumulate_numerators(log_size, (const uint32_t* const*)c.upload_vec(cv), c.upload_vec(ents),
+                                 c.upload_vec(starts), c.upload_vec(Bs), n_batches,
+                                 (uint32_t* const*)c.upload_vec(ov), c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_compute_quotients_and_combine(stwo_hip_ctx* h, const stwo_accumulated_numerators* accs, uint32_t n_accs,
+                                           uint32_t lifting_log_size, uint32_t* const out[4]) {
+  return guarded(h, [&](Ctx& c) {
+    if (lifting_log_size < 1 || lifting_log_size > 30) throw HipError(3, "lifting_log_size out of range");
+    std::vector<QuotPoint> pts;
+    std::vector<QuotPart> parts;
+    for (uint32_t a = 0; a < n_accs; ++a) {
+      if (accs[a].log_size > lifting_log_size) throw HipError(1, "accumulation larger than the lifting domain");
+      cpqm31 p = cp(accs[a].sample_point);
+      QuotPoint qp{};
+      qp.prx = qm_lo(p.x); qp.pix = qm_hi(p.x); qp.pry = qm_lo(p.y); qp.piy = qm_hi(p.y);
+      qp.first_linear = q(accs[a].first_linear_term);
+      qp.part_begin = (uint32_t)parts.size();
+      QuotPart pa{};
+      for (int k = 0; k < 4; ++k) pa.c[k] = accs[a].partial[k];
+      pa.log_size = accs[a].log_size;
+      parts.push_back(pa);
+      qp.part_end = (uint32_t)parts.size();
+      pts.push_back(qp);
+    }
+    launch_quotients(lifting_log_size, c.domain_xy(lifting_log_size), c.upload_vec(pts), n_accs, c.upload_vec(parts),
+                     {{out[0], out[1], out[2], out[3]}}, c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_accumulate(stwo_hip_ctx* h, uint32_t log_size, uint32_t* const dst[4], const uint32_t* const src[4]) {
+  return guarded(h, [&](Ctx& c) {
+    launch_accumulate_lifted(log_size, {{dst[0], dst[1], dst[2], dst[3]}}, log_size, {{src[0], src[1], src[2], src[3]}},
+                             c.stream);
+    c.sync();
+  });
+}
+int stwo_hip_lift_and_accumulate(stwo_hip_ctx* h, const uint32_t* const* cols4, const uint32_t* log_sizes,
+                                 uint32_t n_cols, uint32_t* const out[4]) {
+  return guarded(h, [&](Ctx& c) {
+    if (n_cols == 0) throw HipError(1, "no columns");
+    for (uint32_t i = 1; i < n_cols; ++i)
+      if (log_sizes[i] < log_sizes[i - 1]) throw HipError(1, "columns must be sorted by increasing size");
+    if (log_sizes[0] < 1) throw HipError(1, "A column must be of length at least 2.");
+    uint32_t L = log_sizes[n_cols - 1];
+    Cols4 o = {{out[0], out[1], out[2], out[3]}};
+    for (int k = 0; k < 4; ++k) launch_copy_words(out[k], cols4[4 * (n_cols - 1) + k], (size_t)1 << L, c.stream);
+    for (uint32_t i = 0; i + 1 < n_cols; ++i)
+      launch_accumulate_lifted(L, o, log_sizes[i], {{cols4[4 * i], cols4[4 * i + 1], cols4[4 * i + 2], cols4[4 * i + 3]}},
+                               c.stream);
+    c.sync();
+  });
+}
+int stwo_hip_generate_secure_powers(stwo_hip_ctx* h, stwo_qm31 felt, uint32_t n, stwo_qm31* out) {
+  return guarded(h, [&](Ctx&) {
+    qm31 p = qm_one(), f = q(felt);
+    for (uint32_t i = 0; i < n; ++i) { out[i] = q(p); p = qm_mul(p, f); }
+  });
+}
+
+int stwo_hip_grind(stwo_hip_ctx* h, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce) {
+  return guarded(h, [&](Ctx& c) {
+    if (pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
+    Channel ch;
+    memcpy(ch.digest.data(), digest, 32);
+    Hash pre = ch.pow_prefix(pow_bits);
+    uint32_t pw[8];
+    for (int i = 0; i < 8; ++i) pw[i] = (uint32_t)pre[4 * i] | ((uint32_t)pre[4 * i + 1] << 8) | ((uint32_t)pre[4 * i + 2] << 16) | ((uint32_t)pre[4 * i + 3] << 24);
+    const uint32_t* dpre = (const uint32_t*)c.upload(pw, 32);
+    DevBuf res(&c, 2);
+    unsigned long long none = ~0ull, found = ~0ull;
+    HIP_CHECK(hipMemcpyAsync(res.p, &none, 8, hipMemcpyHostToDevice, c.stream));
+    for (uint32_t hi = 0; found == ~0ull; hi += 16) {
+      launch_grind(dpre, pow_bits, hi, 16, (unsigned long long*)res.p, c.stream);
+      c.d2h(&found, res.p, 8);
+      if (hi > 0x7fffffffu - 16) throw HipError(4, "Grind failed to find a solution.");
+    }
+    *nonce = found;
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* const acc[4],
+                                                     const uint32_t* const* trace, uint32_t n_trace,
+                                                     const uint32_t* const* interaction, uint32_t n_interaction,
+                                                     const stwo_qm31* coeff_powers, uint32_t n_constraints,
+                                                     const uint32_t* denom_inv_host, uint32_t trace_log,
+                                                     uint32_t eval_log, const uint8_t* blob, size_t blob_len,
+                                                     stwo_qm31 cumsum_shift) {
+  return guarded(h, [&](Ctx& c) {
+    if (!blob || blob_len < 4) throw HipError(1, "eval_blob too short");
+    if (eval_log <= trace_log || eval_log > 30) throw HipError(3, "eval_log must exceed trace_log");
+    uint32_t id;
+    memcpy(&id, blob, 4);
+    std::vector<qm31> pows(n_constraints);
+    for (uint32_t i = 0; i < n_constraints; ++i) pows[i] = q(coeff_powers[i]);
+    std::vector<uint32_t> dinv(denom_inv_host, denom_inv_host + (1u << (eval_log - trace_log)));
+    Cols4 a4 = {{acc[0], acc[1], acc[2], acc[3]}};
+    std::vector<const uint32_t*> tv(trace, trace + n_trace);
+    if (id == fnv1a("fibonacci_example")) {
+      if (n_trace < 2 || n_constraints != n_trace - 2) throw HipError(1, "wide-fib: n_constraints must be n_trace - 2");
+      launch_wide_fib_composition(eval_log, trace_log, (const uint32_t* const*)c.upload_vec(tv), n_trace,
+                                  c.upload_vec(pows), c.upload_vec(dinv), a4, c.stream);
+    } else if (id == fnv1a("poseidon_example")) {
+      if (blob_len < 4 + 4 + 32) throw HipError(1, "poseidon eval_blob too short");
+      PoseidonParams pp{};
+      memcpy(&pp.ipr, blob + 4, 4);
+      stwo_qm31 z, al;
+      memcpy(&z, blob + 8, 16);
+      memcpy(&al, blob + 24, 16);
+      if (pp.ipr == 0 || n_trace != 158 * pp.ipr || n_interaction != 4 * pp.ipr || n_constraints != 143 * pp.ipr)
+        throw HipError(1, "poseidon: column or constraint counts do not match instances_per_row");
+      pp.z = q(z);
+      qm31 a = q(al), p = qm_one();
+      for (int i = 0; i < 16; ++i) { pp.alpha_pow[i] = p; p = qm_mul(p, a); }
+      pp.shift = q(cumsum_shift);
+      std::vector<const uint32_t*> iv(interaction, interaction + n_interaction);
+      launch_poseidon_composition(eval_log, trace_log, (const uint32_t* const*)c.upload_vec(tv),
+                                  (const uint32_t* const*)c.upload_vec(iv), (const PoseidonParams*)c.upload(&pp, sizeof(pp)),
+                                  c.upload_vec(pows), c.upload_vec(dinv), a4, c.stream);
+    } else {
+      throw HipError(5, "no device kernel for this component");
+    }
+    c.sync();
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_prove_wide_fibonacci(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_cols, uint32_t log_n_rows,
+                                  const stwo_pcs_config* config, stwo_proof** out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!out) throw HipError(1, "null out");
+    if (n_cols < 3) throw HipError(1, "wide-fib needs at least 3 columns");
+    if (log_n_rows < 3 || log_n_rows > 26) throw HipError(3, "log_n_rows must be in [3, 26]");
+    PcsCfg cfg = cfg_of(config);
+    std::vector<const uint32_t*> t(trace, trace + n_cols);
+    auto* p = new stwo_proof();
+    try {
+      p->r = prove_wide_fib(c, t, log_n_rows, n_cols, cfg);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+int stwo_hip_prove_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr, uint32_t log_n_rows,
+                            const stwo_pcs_config* config, stwo_proof** out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!out) throw HipError(1, "null out");
+    if (ipr < 1 || ipr > 64) throw HipError(1, "instances_per_row must be in [1, 64]");
+    if (log_n_rows < 3 || log_n_rows > 24) throw HipError(3, "log_n_rows must be in [3, 24]");
+    PcsCfg cfg = cfg_of(config);
+    std::vector<const uint32_t*> t(trace, trace + 158 * ipr);
+    auto* p = new stwo_proof();
+    try {
+      p->r = prove_poseidon(c, t, log_n_rows, ipr, cfg);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+int stwo_proof_status(const stwo_proof* p) { return p ? p->r.status : -1; }
+size_t stwo_proof_size(const stwo_proof* p) { return p ? p->r.proof.size() : 0; }
+const uint8_t* stwo_proof_bytes(const stwo_proof* p) { return p ? p->r.proof.data() : nullptr; }
+void stwo_proof_digest(const stwo_proof* p, uint8_t out[32]) { if (p) memcpy(out, p->r.digest.data(), 32); }
+void stwo_proof_claimed_sum(const stwo_proof* p, stwo_qm31* out) { if (p && out) *out = q(p->r.claimed_sum); }
+void stwo_proof_stage_ms(const stwo_proof* p, double out[8]) { if (p) memcpy(out, p->r.stage_ms, sizeof(double) * 8); }
+void stwo_proof_free(stwo_proof* p) { delete p; }
+
+int stwo_hip_gen_trace_wide_fibonacci(stwo_hip_ctx* h, uint32_t* const* cols, uint32_t n_cols, uint32_t log_n_rows,
+                                      uint32_t mode, uint64_t seed) {
+  return guarded(h, [&](Ctx& c) {
+    if (n_cols < 2 || log_n_rows > 28) throw HipError(1, "bad trace shape");
+    std::vector<uint32_t*> v(cols, cols + n_cols);
+    launch_gen_wide_fib(log_n_rows, (uint32_t* const*)c.upload_vec(v), n_cols, mode, seed, c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* h, uint32_t* const* cols, uint32_t ipr, uint32_t log_n_rows, uint32_t mode,
+                                uint64_t seed) {
+  return guarded(h, [&](Ctx& c) {
+    if (ipr < 1 || ipr > 64 || log_n_rows > 26) throw HipError(1, "bad trace shape");
+    std::vector<uint32_t*> v(cols, cols + 158 * ipr);
+    launch_gen_poseidon(log_n_rows, (uint32_t* const*)c.upload_vec(v), ipr, mode, seed, c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+
+}  // extern "C"

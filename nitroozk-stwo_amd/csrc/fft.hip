@@ -1,0 +1,146 @@
+// Circle FFT over batches of same-size M31 columns (PolyOps::interpolate / evaluate).
+//
+// Layer structure (prover/backend/cpu/circle.rs:28-80,190-228,295-335):
+//   stage 0 (circle layer): pairs (2h, 2h+1), twiddle from line layer 0 pairs [x, y] -> [y,-y,-x,x]
+//   stage s >= 1 (line layer s-1): pairs (idx, idx + 2^s), twiddle tree[len - 2^(K-s) + (idx >> (s+1))]
+// interpolate runs stages 0..K-1 with inverse butterflies and inverse twiddles, then scales by
+// 2^-K; evaluate runs stages K-1..0 with forward butterflies. Evaluating k coefficients onto a
+// 2^b-times larger domain skips the top b stages: on zero-padded input they only copy, so the
+// first pass reads the coefficient column with the index masked to 2^k.
+//
+// MI355X mapping: one workgroup owns a tile of 2^S * 2^c words in LDS and runs S consecutive
+// stages on it. The first (lowest) stages use contiguous tiles (c = 0); higher stages use
+// strided tiles made of 2^c adjacent sub-columns so that global loads stay coalesced.
+#include "field.h"
+#include "kernels.h"
+
+namespace sh {
+
+template <bool INV>
+__device__ __forceinline__ void bfly(uint32_t& v0, uint32_t& v1, uint32_t t) {
+  if (INV) {  // ibutterfly(v0, v1, t) = (v0 + v1, (v0 - v1) t)
+    uint32_t a = v0, b = v1;
+    v0 = m31_add(a, b);
+    v1 = m31_mul(m31_sub(a, b), t);
+  } else {  // butterfly(v0, v1, t) = (v0 + v1 t, v0 - v1 t)
+    uint32_t m = m31_mul(v1, t);
+    uint32_t a = v0;
+    v0 = m31_add(a, m);
+    v1 = m31_sub(a, m);
+  }
+}
+
+__device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* __restrict__ lt0, uint32_t h) {
+  uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
+  switch (h & 3) {
+    case 0: return y;
+    case 1: return m31_neg(y);
+    case 2: return m31_neg(x);
+    default: return x;
+  }
+}
+
+template <bool INV>
+__global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __restrict__ src,
+                                                       uint32_t* const* __restrict__ dst, uint32_t log_n,
+                                                       uint32_t in_mask, uint32_t e, uint32_t S, uint32_t c,
+                                                       const uint32_t* __restrict__ tree, uint32_t tree_len,
+                                                       uint32_t scale) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t col = blockIdx.y;
+  const uint32_t* __restrict__ in = src[col];
+  uint32_t* __restrict__ out = dst[col];
+  const uint32_t C = 1u << c;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t lo_blk = tile & ((1u << (e - c)) - 1);
+  const uint32_t hi = tile >> (e - c);
+  const uint32_t base = (hi << (e + S)) | (lo_blk << c);
+  const uint32_t tile_n = 1u << (S + c);
+  // load
+  for (uint32_t l = threadIdx.x; l < tile_n; l += blockDim.x) {
+    uint32_t t = l >> c, lo = l & (C - 1);
+    uint32_t idx = base | (t << e) | lo;
+    lds[l] = in[idx & in_mask];
+  }
+  __syncthreads();
+  const uint32_t npairs = tile_n >> 1;
+  for (uint32_t jj = 0; jj < S; ++jj) {
+    const uint32_t j = INV ? jj : S - 1 - jj;  // local stage bit
+    const uint32_t s = e + j;                  // global stage
+    for (uint32_t q = threadIdx.x; q < npairs; q += blockDim.x) {
+      uint32_t lo = q & (C - 1), r = q >> c;
+      uint32_t t0 = ((r >> j) << (j + 1)) | (r & ((1u << j) - 1));
+      uint32_t t1 = t0 | (1u << j);
+      uint32_t idx0 = base | (t0 << e) | lo;
+      uint32_t tw;
+      if (s == 0) tw = circle_twiddle(tree + tree_len - (1u << (log_n - 1)), idx0 >> 1);
+      else tw = tree[tree_len - (1u << (log_n - s)) + (idx0 >> (s + 1))];
+      uint32_t a = lds[t0 * C + lo], b = lds[t1 * C + lo];
+      bfly<INV>(a, b, tw);
+      lds[t0 * C + lo] = a;
+      lds[t1 * C + lo] = b;
+    }
+    __syncthreads();
+  }
+  for (uint32_t l = threadIdx.x; l < tile_n; l += blockDim.x) {
+    uint32_t t = l >> c, lo = l & (C - 1);
+    uint32_t idx = base | (t << e) | lo;
+    uint32_t v = lds[l];
+    out[idx] = scale ? m31_mul(v, scale) : v;
+  }
+}
+
+// Host-side pass planning shared by interpolate and evaluate.
+struct FftPass { uint32_t e, S, c; };
+static std::vector<FftPass> plan_passes(uint32_t k, uint32_t K) {
+  // stages [0, k) split into a contiguous first chunk and strided higher chunks
+  std::vector<FftPass> p;
+  const uint32_t TILE_LOG = 12;
+  uint32_t first = k < TILE_LOG ? k : TILE_LOG;
+  p.push_back({0, first, 0});
+  uint32_t e = first;
+  while (e < k) {
+    uint32_t S = k - e;
+    if (S > 8) S = 8;
+    uint32_t c = TILE_LOG - S;
+    if (c > e) c = e;
+    // the tile must fit in the column: S + c <= K
+    p.push_back({e, S, c});
+    e += S;
+  }
+  (void)K;
+  return p;
+}
+
+void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
+                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream) {
+  if (n_cols == 0) return;
+  uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
+  auto passes = plan_passes(k, log_n);
+  uint32_t inv_n = inverse ? m31_inv(1u << log_n) : 0;
+  if (inverse) {
+    for (size_t i = 0; i < passes.size(); ++i) {
+      auto& ps = passes[i];
+      uint32_t tiles = 1u << (log_n - ps.S - ps.c);
+      uint32_t tile_n = 1u << (ps.S + ps.c);
+      uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
+      bool last = i + 1 == passes.size();
+      hipLaunchKernelGGL(fft_pass_kernel<true>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream,
+                         i == 0 ? src : (const uint32_t* const*)dst, dst, log_n, (1u << log_n) - 1, ps.e, ps.S, ps.c,
+                         tree, tree_len, last ? inv_n : 0u);
+    }
+  } else {
+    for (size_t ii = passes.size(); ii-- > 0;) {
+      auto& ps = passes[ii];
+      uint32_t tiles = 1u << (log_n - ps.S - ps.c);
+      uint32_t tile_n = 1u << (ps.S + ps.c);
+      uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
+      bool first = ii + 1 == passes.size();
+      hipLaunchKernelGGL(fft_pass_kernel<false>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream,
+                         first ? src : (const uint32_t* const*)dst, dst, log_n,
+                         first ? (1u << log_in) - 1 : (1u << log_n) - 1, ps.e, ps.S, ps.c, tree, tree_len, 0u);
+    }
+  }
+}
+
+}  // namespace sh

@@ -1,0 +1,74 @@
+// Launch wrappers for the HIP kernels of libstwo_hip. All take device pointers and a stream;
+// none allocates or synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <vector>
+#include "field.h"
+
+namespace sh {
+
+struct Cols4 { uint32_t* c[4]; };
+struct CCols4 { const uint32_t* c[4]; };
+
+// fft.hip
+void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
+                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream);
+
+// merkle.hip
+void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
+                         uint32_t* out, hipStream_t stream);
+void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint32_t hi_base, uint32_t n_hi,
+                  unsigned long long* result, hipStream_t stream);
+
+// ops.hip
+void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
+                                  uint32_t tree_len, hipStream_t stream);
+void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
+                      hipStream_t stream);
+
+// Fused accumulate_numerators for one size group. Entries of batch b are
+// [batch_start[b], batch_start[b+1]); out holds 4 coordinate pointers per batch.
+struct NumEntry { uint32_t col; uint32_t pad[3]; qm31 c; };
+void launch_accumulate_numerators(uint32_t log_size, const uint32_t* const* cols, const NumEntry* entries,
+                                  const uint32_t* batch_start, const qm31* batch_b, uint32_t n_batches,
+                                  uint32_t* const* out, hipStream_t stream);
+
+struct QuotPoint { cm31 prx, pix, pry, piy; qm31 first_linear; uint32_t part_begin, part_end, pad[2]; };
+struct QuotPart { const uint32_t* c[4]; uint32_t log_size; uint32_t pad[3]; };
+void launch_quotients(uint32_t lifting_log, const uint32_t* domain_xy, const QuotPoint* points, uint32_t n_points,
+                      const QuotPart* parts, Cols4 out, hipStream_t stream);
+
+void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCols4 src, hipStream_t stream);
+
+// OODS: value of FFT-basis coefficient columns at QM31 points. factors[t*32 + b] holds the
+// folding factor of bit b for task t (y, x, pi(x), ...).
+struct OodsTask { const uint32_t* coeffs; uint32_t log_size; uint32_t out_index; uint32_t pad[2]; };
+void launch_oods(const OodsTask* tasks, uint32_t n_tasks, uint32_t max_log, const qm31* factors, qm31* partials,
+                 qm31* out, hipStream_t stream);
+
+void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
+                         uint32_t* out, hipStream_t stream);
+
+// Composition (constraint quotients on the evaluation domain), accumulated into acc.
+void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* cols, uint32_t n_cols,
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
+struct PoseidonParams { qm31 z; qm31 alpha_pow[16]; qm31 shift; uint32_t ipr; uint32_t pad[3]; };
+void launch_poseidon_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* trace_cols,
+                                 const uint32_t* const* inter_cols, const PoseidonParams* params, const qm31* pows,
+                                 const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
+
+// Trace generation and the Poseidon LogUp interaction trace.
+void launch_gen_wide_fib(uint32_t log_n, uint32_t* const* cols, uint32_t n_cols, uint32_t mode, uint64_t seed,
+                         hipStream_t stream);
+void launch_gen_poseidon(uint32_t log_n, uint32_t* const* cols, uint32_t ipr, uint32_t mode, uint64_t seed,
+                         hipStream_t stream);
+void launch_poseidon_logup_fracs(uint32_t log_n, const uint32_t* const* trace_cols, uint32_t* const* inter_cols,
+                                 const PoseidonParams* params, unsigned long long* sums, hipStream_t stream);
+void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, uint32_t* scratch,
+                             hipStream_t stream);
+
+void launch_fill_zero(uint32_t* p, size_t n, hipStream_t stream);
+void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t stream);
+
+}  // namespace sh

@@ -1,0 +1,270 @@
+// FRI folds, DEEP quotients, accumulation, out-of-domain sampling and gathers.
+//
+//   fold_circle_into_line  prover/fri.rs:46-66; core/fri.rs:751-780
+//   fold_line              prover/fri.rs:31-44; core/fri.rs:702-725
+//   accumulate_numerators  prover/pcs/quotient_ops.rs:19-43; core/pcs/quotients.rs:200-298
+//   compute_quotients_and_combine  prover/backend/cpu/quotients.rs:45-80
+//   lift_and_accumulate    prover/backend/cpu/accumulation.rs (lifted index (i >> (r+1) << 1) + (i & 1))
+//   eval_at_point          prover/backend/cpu/circle.rs:82-99; core/poly/utils.rs:27-70
+// All kernels are HBM streaming kernels: one lane per row, coordinate columns read coalesced.
+#include "field.h"
+#include "kernels.h"
+
+namespace sh {
+
+__device__ __forceinline__ qm31 load_q(const uint32_t* const c[4], uint32_t i) {
+  return {c[0][i], c[1][i], c[2][i], c[3][i]};
+}
+__device__ __forceinline__ void store_q(uint32_t* const c[4], uint32_t i, qm31 v) {
+  c[0][i] = v.a; c[1][i] = v.b; c[2][i] = v.c; c[3][i] = v.d;
+}
+
+// Inverse circle twiddle of pair h: line layer 0 pairs [x, y] -> [y, -y, -x, x].
+__device__ __forceinline__ uint32_t circle_itw(const uint32_t* __restrict__ lt0, uint32_t h) {
+  uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
+  switch (h & 3) {
+    case 0: return y;
+    case 1: return m31_neg(y);
+    case 2: return m31_neg(x);
+    default: return x;
+  }
+}
+
+__global__ void __launch_bounds__(256) fold_circle_kernel(uint32_t n_out, CCols4 src, Cols4 dst, qm31 alpha,
+                                                          qm31 alpha2, const uint32_t* __restrict__ lt0) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  qm31 fp = load_q(src.c, 2 * i), fn = load_q(src.c, 2 * i + 1);
+  uint32_t t = circle_itw(lt0, i);  // 1 / p.y for p = domain.at(bitrev(2i))
+  qm31 f0 = qm_add(fp, fn);
+  qm31 f1 = qm_mul_m31(qm_sub(fp, fn), t);
+  qm31 fr = qm_add(f0, qm_mul(alpha, f1));
+  qm31 d = load_q((const uint32_t* const*)dst.c, i);
+  store_q(dst.c, i, qm_add(qm_mul(d, alpha2), fr));
+}
+
+void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
+                                  uint32_t tree_len, hipStream_t stream) {
+  uint32_t n = 1u << (log_src - 1);
+  const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
+  hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha,
+                     qm_mul(alpha, alpha), lt0);
+}
+
+__global__ void __launch_bounds__(256) fold_line_kernel(uint32_t n_out, CCols4 src, Cols4 dst, qm31 alpha,
+                                                        const uint32_t* __restrict__ lt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  qm31 fx = load_q(src.c, 2 * i), fnx = load_q(src.c, 2 * i + 1);
+  uint32_t xi = lt[i];  // 1 / x for x = domain.at(bitrev(2i))
+  qm31 f0 = qm_add(fx, fnx);
+  qm31 f1 = qm_mul_m31(qm_sub(fx, fnx), xi);
+  store_q(dst.c, i, qm_add(f0, qm_mul(alpha, f1)));
+}
+
+void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
+                      hipStream_t stream) {
+  uint32_t n = 1u << (log_src - 1);
+  const uint32_t* lt = itree + tree_len - (1u << log_src);
+  hipLaunchKernelGGL(fold_line_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha, lt);
+}
+
+// partial_b[row] = sum_{j in batch b} c_j * col_j[row] - sum_j b_j
+__global__ void __launch_bounds__(256) accumulate_numerators_kernel(uint32_t n_rows,
+                                                                    const uint32_t* const* __restrict__ cols,
+                                                                    const NumEntry* __restrict__ entries,
+                                                                    const uint32_t* __restrict__ batch_start,
+                                                                    const qm31* __restrict__ batch_b,
+                                                                    uint32_t n_batches, uint32_t* const* __restrict__ out) {
+  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  for (uint32_t b = 0; b < n_batches; ++b) {
+    qacc acc = qacc_zero();
+    uint32_t e_end = batch_start[b + 1];
+    for (uint32_t e = batch_start[b]; e < e_end; ++e) {
+      NumEntry en = entries[e];
+      qacc_mac(acc, en.c, cols[en.col][row]);
+    }
+    qm31 r = qm_sub(qacc_reduce(acc), batch_b[b]);
+    out[4 * b][row] = r.a;
+    out[4 * b + 1][row] = r.b;
+    out[4 * b + 2][row] = r.c;
+    out[4 * b + 3][row] = r.d;
+  }
+}
+
+void launch_accumulate_numerators(uint32_t log_size, const uint32_t* const* cols, const NumEntry* entries,
+                                  const uint32_t* batch_start, const qm31* batch_b, uint32_t n_batches,
+                                  uint32_t* const* out, hipStream_t stream) {
+  uint32_t n = 1u << log_size;
+  hipLaunchKernelGGL(accumulate_numerators_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, cols, entries,
+                     batch_start, batch_b, n_batches, out);
+}
+
+// q[row] = sum_a (sum_parts lift(partial)[row] - first_linear_a * y) / den_a(row),
+// den_a = (Re p.x - x) Im p.y - (Re p.y - y) Im p.x  (a CM31 value; core/pcs/quotients.rs:200-226)
+__global__ void __launch_bounds__(256) quotients_kernel(uint32_t lifting_log, const uint32_t* __restrict__ domain_xy,
+                                                        const QuotPoint* __restrict__ points, uint32_t n_points,
+                                                        const QuotPart* __restrict__ parts, Cols4 out) {
+  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (1u << lifting_log)) return;
+  uint32_t x = domain_xy[2 * row], y = domain_xy[2 * row + 1];
+  qm31 q = qm_zero();
+  for (uint32_t a = 0; a < n_points; ++a) {
+    QuotPoint pt = points[a];
+    cm31 dx = {m31_sub(pt.prx.a, x), pt.prx.b};
+    cm31 dy = {m31_sub(pt.pry.a, y), pt.pry.b};
+    cm31 den = cm_sub(cm_mul(dx, pt.piy), cm_mul(dy, pt.pix));
+    cm31 dinv = cm_inv(den);
+    qm31 num = qm_zero();
+    for (uint32_t p = pt.part_begin; p < pt.part_end; ++p) {
+      QuotPart pa = parts[p];
+      uint32_t r = lifting_log - pa.log_size;
+      uint32_t li = ((row >> (r + 1)) << 1) + (row & 1);
+      num = qm_add(num, load_q(pa.c, li));
+    }
+    num = qm_sub(num, qm_mul_m31(pt.first_linear, y));
+    q = qm_add(q, qm_mul_cm31(num, dinv));
+  }
+  store_q(out.c, row, q);
+}
+
+void launch_quotients(uint32_t lifting_log, const uint32_t* domain_xy, const QuotPoint* points, uint32_t n_points,
+                      const QuotPart* parts, Cols4 out, hipStream_t stream) {
+  uint32_t n = 1u << lifting_log;
+  hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, lifting_log, domain_xy, points,
+                     n_points, parts, out);
+}
+
+// dst[i] += src[(i >> (r+1) << 1) + (i & 1)], r = log_dst - log_src (r = 0 is a plain add)
+__global__ void __launch_bounds__(256) accumulate_lifted_kernel(uint32_t log_dst, Cols4 dst, uint32_t log_src,
+                                                                CCols4 src) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1u << log_dst)) return;
+  uint32_t r = log_dst - log_src;
+  uint32_t li = r ? ((i >> (r + 1)) << 1) + (i & 1) : i;
+  qm31 v = qm_add(load_q((const uint32_t* const*)dst.c, i), load_q(src.c, li));
+  store_q(dst.c, i, v);
+}
+
+void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCols4 src, hipStream_t stream) {
+  uint32_t n = 1u << log_dst;
+  hipLaunchKernelGGL(accumulate_lifted_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, log_dst, dst, log_src,
+                     src);
+}
+
+// ---- out-of-domain sampling on coefficients ------------------------------------------------
+// Chunk of 2^11 coefficients per workgroup: lane t owns 8 consecutive coefficients (index
+// t*8 + j). The basis weight of index lo within the chunk factors as W3[j] * base(t), where W3
+// uses the factors of bits 0..2 and base those of bits 3..10; the chunk weight uses bits >= 11.
+constexpr uint32_t OODS_CHUNK_LOG = 11;
+
+__global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restrict__ tasks,
+                                                         const qm31* __restrict__ factors, qm31* __restrict__ partials,
+                                                         uint32_t max_chunks) {
+  __shared__ qm31 red[256];
+  const OodsTask task = tasks[blockIdx.y];
+  const uint32_t k = task.log_size;
+  const uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
+  const uint32_t n_chunks = 1u << (k - m);
+  const uint32_t chunk = blockIdx.x;
+  const qm31* f = factors + (size_t)blockIdx.y * 32;
+  qm31 out = qm_zero();
+  if (chunk < n_chunks) {
+    const uint32_t t = threadIdx.x;
+    qm31 w3[8];
+    w3[0] = qm_one();
+#pragma unroll
+    for (uint32_t j = 1; j < 8; ++j) {
+      qm31 w = qm_one();
+      for (uint32_t b = 0; b < 3; ++b)
+        if (((j >> b) & 1) && b < m) w = qm_mul(w, f[b]);
+      w3[j] = w;
+    }
+    qacc acc = qacc_zero();
+    const uint32_t* c = task.coeffs + ((size_t)chunk << m);
+    uint32_t base_lo = t * 8;
+    bool active = base_lo < (1u << m);
+    if (active) {
+      if (m >= 3) {
+        uint4 v0 = *reinterpret_cast<const uint4*>(c + base_lo);
+        uint4 v1 = *reinterpret_cast<const uint4*>(c + base_lo + 4);
+        uint32_t vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) qacc_mac(acc, w3[j], vv[j]);
+      } else {
+        for (uint32_t j = 0; j < (1u << m); ++j) qacc_mac(acc, w3[j], c[j]);
+      }
+      qm31 s = qacc_reduce(acc);
+      for (uint32_t b = 3; b < m; ++b)
+        if ((t >> (b - 3)) & 1) s = qm_mul(s, f[b]);
+      out = s;
+    }
+  }
+  red[threadIdx.x] = out;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = qm_add(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && chunk < n_chunks) {
+    qm31 s = red[0];
+    for (uint32_t b = m; b < k; ++b)
+      if ((chunk >> (b - m)) & 1) s = qm_mul(s, f[b]);
+    partials[(size_t)blockIdx.y * max_chunks + chunk] = s;
+  }
+}
+
+__global__ void oods_reduce_kernel(const OodsTask* __restrict__ tasks, uint32_t n_tasks,
+                                   const qm31* __restrict__ partials, uint32_t max_chunks, qm31* __restrict__ out) {
+  uint32_t tsk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tsk >= n_tasks) return;
+  uint32_t k = tasks[tsk].log_size;
+  uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
+  uint32_t n_chunks = 1u << (k - m);
+  qm31 s = qm_zero();
+  for (uint32_t i = 0; i < n_chunks; ++i) s = qm_add(s, partials[(size_t)tsk * max_chunks + i]);
+  out[tasks[tsk].out_index] = s;
+}
+
+void launch_oods(const OodsTask* tasks, uint32_t n_tasks, uint32_t max_log, const qm31* factors, qm31* partials,
+                 qm31* out, hipStream_t stream) {
+  if (n_tasks == 0) return;
+  uint32_t max_chunks = max_log > OODS_CHUNK_LOG ? 1u << (max_log - OODS_CHUNK_LOG) : 1u;
+  hipLaunchKernelGGL(oods_chunk_kernel, dim3(max_chunks, n_tasks), dim3(256), 0, stream, tasks, factors, partials,
+                     max_chunks);
+  hipLaunchKernelGGL(oods_reduce_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, stream, tasks, n_tasks, partials,
+                     max_chunks, out);
+}
+
+__global__ void gather_words_kernel(const uint32_t* const* __restrict__ bases, const uint32_t* __restrict__ base_idx,
+                                    const uint32_t* __restrict__ offsets, uint32_t n, uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = bases[base_idx[i]][offsets[i]];
+}
+
+void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
+                         uint32_t* out, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(gather_words_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, bases, base_idx, offsets, n,
+                     out);
+}
+
+__global__ void fill_zero_kernel(uint32_t* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+void launch_fill_zero(uint32_t* p, size_t n, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n);
+}
+__global__ void copy_words_kernel(uint32_t* dst, const uint32_t* src, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(copy_words_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, dst, src, n);
+}
+
+}  // namespace sh

@@ -1,0 +1,1006 @@
+// Device-resident prover of libstwo_hip.
+//
+// Follows the reference protocol step by step (paths relative to crates/stwo/src):
+//   prove / prove_ex                   prover/mod.rs:28-150
+//   CommitmentSchemeProver             prover/pcs/mod.rs:36-494
+//   compute_fri_quotients              prover/pcs/quotient_ops.rs:73-145
+//   samples, batches, line coeffs      core/pcs/quotients.rs:71-366; core/constraints.rs:119-140
+//   FriProver commit / decommit        prover/fri.rs:100-412
+//   MerkleProver commit / decommit     prover/vcs/prover.rs:39-221
+//   composition accumulator            prover/air/accumulation.rs:21-125
+//   component domain evaluation        crates/constraint-framework/src/prover/component_prover.rs:32-169
+// Column data, Merkle layers, FRI layers and quotients stay in HBM; the host holds the
+// Fiat-Shamir channel and reads back only roots, sampled values, the last FRI layer, the
+// proof-of-work nonce and the decommitted values.
+#include "prover.h"
+
+#include <algorithm>
+#include <chrono>
+#include <set>
+
+namespace sh {
+
+namespace {
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double lap() {
+    auto t = std::chrono::steady_clock::now();
+    double ms = std::chrono::duration<double, std::milli>(t - t0).count();
+    t0 = t;
+    return ms;
+  }
+};
+
+Hash read_hash(Ctx& ctx, const uint32_t* dev) {
+  uint32_t w[8];
+  ctx.d2h(w, dev, 32);
+  Hash h;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 4; ++j) h[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
+  return h;
+}
+
+// Batched gather of device words: requests are collected first, then served by one kernel.
+struct Gather {
+  std::vector<const uint32_t*> bases;
+  std::map<const uint32_t*, uint32_t> base_of;
+  std::vector<uint32_t> bidx, offs;
+  uint32_t add(const uint32_t* base, uint32_t off) {
+    auto it = base_of.find(base);
+    uint32_t b;
+    if (it == base_of.end()) { b = (uint32_t)bases.size(); bases.push_back(base); base_of[base] = b; }
+    else b = it->second;
+    bidx.push_back(b);
+    offs.push_back(off);
+    return (uint32_t)offs.size() - 1;
+  }
+  uint32_t add_hash(const uint32_t* layer, uint32_t node) {
+    uint32_t first = add(layer, node * 8);
+    for (uint32_t w = 1; w < 8; ++w) add(layer, node * 8 + w);
+    return first;
+  }
+  std::vector<uint32_t> run(Ctx& ctx) {
+    std::vector<uint32_t> out(offs.size());
+    if (offs.empty()) return out;
+    const uint32_t* const* b = (const uint32_t* const*)ctx.upload(bases.data(), bases.size() * sizeof(void*));
+    const uint32_t* bi = ctx.upload_vec(bidx);
+    const uint32_t* of = ctx.upload_vec(offs);
+    DevBuf o(&ctx, offs.size());
+    launch_gather_words(b, bi, of, (uint32_t)offs.size(), o.p, ctx.stream);
+    ctx.d2h(out.data(), o.p, out.size() * 4);
+    return out;
+  }
+};
+Hash words_to_hash(const std::vector<uint32_t>& w, uint32_t at) {
+  Hash h;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 4; ++j) h[4 * i + j] = (uint8_t)(w[at + i] >> (8 * j));
+  return h;
+}
+
+// ---- proof encoding (little endian, counts as u32) ---------------------------------------------
+struct Writer {
+  std::vector<uint8_t> b;
+  void u32(uint32_t v) { for (int i = 0; i < 4; ++i) b.push_back((uint8_t)(v >> (8 * i))); }
+  void u64(uint64_t v) { u32((uint32_t)v); u32((uint32_t)(v >> 32)); }
+  void hash(const Hash& h) { b.insert(b.end(), h.begin(), h.end()); }
+  void q(const qm31& x) { u32(x.a); u32(x.b); u32(x.c); u32(x.d); }
+};
+struct MerkleDec {
+  std::vector<Hash> hash_witness;
+  std::vector<uint32_t> column_witness;
+};
+struct FriLayer {
+  std::vector<qm31> witness;
+  MerkleDec dec;
+  Hash commitment;
+};
+void write_dec(Writer& w, const MerkleDec& d) {
+  w.u32((uint32_t)d.hash_witness.size());
+  for (auto& h : d.hash_witness) w.hash(h);
+  w.u32((uint32_t)d.column_witness.size());
+  for (auto v : d.column_witness) w.u32(v);
+}
+void write_layer(Writer& w, const FriLayer& l) {
+  w.u32((uint32_t)l.witness.size());
+  for (auto& x : l.witness) w.q(x);
+  write_dec(w, l.dec);
+  w.hash(l.commitment);
+}
+
+std::vector<size_t> sorted_unique(std::vector<size_t> v) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  return v;
+}
+std::vector<size_t> fold_queries(const std::vector<size_t>& q, uint32_t n) {
+  std::vector<size_t> r;
+  for (size_t x : q) { size_t y = x >> n; if (r.empty() || r.back() != y) r.push_back(y); }
+  return r;
+}
+// core/pcs/utils.rs prepare_preprocessed_query_positions
+std::vector<size_t> tree_local_positions(const std::vector<size_t>& q, uint32_t max_log, uint32_t tree_max) {
+  std::vector<size_t> r;
+  if (tree_max == 0) return r;
+  if (max_log < tree_max) {
+    for (size_t p : q) r.push_back(((p >> 1) << (tree_max - max_log + 1)) + (p & 1));
+    return r;
+  }
+  for (size_t p : q) r.push_back(((p >> (max_log - tree_max + 1)) << 1) + (p & 1));
+  return r;
+}
+
+// Columns in the order MerkleProver hashes them: stable by decreasing size.
+std::vector<size_t> merkle_order(const std::vector<uint32_t>& logs) {
+  std::vector<size_t> o(logs.size());
+  for (size_t i = 0; i < o.size(); ++i) o[i] = i;
+  std::stable_sort(o.begin(), o.end(), [&](size_t a, size_t b) { return logs[a] > logs[b]; });
+  return o;
+}
+
+// MerkleProver::decommit (prover/vcs/prover.rs:84-217): returns queried values and decommitment.
+std::vector<uint32_t> merkle_decommit(Ctx& ctx, const DevTree& t, const std::map<uint32_t, std::vector<size_t>>& qpl,
+                                      MerkleDec& dec) {
+  auto order = merkle_order(t.eval_logs);
+  Gather g;
+  std::vector<uint32_t> hw_idx, q_idx, w_idx;
+  size_t ci = 0;
+  std::vector<size_t> last;
+  for (int ls = (int)t.layers.size() - 1; ls >= 0; --ls) {
+    bool has_prev = (size_t)ls + 1 < t.layers.size();
+    std::set<size_t> prev_set(last.begin(), last.end());
+    std::set<size_t> colq;
+    auto it = qpl.find((uint32_t)ls);
+    if (it != qpl.end()) colq.insert(it->second.begin(), it->second.end());
+    std::set<size_t> merged;
+    for (size_t q : last) merged.insert(q / 2);
+    merged.insert(colq.begin(), colq.end());
+    std::vector<size_t> lc;
+    while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(order[ci++]);
+    std::vector<size_t> qn, wn;
+    for (size_t node : merged) {
+      if (has_prev) {
+        const uint32_t* prev = t.layers[ls + 1].p;
+        if (!prev_set.count(2 * node)) hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node)));
+        if (!prev_set.count(2 * node + 1)) hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node + 1)));
+      }
+      if (colq.count(node)) qn.push_back(node); else wn.push_back(node);
+    }
+    for (size_t node : qn) for (size_t c : lc) q_idx.push_back(g.add(t.evals[c], (uint32_t)node));
+    for (size_t node : wn) for (size_t c : lc) w_idx.push_back(g.add(t.evals[c], (uint32_t)node));
+    last.assign(merged.begin(), merged.end());
+  }
+  auto vals = g.run(ctx);
+  for (auto i : hw_idx) dec.hash_witness.push_back(words_to_hash(vals, i));
+  for (auto i : w_idx) dec.column_witness.push_back(vals[i]);
+  std::vector<uint32_t> queried;
+  for (auto i : q_idx) queried.push_back(vals[i]);
+  return queried;
+}
+
+qm31 qm_pow_u(qm31 b, uint64_t e) { qm31 r = qm_one(); while (e) { if (e & 1) r = qm_mul(r, b); b = qm_mul(b, b); e >>= 1; } return r; }
+
+bool point_less(const cpqm31& a, const cpqm31& b) {
+  uint32_t x[8] = {a.x.a, a.x.b, a.x.c, a.x.d, a.y.a, a.y.b, a.y.c, a.y.d};
+  uint32_t y[8] = {b.x.a, b.x.b, b.x.c, b.x.d, b.y.a, b.y.b, b.y.c, b.y.d};
+  for (int i = 0; i < 8; ++i) if (x[i] != y[i]) return x[i] < y[i];
+  return false;
+}
+bool point_eq(const cpqm31& a, const cpqm31& b) { return qm_eq(a.x, b.x) && qm_eq(a.y, b.y); }
+cpqm31 cpq_repeated_double(cpqm31 p, uint32_t n) { for (uint32_t i = 0; i < n; ++i) p = cpq_double(p); return p; }
+
+qm31 from_partial_evals(qm31 e0, qm31 e1, qm31 e2, qm31 e3) {
+  qm31 r = e0;
+  r = qm_add(r, qm_mul(e1, qm31{0, 1, 0, 0}));
+  r = qm_add(r, qm_mul(e2, qm31{0, 0, 1, 0}));
+  r = qm_add(r, qm_mul(e3, qm31{0, 0, 0, 1}));
+  return r;
+}
+
+// ---- host point evaluation of the AIRs (sanity check of prove_ex) ------------------------------
+struct PointEval {
+  const std::vector<std::vector<std::vector<qm31>>>* sv;
+  const Component* c;
+  size_t ti = 0, ii = 0;
+  qm31* acc;
+  qm31 rc, dinv;
+  qm31 next_trace() { return (*sv)[1][c->trace_start + ti++][0]; }
+  void add(qm31 x) { *acc = qm_add(qm_mul(*acc, rc), qm_mul(dinv, x)); }
+  std::vector<qm31> next_ext(size_t n_off) {
+    const std::vector<qm31>* m[4];
+    for (int k = 0; k < 4; ++k) m[k] = &(*sv)[2][c->inter_start + ii++];
+    std::vector<qm31> r;
+    for (size_t o = 0; o < n_off; ++o) r.push_back(from_partial_evals((*m[0])[o], (*m[1])[o], (*m[2])[o], (*m[3])[o]));
+    return r;
+  }
+};
+qm31 pow5q(qm31 x) { qm31 x2 = qm_sqr(x); return qm_mul(qm_sqr(x2), x); }
+void m4q(qm31* x) {
+  qm31 t0 = qm_add(x[0], x[1]), t02 = qm_add(t0, t0), t1 = qm_add(x[2], x[3]), t12 = qm_add(t1, t1);
+  qm31 t2 = qm_add(qm_add(x[1], x[1]), t1), t3 = qm_add(qm_add(x[3], x[3]), t0);
+  qm31 t4 = qm_add(qm_add(t12, t12), t3), t5 = qm_add(qm_add(t02, t02), t2);
+  x[0] = qm_add(t3, t5); x[1] = t5; x[2] = qm_add(t2, t4); x[3] = t4;
+}
+void ext_q(qm31* s) {
+  for (int i = 0; i < 4; ++i) m4q(s + 4 * i);
+  for (int j = 0; j < 4; ++j) {
+    qm31 t = qm_add(qm_add(s[j], s[j + 4]), qm_add(s[j + 8], s[j + 12]));
+    for (int i = 0; i < 4; ++i) s[4 * i + j] = qm_add(s[4 * i + j], t);
+  }
+}
+void int_q(qm31* s) {
+  qm31 sum = s[0];
+  for (int i = 1; i < 16; ++i) sum = qm_add(sum, s[i]);
+  for (int i = 0; i < 16; ++i) s[i] = qm_add(qm_mul_m31(s[i], 1u << (i + 1)), sum);
+}
+qm31 combine_q(const Component& c, const qm31* v) {
+  qm31 acc = qm_zero();
+  for (int i = 0; i < 16; ++i) acc = qm_add(acc, qm_mul(c.alpha_pow[i], v[i]));
+  return qm_sub(acc, c.z);
+}
+void eval_at_point(PointEval& e) {
+  const Component& c = *e.c;
+  if (c.kind == AIR_WIDE_FIB) {
+    qm31 a = e.next_trace(), b = e.next_trace();
+    for (uint32_t i = 2; i < c.n_cols; ++i) {
+      qm31 x = e.next_trace();
+      e.add(qm_sub(x, qm_add(qm_sqr(a), qm_sqr(b))));
+      a = b;
+      b = x;
+    }
+    return;
+  }
+  std::vector<std::pair<qm31, qm31>> fr;
+  for (uint32_t rep = 0; rep < c.ipr; ++rep) {
+    qm31 s[16], init[16];
+    for (int i = 0; i < 16; ++i) { s[i] = e.next_trace(); init[i] = s[i]; }
+    auto full = [&]() {
+      for (int i = 0; i < 16; ++i) s[i] = qm_add_m31(s[i], 1234);
+      ext_q(s);
+      for (int i = 0; i < 16; ++i) s[i] = pow5q(s[i]);
+      for (int i = 0; i < 16; ++i) { qm31 m = e.next_trace(); e.add(qm_sub(s[i], m)); s[i] = m; }
+    };
+    for (int r = 0; r < 4; ++r) full();
+    for (int r = 0; r < 14; ++r) {
+      s[0] = qm_add_m31(s[0], 1234);
+      int_q(s);
+      s[0] = pow5q(s[0]);
+      qm31 m = e.next_trace();
+      e.add(qm_sub(s[0], m));
+      s[0] = m;
+    }
+    for (int r = 0; r < 4; ++r) full();
+    qm31 d0 = combine_q(c, init), d1 = combine_q(c, s);
+    fr.push_back({qm_sub(d1, d0), qm_mul(d0, d1)});
+  }
+  qm31 prev = qm_zero();
+  for (uint32_t b = 0; b + 1 < c.ipr; ++b) {
+    qm31 cur = e.next_ext(1)[0];
+    qm31 diff = qm_sub(cur, prev);
+    prev = cur;
+    e.add(qm_sub(qm_mul(diff, fr[b].second), fr[b].first));
+  }
+  auto m = e.next_ext(2);
+  qm31 diff = qm_add(qm_sub(qm_sub(m[1], m[0]), prev), c.cumsum_shift());
+  e.add(qm_sub(qm_mul(diff, fr.back().second), fr.back().first));
+}
+
+// mask offsets per tree (0 preprocessed, 1 trace, 2 interaction), per column of the component
+std::vector<std::vector<std::vector<long>>> mask_offsets(const Component& c) {
+  std::vector<std::vector<std::vector<long>>> r(3);
+  r[1].assign(c.n_trace_cols(), {0});
+  if (c.kind == AIR_POSEIDON) {
+    for (uint32_t i = 0; i < 4 * (c.ipr - 1); ++i) r[2].push_back({0});
+    for (uint32_t i = 0; i < 4; ++i) r[2].push_back({-1, 0});
+  }
+  return r;
+}
+
+struct PointSample { cpqm31 point; qm31 value; qm31 rand; };
+
+// Line coefficients (a, b, c) * alpha (core/constraints.rs complex_conjugate_line_coeffs).
+void line_coeffs(const cpqm31& p, qm31 value, qm31 alpha, qm31& a, qm31& b, qm31& c) {
+  qm31 aa = qm_sub(qm_conj(value), value);
+  qm31 cc = qm_sub(qm_conj(p.y), p.y);
+  qm31 bb = qm_sub(qm_mul(value, cc), qm_mul(aa, p.y));
+  a = qm_mul(alpha, aa);
+  b = qm_mul(alpha, bb);
+  c = qm_mul(alpha, cc);
+}
+
+void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const std::vector<uint32_t>& logs,
+                   uint32_t target_log, std::vector<DevBuf>& out) {
+  // evaluates every column onto CanonicCoset(target_log), grouping equal sizes into one batch
+  std::map<uint32_t, std::vector<size_t>> by;
+  for (size_t i = 0; i < coeffs.size(); ++i) by[logs[i]].push_back(i);
+  out.resize(coeffs.size());
+  ctx.ensure_twiddles(target_log);
+  for (auto& [lg, idx] : by) {
+    std::vector<const uint32_t*> src;
+    std::vector<uint32_t*> dst;
+    for (size_t i : idx) {
+      out[i] = DevBuf(&ctx, (size_t)1 << target_log);
+      src.push_back(coeffs[i]);
+      dst.push_back(out[i].p);
+    }
+    launch_fft(false, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
+               (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), ctx.stream);
+  }
+}
+
+}  // namespace
+
+void merkle_commit(Ctx& ctx, DevTree& t) {
+  t.layers.clear();
+  if (t.evals.empty()) {
+    t.layers.emplace_back(&ctx, 8);
+    launch_merkle_layer(0, nullptr, nullptr, 0, t.layers[0].p, ctx.stream);
+    t.root = read_hash(ctx, t.layers[0].p);
+    return;
+  }
+  auto order = merkle_order(t.eval_logs);
+  uint32_t max_log = t.eval_logs[order[0]];
+  std::vector<DevBuf> rev;
+  size_t ci = 0;
+  for (int ls = (int)max_log; ls >= 0; --ls) {
+    std::vector<const uint32_t*> lc;
+    while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(t.evals[order[ci++]]);
+    DevBuf layer(&ctx, (size_t)8 << ls);
+    const uint32_t* prev = rev.empty() ? nullptr : rev.back().p;
+    const uint32_t* const* cp = lc.empty() ? nullptr : (const uint32_t* const*)ctx.upload_vec(lc);
+    launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
+    rev.push_back(std::move(layer));
+  }
+  for (size_t i = rev.size(); i-- > 0;) t.layers.push_back(std::move(rev[i]));
+  t.root = read_hash(ctx, t.layers[0].p);
+}
+
+void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, const std::vector<uint32_t>& logs,
+                         DevTree& into) {
+  std::map<uint32_t, std::vector<size_t>> by;
+  for (size_t i = 0; i < evals.size(); ++i) by[logs[i]].push_back(i);
+  size_t base = into.coeffs.size();
+  into.coeffs.resize(base + evals.size());
+  into.coeff_logs.resize(base + evals.size());
+  for (auto& [lg, idx] : by) {
+    if (lg < 3) throw HipError(3, "interpolate: log size < 3 is not supported");
+    ctx.ensure_twiddles(lg);
+    std::vector<const uint32_t*> src;
+    std::vector<uint32_t*> dst;
+    for (size_t i : idx) {
+      into.owned.emplace_back(&ctx, (size_t)1 << lg);
+      src.push_back(evals[i]);
+      dst.push_back(into.owned.back().p);
+      into.coeffs[base + i] = into.owned.back().p;
+      into.coeff_logs[base + i] = lg;
+    }
+    launch_fft(true, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
+               (uint32_t)idx.size(), lg, lg, ctx.itw.p, ctx.tree_len(), ctx.stream);
+  }
+}
+
+void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
+  std::map<uint32_t, std::vector<size_t>> by;
+  for (size_t i = 0; i < t.coeffs.size(); ++i) by[t.coeff_logs[i]].push_back(i);
+  t.evals.assign(t.coeffs.size(), nullptr);
+  t.eval_logs.assign(t.coeffs.size(), 0);
+  for (auto& [lg, idx] : by) {
+    uint32_t K = lg + log_blowup;
+    ctx.ensure_twiddles(K);
+    std::vector<const uint32_t*> src;
+    std::vector<uint32_t*> dst;
+    for (size_t i : idx) {
+      t.owned.emplace_back(&ctx, (size_t)1 << K);
+      src.push_back(t.coeffs[i]);
+      dst.push_back(t.owned.back().p);
+      t.evals[i] = t.owned.back().p;
+      t.eval_logs[i] = K;
+    }
+    launch_fft(false, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
+               (uint32_t)idx.size(), lg, K, ctx.tw.p, ctx.tree_len(), ctx.stream);
+  }
+}
+
+namespace {
+
+// CommitmentSchemeProver::commit + TreeBuilder::commit for trees given as coefficients.
+void commit_tree(Ctx& ctx, DevTree& t, const PcsCfg& cfg, Channel& ch) {
+  evaluate_polynomials(ctx, t, cfg.log_blowup);
+  merkle_commit(ctx, t);
+  ch.mix_root(t.root);
+  ctx.reset_arena();  // merkle_commit synchronised the stream
+}
+
+struct ProofParts {
+  std::vector<Hash> commitments;
+  std::vector<std::vector<std::vector<qm31>>> sampled;
+  std::vector<MerkleDec> decs;
+  std::vector<std::vector<uint32_t>> queried;
+  uint64_t pow = 0;
+  FriLayer first;
+  std::vector<FriLayer> inner;
+  std::vector<qm31> last_poly;
+};
+
+std::vector<uint8_t> encode(const PcsCfg& cfg, const ProofParts& p) {
+  Writer w;
+  w.u32(cfg.pow_bits); w.u32(cfg.log_blowup); w.u32(cfg.log_last_layer); w.u32(cfg.n_queries);
+  w.u32(cfg.fold_step); w.u32(cfg.lifting_log_size);
+  w.u32((uint32_t)p.commitments.size());
+  for (auto& h : p.commitments) w.hash(h);
+  w.u32((uint32_t)p.sampled.size());
+  for (auto& t : p.sampled) {
+    w.u32((uint32_t)t.size());
+    for (auto& c : t) { w.u32((uint32_t)c.size()); for (auto& q : c) w.q(q); }
+  }
+  w.u32((uint32_t)p.decs.size());
+  for (auto& d : p.decs) write_dec(w, d);
+  w.u32((uint32_t)p.queried.size());
+  for (auto& t : p.queried) { w.u32((uint32_t)t.size()); for (auto v : t) w.u32(v); }
+  w.u64(p.pow);
+  write_layer(w, p.first);
+  w.u32((uint32_t)p.inner.size());
+  for (auto& l : p.inner) write_layer(w, l);
+  w.u32((uint32_t)p.last_poly.size());
+  for (auto& q : p.last_poly) w.q(q);
+  return w.b;
+}
+
+// A FRI layer column (4 coordinate columns of 2^log) with its Merkle tree.
+struct FriCol {
+  std::vector<DevBuf> c;  // 4 coordinate columns
+  uint32_t log = 0;
+  DevTree tree;
+  uint32_t fold_step = 1;
+  CCols4 cc() const { return {{c[0].p, c[1].p, c[2].p, c[3].p}}; }
+  Cols4 mc() { return {{c[0].p, c[1].p, c[2].p, c[3].p}}; }
+};
+void fri_commit_col(Ctx& ctx, FriCol& f) {
+  f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
+  f.tree.eval_logs.assign(4, f.log);
+  merkle_commit(ctx, f.tree);
+  ctx.reset_arena();
+}
+FriLayer fri_decommit_layer(Ctx& ctx, const FriCol& f, const std::vector<size_t>& q, uint32_t fold_step) {
+  FriLayer L;
+  std::vector<size_t> pos;
+  std::vector<bool> is_witness;
+  size_t i = 0;
+  while (i < q.size()) {
+    size_t j = i;
+    while (j < q.size() && (q[j] >> fold_step) == (q[i] >> fold_step)) ++j;
+    size_t start = (q[i] >> fold_step) << fold_step;
+    size_t k = i;
+    for (size_t p = start; p < start + ((size_t)1 << fold_step); ++p) {
+      pos.push_back(p);
+      if (k < j && q[k] == p) { ++k; is_witness.push_back(false); }
+      else is_witness.push_back(true);
+    }
+    i = j;
+  }
+  Gather g;
+  std::vector<uint32_t> at;
+  for (size_t p : pos) for (int k = 0; k < 4; ++k) at.push_back(g.add(f.c[k].p, (uint32_t)p));
+  auto v = g.run(ctx);
+  for (size_t n = 0; n < pos.size(); ++n)
+    if (is_witness[n]) L.witness.push_back({v[at[4 * n]], v[at[4 * n + 1]], v[at[4 * n + 2]], v[at[4 * n + 3]]});
+  std::map<uint32_t, std::vector<size_t>> qpl{{f.log, pos}};
+  merkle_decommit(ctx, f.tree, qpl, L.dec);
+  L.commitment = f.tree.root;
+  return L;
+}
+
+// Host inverse line FFT of the last FRI layer (prover/line.rs:44-64); returns stored-order coeffs.
+std::vector<qm31> line_interpolate(std::vector<qm31> v, HCoset dom) {
+  size_t n = v.size();
+  uint32_t lg = 0;
+  while (((size_t)1 << lg) < n) ++lg;
+  for (size_t i = 0; i < n; ++i) { size_t j = bitrev((uint32_t)i, lg); if (j > i) std::swap(v[i], v[j]); }
+  HCoset d = dom;
+  while (((size_t)1 << d.log_size) > 1) {
+    size_t ds = (size_t)1 << d.log_size, half = ds / 2;
+    std::vector<uint32_t> xi(half);
+    for (size_t i = 0; i < half; ++i) xi[i] = m31_inv(d.at(i).x);
+    for (size_t ch = 0; ch < n; ch += ds)
+      for (size_t i = 0; i < half; ++i) {
+        qm31 a = v[ch + i], b = v[ch + half + i];
+        v[ch + i] = qm_add(a, b);
+        v[ch + half + i] = qm_mul_m31(qm_sub(a, b), xi[i]);
+      }
+    d = d.doubled();
+  }
+  uint32_t li = m31_inv((uint32_t)n);
+  for (auto& x : v) x = qm_mul_m31(x, li);
+  return v;
+}
+
+// prove_ex + prove_values. `trees` holds the committed preprocessed, trace (and interaction) trees.
+int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vector<Component>& comps,
+             const PcsCfg& cfg, ProofParts& out, double* stage_ms) {
+  Timer tm;
+  // ---------------- composition ----------------
+  qm31 random_coeff = ch.draw_secure_felt();
+  uint32_t max_log = 0;
+  size_t T = 0;
+  for (auto& c : comps) { max_log = std::max(max_log, c.eval_log()); T += c.n_constraints(); }
+  std::vector<qm31> powers(T);
+  { qm31 p = qm_one(); for (size_t i = 0; i < T; ++i) { powers[i] = p; p = qm_mul(p, random_coeff); } }
+  std::map<uint32_t, std::vector<DevBuf>> accs;
+  for (auto& c : comps) {
+    size_t nc = c.n_constraints();
+    std::vector<qm31> cp(powers.end() - nc, powers.end());
+    powers.resize(powers.size() - nc);
+    std::reverse(cp.begin(), cp.end());
+    uint32_t E = c.eval_log();
+    ctx.ensure_twiddles(E);
+    auto& acc = accs[E];
+    if (acc.empty()) for (int k = 0; k < 4; ++k) acc.emplace_back(&ctx, (size_t)1 << E, true);
+    Cols4 a4 = {{acc[0].p, acc[1].p, acc[2].p, acc[3].p}};
+    // columns on the evaluation domain: committed evaluations when sizes match, else extended
+    std::vector<DevBuf> ext_t, ext_i;
+    auto cols_on_domain = [&](const DevTree& t, size_t start, size_t n, std::vector<DevBuf>& ext) {
+      std::vector<const uint32_t*> r(n);
+      bool need = false;
+      for (size_t i = 0; i < n; ++i) need |= t.eval_logs[start + i] != E;
+      if (!need) { for (size_t i = 0; i < n; ++i) r[i] = t.evals[start + i]; return r; }
+      std::vector<const uint32_t*> cf(t.coeffs.begin() + start, t.coeffs.begin() + start + n);
+      std::vector<uint32_t> lg(t.coeff_logs.begin() + start, t.coeff_logs.begin() + start + n);
+      evaluate_into(ctx, cf, lg, E, ext);
+      for (size_t i = 0; i < n; ++i) r[i] = ext[i].p;
+      return r;
+    };
+    auto tcols = cols_on_domain(trees[1], c.trace_start, c.n_trace_cols(), ext_t);
+    uint32_t le = E - c.log_n;
+    std::vector<uint32_t> dinv(1u << le);
+    for (uint32_t i = 0; i < (1u << le); ++i) dinv[i] = m31_inv(coset_vanishing_m31(c.log_n, canonic_domain_at(E, i)));
+    std::vector<uint32_t> dinv_br(dinv.size());
+    for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
+    const uint32_t* d_dinv = ctx.upload_vec(dinv_br);
+    const qm31* d_pows = ctx.upload_vec(cp);
+    if (c.kind == AIR_WIDE_FIB) {
+      launch_wide_fib_composition(E, c.log_n, (const uint32_t* const*)ctx.upload_vec(tcols), c.n_cols, d_pows, d_dinv,
+                                  a4, ctx.stream);
+    } else {
+      auto icols = cols_on_domain(trees[2], c.inter_start, c.n_inter_cols(), ext_i);
+      PoseidonParams pp{};
+      pp.z = c.z;
+      for (int i = 0; i < 16; ++i) pp.alpha_pow[i] = c.alpha_pow[i];
+      pp.shift = c.cumsum_shift();
+      pp.ipr = c.ipr;
+      launch_poseidon_composition(E, c.log_n, (const uint32_t* const*)ctx.upload_vec(tcols),
+                                  (const uint32_t* const*)ctx.upload_vec(icols),
+                                  (const PoseidonParams*)ctx.upload(&pp, sizeof(pp)), d_pows, d_dinv, a4, ctx.stream);
+    }
+    ctx.sync();
+    ctx.reset_arena();
+  }
+  // finalize: lift smaller accumulations into the largest, then interpolate 4 coordinate columns
+  auto& big = accs[max_log];
+  Cols4 b4 = {{big[0].p, big[1].p, big[2].p, big[3].p}};
+  for (auto& [lg, a] : accs)
+    if (lg != max_log) launch_accumulate_lifted(max_log, b4, lg, {{a[0].p, a[1].p, a[2].p, a[3].p}}, ctx.stream);
+  DevTree comp;
+  {
+    std::vector<const uint32_t*> ev = {big[0].p, big[1].p, big[2].p, big[3].p};
+    std::vector<uint32_t> lg(4, max_log);
+    DevTree full;
+    interpolate_columns(ctx, ev, lg, full);
+    // split_at_mid: left halves then right halves, as views into the coefficient buffers
+    size_t half = (size_t)1 << (max_log - 1);
+    for (int k = 0; k < 4; ++k) { comp.coeffs.push_back(full.coeffs[k]); comp.coeff_logs.push_back(max_log - 1); }
+    for (int k = 0; k < 4; ++k) { comp.coeffs.push_back(full.coeffs[k] + half); comp.coeff_logs.push_back(max_log - 1); }
+    for (auto& o : full.owned) comp.owned.push_back(std::move(o));
+  }
+  accs.clear();
+  commit_tree(ctx, comp, cfg, ch);
+  trees.push_back(std::move(comp));
+  stage_ms[2] = tm.lap();
+
+  // ---------------- OODS ----------------
+  cpqm31 oods;
+  {
+    qm31 t = ch.draw_secure_felt();
+    qm31 t2 = qm_sqr(t);
+    qm31 d = qm_inv(qm_add(t2, qm_one()));
+    oods = {qm_mul(qm_sub(qm_one(), t2), d), qm_mul(qm_add(t, t), d)};
+  }
+  uint32_t split_log = trees.back().max_log();
+  uint32_t L = cfg.lifting_log_size ? cfg.lifting_log_size : split_log;
+  if (L < split_log) throw HipError(3, "lifting log size below the composition size");
+  uint32_t max_deg = L - cfg.log_blowup;
+  std::vector<std::vector<std::vector<cpqm31>>> spts(trees.size());
+  for (size_t t = 0; t < trees.size(); ++t) spts[t].resize(trees[t].evals.size());
+  cpm31 step = index_to_point(subgroup_gen(max_deg));
+  for (auto& c : comps) {
+    auto offs = mask_offsets(c);
+    for (size_t t = 1; t < 3 && t + 1 < trees.size(); ++t) {
+      size_t start = t == 1 ? c.trace_start : c.inter_start;
+      for (size_t i = 0; i < offs[t].size(); ++i)
+        for (long o : offs[t][i]) spts[t][start + i].push_back(cpq_add(oods, cpq_mul_signed(step, o)));
+    }
+  }
+  for (auto& v : spts.back()) v = {oods};
+  uint32_t Lc = trees.back().max_log();  // prove_values lifting size: the last tree
+  std::vector<OodsTask> tasks;
+  std::vector<qm31> factors;
+  uint32_t max_coeff_log = 0;
+  size_t n_vals = 0;
+  for (size_t t = 0; t < trees.size(); ++t)
+    for (size_t c = 0; c < spts[t].size(); ++c)
+      for (auto& p : spts[t][c]) {
+        cpqm31 q = cpq_repeated_double(p, Lc - trees[t].eval_logs[c]);
+        uint32_t k = trees[t].coeff_logs[c];
+        OodsTask tk{trees[t].coeffs[c], k, (uint32_t)n_vals, {0, 0}};
+        tasks.push_back(tk);
+        std::vector<qm31> f(32, qm_zero());
+        f[0] = q.y;
+        qm31 x = q.x;
+        for (uint32_t b = 1; b < k; ++b) { f[b] = x; x = qm_double_x(x); }
+        factors.insert(factors.end(), f.begin(), f.end());
+        max_coeff_log = std::max(max_coeff_log, k);
+        ++n_vals;
+      }
+  std::vector<qm31> values(n_vals);
+  if (n_vals) {
+    uint32_t max_chunks = max_coeff_log > 11 ? 1u << (max_coeff_log - 11) : 1u;
+    DevBuf partials(&ctx, (size_t)n_vals * max_chunks * 4), dvals(&ctx, n_vals * 4);
+    launch_oods(ctx.upload_vec(tasks), (uint32_t)n_vals, max_coeff_log, ctx.upload_vec(factors), (qm31*)partials.p,
+                (qm31*)dvals.p, ctx.stream);
+    ctx.d2h(values.data(), dvals.p, n_vals * 16);
+    ctx.reset_arena();
+  }
+  out.sampled.resize(trees.size());
+  std::vector<std::vector<std::vector<PointSample>>> samples(trees.size());
+  {
+    size_t vi = 0;
+    for (size_t t = 0; t < trees.size(); ++t) {
+      samples[t].resize(spts[t].size());
+      for (size_t c = 0; c < spts[t].size(); ++c) {
+        std::vector<qm31> v;
+        for (auto& p : spts[t][c]) { samples[t][c].push_back({p, values[vi], qm_zero()}); v.push_back(values[vi]); ++vi; }
+        out.sampled[t].push_back(v);
+      }
+    }
+  }
+  ch.mix_felts(values);
+  stage_ms[3] = tm.lap();
+
+  // ---------------- FRI quotients ----------------
+  qm31 qrc = ch.draw_secure_felt();
+  std::vector<std::vector<std::vector<PointSample>>> swr(trees.size());
+  {
+    qm31 pw = qm_one();
+    auto next = [&]() { qm31 r = pw; pw = qm_mul(pw, qrc); return r; };
+    cpm31 gen = index_to_point(subgroup_gen(Lc));
+    for (size_t t = 0; t < trees.size(); ++t)
+      for (size_t c = 0; c < samples[t].size(); ++c) {
+        std::vector<PointSample> ns;
+        auto& sc = samples[t][c];
+        if (!sc.empty()) {
+          if (sc.size() == 2) {
+            cpm31 pg = gen;
+            for (uint32_t i = 0; i < trees[t].eval_logs[c]; ++i) pg = cp_double(pg);
+            ns.push_back({cpq_add(sc[1].point, cpq_from(pg)), sc[1].value, next()});
+          }
+          for (auto s : sc) { s.rand = next(); ns.push_back(s); }
+        }
+        swr[t].push_back(ns);
+      }
+  }
+  struct Item { size_t t, c; uint32_t ls; };
+  std::vector<Item> items;
+  for (size_t t = 0; t < trees.size(); ++t)
+    for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
+  struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; qm31 first_linear; };
+  std::vector<Acc> acc_list;
+  std::vector<DevBuf> partial_bufs;
+  for (size_t i = 0; i < items.size();) {
+    size_t j = i;
+    while (j < items.size() && items[j].ls == items[i].ls) ++j;
+    uint32_t ls = items[i].ls;
+    std::vector<const uint32_t*> gcols;
+    struct Batch { cpqm31 point; std::vector<NumEntry> entries; qm31 B, A; };
+    std::vector<Batch> batches;
+    for (size_t k = i; k < j; ++k) {
+      uint32_t gi = (uint32_t)gcols.size();
+      gcols.push_back(trees[items[k].t].evals[items[k].c]);
+      for (auto& s : swr[items[k].t][items[k].c]) {
+        Batch* b = nullptr;
+        for (auto& e : batches) if (point_eq(e.point, s.point)) { b = &e; break; }
+        if (!b) { batches.push_back({s.point, {}, qm_zero(), qm_zero()}); b = &batches.back(); }
+        qm31 la, lb, lc;
+        line_coeffs(s.point, s.value, s.rand, la, lb, lc);
+        NumEntry ne{};
+        ne.col = gi;
+        ne.c = lc;
+        b->entries.push_back(ne);
+        b->B = qm_add(b->B, lb);
+        b->A = qm_add(b->A, la);
+      }
+    }
+    if (!batches.empty()) {
+      std::vector<NumEntry> ents;
+      std::vector<uint32_t> starts;
+      std::vector<qm31> Bs;
+      std::vector<uint32_t*> outs;
+      for (auto& b : batches) {
+        starts.push_back((uint32_t)ents.size());
+        ents.insert(ents.end(), b.entries.begin(), b.entries.end());
+        Bs.push_back(b.B);
+        Acc a;
+        a.point = b.point;
+        a.log = ls;
+        a.first_linear = b.A;
+        for (int k = 0; k < 4; ++k) {
+          partial_bufs.emplace_back(&ctx, (size_t)1 << ls);
+          outs.push_back(partial_bufs.back().p);
+          a.c[k] = partial_bufs.back().p;
+        }
+        acc_list.push_back(a);
+      }
+      starts.push_back((uint32_t)ents.size());
+      launch_accumulate_numerators(ls, (const uint32_t* const*)ctx.upload_vec(gcols), ctx.upload_vec(ents),
+                                   ctx.upload_vec(starts), ctx.upload_vec(Bs), (uint32_t)batches.size(),
+                                   (uint32_t* const*)ctx.upload_vec(outs), ctx.stream);
+    }
+    i = j;
+  }
+  std::stable_sort(acc_list.begin(), acc_list.end(), [](const Acc& a, const Acc& b) { return point_less(a.point, b.point); });
+  std::vector<QuotPoint> qpts;
+  std::vector<QuotPart> qparts;
+  for (size_t i = 0; i < acc_list.size();) {
+    size_t j = i;
+    while (j < acc_list.size() && point_eq(acc_list[j].point, acc_list[i].point)) ++j;
+    QuotPoint qp{};
+    const cpqm31& p = acc_list[i].point;
+    qp.prx = qm_lo(p.x); qp.pix = qm_hi(p.x); qp.pry = qm_lo(p.y); qp.piy = qm_hi(p.y);
+    qp.first_linear = qm_zero();
+    qp.part_begin = (uint32_t)qparts.size();
+    for (size_t k = i; k < j; ++k) {
+      qp.first_linear = qm_add(qp.first_linear, acc_list[k].first_linear);
+      QuotPart pa{};
+      for (int c = 0; c < 4; ++c) pa.c[c] = acc_list[k].c[c];
+      pa.log_size = acc_list[k].log;
+      qparts.push_back(pa);
+    }
+    qp.part_end = (uint32_t)qparts.size();
+    qpts.push_back(qp);
+    i = j;
+  }
+  FriCol first;
+  first.log = Lc;
+  for (int k = 0; k < 4; ++k) first.c.emplace_back(&ctx, (size_t)1 << Lc);
+  launch_quotients(Lc, ctx.domain_xy(Lc), ctx.upload_vec(qpts), (uint32_t)qpts.size(), ctx.upload_vec(qparts),
+                   first.mc(), ctx.stream);
+  ctx.sync();
+  ctx.reset_arena();
+  partial_bufs.clear();
+  stage_ms[4] = tm.lap();
+
+  // ---------------- FRI commit ----------------
+  ctx.ensure_twiddles(Lc);
+  fri_commit_col(ctx, first);
+  ch.mix_root(first.tree.root);
+  std::vector<FriCol> inner;
+  FriCol cur;
+  cur.log = Lc - 1;
+  for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log, true);
+  qm31 alpha = ch.draw_secure_felt();
+  launch_fold_circle_into_line(Lc, first.cc(), cur.mc(), alpha, ctx.itw.p, ctx.tree_len(), ctx.stream);
+  uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
+  auto fold_layer = [&](FriCol& src, uint32_t fs, qm31 a) {
+    FriCol res;
+    FriCol* s = &src;
+    FriCol tmp;
+    for (uint32_t st = 0; st < fs; ++st) {
+      FriCol nx;
+      nx.log = s->log - 1;
+      for (int k = 0; k < 4; ++k) nx.c.emplace_back(&ctx, (size_t)1 << nx.log);
+      launch_fold_line(s->log, s->cc(), nx.mc(), a, ctx.itw.p, ctx.tree_len(), ctx.stream);
+      a = qm_sqr(a);
+      tmp = std::move(nx);
+      s = &tmp;
+    }
+    return tmp;
+  };
+  if (cur.log != last_log) {
+    while (cur.log > last_log + cfg.fold_step) {
+      cur.fold_step = cfg.fold_step;
+      fri_commit_col(ctx, cur);
+      ch.mix_root(cur.tree.root);
+      qm31 a = ch.draw_secure_felt();
+      FriCol nx = fold_layer(cur, cfg.fold_step, a);
+      inner.push_back(std::move(cur));
+      cur = std::move(nx);
+    }
+    uint32_t ls = cur.log - last_log;
+    cur.fold_step = ls;
+    fri_commit_col(ctx, cur);
+    ch.mix_root(cur.tree.root);
+    qm31 a = ch.draw_secure_felt();
+    FriCol nx = fold_layer(cur, ls, a);
+    inner.push_back(std::move(cur));
+    cur = std::move(nx);
+  }
+  {
+    size_t n = (size_t)1 << cur.log;
+    std::vector<uint32_t> w(4 * n);
+    for (int k = 0; k < 4; ++k) ctx.d2h(w.data() + k * n, cur.c[k].p, n * 4);
+    std::vector<qm31> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = {w[i], w[n + i], w[2 * n + i], w[3 * n + i]};
+    HCoset dom = HCoset::half_odds(Lc - 1);
+    for (uint32_t i = Lc - 1; i > cur.log; --i) dom = dom.doubled();
+    auto coeffs = line_interpolate(v, dom);
+    std::vector<qm31> ordered(coeffs);
+    uint32_t lg = cur.log;
+    for (size_t i = 0; i < n; ++i) { size_t j = bitrev((uint32_t)i, lg); if (j > i) std::swap(ordered[i], ordered[j]); }
+    size_t bound = (size_t)1 << cfg.log_last_layer;
+    for (size_t i = bound; i < n; ++i)
+      if (!qm_eq(ordered[i], qm_zero())) throw HipError(4, "invalid degree");
+    ordered.resize(bound);
+    uint32_t bl = cfg.log_last_layer;
+    for (size_t i = 0; i < bound; ++i) { size_t j = bitrev((uint32_t)i, bl); if (j > i) std::swap(ordered[i], ordered[j]); }
+    out.last_poly = ordered;
+    ch.mix_felts(out.last_poly);
+  }
+  stage_ms[5] = tm.lap();
+
+  // ---------------- proof of work ----------------
+  {
+    if (cfg.pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
+    Hash pre = ch.pow_prefix(cfg.pow_bits);
+    uint32_t pw[8];
+    for (int i = 0; i < 8; ++i) pw[i] = (uint32_t)pre[4 * i] | ((uint32_t)pre[4 * i + 1] << 8) | ((uint32_t)pre[4 * i + 2] << 16) | ((uint32_t)pre[4 * i + 3] << 24);
+    const uint32_t* dpre = (const uint32_t*)ctx.upload(pw, 32);
+    DevBuf res(&ctx, 2);
+    unsigned long long none = ~0ull, found = ~0ull;
+    HIP_CHECK(hipMemcpyAsync(res.p, &none, 8, hipMemcpyHostToDevice, ctx.stream));
+    const uint32_t per = 16;
+    for (uint32_t hi = 0; found == ~0ull; hi += per) {
+      launch_grind(dpre, cfg.pow_bits, hi, per, (unsigned long long*)res.p, ctx.stream);
+      ctx.d2h(&found, res.p, 8);
+      if (hi > 0x7fffffffu - per) throw HipError(4, "Grind failed to find a solution.");
+    }
+    out.pow = found;
+    ch.mix_u64(found);
+    ctx.reset_arena();
+  }
+  stage_ms[6] = tm.lap();
+
+  // ---------------- decommit ----------------
+  std::vector<size_t> unsorted;
+  {
+    uint32_t mask = (uint32_t)(((uint64_t)1 << Lc) - 1);
+    while (unsorted.size() < cfg.n_queries) {
+      auto w = ch.draw_u32s();
+      for (auto x : w) { if (unsorted.size() < cfg.n_queries) unsorted.push_back(x & mask); }
+    }
+  }
+  auto qpos = sorted_unique(unsorted);
+  out.first = fri_decommit_layer(ctx, first, qpos, 1);
+  auto lq = fold_queries(qpos, 1);
+  for (auto& l : inner) {
+    out.inner.push_back(fri_decommit_layer(ctx, l, lq, l.fold_step));
+    lq = fold_queries(lq, l.fold_step);
+  }
+  for (auto& t : trees) {
+    auto q = tree_local_positions(qpos, Lc, t.max_log());
+    std::map<uint32_t, std::vector<size_t>> qpl;
+    uint32_t mx = t.max_log();
+    for (auto ls : std::set<uint32_t>(t.eval_logs.begin(), t.eval_logs.end())) {
+      uint32_t shift = mx - ls;
+      std::vector<size_t> v;
+      for (size_t p : q) v.push_back(((p >> (shift + 1)) << 1) + (p & 1));
+      qpl[ls] = sorted_unique(v);
+    }
+    MerkleDec d;
+    out.queried.push_back(merkle_decommit(ctx, t, qpl, d));
+    out.decs.push_back(d);
+    out.commitments.push_back(t.root);
+  }
+  ctx.reset_arena();
+  stage_ms[7] = tm.lap();
+
+  // ---------------- sanity check (prover/mod.rs:127-144) ----------------
+  auto& last = out.sampled.back();
+  qm31 lft = from_partial_evals(last[0][0], last[1][0], last[2][0], last[3][0]);
+  qm31 rgt = from_partial_evals(last[4][0], last[5][0], last[6][0], last[7][0]);
+  qm31 committed = qm_add(lft, qm_mul(cpq_repeated_double(oods, max_deg - 1).x, rgt));
+  qm31 acc = qm_zero();
+  for (auto& c : comps) {
+    PointEval e;
+    e.sv = &out.sampled;
+    e.c = &c;
+    e.acc = &acc;
+    e.rc = random_coeff;
+    e.dinv = qm_inv(coset_vanishing_qm31(max_deg, oods));
+    eval_at_point(e);
+  }
+  return qm_eq(committed, acc) ? 0 : 2;
+}
+
+}  // namespace
+
+ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t n_cols,
+                           const PcsCfg& cfg) {
+  ProveResult r;
+  Timer tm;
+  Channel ch;
+  std::vector<DevTree> trees(2);
+  ctx.ensure_twiddles(log_n + 1 + cfg.log_blowup);
+  commit_tree(ctx, trees[0], cfg, ch);  // empty preprocessed tree
+  interpolate_columns(ctx, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
+  commit_tree(ctx, trees[1], cfg, ch);
+  r.stage_ms[0] = tm.lap();
+  Component c;
+  c.kind = AIR_WIDE_FIB;
+  c.log_n = log_n;
+  c.n_cols = n_cols;
+  ProofParts parts;
+  r.status = prove_ex(ctx, ch, trees, {c}, cfg, parts, r.stage_ms);
+  r.proof = encode(cfg, parts);
+  r.digest = ch.digest;
+  return r;
+}
+
+ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
+                           const PcsCfg& cfg) {
+  ProveResult r;
+  Timer tm;
+  Channel ch;
+  std::vector<DevTree> trees(2);
+  ctx.ensure_twiddles(log_n + 2 + cfg.log_blowup);
+  commit_tree(ctx, trees[0], cfg, ch);
+  interpolate_columns(ctx, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
+  commit_tree(ctx, trees[1], cfg, ch);
+  r.stage_ms[0] = tm.lap();
+  Component c;
+  c.kind = AIR_POSEIDON;
+  c.log_n = log_n;
+  c.ipr = ipr;
+  auto le = ch.draw_secure_felts(2);
+  c.z = le[0];
+  c.alpha = le[1];
+  { qm31 p = qm_one(); for (int i = 0; i < 16; ++i) { c.alpha_pow[i] = p; p = qm_mul(p, c.alpha); } }
+  // LogUp interaction trace on the device
+  DevTree inter_src;
+  std::vector<DevBuf> icols;
+  for (uint32_t i = 0; i < 4 * ipr; ++i) icols.emplace_back(&ctx, (size_t)1 << log_n);
+  {
+    PoseidonParams pp{};
+    pp.z = c.z;
+    for (int i = 0; i < 16; ++i) pp.alpha_pow[i] = c.alpha_pow[i];
+    pp.ipr = ipr;
+    std::vector<uint32_t*> ip;
+    for (auto& b : icols) ip.push_back(b.p);
+    DevBuf sums(&ctx, 8, true);
+    launch_poseidon_logup_fracs(log_n, (const uint32_t* const*)ctx.upload_vec(trace),
+                                (uint32_t* const*)ctx.upload_vec(ip), (const PoseidonParams*)ctx.upload(&pp, sizeof(pp)),
+                                (unsigned long long*)sums.p, ctx.stream);
+    unsigned long long s[4];
+    ctx.d2h(s, sums.p, 32);
+    c.claimed_sum = {m31_reduce64(s[0] % P), m31_reduce64(s[1] % P), m31_reduce64(s[2] % P), m31_reduce64(s[3] % P)};
+    qm31 shift = c.cumsum_shift();
+    std::vector<uint32_t*> last4(ip.end() - 4, ip.end());
+    uint32_t nb = (1u << log_n) >= 1024 ? (1u << log_n) / 1024 : 1;
+    DevBuf scratch(&ctx, 4 * nb);
+    launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.upload_vec(last4), shift, scratch.p, ctx.stream);
+    std::vector<const uint32_t*> ev;
+    for (auto& b : icols) ev.push_back(b.p);
+    trees.emplace_back();
+    interpolate_columns(ctx, ev, std::vector<uint32_t>(ev.size(), log_n), trees[2]);
+    commit_tree(ctx, trees[2], cfg, ch);
+  }
+  icols.clear();
+  r.stage_ms[1] = tm.lap();
+  r.claimed_sum = c.claimed_sum;
+  ProofParts parts;
+  r.status = prove_ex(ctx, ch, trees, {c}, cfg, parts, r.stage_ms);
+  r.proof = encode(cfg, parts);
+  r.digest = ch.digest;
+  return r;
+}
+
+}  // namespace sh

@@ -1,0 +1,68 @@
+// Device-resident prover of libstwo_hip: the reference protocol (prove / prove_ex,
+// CommitmentSchemeProver, FriProver, MerkleProver) orchestrated over HIP kernels.
+#pragma once
+#include "runtime.h"
+
+namespace sh {
+
+struct PcsCfg {
+  uint32_t pow_bits = 5;
+  uint32_t log_blowup = 1;
+  uint32_t log_last_layer = 0;
+  uint32_t n_queries = 3;
+  uint32_t fold_step = 1;
+  uint32_t lifting_log_size = 0;  // 0 = None
+};
+
+enum AirKind { AIR_WIDE_FIB = 0, AIR_POSEIDON = 1 };
+
+struct Component {
+  AirKind kind = AIR_WIDE_FIB;
+  uint32_t log_n = 0;
+  uint32_t n_cols = 0;  // wide-fib sequence length
+  uint32_t ipr = 1;     // Poseidon instances per row
+  qm31 z{}, alpha{};
+  qm31 alpha_pow[16]{};
+  qm31 claimed_sum{};
+  size_t trace_start = 0, inter_start = 0;
+  uint32_t n_trace_cols() const { return kind == AIR_WIDE_FIB ? n_cols : ipr * 158; }
+  uint32_t n_inter_cols() const { return kind == AIR_WIDE_FIB ? 0 : 4 * ipr; }
+  uint32_t n_constraints() const { return kind == AIR_WIDE_FIB ? n_cols - 2 : ipr * 143; }
+  uint32_t eval_log() const { return kind == AIR_WIDE_FIB ? log_n + 1 : log_n + 2; }
+  qm31 cumsum_shift() const { return qm_mul_m31(claimed_sum, m31_inv(1u << log_n)); }
+};
+
+// A committed tree: columns (evaluations, optionally coefficients) and its Merkle layers.
+struct DevTree {
+  std::vector<const uint32_t*> coeffs;  // per column, null when not kept
+  std::vector<uint32_t> coeff_logs;
+  std::vector<const uint32_t*> evals;
+  std::vector<uint32_t> eval_logs;
+  std::vector<DevBuf> owned;
+  std::vector<DevBuf> layers;  // layers[k]: 2^k nodes of 8 words; layers[0] is the root
+  Hash root{};
+  uint32_t max_log() const { return (uint32_t)layers.size() - 1; }
+};
+
+struct ProveResult {
+  int status = 0;  // 0 ok, 2 ConstraintsNotSatisfied (reference prove_ex sanity check)
+  std::vector<uint8_t> proof;
+  Hash digest{};
+  qm31 claimed_sum{};
+  double stage_ms[8] = {0};  // trace commit, interaction, composition, oods, quotients, fri, pow, decommit
+};
+
+// Builds the Merkle layers of a tree from its evaluation columns (MerkleProver::commit).
+void merkle_commit(Ctx& ctx, DevTree& t);
+// Interpolates columns (any sizes) into new coefficient buffers owned by `into`.
+void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, const std::vector<uint32_t>& logs,
+                         DevTree& into);
+// Evaluates the tree's coefficients with blowup into new evaluation buffers.
+void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup);
+
+ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t n_cols,
+                           const PcsCfg& cfg);
+ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
+                           const PcsCfg& cfg);
+
+}  // namespace sh

@@ -1,0 +1,265 @@
+// Host runtime of libstwo_hip (see runtime.h).
+#include "runtime.h"
+
+#include <algorithm>
+
+namespace sh {
+
+DevBuf::DevBuf(Ctx* c, size_t words, bool zero) : n(words), ctx(c) {
+  if (words) {
+    HIP_CHECK(hipMallocAsync((void**)&p, words * 4, c->stream));
+    if (zero) launch_fill_zero(p, words, c->stream);
+  }
+}
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    reset();
+    p = o.p; n = o.n; ctx = o.ctx;
+    o.p = nullptr; o.n = 0;
+  }
+  return *this;
+}
+void DevBuf::reset() {
+  if (p) (void)hipFreeAsync(p, ctx->stream);
+  p = nullptr;
+  n = 0;
+}
+DevBuf::~DevBuf() { reset(); }
+
+Ctx::Ctx(int dev) : device(dev) {
+  HIP_CHECK(hipSetDevice(dev));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  hipMemPool_t pool;
+  HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t thr = UINT64_MAX;  // keep freed blocks cached in the pool
+  HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  pinned_cap = 64u << 20;
+  HIP_CHECK(hipHostMalloc((void**)&pinned, pinned_cap, hipHostMallocDefault));
+  arena_cap = 64u << 20;
+  HIP_CHECK(hipMalloc((void**)&dev_arena, arena_cap));
+}
+Ctx::~Ctx() {
+  (void)hipStreamSynchronize(stream);
+  tw.reset();
+  itw.reset();
+  domain_points.clear();
+  for (auto* p : overflow) (void)hipFree(p);
+  (void)hipStreamSynchronize(stream);
+  if (dev_arena) (void)hipFree(dev_arena);
+  if (pinned) (void)hipHostFree(pinned);
+  (void)hipStreamDestroy(stream);
+}
+
+void* Ctx::upload(const void* src, size_t bytes) {
+  size_t a = (bytes + 255) & ~(size_t)255;
+  if (a == 0) a = 256;
+  if (pinned_off + a > pinned_cap || arena_off + a > arena_cap) {
+    // rare large upload: dedicated allocation, synchronous copy
+    uint8_t* d;
+    HIP_CHECK(hipMalloc((void**)&d, a));
+    HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    overflow.push_back(d);
+    return d;
+  }
+  uint8_t* h = pinned + pinned_off;
+  uint8_t* d = dev_arena + arena_off;
+  memcpy(h, src, bytes);
+  HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+  pinned_off += a;
+  arena_off += a;
+  return d;
+}
+void Ctx::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+void Ctx::reset_arena() {
+  for (auto* p : overflow) (void)hipFree(p);
+  overflow.clear();
+  pinned_off = 0;
+  arena_off = 0;
+}
+void Ctx::d2h(void* dst, const void* src, size_t bytes) {
+  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// Twiddle tree of the half coset of CanonicCoset(log_domain) (prover/backend/cpu/circle.rs:274-291):
+// layer j = x-coordinates of the first half of the coset doubled j times, bit-reversed; pad 1.
+// A tree built for a larger domain serves every smaller canonic domain: line twiddles are read
+// from the end of the buffer (core/poly/utils.rs:98-114).
+void Ctx::ensure_twiddles(uint32_t log_domain) {
+  if (log_domain <= tw_log) return;
+  uint32_t R = log_domain - 1;  // root coset log size
+  size_t len = (size_t)1 << R;
+  std::vector<uint32_t> t, it;
+  t.reserve(len);
+  HCoset c = HCoset::half_odds(R);
+  for (uint32_t j = 0; j < R; ++j) {
+    size_t half = (size_t)1 << (c.log_size - 1);
+    size_t i0 = t.size();
+    cpm31 p = index_to_point(c.initial_index), st = index_to_point(c.step_size);
+    for (size_t i = 0; i < half; ++i) { t.push_back(p.x); p = cp_add(p, st); }
+    uint32_t lg = c.log_size - 1;
+    for (size_t i = 0; i < half; ++i) {
+      size_t k = bitrev((uint32_t)i, lg);
+      if (k > i) std::swap(t[i0 + i], t[i0 + k]);
+    }
+    c = c.doubled();
+  }
+  t.push_back(1);
+  // batch inverse
+  it.resize(t.size());
+  std::vector<uint32_t> pre(t.size());
+  uint32_t acc = 1;
+  for (size_t i = 0; i < t.size(); ++i) { pre[i] = acc; acc = m31_mul(acc, t[i]); }
+  uint32_t inv = m31_inv(acc);
+  for (size_t i = t.size(); i-- > 0;) { it[i] = m31_mul(inv, pre[i]); inv = m31_mul(inv, t[i]); }
+  tw = DevBuf(this, len);
+  itw = DevBuf(this, len);
+  HIP_CHECK(hipMemcpyAsync(tw.p, t.data(), len * 4, hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipMemcpyAsync(itw.p, it.data(), len * 4, hipMemcpyHostToDevice, stream));
+  sync();
+  tw_log = log_domain;
+}
+
+cpm31 canonic_domain_at(uint32_t log, size_t i) {
+  HCoset hc = HCoset::half_odds(log - 1);
+  size_t h = (size_t)1 << (log - 1);
+  if (i < h) return hc.at(i);
+  return index_to_point((0x80000000u - hc.index_at(i - h)) & 0x7fffffffu);
+}
+
+const uint32_t* Ctx::domain_xy(uint32_t log) {
+  auto it = domain_points.find(log);
+  if (it != domain_points.end()) return it->second.p;
+  size_t n = (size_t)1 << log, h = n / 2;
+  std::vector<uint32_t> xy(2 * n);
+  HCoset hc = HCoset::half_odds(log - 1);
+  cpm31 p = index_to_point(hc.initial_index), st = index_to_point(hc.step_size);
+  for (size_t j = 0; j < h; ++j) {
+    size_t r0 = bitrev((uint32_t)j, log), r1 = bitrev((uint32_t)(j + h), log);
+    xy[2 * r0] = p.x; xy[2 * r0 + 1] = p.y;
+    xy[2 * r1] = p.x; xy[2 * r1 + 1] = m31_neg(p.y);  // conjugate half
+    p = cp_add(p, st);
+  }
+  DevBuf b(this, 2 * n);
+  HIP_CHECK(hipMemcpyAsync(b.p, xy.data(), 2 * n * 4, hipMemcpyHostToDevice, stream));
+  sync();
+  const uint32_t* r = b.p;
+  domain_points.emplace(log, std::move(b));
+  return r;
+}
+
+uint32_t coset_vanishing_m31(uint32_t coset_log, cpm31 p) {
+  HCoset c = HCoset::odds(coset_log);
+  cpm31 init = index_to_point(c.initial_index);
+  cpm31 half = index_to_point(c.step_size >> 1);
+  p = cp_add(cp_add(p, cp_conj(init)), half);
+  uint32_t x = p.x;
+  for (uint32_t i = 1; i < coset_log; ++i) x = m31_double_x(x);
+  return x;
+}
+qm31 coset_vanishing_qm31(uint32_t coset_log, cpqm31 p) {
+  HCoset c = HCoset::odds(coset_log);
+  cpm31 init = index_to_point(c.initial_index);
+  cpm31 half = index_to_point(c.step_size >> 1);
+  p = cpq_add(cpq_add(p, cpq_from(cp_conj(init))), cpq_from(half));
+  qm31 x = p.x;
+  for (uint32_t i = 1; i < coset_log; ++i) x = qm_double_x(x);
+  return x;
+}
+cpqm31 cpq_mul_signed(cpm31 step, long off) {
+  cpm31 base = off > 0 ? step : cp_conj(step);
+  unsigned long k = off > 0 ? off : -off;
+  cpm31 r = {1, 0};
+  while (k) { if (k & 1) r = cp_add(r, base); base = cp_double(base); k >>= 1; }
+  return cpq_from(r);
+}
+
+// ---- Blake2s (host) -----------------------------------------------------------------------
+static void words_from_bytes(const uint8_t* b, uint32_t m[16]) {
+  for (int i = 0; i < 16; ++i)
+    m[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) | ((uint32_t)b[4 * i + 3] << 24);
+}
+void Blake2sHost::update(const void* data, size_t len) {
+  const uint8_t* p = (const uint8_t*)data;
+  while (len) {
+    if (buflen == 64) {
+      t += 64;
+      uint32_t m[16];
+      words_from_bytes(buf, m);
+      blake2s_compress(h, m, (uint32_t)t, false);
+      buflen = 0;
+    }
+    size_t k = std::min(len, 64 - buflen);
+    memcpy(buf + buflen, p, k);
+    buflen += k;
+    p += k;
+    len -= k;
+  }
+}
+Hash Blake2sHost::finalize() {
+  t += buflen;
+  memset(buf + buflen, 0, 64 - buflen);
+  uint32_t m[16];
+  words_from_bytes(buf, m);
+  blake2s_compress(h, m, (uint32_t)t, true);
+  Hash out;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = (uint8_t)(h[i] >> (8 * j));
+  return out;
+}
+Hash blake2s_hash(const void* data, size_t len) { Blake2sHost b; b.update(data, len); return b.finalize(); }
+
+void Channel::mix_root(const Hash& root) {
+  Blake2sHost b; b.update(digest.data(), 32); b.update(root.data(), 32); update_digest(b.finalize());
+}
+void Channel::mix_u32s(const uint32_t* data, size_t n) {
+  Blake2sHost b; b.update(digest.data(), 32);
+  for (size_t i = 0; i < n; ++i) b.update_u32(data[i]);
+  update_digest(b.finalize());
+}
+void Channel::mix_felts(const std::vector<qm31>& f) {
+  Blake2sHost b; b.update(digest.data(), 32);
+  for (auto& q : f) { b.update_u32(q.a); b.update_u32(q.b); b.update_u32(q.c); b.update_u32(q.d); }
+  update_digest(b.finalize());
+}
+std::array<uint32_t, 8> Channel::draw_u32s() {
+  uint8_t in[37];
+  memcpy(in, digest.data(), 32);
+  for (int i = 0; i < 4; ++i) in[32 + i] = (uint8_t)(n_draws >> (8 * i));
+  in[36] = 0;
+  n_draws++;
+  Hash h = blake2s_hash(in, 37);
+  std::array<uint32_t, 8> r;
+  for (int i = 0; i < 8; ++i) r[i] = (uint32_t)h[4 * i] | ((uint32_t)h[4 * i + 1] << 8) | ((uint32_t)h[4 * i + 2] << 16) | ((uint32_t)h[4 * i + 3] << 24);
+  return r;
+}
+std::array<uint32_t, 8> Channel::draw_base_felts() {
+  while (true) {
+    auto u = draw_u32s();
+    bool ok = true;
+    for (auto x : u) ok &= x < 2 * P;
+    if (ok) { for (auto& x : u) x = x >= P ? x - P : x; return u; }
+  }
+}
+qm31 Channel::draw_secure_felt() { auto f = draw_base_felts(); return {f[0], f[1], f[2], f[3]}; }
+std::vector<qm31> Channel::draw_secure_felts(size_t n) {
+  std::vector<qm31> r;
+  while (r.size() < n) {
+    auto f = draw_base_felts();
+    r.push_back({f[0], f[1], f[2], f[3]});
+    if (r.size() < n) r.push_back({f[4], f[5], f[6], f[7]});
+  }
+  return r;
+}
+Hash Channel::pow_prefix(uint32_t pow_bits) const {
+  Blake2sHost b;
+  b.update_u32(0x12345678u);
+  uint8_t z[12] = {0};
+  b.update(z, 12);
+  b.update(digest.data(), 32);
+  b.update_u32(pow_bits);
+  return b.finalize();
+}
+
+}  // namespace sh

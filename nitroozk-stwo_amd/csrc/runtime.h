@@ -1,0 +1,126 @@
+// Host runtime of libstwo_hip: device context, stream-ordered memory, pinned staging,
+// twiddle / domain caches, and the host-side Fiat-Shamir channel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <array>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "blake2s.h"
+#include "field.h"
+#include "kernels.h"
+
+namespace sh {
+
+struct HipError : std::runtime_error {
+  int code;
+  HipError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+#define HIP_CHECK(x)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess) throw HipError(2, std::string(#x " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Ctx;
+
+// Owning device buffer of u32 words, freed stream-ordered.
+struct DevBuf {
+  uint32_t* p = nullptr;
+  size_t n = 0;
+  Ctx* ctx = nullptr;
+  DevBuf() = default;
+  DevBuf(Ctx* c, size_t words, bool zero = false);
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept;
+  ~DevBuf();
+  void reset();
+};
+
+using Hash = std::array<uint8_t, 32>;
+
+struct StageTimes {
+  double ms[16] = {0};
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // twiddle tree for the half coset of CanonicCoset(tw_log): 2^(tw_log-1) words each
+  uint32_t tw_log = 0;
+  DevBuf tw, itw;
+  std::map<uint32_t, DevBuf> domain_points;  // bit-reversed (x, y) of CanonicCoset(log)
+  // small-upload arena (device) and pinned staging (host)
+  uint8_t* pinned = nullptr;
+  size_t pinned_cap = 0, pinned_off = 0;
+  uint8_t* dev_arena = nullptr;
+  size_t arena_cap = 0, arena_off = 0;
+  std::vector<uint8_t*> overflow;  // arena overflow allocations, freed at reset
+
+  explicit Ctx(int dev);
+  ~Ctx();
+  // Copies host bytes to device through pinned staging; valid on the stream until reset_arena().
+  void* upload(const void* src, size_t bytes);
+  template <class T> const T* upload_vec(const std::vector<T>& v) { return (const T*)upload(v.data(), v.size() * sizeof(T)); }
+  void sync();
+  void reset_arena();  // after sync
+  void ensure_twiddles(uint32_t log_domain);
+  const uint32_t* domain_xy(uint32_t log);
+  uint32_t tree_len() const { return 1u << (tw_log - 1); }
+  void d2h(void* dst, const void* src, size_t bytes);  // synchronous
+};
+
+// ---- host Blake2s and channel ------------------------------------------------------------
+struct Blake2sHost {
+  uint32_t h[8];
+  uint8_t buf[64];
+  size_t buflen = 0;
+  uint64_t t = 0;
+  Blake2sHost() { blake2s_init(h); }
+  void update(const void* data, size_t len);
+  void update_u32(uint32_t w) { uint8_t b[4] = {(uint8_t)w, (uint8_t)(w >> 8), (uint8_t)(w >> 16), (uint8_t)(w >> 24)}; update(b, 4); }
+  Hash finalize();
+};
+Hash blake2s_hash(const void* data, size_t len);
+
+// Blake2sChannel (crates/stwo/src/core/channel/blake2s.rs:19-140).
+struct Channel {
+  Hash digest{};
+  uint32_t n_draws = 0;
+  void update_digest(const Hash& d) { digest = d; n_draws = 0; }
+  void mix_root(const Hash& root);
+  void mix_u32s(const uint32_t* data, size_t n);
+  void mix_u64(uint64_t v) { uint32_t d[2] = {(uint32_t)v, (uint32_t)(v >> 32)}; mix_u32s(d, 2); }
+  void mix_felts(const std::vector<qm31>& f);
+  std::array<uint32_t, 8> draw_u32s();
+  std::array<uint32_t, 8> draw_base_felts();
+  qm31 draw_secure_felt();
+  std::vector<qm31> draw_secure_felts(size_t n);
+  Hash pow_prefix(uint32_t pow_bits) const;
+};
+
+// ---- host circle-domain helpers -------------------------------------------------------------
+inline uint32_t subgroup_gen(uint32_t log) { return 1u << (31 - log); }
+struct HCoset {
+  uint32_t initial_index, step_size, log_size;
+  static HCoset make(uint32_t init, uint32_t log) { return {init & 0x7fffffffu, subgroup_gen(log), log}; }
+  static HCoset odds(uint32_t log) { return make(subgroup_gen(log + 1), log); }
+  static HCoset half_odds(uint32_t log) { return make(subgroup_gen(log + 2), log); }
+  uint32_t index_at(size_t i) const { return (uint32_t)(((uint64_t)initial_index + (uint64_t)step_size * i) & 0x7fffffffu); }
+  cpm31 at(size_t i) const { return index_to_point(index_at(i)); }
+  HCoset doubled() const { return {(initial_index * 2) & 0x7fffffffu, (step_size * 2) & 0x7fffffffu, log_size ? log_size - 1 : 0}; }
+};
+// Point i of the circle domain of CanonicCoset(log) (natural order).
+cpm31 canonic_domain_at(uint32_t log, size_t i);
+// Vanishing polynomial of CanonicCoset(log) (core/constraints.rs coset_vanishing).
+uint32_t coset_vanishing_m31(uint32_t coset_log, cpm31 p);
+qm31 coset_vanishing_qm31(uint32_t coset_log, cpqm31 p);
+cpqm31 cpq_mul_signed(cpm31 step, long off);
+
+}  // namespace sh

@@ -1,0 +1,387 @@
+"""Python host side of libstwo_hip: a HipBackend with the reference's prover-backend trait
+methods (PolyOps, MerkleOps<Blake2sMerkleHasher>, FriOps, QuotientOps, AccumulationOps,
+GrindOps, ComponentProver) and the two example provers, over the C ABI in include/stwo_hip.h.
+
+Device memory is owned by DeviceColumn objects (the analogue of BaseFieldVec,
+crates/stwo/src/stwo_cuda/base_field_vec.rs:7-140). Every call goes to the HIP library; there is
+no CPU fallback, and constructing a backend fails loudly when the library or the GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libstwo_hip.so")
+P = (1 << 31) - 1
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "unsupported size", 4: "protocol failure",
+          5: "unknown component"}
+
+
+class StwoHipError(RuntimeError):
+    pass
+
+
+class QM31(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_uint32), ("b", ctypes.c_uint32), ("c", ctypes.c_uint32), ("d", ctypes.c_uint32)]
+
+    @staticmethod
+    def of(v):
+        return QM31(*[int(x) for x in v])
+
+    def tuple(self):
+        return (self.a, self.b, self.c, self.d)
+
+
+class CirclePointQM31(ctypes.Structure):
+    _fields_ = [("x", QM31), ("y", QM31)]
+
+
+class PcsConfig(ctypes.Structure):
+    """PcsConfig + FriConfig (crates/stwo/src/core/pcs/mod.rs:29-80, core/fri.rs:32-81)."""
+    _fields_ = [("pow_bits", ctypes.c_uint32), ("log_blowup_factor", ctypes.c_uint32),
+                ("log_last_layer_degree_bound", ctypes.c_uint32), ("n_queries", ctypes.c_uint32),
+                ("line_fold_step", ctypes.c_uint32), ("lifting_log_size", ctypes.c_uint32)]
+
+    @staticmethod
+    def make(pow_bits=5, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3, line_fold_step=1,
+             lifting_log_size=0):
+        return PcsConfig(pow_bits, log_blowup_factor, log_last_layer_degree_bound, n_queries, line_fold_step,
+                         lifting_log_size)
+
+
+class SampleBatch(ctypes.Structure):
+    _fields_ = [("point", CirclePointQM31), ("n_columns", ctypes.c_uint32),
+                ("column_indices", ctypes.POINTER(ctypes.c_uint32)), ("sample_values", ctypes.POINTER(QM31)),
+                ("random_coeffs", ctypes.POINTER(QM31))]
+
+
+class AccumulatedNumerators(ctypes.Structure):
+    _fields_ = [("sample_point", CirclePointQM31), ("partial", ctypes.c_void_p * 4), ("log_size", ctypes.c_uint32),
+                ("first_linear_term", QM31)]
+
+
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_VPP = ctypes.POINTER(ctypes.c_void_p)
+
+EXPORTS = [
+    "stwo_hip_create", "stwo_hip_destroy", "stwo_hip_synchronize", "stwo_hip_last_error", "stwo_hip_alloc",
+    "stwo_hip_free", "stwo_hip_copy_h2d", "stwo_hip_copy_d2h", "stwo_hip_batch_at_multi",
+    "stwo_hip_precompute_twiddles", "stwo_hip_interpolate_columns", "stwo_hip_evaluate_polynomials",
+    "stwo_hip_batch_eval_at_point", "stwo_hip_commit_on_layer", "stwo_hip_fold_circle_into_line",
+    "stwo_hip_fold_line", "stwo_hip_accumulate_numerators", "stwo_hip_compute_quotients_and_combine",
+    "stwo_hip_accumulate", "stwo_hip_lift_and_accumulate", "stwo_hip_generate_secure_powers", "stwo_hip_grind",
+    "stwo_hip_evaluate_constraint_quotients_on_domain", "stwo_hip_prove_wide_fibonacci", "stwo_hip_prove_poseidon",
+    "stwo_proof_status", "stwo_proof_size", "stwo_proof_bytes", "stwo_proof_digest", "stwo_proof_claimed_sum",
+    "stwo_proof_stage_ms", "stwo_proof_free", "stwo_hip_gen_trace_wide_fibonacci", "stwo_hip_gen_trace_poseidon",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libstwo_hip.so; raises when it is missing (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise StwoHipError("libstwo_hip.so not built: run __graft_entry__.build() (expected at %s)" % path)
+    L = ctypes.CDLL(path)
+    c = ctypes
+    L.stwo_hip_create.argtypes = [c.c_int, c.POINTER(c.c_void_p)]
+    L.stwo_hip_destroy.argtypes = [c.c_void_p]
+    L.stwo_hip_destroy.restype = None
+    L.stwo_hip_synchronize.argtypes = [c.c_void_p]
+    L.stwo_hip_last_error.argtypes = [c.c_void_p]
+    L.stwo_hip_last_error.restype = c.c_char_p
+    L.stwo_hip_alloc.argtypes = [c.c_void_p, c.c_size_t, c.c_int, c.POINTER(c.c_void_p)]
+    L.stwo_hip_free.argtypes = [c.c_void_p, c.c_void_p]
+    L.stwo_hip_copy_h2d.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_size_t]
+    L.stwo_hip_copy_d2h.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_size_t]
+    L.stwo_hip_batch_at_multi.argtypes = [c.c_void_p, _VPP, c.c_uint32, _U32P, c.c_uint32, _U32P]
+    L.stwo_hip_precompute_twiddles.argtypes = [c.c_void_p, c.c_uint32, _U32P, _U32P]
+    L.stwo_hip_interpolate_columns.argtypes = [c.c_void_p, _VPP, _VPP, c.c_uint32, c.c_uint32]
+    L.stwo_hip_evaluate_polynomials.argtypes = [c.c_void_p, _VPP, _VPP, c.c_uint32, c.c_uint32, c.c_uint32]
+    L.stwo_hip_batch_eval_at_point.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, CirclePointQM31,
+                                               c.POINTER(QM31)]
+    L.stwo_hip_commit_on_layer.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32, c.c_void_p]
+    L.stwo_hip_fold_circle_into_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
+    L.stwo_hip_fold_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
+    L.stwo_hip_accumulate_numerators.argtypes = [c.c_void_p, c.c_uint32, _VPP, c.c_uint32, c.POINTER(SampleBatch),
+                                                 c.c_uint32, _VPP, c.POINTER(QM31)]
+    L.stwo_hip_compute_quotients_and_combine.argtypes = [c.c_void_p, c.POINTER(AccumulatedNumerators), c.c_uint32,
+                                                         c.c_uint32, _VPP]
+    L.stwo_hip_accumulate.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP]
+    L.stwo_hip_lift_and_accumulate.argtypes = [c.c_void_p, _VPP, _U32P, c.c_uint32, _VPP]
+    L.stwo_hip_generate_secure_powers.argtypes = [c.c_void_p, QM31, c.c_uint32, c.POINTER(QM31)]
+    L.stwo_hip_grind.argtypes = [c.c_void_p, c.c_char_p, c.c_uint32, c.POINTER(c.c_uint64)]
+    L.stwo_hip_evaluate_constraint_quotients_on_domain.argtypes = [
+        c.c_void_p, _VPP, _VPP, c.c_uint32, _VPP, c.c_uint32, c.POINTER(QM31), c.c_uint32, _U32P, c.c_uint32,
+        c.c_uint32, c.c_char_p, c.c_size_t, QM31]
+    L.stwo_hip_prove_wide_fibonacci.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.POINTER(PcsConfig),
+                                                c.POINTER(c.c_void_p)]
+    L.stwo_hip_prove_poseidon.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.POINTER(PcsConfig),
+                                          c.POINTER(c.c_void_p)]
+    L.stwo_proof_status.argtypes = [c.c_void_p]
+    L.stwo_proof_size.argtypes = [c.c_void_p]
+    L.stwo_proof_size.restype = c.c_size_t
+    L.stwo_proof_bytes.argtypes = [c.c_void_p]
+    L.stwo_proof_bytes.restype = c.c_void_p
+    L.stwo_proof_digest.argtypes = [c.c_void_p, c.c_char_p]
+    L.stwo_proof_digest.restype = None
+    L.stwo_proof_claimed_sum.argtypes = [c.c_void_p, c.POINTER(QM31)]
+    L.stwo_proof_claimed_sum.restype = None
+    L.stwo_proof_stage_ms.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
+    L.stwo_proof_stage_ms.restype = None
+    L.stwo_proof_free.argtypes = [c.c_void_p]
+    L.stwo_proof_free.restype = None
+    L.stwo_hip_gen_trace_wide_fibonacci.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.c_uint32,
+                                                    c.c_uint64]
+    L.stwo_hip_gen_trace_poseidon.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint64]
+    _lib = L
+    return L
+
+
+def _ptrs(cols):
+    arr = (ctypes.c_void_p * max(1, len(cols)))()
+    for i, col in enumerate(cols):
+        arr[i] = col.ptr if isinstance(col, DeviceColumn) else col
+    return arr
+
+
+class DeviceColumn:
+    """A device-resident column of u32 words (M31 values), freed with its backend."""
+
+    def __init__(self, backend, n_words, zeroed=False):
+        self.backend = backend
+        self.n = int(n_words)
+        p = ctypes.c_void_p()
+        backend._check(backend.lib.stwo_hip_alloc(backend.ctx, self.n, int(zeroed), ctypes.byref(p)))
+        self.ptr = p.value
+
+    @property
+    def log_size(self):
+        return self.n.bit_length() - 1
+
+    def free(self):
+        if self.ptr:
+            self.backend.lib.stwo_hip_free(self.backend.ctx, self.ptr)
+            self.ptr = None
+
+    def to_cpu(self):
+        out = np.empty(self.n, np.uint32)
+        self.backend._check(self.backend.lib.stwo_hip_copy_d2h(self.backend.ctx, out.ctypes.data, self.ptr, self.n))
+        return out
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Proof:
+    def __init__(self, backend, handle):
+        L = backend.lib
+        self.status = L.stwo_proof_status(handle)
+        n = L.stwo_proof_size(handle)
+        self.bytes = ctypes.string_at(L.stwo_proof_bytes(handle), n)
+        d = ctypes.create_string_buffer(32)
+        L.stwo_proof_digest(handle, d)
+        self.digest = d.raw
+        cs = QM31()
+        L.stwo_proof_claimed_sum(handle, ctypes.byref(cs))
+        self.claimed_sum = cs.tuple()
+        ms = (ctypes.c_double * 8)()
+        L.stwo_proof_stage_ms(handle, ms)
+        self.stage_ms = dict(zip(["trace_commit", "interaction", "composition", "oods", "quotients", "fri", "pow",
+                                  "decommit"], list(ms)))
+        L.stwo_proof_free(handle)
+
+
+class HipBackend:
+    """Drop-in counterpart of SimdBackend's trait methods (crates/stwo/src/prover/backend/mod.rs:20-37)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        p = ctypes.c_void_p()
+        st = self.lib.stwo_hip_create(device, ctypes.byref(p))
+        if st != 0:
+            raise StwoHipError("stwo_hip_create failed (status %d): is a GPU visible?" % st)
+        self.ctx = p.value
+
+    def close(self):
+        if self.ctx:
+            self.lib.stwo_hip_destroy(self.ctx)
+            self.ctx = None
+
+    def _check(self, st):
+        if st != 0:
+            msg = self.lib.stwo_hip_last_error(self.ctx)
+            raise StwoHipError("%s: %s" % (STATUS.get(st, st), msg.decode() if msg else ""))
+
+    # ---- columns ----
+    def column(self, n_words, zeroed=False):
+        return DeviceColumn(self, n_words, zeroed)
+
+    def from_cpu(self, arr):
+        a = np.ascontiguousarray(arr, np.uint32)
+        col = DeviceColumn(self, a.size)
+        self._check(self.lib.stwo_hip_copy_h2d(self.ctx, col.ptr, a.ctypes.data, a.size))
+        return col
+
+    def batch_at_multi(self, cols, rows):
+        rows = np.ascontiguousarray(rows, np.uint32)
+        out = np.empty(len(rows) * len(cols), np.uint32)
+        self._check(self.lib.stwo_hip_batch_at_multi(self.ctx, _ptrs(cols), len(cols), rows.ctypes.data_as(_U32P),
+                                                     len(rows), out.ctypes.data_as(_U32P)))
+        return out
+
+    def synchronize(self):
+        self._check(self.lib.stwo_hip_synchronize(self.ctx))
+
+    # ---- PolyOps ----
+    def precompute_twiddles(self, log_domain):
+        n = 1 << (log_domain - 1)
+        tw = np.empty(n, np.uint32)
+        itw = np.empty(n, np.uint32)
+        self._check(self.lib.stwo_hip_precompute_twiddles(self.ctx, log_domain, tw.ctypes.data_as(_U32P),
+                                                          itw.ctypes.data_as(_U32P)))
+        return tw, itw
+
+    def interpolate_columns(self, evals, log_size, out=None):
+        out = out if out is not None else [self.column(1 << log_size) for _ in evals]
+        self._check(self.lib.stwo_hip_interpolate_columns(self.ctx, _ptrs(evals), _ptrs(out), len(evals), log_size))
+        return out
+
+    def evaluate_polynomials(self, coeffs, log_size, log_blowup):
+        out = [self.column(1 << (log_size + log_blowup)) for _ in coeffs]
+        self._check(self.lib.stwo_hip_evaluate_polynomials(self.ctx, _ptrs(coeffs), _ptrs(out), len(coeffs), log_size,
+                                                           log_blowup))
+        return out
+
+    def batch_eval_at_point(self, coeffs, log_size, point):
+        out = (QM31 * len(coeffs))()
+        pt = CirclePointQM31(QM31.of(point[:4]), QM31.of(point[4:]))
+        self._check(self.lib.stwo_hip_batch_eval_at_point(self.ctx, _ptrs(coeffs), len(coeffs), log_size, pt, out))
+        return [o.tuple() for o in out]
+
+    # ---- MerkleOps ----
+    def commit_on_layer(self, log_size, prev, cols):
+        out = self.column(8 << log_size)
+        self._check(self.lib.stwo_hip_commit_on_layer(self.ctx, log_size, prev.ptr if prev is not None else None,
+                                                      _ptrs(cols), len(cols), out.ptr))
+        return out
+
+    # ---- FriOps ----
+    def fold_circle_into_line(self, dst4, src4, alpha, log_src):
+        self._check(self.lib.stwo_hip_fold_circle_into_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst4),
+                                                            QM31.of(alpha)))
+
+    def fold_line(self, src4, alpha, log_src):
+        dst = [self.column(1 << (log_src - 1)) for _ in range(4)]
+        self._check(self.lib.stwo_hip_fold_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst), QM31.of(alpha)))
+        return dst
+
+    # ---- QuotientOps ----
+    def accumulate_numerators(self, cols, log_size, batches):
+        """batches: list of (point8, [(col_idx, value4, rand4), ...]); returns (partials, first_linear)."""
+        keep = []
+        arr = (SampleBatch * len(batches))()
+        for i, (pt, entries) in enumerate(batches):
+            idx = np.array([e[0] for e in entries], np.uint32)
+            vals = (QM31 * len(entries))(*[QM31.of(e[1]) for e in entries])
+            rnds = (QM31 * len(entries))(*[QM31.of(e[2]) for e in entries])
+            keep += [idx, vals, rnds]
+            arr[i].point = CirclePointQM31(QM31.of(pt[:4]), QM31.of(pt[4:]))
+            arr[i].n_columns = len(entries)
+            arr[i].column_indices = idx.ctypes.data_as(_U32P)
+            arr[i].sample_values = vals
+            arr[i].random_coeffs = rnds
+        outs = [self.column(1 << log_size) for _ in range(4 * len(batches))]
+        fl = (QM31 * len(batches))()
+        self._check(self.lib.stwo_hip_accumulate_numerators(self.ctx, log_size, _ptrs(cols), len(cols), arr,
+                                                            len(batches), _ptrs(outs), fl))
+        return [outs[4 * b:4 * b + 4] for b in range(len(batches))], [f.tuple() for f in fl]
+
+    def compute_quotients_and_combine(self, accs, lifting_log_size):
+        """accs: list of (point8, partial4 columns, log_size, first_linear4)."""
+        arr = (AccumulatedNumerators * len(accs))()
+        for i, (pt, part, ls, fl) in enumerate(accs):
+            arr[i].sample_point = CirclePointQM31(QM31.of(pt[:4]), QM31.of(pt[4:]))
+            for k in range(4):
+                arr[i].partial[k] = part[k].ptr
+            arr[i].log_size = ls
+            arr[i].first_linear_term = QM31.of(fl)
+        out = [self.column(1 << lifting_log_size) for _ in range(4)]
+        self._check(self.lib.stwo_hip_compute_quotients_and_combine(self.ctx, arr, len(accs), lifting_log_size,
+                                                                    _ptrs(out)))
+        return out
+
+    # ---- AccumulationOps ----
+    def accumulate(self, dst4, src4, log_size):
+        self._check(self.lib.stwo_hip_accumulate(self.ctx, log_size, _ptrs(dst4), _ptrs(src4)))
+
+    def lift_and_accumulate(self, cols4_list, log_sizes):
+        flat = [c for cols4 in cols4_list for c in cols4]
+        ls = np.array(log_sizes, np.uint32)
+        out = [self.column(1 << int(max(log_sizes))) for _ in range(4)]
+        self._check(self.lib.stwo_hip_lift_and_accumulate(self.ctx, _ptrs(flat), ls.ctypes.data_as(_U32P), len(ls),
+                                                          _ptrs(out)))
+        return out
+
+    def generate_secure_powers(self, felt, n):
+        out = (QM31 * n)()
+        self._check(self.lib.stwo_hip_generate_secure_powers(self.ctx, QM31.of(felt), n, out))
+        return [o.tuple() for o in out]
+
+    # ---- GrindOps ----
+    def grind(self, digest, pow_bits):
+        nonce = ctypes.c_uint64()
+        self._check(self.lib.stwo_hip_grind(self.ctx, digest, pow_bits, ctypes.byref(nonce)))
+        return nonce.value
+
+    # ---- ComponentProver ----
+    def evaluate_constraint_quotients_on_domain(self, acc4, trace, interaction, coeff_powers, denom_inv, trace_log,
+                                                eval_log, eval_blob, cumsum_shift=(0, 0, 0, 0)):
+        pows = (QM31 * len(coeff_powers))(*[QM31.of(p) for p in coeff_powers])
+        dinv = np.ascontiguousarray(denom_inv, np.uint32)
+        self._check(self.lib.stwo_hip_evaluate_constraint_quotients_on_domain(
+            self.ctx, _ptrs(acc4), _ptrs(trace), len(trace), _ptrs(interaction), len(interaction), pows,
+            len(coeff_powers), dinv.ctypes.data_as(_U32P), trace_log, eval_log, eval_blob, len(eval_blob),
+            QM31.of(cumsum_shift)))
+
+    # ---- trace generation and whole proofs ----
+    def gen_trace_wide_fibonacci(self, n_cols, log_n_rows, mode=1, seed=0):
+        cols = [self.column(1 << log_n_rows) for _ in range(n_cols)]
+        self._check(self.lib.stwo_hip_gen_trace_wide_fibonacci(self.ctx, _ptrs(cols), n_cols, log_n_rows, mode,
+                                                               seed))
+        return cols
+
+    def gen_trace_poseidon(self, instances_per_row, log_n_rows, mode=1, seed=0):
+        cols = [self.column(1 << log_n_rows) for _ in range(158 * instances_per_row)]
+        self._check(self.lib.stwo_hip_gen_trace_poseidon(self.ctx, _ptrs(cols), instances_per_row, log_n_rows, mode,
+                                                         seed))
+        return cols
+
+    def prove_wide_fibonacci(self, trace, log_n_rows, config):
+        h = ctypes.c_void_p()
+        self._check(self.lib.stwo_hip_prove_wide_fibonacci(self.ctx, _ptrs(trace), len(trace), log_n_rows,
+                                                           ctypes.byref(config), ctypes.byref(h)))
+        return Proof(self, h.value)
+
+    def prove_poseidon(self, trace, instances_per_row, log_n_rows, config):
+        h = ctypes.c_void_p()
+        self._check(self.lib.stwo_hip_prove_poseidon(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
+                                                     ctypes.byref(config), ctypes.byref(h)))
+        return Proof(self, h.value)
+
+
+def fnv1a(name: str) -> int:
+    """Component id of the reference's dispatcher (crates/constraint-framework/src/lib.rs fnv1a_eval_id_gen)."""
+    h = 0x811C9DC5
+    for b in name.encode():
+        h ^= b
+        h = (h * 0x01000193) & 0xFFFFFFFF
+    return h
