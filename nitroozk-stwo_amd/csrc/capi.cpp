@@ -447,4 +447,23 @@ int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* h, uint32_t* const* cols, uint32_t
   });
 }
 
+int stwo_hip_set_kernel_timing(stwo_hip_ctx* h, int enable) {
+  return guarded(h, [&](Ctx& c) { c.sync(); c.timing = enable != 0; });
+}
+int stwo_hip_kernel_stats(stwo_hip_ctx* h, uint32_t cls, double* ms, double* bytes, uint64_t* launches) {
+  return guarded(h, [&](Ctx& c) {
+    if (cls >= K_NUM_CLASSES) throw HipError(1, "kernel class out of range");
+    c.sync();
+    if (ms) *ms = c.kstats[cls].ms;
+    if (bytes) *bytes = c.kstats[cls].bytes;
+    if (launches) *launches = c.kstats[cls].launches;
+  });
+}
+int stwo_hip_reset_kernel_stats(stwo_hip_ctx* h) {
+  return guarded(h, [&](Ctx& c) {
+    c.sync();
+    for (auto& k : c.kstats) k = KernelStat();
+  });
+}
+
 }  // extern "C"

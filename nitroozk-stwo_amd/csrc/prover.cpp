@@ -324,6 +324,7 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
       src.push_back(coeffs[i]);
       dst.push_back(out[i].p);
     }
+    KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)));
     launch_fft(false, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
                (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), ctx.stream);
   }
@@ -349,6 +350,7 @@ void merkle_commit(Ctx& ctx, DevTree& t) {
     DevBuf layer(&ctx, (size_t)8 << ls);
     const uint32_t* prev = rev.empty() ? nullptr : rev.back().p;
     const uint32_t* const* cp = lc.empty() ? nullptr : (const uint32_t* const*)ctx.upload_vec(lc);
+    KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
     launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
     rev.push_back(std::move(layer));
   }
@@ -375,6 +377,7 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
       into.coeffs[base + i] = into.owned.back().p;
       into.coeff_logs[base + i] = lg;
     }
+    KScope ks(ctx, K_INTERPOLATE, (double)idx.size() * 8.0 * (1u << lg));
     launch_fft(true, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
                (uint32_t)idx.size(), lg, lg, ctx.itw.p, ctx.tree_len(), ctx.stream);
   }
@@ -397,6 +400,7 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
       t.evals[i] = t.owned.back().p;
       t.eval_logs[i] = K;
     }
+    KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << K)));
     launch_fft(false, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
                (uint32_t)idx.size(), lg, K, ctx.tw.p, ctx.tree_len(), ctx.stream);
   }
@@ -558,6 +562,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
     const uint32_t* d_dinv = ctx.upload_vec(dinv_br);
     const qm31* d_pows = ctx.upload_vec(cp);
+    KScope ks(ctx, K_COMPOSITION, (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E));
     if (c.kind == AIR_WIDE_FIB) {
       launch_wide_fib_composition(E, c.log_n, (const uint32_t* const*)ctx.upload_vec(tcols), c.n_cols, d_pows, d_dinv,
                                   a4, ctx.stream);
@@ -645,8 +650,14 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   if (n_vals) {
     uint32_t max_chunks = max_coeff_log > 11 ? 1u << (max_coeff_log - 11) : 1u;
     DevBuf partials(&ctx, (size_t)n_vals * max_chunks * 4), dvals(&ctx, n_vals * 4);
-    launch_oods(ctx.upload_vec(tasks), (uint32_t)n_vals, max_coeff_log, ctx.upload_vec(factors), (qm31*)partials.p,
-                (qm31*)dvals.p, ctx.stream);
+    double obytes = 0;
+    for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
+    const OodsTask* dt = ctx.upload_vec(tasks);
+    const qm31* df = ctx.upload_vec(factors);
+    {
+      KScope ks(ctx, K_OODS, obytes);
+      launch_oods(dt, (uint32_t)n_vals, max_coeff_log, df, (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
+    }
     ctx.d2h(values.data(), dvals.p, n_vals * 16);
     ctx.reset_arena();
   }
@@ -741,6 +752,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         acc_list.push_back(a);
       }
       starts.push_back((uint32_t)ents.size());
+      KScope ks(ctx, K_NUMERATORS, (4.0 * gcols.size() + 16.0 * batches.size()) * (double)(1u << ls));
       launch_accumulate_numerators(ls, (const uint32_t* const*)ctx.upload_vec(gcols), ctx.upload_vec(ents),
                                    ctx.upload_vec(starts), ctx.upload_vec(Bs), (uint32_t)batches.size(),
                                    (uint32_t* const*)ctx.upload_vec(outs), ctx.stream);
@@ -772,7 +784,11 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   FriCol first;
   first.log = Lc;
   for (int k = 0; k < 4; ++k) first.c.emplace_back(&ctx, (size_t)1 << Lc);
-  launch_quotients(Lc, ctx.domain_xy(Lc), ctx.upload_vec(qpts), (uint32_t)qpts.size(), ctx.upload_vec(qparts),
+  const uint32_t* dxy = ctx.domain_xy(Lc);
+  double qbytes = 24.0 * (double)(1u << Lc);
+  for (auto& pa : qparts) qbytes += 16.0 * (double)(1u << pa.log_size);
+  KScope ksq(ctx, K_QUOTIENTS, qbytes);
+  launch_quotients(Lc, dxy, ctx.upload_vec(qpts), (uint32_t)qpts.size(), ctx.upload_vec(qparts),
                    first.mc(), ctx.stream);
   ctx.sync();
   ctx.reset_arena();
@@ -788,7 +804,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   cur.log = Lc - 1;
   for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log, true);
   qm31 alpha = ch.draw_secure_felt();
-  launch_fold_circle_into_line(Lc, first.cc(), cur.mc(), alpha, ctx.itw.p, ctx.tree_len(), ctx.stream);
+  {
+    KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 32.0 * (double)(1u << (Lc - 1)));
+    launch_fold_circle_into_line(Lc, first.cc(), cur.mc(), alpha, ctx.itw.p, ctx.tree_len(), ctx.stream);
+  }
   uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
   auto fold_layer = [&](FriCol& src, uint32_t fs, qm31 a) {
     FriCol res;
@@ -798,6 +817,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       FriCol nx;
       nx.log = s->log - 1;
       for (int k = 0; k < 4; ++k) nx.c.emplace_back(&ctx, (size_t)1 << nx.log);
+      KScope ks(ctx, K_FRI_FOLD, 24.0 * (double)(1u << s->log));
       launch_fold_line(s->log, s->cc(), nx.mc(), a, ctx.itw.p, ctx.tree_len(), ctx.stream);
       a = qm_sqr(a);
       tmp = std::move(nx);
@@ -859,7 +879,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     HIP_CHECK(hipMemcpyAsync(res.p, &none, 8, hipMemcpyHostToDevice, ctx.stream));
     const uint32_t per = 16;
     for (uint32_t hi = 0; found == ~0ull; hi += per) {
-      launch_grind(dpre, cfg.pow_bits, hi, per, (unsigned long long*)res.p, ctx.stream);
+      {
+        KScope ks(ctx, K_GRIND, 0.0);
+        launch_grind(dpre, cfg.pow_bits, hi, per, (unsigned long long*)res.p, ctx.stream);
+      }
       ctx.d2h(&found, res.p, 8);
       if (hi > 0x7fffffffu - per) throw HipError(4, "Grind failed to find a solution.");
     }
@@ -976,6 +999,7 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     std::vector<uint32_t*> ip;
     for (auto& b : icols) ip.push_back(b.p);
     DevBuf sums(&ctx, 8, true);
+    KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
     launch_poseidon_logup_fracs(log_n, (const uint32_t* const*)ctx.upload_vec(trace),
                                 (uint32_t* const*)ctx.upload_vec(ip), (const PoseidonParams*)ctx.upload(&pp, sizeof(pp)),
                                 (unsigned long long*)sums.p, ctx.stream);
@@ -986,7 +1010,10 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     std::vector<uint32_t*> last4(ip.end() - 4, ip.end());
     uint32_t nb = (1u << log_n) >= 1024 ? (1u << log_n) / 1024 : 1;
     DevBuf scratch(&ctx, 4 * nb);
-    launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.upload_vec(last4), shift, scratch.p, ctx.stream);
+    {
+      KScope ks2(ctx, K_LOGUP, 32.0 * (double)(1u << log_n));
+      launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.upload_vec(last4), shift, scratch.p, ctx.stream);
+    }
     std::vector<const uint32_t*> ev;
     for (auto& b : icols) ev.push_back(b.p);
     trees.emplace_back();

@@ -40,6 +40,8 @@ Ctx::Ctx(int dev) : device(dev) {
 }
 Ctx::~Ctx() {
   (void)hipStreamSynchronize(stream);
+  for (auto& p : pending) { event_pool.push_back(p.a); event_pool.push_back(p.b); }
+  for (auto e : event_pool) (void)hipEventDestroy(e);
   tw.reset();
   itw.reset();
   domain_points.clear();
@@ -70,7 +72,29 @@ void* Ctx::upload(const void* src, size_t bytes) {
   arena_off += a;
   return d;
 }
-void Ctx::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+void Ctx::sync() {
+  HIP_CHECK(hipStreamSynchronize(stream));
+  if (!pending.empty()) collect_timing();
+}
+hipEvent_t Ctx::get_event() {
+  if (!event_pool.empty()) { hipEvent_t e = event_pool.back(); event_pool.pop_back(); return e; }
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+void Ctx::collect_timing() {
+  for (auto& p : pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      kstats[p.cls].ms += ms;
+      kstats[p.cls].bytes += p.bytes;
+      kstats[p.cls].launches += 1;
+    }
+    event_pool.push_back(p.a);
+    event_pool.push_back(p.b);
+  }
+  pending.clear();
+}
 void Ctx::reset_arena() {
   for (auto* p : overflow) (void)hipFree(p);
   overflow.clear();
@@ -79,7 +103,7 @@ void Ctx::reset_arena() {
 }
 void Ctx::d2h(void* dst, const void* src, size_t bytes) {
   HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
-  HIP_CHECK(hipStreamSynchronize(stream));
+  sync();
 }
 
 // Twiddle tree of the half coset of CanonicCoset(log_domain) (prover/backend/cpu/circle.rs:274-291):

@@ -48,8 +48,26 @@ struct StageTimes {
   double ms[16] = {0};
 };
 
+// Kernel classes timed with HIP events when timing is enabled (stwo_hip_set_kernel_timing).
+enum KernelClass {
+  K_INTERPOLATE = 0, K_EVALUATE, K_MERKLE, K_COMPOSITION, K_NUMERATORS, K_QUOTIENTS, K_FRI_FOLD, K_OODS,
+  K_LOGUP, K_GRIND, K_MISC, K_NUM_CLASSES
+};
+struct KernelStat {
+  double ms = 0;
+  double bytes = 0;  // algorithmic HBM bytes
+  uint64_t launches = 0;
+};
+
 struct Ctx {
   int device = 0;
+  bool timing = false;
+  struct Pending { int cls; double bytes; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  KernelStat kstats[K_NUM_CLASSES];
+  hipEvent_t get_event();
+  void collect_timing();  // after a sync: folds finished events into kstats
   hipStream_t stream = nullptr;
   std::string last_error;
   // twiddle tree for the half coset of CanonicCoset(tw_log): 2^(tw_log-1) words each
@@ -74,6 +92,24 @@ struct Ctx {
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
   void d2h(void* dst, const void* src, size_t bytes);  // synchronous
+};
+
+// Times the kernels launched during its lifetime on the context stream (class, algorithmic bytes).
+struct KScope {
+  Ctx& c;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  KScope(Ctx& ctx, int k, double b) : c(ctx), cls(k), bytes(b) {
+    if (c.timing) { a = c.get_event(); (void)hipEventRecord(a, c.stream); }
+  }
+  ~KScope() {
+    if (c.timing) {
+      hipEvent_t b = c.get_event();
+      (void)hipEventRecord(b, c.stream);
+      c.pending.push_back({cls, bytes, a, b});
+    }
+  }
 };
 
 // ---- host Blake2s and channel ------------------------------------------------------------

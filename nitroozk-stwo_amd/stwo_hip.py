@@ -75,7 +75,11 @@ EXPORTS = [
     "stwo_hip_evaluate_constraint_quotients_on_domain", "stwo_hip_prove_wide_fibonacci", "stwo_hip_prove_poseidon",
     "stwo_proof_status", "stwo_proof_size", "stwo_proof_bytes", "stwo_proof_digest", "stwo_proof_claimed_sum",
     "stwo_proof_stage_ms", "stwo_proof_free", "stwo_hip_gen_trace_wide_fibonacci", "stwo_hip_gen_trace_poseidon",
+    "stwo_hip_set_kernel_timing", "stwo_hip_kernel_stats", "stwo_hip_reset_kernel_stats",
 ]
+
+KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
+                  "fri_fold", "oods", "logup", "grind", "misc"]
 
 _lib = None
 
@@ -139,6 +143,10 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_gen_trace_wide_fibonacci.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.c_uint32,
                                                     c.c_uint64]
     L.stwo_hip_gen_trace_poseidon.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint64]
+    L.stwo_hip_set_kernel_timing.argtypes = [c.c_void_p, c.c_int]
+    L.stwo_hip_kernel_stats.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double), c.POINTER(c.c_double),
+                                        c.POINTER(c.c_uint64)]
+    L.stwo_hip_reset_kernel_stats.argtypes = [c.c_void_p]
     _lib = L
     return L
 
@@ -351,6 +359,24 @@ class HipBackend:
             self.ctx, _ptrs(acc4), _ptrs(trace), len(trace), _ptrs(interaction), len(interaction), pows,
             len(coeff_powers), dinv.ctypes.data_as(_U32P), trace_log, eval_log, eval_blob, len(eval_blob),
             QM31.of(cumsum_shift)))
+
+    # ---- measurement ----
+    def set_kernel_timing(self, enable):
+        self._check(self.lib.stwo_hip_set_kernel_timing(self.ctx, int(enable)))
+
+    def reset_kernel_stats(self):
+        self._check(self.lib.stwo_hip_reset_kernel_stats(self.ctx))
+
+    def kernel_stats(self):
+        """{class: (total_ms, algorithmic_bytes, launches)} for the classes that ran."""
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+            self._check(self.lib.stwo_hip_kernel_stats(self.ctx, i, ctypes.byref(ms), ctypes.byref(by),
+                                                       ctypes.byref(n)))
+            if n.value:
+                out[name] = (ms.value, by.value, n.value)
+        return out
 
     # ---- trace generation and whole proofs ----
     def gen_trace_wide_fibonacci(self, n_cols, log_n_rows, mode=1, seed=0):
