@@ -562,10 +562,11 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
     const uint32_t* d_dinv = ctx.upload_vec(dinv_br);
     const qm31* d_pows = ctx.upload_vec(cp);
-    KScope ks(ctx, K_COMPOSITION, (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E));
+    const uint32_t* const* d_t = (const uint32_t* const*)ctx.upload_vec(tcols);
+    double cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
     if (c.kind == AIR_WIDE_FIB) {
-      launch_wide_fib_composition(E, c.log_n, (const uint32_t* const*)ctx.upload_vec(tcols), c.n_cols, d_pows, d_dinv,
-                                  a4, ctx.stream);
+      KScope ks(ctx, K_COMPOSITION, cbytes);
+      launch_wide_fib_composition(E, c.log_n, d_t, c.n_cols, d_pows, d_dinv, a4, ctx.stream);
     } else {
       auto icols = cols_on_domain(trees[2], c.inter_start, c.n_inter_cols(), ext_i);
       PoseidonParams pp{};
@@ -573,9 +574,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (int i = 0; i < 16; ++i) pp.alpha_pow[i] = c.alpha_pow[i];
       pp.shift = c.cumsum_shift();
       pp.ipr = c.ipr;
-      launch_poseidon_composition(E, c.log_n, (const uint32_t* const*)ctx.upload_vec(tcols),
-                                  (const uint32_t* const*)ctx.upload_vec(icols),
-                                  (const PoseidonParams*)ctx.upload(&pp, sizeof(pp)), d_pows, d_dinv, a4, ctx.stream);
+      const uint32_t* const* d_i = (const uint32_t* const*)ctx.upload_vec(icols);
+      const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
+      KScope ks(ctx, K_COMPOSITION, cbytes);
+      launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, d_pows, d_dinv, a4, ctx.stream);
     }
     ctx.sync();
     ctx.reset_arena();
@@ -787,9 +789,12 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   const uint32_t* dxy = ctx.domain_xy(Lc);
   double qbytes = 24.0 * (double)(1u << Lc);
   for (auto& pa : qparts) qbytes += 16.0 * (double)(1u << pa.log_size);
-  KScope ksq(ctx, K_QUOTIENTS, qbytes);
-  launch_quotients(Lc, dxy, ctx.upload_vec(qpts), (uint32_t)qpts.size(), ctx.upload_vec(qparts),
-                   first.mc(), ctx.stream);
+  const QuotPoint* dq = ctx.upload_vec(qpts);
+  const QuotPart* dp = ctx.upload_vec(qparts);
+  {
+    KScope ksq(ctx, K_QUOTIENTS, qbytes);
+    launch_quotients(Lc, dxy, dq, (uint32_t)qpts.size(), dp, first.mc(), ctx.stream);
+  }
   ctx.sync();
   ctx.reset_arena();
   partial_bufs.clear();
@@ -948,6 +953,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
 
 ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t n_cols,
                            const PcsCfg& cfg) {
+  ProofArenaScope arena(ctx);
   ProveResult r;
   Timer tm;
   Channel ch;
@@ -970,6 +976,7 @@ ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
 
 ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
                            const PcsCfg& cfg) {
+  ProofArenaScope arena(ctx);
   ProveResult r;
   Timer tm;
   Channel ch;
@@ -999,10 +1006,13 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     std::vector<uint32_t*> ip;
     for (auto& b : icols) ip.push_back(b.p);
     DevBuf sums(&ctx, 8, true);
-    KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
-    launch_poseidon_logup_fracs(log_n, (const uint32_t* const*)ctx.upload_vec(trace),
-                                (uint32_t* const*)ctx.upload_vec(ip), (const PoseidonParams*)ctx.upload(&pp, sizeof(pp)),
-                                (unsigned long long*)sums.p, ctx.stream);
+    const uint32_t* const* d_tr = (const uint32_t* const*)ctx.upload_vec(trace);
+    uint32_t* const* d_ip = (uint32_t* const*)ctx.upload_vec(ip);
+    const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
+    {
+      KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
+      launch_poseidon_logup_fracs(log_n, d_tr, d_ip, d_pp, (unsigned long long*)sums.p, ctx.stream);
+    }
     unsigned long long s[4];
     ctx.d2h(s, sums.p, 32);
     c.claimed_sum = {m31_reduce64(s[0] % P), m31_reduce64(s[1] % P), m31_reduce64(s[2] % P), m31_reduce64(s[3] % P)};

@@ -7,20 +7,45 @@ namespace sh {
 
 DevBuf::DevBuf(Ctx* c, size_t words, bool zero) : n(words), ctx(c) {
   if (words) {
-    HIP_CHECK(hipMallocAsync((void**)&p, words * 4, c->stream));
+    if (c->bump_depth > 0) {
+      p = (uint32_t*)c->bump_alloc(words * 4);
+      owned = false;
+    } else {
+      HIP_CHECK(hipMallocAsync((void**)&p, words * 4, c->stream));
+    }
     if (zero) launch_fill_zero(p, words, c->stream);
   }
+}
+void* Ctx::bump_alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  while (bump_chunk < bump_chunks.size()) {
+    auto& ch = bump_chunks[bump_chunk];
+    if (bump_off + bytes <= ch.second) {
+      void* r = ch.first + bump_off;
+      bump_off += bytes;
+      return r;
+    }
+    ++bump_chunk;
+    bump_off = 0;
+  }
+  size_t cap = bytes > ((size_t)1 << 30) ? bytes : ((size_t)1 << 30);
+  uint8_t* d;
+  HIP_CHECK(hipMalloc((void**)&d, cap));
+  bump_chunks.push_back({d, cap});
+  bump_chunk = bump_chunks.size() - 1;
+  bump_off = bytes;
+  return d;
 }
 DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
   if (this != &o) {
     reset();
-    p = o.p; n = o.n; ctx = o.ctx;
+    p = o.p; n = o.n; ctx = o.ctx; owned = o.owned;
     o.p = nullptr; o.n = 0;
   }
   return *this;
 }
 void DevBuf::reset() {
-  if (p) (void)hipFreeAsync(p, ctx->stream);
+  if (p && owned) (void)hipFreeAsync(p, ctx->stream);
   p = nullptr;
   n = 0;
 }
@@ -47,6 +72,7 @@ Ctx::~Ctx() {
   domain_points.clear();
   for (auto* p : overflow) (void)hipFree(p);
   (void)hipStreamSynchronize(stream);
+  for (auto& ch : bump_chunks) (void)hipFree(ch.first);
   if (dev_arena) (void)hipFree(dev_arena);
   if (pinned) (void)hipHostFree(pinned);
   (void)hipStreamDestroy(stream);
@@ -112,6 +138,7 @@ void Ctx::d2h(void* dst, const void* src, size_t bytes) {
 // from the end of the buffer (core/poly/utils.rs:98-114).
 void Ctx::ensure_twiddles(uint32_t log_domain) {
   if (log_domain <= tw_log) return;
+  PersistentAllocScope keep(*this);
   uint32_t R = log_domain - 1;  // root coset log size
   size_t len = (size_t)1 << R;
   std::vector<uint32_t> t, it;
@@ -155,6 +182,7 @@ cpm31 canonic_domain_at(uint32_t log, size_t i) {
 const uint32_t* Ctx::domain_xy(uint32_t log) {
   auto it = domain_points.find(log);
   if (it != domain_points.end()) return it->second.p;
+  PersistentAllocScope keep(*this);
   size_t n = (size_t)1 << log, h = n / 2;
   std::vector<uint32_t> xy(2 * n);
   HCoset hc = HCoset::half_odds(log - 1);

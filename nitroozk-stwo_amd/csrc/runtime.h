@@ -32,6 +32,7 @@ struct DevBuf {
   uint32_t* p = nullptr;
   size_t n = 0;
   Ctx* ctx = nullptr;
+  bool owned = true;  // false for proof-arena memory, released all at once at the end of a proof
   DevBuf() = default;
   DevBuf(Ctx* c, size_t words, bool zero = false);
   DevBuf(const DevBuf&) = delete;
@@ -80,6 +81,12 @@ struct Ctx {
   uint8_t* dev_arena = nullptr;
   size_t arena_cap = 0, arena_off = 0;
   std::vector<uint8_t*> overflow;  // arena overflow allocations, freed at reset
+  // proof arena: HBM chunks kept across proofs; a proof bump-allocates its buffers from them
+  std::vector<std::pair<uint8_t*, size_t>> bump_chunks;
+  size_t bump_chunk = 0, bump_off = 0;
+  int bump_depth = 0;
+  void* bump_alloc(size_t bytes);
+  void bump_reset() { bump_chunk = 0; bump_off = 0; }
 
   explicit Ctx(int dev);
   ~Ctx();
@@ -92,6 +99,26 @@ struct Ctx {
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
   void d2h(void* dst, const void* src, size_t bytes);  // synchronous
+};
+
+// While alive, DevBufs of the context come from the proof arena; the arena is rewound (after a
+// stream sync) when the outermost scope ends.
+struct ProofArenaScope {
+  Ctx& c;
+  explicit ProofArenaScope(Ctx& ctx) : c(ctx) { c.bump_depth++; }
+  ~ProofArenaScope() {
+    if (--c.bump_depth == 0) {
+      (void)hipStreamSynchronize(c.stream);
+      c.bump_reset();
+    }
+  }
+};
+// Suspends the proof arena (for cached buffers that outlive a proof).
+struct PersistentAllocScope {
+  Ctx& c;
+  int saved;
+  explicit PersistentAllocScope(Ctx& ctx) : c(ctx), saved(ctx.bump_depth) { c.bump_depth = 0; }
+  ~PersistentAllocScope() { c.bump_depth = saved; }
 };
 
 // Times the kernels launched during its lifetime on the context stream (class, algorithmic bytes).
