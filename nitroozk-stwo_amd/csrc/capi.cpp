@@ -170,15 +170,12 @@ int stwo_hip_batch_eval_at_point(stwo_hip_ctx* h, const uint32_t* const* coeffs,
     f[0] = p.y;
     qm31 x = p.x;
     for (uint32_t b = 1; b < log_size; ++b) { f[b] = x; x = qm_double_x(x); }
-    std::vector<qm31> factors;
-    for (uint32_t i = 0; i < n_cols; ++i) {
-      tasks.push_back({coeffs[i], log_size, i, {0, 0}});
-      factors.insert(factors.end(), f.begin(), f.end());
-    }
+    std::vector<uint32_t> tg(n_cols, 0), gl(1, log_size);
+    for (uint32_t i = 0; i < n_cols; ++i) tasks.push_back({coeffs[i], log_size, i, {0, 0}});
     uint32_t max_chunks = log_size > 11 ? 1u << (log_size - 11) : 1u;
-    DevBuf partials(&c, (size_t)n_cols * max_chunks * 4), vals(&c, (size_t)n_cols * 4);
-    launch_oods(c.upload_vec(tasks), n_cols, log_size, c.upload_vec(factors), (qm31*)partials.p, (qm31*)vals.p,
-                c.stream);
+    DevBuf partials(&c, (size_t)n_cols * max_chunks * 4), vals(&c, (size_t)n_cols * 4), w(&c, (size_t)4 << 11);
+    launch_oods(c.upload_vec(tasks), c.upload_vec(tg), n_cols, log_size, c.upload_vec(f), c.upload_vec(gl), 1,
+                (qm31*)w.p, (qm31*)partials.p, (qm31*)vals.p, c.stream);
     c.d2h(out_host, vals.p, (size_t)n_cols * 16);
     c.reset_arena();
   });
@@ -316,23 +313,9 @@ int stwo_hip_generate_secure_powers(stwo_hip_ctx* h, stwo_qm31 felt, uint32_t n,
 
 int stwo_hip_grind(stwo_hip_ctx* h, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce) {
   return guarded(h, [&](Ctx& c) {
-    if (pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
     Channel ch;
     memcpy(ch.digest.data(), digest, 32);
-    Hash pre = ch.pow_prefix(pow_bits);
-    uint32_t pw[8];
-    for (int i = 0; i < 8; ++i) pw[i] = (uint32_t)pre[4 * i] | ((uint32_t)pre[4 * i + 1] << 8) | ((uint32_t)pre[4 * i + 2] << 16) | ((uint32_t)pre[4 * i + 3] << 24);
-    const uint32_t* dpre = (const uint32_t*)c.upload(pw, 32);
-    DevBuf res(&c, 2);
-    unsigned long long none = ~0ull, found = ~0ull;
-    HIP_CHECK(hipMemcpyAsync(res.p, &none, 8, hipMemcpyHostToDevice, c.stream));
-    for (uint32_t hi = 0; found == ~0ull; hi += 16) {
-      launch_grind(dpre, pow_bits, hi, 16, (unsigned long long*)res.p, c.stream);
-      c.d2h(&found, res.p, 8);
-      if (hi > 0x7fffffffu - 16) throw HipError(4, "Grind failed to find a solution.");
-    }
-    *nonce = found;
-    c.reset_arena();
+    *nonce = device_grind(c, ch.pow_prefix(pow_bits), pow_bits);
   });
 }
 

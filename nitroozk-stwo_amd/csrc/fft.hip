@@ -40,6 +40,49 @@ __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* __restrict__ 
   }
 }
 
+// Twiddle of the pair whose lower element has global index idx0 at stage s (see top of file).
+__device__ __forceinline__ uint32_t stage_twiddle(const uint32_t* __restrict__ tree, uint32_t tree_len,
+                                                  uint32_t log_n, uint32_t s, uint32_t idx0) {
+  if (s == 0) return circle_twiddle(tree + tree_len - (1u << (log_n - 1)), idx0 >> 1);
+  return tree[tree_len - (1u << (log_n - s)) + (idx0 >> (s + 1))];
+}
+
+// One round of R consecutive stages (local bits j0 .. j0+R-1 of t) on 2^R elements held in
+// registers: 2^(R-1) * R butterflies, one twiddle per distinct butterfly block.
+template <bool INV, int R>
+__device__ __forceinline__ void radix_round(uint32_t* lds, uint32_t C, uint32_t c, uint32_t e, uint32_t j0,
+                                            uint32_t base, uint32_t tile_n, const uint32_t* __restrict__ tree,
+                                            uint32_t tree_len, uint32_t log_n) {
+  constexpr uint32_t G = 1u << R;
+  const uint32_t n_groups = tile_n >> R;
+  for (uint32_t g = threadIdx.x; g < n_groups; g += blockDim.x) {
+    uint32_t lo = g & (C - 1), r = g >> c;
+    // t with R zero bits inserted at j0
+    uint32_t t0 = ((r >> j0) << (j0 + R)) | (r & ((1u << j0) - 1));
+    uint32_t v[G];
+#pragma unroll
+    for (uint32_t q = 0; q < G; ++q) v[q] = lds[(t0 | (q << j0)) * C + lo];
+    const uint32_t idx_base = base | (t0 << e) | lo;
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+      const int k = INV ? kk : R - 1 - kk;  // bit within the group
+      const uint32_t s = e + j0 + k;
+      // the twiddle depends only on the group bits above k: 2^(R-1-k) distinct values
+#pragma unroll
+      for (uint32_t u = 0; u < (G >> (k + 1)); ++u) {
+        uint32_t tw = stage_twiddle(tree, tree_len, log_n, s, idx_base | ((u << (k + 1)) << (j0 + e)));
+#pragma unroll
+        for (uint32_t l = 0; l < (1u << k); ++l) {
+          uint32_t q = (u << (k + 1)) | l;
+          bfly<INV>(v[q], v[q | (1u << k)], tw);
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < G; ++q) lds[(t0 | (q << j0)) * C + lo] = v[q];
+  }
+}
+
 template <bool INV>
 __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __restrict__ src,
                                                        uint32_t* const* __restrict__ dst, uint32_t log_n,
@@ -56,37 +99,26 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
   const uint32_t hi = tile >> (e - c);
   const uint32_t base = (hi << (e + S)) | (lo_blk << c);
   const uint32_t tile_n = 1u << (S + c);
-  // load
   for (uint32_t l = threadIdx.x; l < tile_n; l += blockDim.x) {
     uint32_t t = l >> c, lo = l & (C - 1);
-    uint32_t idx = base | (t << e) | lo;
-    lds[l] = in[idx & in_mask];
+    lds[l] = in[(base | (t << e) | lo) & in_mask];
   }
   __syncthreads();
-  const uint32_t npairs = tile_n >> 1;
-  for (uint32_t jj = 0; jj < S; ++jj) {
-    const uint32_t j = INV ? jj : S - 1 - jj;  // local stage bit
-    const uint32_t s = e + j;                  // global stage
-    for (uint32_t q = threadIdx.x; q < npairs; q += blockDim.x) {
-      uint32_t lo = q & (C - 1), r = q >> c;
-      uint32_t t0 = ((r >> j) << (j + 1)) | (r & ((1u << j) - 1));
-      uint32_t t1 = t0 | (1u << j);
-      uint32_t idx0 = base | (t0 << e) | lo;
-      uint32_t tw;
-      if (s == 0) tw = circle_twiddle(tree + tree_len - (1u << (log_n - 1)), idx0 >> 1);
-      else tw = tree[tree_len - (1u << (log_n - s)) + (idx0 >> (s + 1))];
-      uint32_t a = lds[t0 * C + lo], b = lds[t1 * C + lo];
-      bfly<INV>(a, b, tw);
-      lds[t0 * C + lo] = a;
-      lds[t1 * C + lo] = b;
-    }
+  // stages in rounds of up to 3: ascending for the inverse transform, descending for the forward
+  uint32_t done = 0;
+  while (done < S) {
+    uint32_t R = S - done < 3 ? S - done : 3;
+    uint32_t j0 = INV ? done : S - done - R;
+    if (R == 3) radix_round<INV, 3>(lds, C, c, e, j0, base, tile_n, tree, tree_len, log_n);
+    else if (R == 2) radix_round<INV, 2>(lds, C, c, e, j0, base, tile_n, tree, tree_len, log_n);
+    else radix_round<INV, 1>(lds, C, c, e, j0, base, tile_n, tree, tree_len, log_n);
+    done += R;
     __syncthreads();
   }
   for (uint32_t l = threadIdx.x; l < tile_n; l += blockDim.x) {
     uint32_t t = l >> c, lo = l & (C - 1);
-    uint32_t idx = base | (t << e) | lo;
     uint32_t v = lds[l];
-    out[idx] = scale ? m31_mul(v, scale) : v;
+    out[base | (t << e) | lo] = scale ? m31_mul(v, scale) : v;
   }
 }
 

@@ -18,7 +18,13 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
 // merkle.hip
 void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
                          uint32_t* out, hipStream_t stream);
-void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint32_t hi_base, uint32_t n_hi,
+// `levels` (<= 8, <= log_child) column-free levels above a layer of 2^log_child nodes; outs[j]
+// receives level log_child - 1 - j.
+void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t levels, uint32_t* const* outs,
+                           hipStream_t stream);
+// Scans nonce candidates [start, start + count) in search order (hi * 2^20 + lo, lo < 2^20) and
+// atomically keeps the smallest solution in *result.
+void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream);
 
 // ops.hip
@@ -26,6 +32,16 @@ void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 
                                   uint32_t tree_len, hipStream_t stream);
 void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
                       hipStream_t stream);
+
+// Variants that read the folding coefficient from device memory (drawn by the device channel);
+// fold_line squares it n_squarings times first (fold steps > 1).
+void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev,
+                                      const uint32_t* itree, uint32_t tree_len, hipStream_t stream);
+void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev, uint32_t n_squarings,
+                          const uint32_t* itree, uint32_t tree_len, hipStream_t stream);
+// Device channel (state = 8 digest words + n_draws): mix a root, draw one QM31. record[0..8) gets
+// the root, record[8..12) the drawn QM31.
+void launch_channel_mix_root_draw(uint32_t* state, const uint32_t* root, uint32_t* record, hipStream_t stream);
 
 // Fused accumulate_numerators for one size group. Entries of batch b are
 // [batch_start[b], batch_start[b+1]); out holds 4 coordinate pointers per batch.
@@ -41,11 +57,13 @@ void launch_quotients(uint32_t lifting_log, const uint32_t* domain_xy, const Quo
 
 void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCols4 src, hipStream_t stream);
 
-// OODS: value of FFT-basis coefficient columns at QM31 points. factors[t*32 + b] holds the
-// folding factor of bit b for task t (y, x, pi(x), ...).
+// OODS: value of FFT-basis coefficient columns at QM31 points. Tasks are grouped by (point, size):
+// group_factors[g*32 + b] is the folding factor of bit b (y, x, pi(x), ...) and weights holds
+// 2^11 QM31 per group (scratch).
 struct OodsTask { const uint32_t* coeffs; uint32_t log_size; uint32_t out_index; uint32_t pad[2]; };
-void launch_oods(const OodsTask* tasks, uint32_t n_tasks, uint32_t max_log, const qm31* factors, qm31* partials,
-                 qm31* out, hipStream_t stream);
+void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, uint32_t max_log,
+                 const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,
+                 qm31* partials, qm31* out, hipStream_t stream);
 
 void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
                          uint32_t* out, hipStream_t stream);

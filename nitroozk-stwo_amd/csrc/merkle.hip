@@ -38,12 +38,23 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(uint32_t n_nodes, con
     for (int k = 0; k < 16; ++k) m[k] = 0;
     blake2s_compress(h, m, 0, true);
   }
-  for (uint32_t cs = 0; cs < n_cols; cs += 16) {
-    uint32_t nw = n_cols - cs < 16 ? n_cols - cs : 16;
+  if (n_cols) {
+    // software pipeline: the next block's 16 column words are loaded while this block compresses
+    uint32_t nw = n_cols < 16 ? n_cols : 16;
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) m[k] = k < nw ? cols[cs + k][i] : 0u;
-    t += 4 * nw;
-    blake2s_compress(h, m, t, cs + 16 >= n_cols);
+    for (uint32_t k = 0; k < 16; ++k) m[k] = k < nw ? cols[k][i] : 0u;
+    for (uint32_t cs = 0; cs < n_cols; cs += 16) {
+      uint32_t nx[16];
+      uint32_t ncs = cs + 16;
+      uint32_t nnw = ncs < n_cols ? (n_cols - ncs < 16 ? n_cols - ncs : 16) : 0;
+#pragma unroll
+      for (uint32_t k = 0; k < 16; ++k) nx[k] = k < nnw ? cols[ncs + k][i] : 0u;
+      t += 4 * nw;
+      blake2s_compress(h, m, t, ncs >= n_cols);
+#pragma unroll
+      for (uint32_t k = 0; k < 16; ++k) m[k] = nx[k];
+      nw = nnw;
+    }
   }
   uint4* o4 = reinterpret_cast<uint4*>(out + (size_t)i * 8);
   o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
@@ -58,11 +69,76 @@ void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t
                      n_cols, out);
 }
 
+// Up to 8 levels of a Merkle tree with no injected columns in one launch. Workgroup w reads the
+// 2^L children [w 2^L, (w+1) 2^L) of level log_child and writes its 2^(L-j) nodes of level
+// log_child - j (j = 1..L); the levels in between stay in LDS.
+struct LayerPtrs { uint32_t* p[8]; };
+__global__ void __launch_bounds__(128) merkle_subtree_kernel(const uint32_t* __restrict__ child, uint32_t L,
+                                                             LayerPtrs outs) {
+  __shared__ uint32_t lds[128 * 8];
+  const uint32_t w = blockIdx.x, i = threadIdx.x;
+  uint32_t n = 1u << (L - 1);  // nodes of the first level computed by this workgroup
+  if (i < n) {
+    const uint4* p4 = reinterpret_cast<const uint4*>(child + ((size_t)w * 2 * n + 2 * i) * 8);
+    uint32_t m[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint4 v = p4[k];
+      m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
+    }
+    uint32_t h[8];
+    blake2s_init(h);
+    blake2s_compress(h, m, 64, true);
+    uint4* o4 = reinterpret_cast<uint4*>(outs.p[0] + ((size_t)w * n + i) * 8);
+    o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lds[i * 8 + k] = h[k];
+  }
+  __syncthreads();
+  for (uint32_t j = 1; j < L; ++j) {
+    n >>= 1;
+    uint32_t h[8];
+    if (i < n) {
+      uint32_t m[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) m[k] = lds[2 * i * 8 + k];
+      blake2s_init(h);
+      blake2s_compress(h, m, 64, true);
+    }
+    __syncthreads();
+    if (i < n) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) lds[i * 8 + k] = h[k];
+      uint4* o4 = reinterpret_cast<uint4*>(outs.p[j] + ((size_t)w * n + i) * 8);
+      o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
+      o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    }
+    __syncthreads();
+  }
+}
+
+void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t levels, uint32_t* const* outs,
+                           hipStream_t stream) {
+  // levels <= 8 and <= log_child; outs[j] receives level log_child - 1 - j
+  LayerPtrs lp{};
+  for (uint32_t j = 0; j < levels; ++j) lp.p[j] = outs[j];
+  uint32_t wgs = 1u << (log_child - levels);
+  uint32_t threads = 1u << (levels - 1);
+  if (threads < 64) threads = 64;
+  hipLaunchKernelGGL(merkle_subtree_kernel, dim3(wgs), dim3(threads), 0, stream, child, levels, lp);
+}
+
+// Candidates are numbered g = hi * 2^20 + lo (lo < 2^20), i.e. in the order of the smallest-nonce
+// search; one launch covers [start, start + count).
 __global__ void __launch_bounds__(256) grind_kernel(const uint32_t* __restrict__ prefix, uint32_t pow_bits,
-                                                    uint32_t hi_base, unsigned long long* __restrict__ result) {
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t lo = g & ((1u << 20) - 1);
-  uint32_t hi = hi_base + (g >> 20);
+                                                    uint64_t start, uint32_t count,
+                                                    unsigned long long* __restrict__ result) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint64_t g = start + i;
+  uint32_t lo = (uint32_t)(g & ((1u << 20) - 1));
+  uint32_t hi = (uint32_t)(g >> 20);
   uint32_t m[16];
 #pragma unroll
   for (int k = 0; k < 8; ++k) m[k] = prefix[k];
@@ -77,10 +153,65 @@ __global__ void __launch_bounds__(256) grind_kernel(const uint32_t* __restrict__
   if (tz >= pow_bits) atomicMin(result, ((unsigned long long)hi << 32) | lo);
 }
 
-void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint32_t hi_base, uint32_t n_hi,
+void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream) {
-  uint32_t total = n_hi << 20;
-  hipLaunchKernelGGL(grind_kernel, dim3(total / 256), dim3(256), 0, stream, prefix, pow_bits, hi_base, result);
+  hipLaunchKernelGGL(grind_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, prefix, pow_bits, start, count,
+                     result);
+}
+
+// ---- device-side Fiat-Shamir channel (crates/stwo/src/core/channel/blake2s.rs:19-140) --------
+// state = digest (8 words) + n_draws. One lane runs it so that FRI can mix each layer root and
+// draw the next folding coefficient without a host round trip.
+__device__ void dev_channel_draw_u32s(uint32_t* st, uint32_t out[8]) {
+  uint32_t m[16];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) m[k] = st[k];
+  m[8] = st[8];  // n_draws, little endian
+  m[9] = 0;      // the trailing zero byte, then zero padding
+#pragma unroll
+  for (int k = 10; k < 16; ++k) m[k] = 0;
+  st[8] += 1;
+  uint32_t h[8];
+  blake2s_init(h);
+  blake2s_compress(h, m, 37, true);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) out[k] = h[k];
+}
+
+// Mixes a Merkle root (8 words) into the channel and draws one QM31 (draw_secure_felt). The
+// record receives the root (words 0..7) and the drawn value (words 8..11) so that the host can
+// replay the transcript.
+__global__ void channel_mix_root_draw_kernel(uint32_t* __restrict__ st, const uint32_t* __restrict__ root,
+                                             uint32_t* __restrict__ rec) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t m[16];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) { m[k] = st[k]; m[8 + k] = root[k]; rec[k] = root[k]; }
+  uint32_t h[8];
+  blake2s_init(h);
+  blake2s_compress(h, m, 64, true);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) st[k] = h[k];
+  st[8] = 0;
+  // draw_base_felts: redraw until all 8 words are below 2P; the value uses the first 4
+  for (int attempt = 0; attempt < 64; ++attempt) {
+    uint32_t u[8];
+    dev_channel_draw_u32s(st, u);
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ok &= u[k] < 2 * P;
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rec[8 + k] = u[k] >= P ? u[k] - P : u[k];
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) rec[8 + k] = P;  // not canonical: the host replay reports it
+}
+
+void launch_channel_mix_root_draw(uint32_t* state, const uint32_t* root, uint32_t* record, hipStream_t stream) {
+  hipLaunchKernelGGL(channel_mix_root_draw_kernel, dim3(1), dim3(64), 0, stream, state, root, record);
 }
 
 }  // namespace sh

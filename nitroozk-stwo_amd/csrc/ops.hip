@@ -31,9 +31,14 @@ __device__ __forceinline__ uint32_t circle_itw(const uint32_t* __restrict__ lt0,
 }
 
 __global__ void __launch_bounds__(256) fold_circle_kernel(uint32_t n_out, CCols4 src, Cols4 dst, qm31 alpha,
-                                                          qm31 alpha2, const uint32_t* __restrict__ lt0) {
+                                                          qm31 alpha2, const uint32_t* __restrict__ lt0,
+                                                          const qm31* __restrict__ alpha_dev) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
+  if (alpha_dev) {  // folding coefficient drawn on the device
+    alpha = *alpha_dev;
+    alpha2 = qm_mul(alpha, alpha);
+  }
   qm31 fp = load_q(src.c, 2 * i), fn = load_q(src.c, 2 * i + 1);
   uint32_t t = circle_itw(lt0, i);  // 1 / p.y for p = domain.at(bitrev(2i))
   qm31 f0 = qm_add(fp, fn);
@@ -48,13 +53,25 @@ void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 
   uint32_t n = 1u << (log_src - 1);
   const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
   hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha,
-                     qm_mul(alpha, alpha), lt0);
+                     qm_mul(alpha, alpha), lt0, (const qm31*)nullptr);
+}
+void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev,
+                                      const uint32_t* itree, uint32_t tree_len, hipStream_t stream) {
+  uint32_t n = 1u << (log_src - 1);
+  const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
+  hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, qm_zero(),
+                     qm_zero(), lt0, alpha_dev);
 }
 
 __global__ void __launch_bounds__(256) fold_line_kernel(uint32_t n_out, CCols4 src, Cols4 dst, qm31 alpha,
-                                                        const uint32_t* __restrict__ lt) {
+                                                        const uint32_t* __restrict__ lt,
+                                                        const qm31* __restrict__ alpha_dev, uint32_t n_squarings) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
+  if (alpha_dev) {  // folding coefficient drawn on the device, squared once per earlier fold step
+    alpha = *alpha_dev;
+    for (uint32_t k = 0; k < n_squarings; ++k) alpha = qm_mul(alpha, alpha);
+  }
   qm31 fx = load_q(src.c, 2 * i), fnx = load_q(src.c, 2 * i + 1);
   uint32_t xi = lt[i];  // 1 / x for x = domain.at(bitrev(2i))
   qm31 f0 = qm_add(fx, fnx);
@@ -66,30 +83,50 @@ void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const
                       hipStream_t stream) {
   uint32_t n = 1u << (log_src - 1);
   const uint32_t* lt = itree + tree_len - (1u << log_src);
-  hipLaunchKernelGGL(fold_line_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha, lt);
+  hipLaunchKernelGGL(fold_line_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha, lt,
+                     (const qm31*)nullptr, 0u);
+}
+void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev, uint32_t n_squarings,
+                          const uint32_t* itree, uint32_t tree_len, hipStream_t stream) {
+  uint32_t n = 1u << (log_src - 1);
+  const uint32_t* lt = itree + tree_len - (1u << log_src);
+  hipLaunchKernelGGL(fold_line_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, qm_zero(), lt,
+                     alpha_dev, n_squarings);
 }
 
-// partial_b[row] = sum_{j in batch b} c_j * col_j[row] - sum_j b_j
+// partial_b[row] = sum_{j in batch b} c_j * col_j[row] - sum_j b_j.
+// A block covers 64 rows; its 4 waves split each batch's columns (entry e goes to wave e mod 4)
+// so that a 2^16-row group still fills every CU, then the 4 partial sums are added in LDS.
+constexpr uint32_t NUM_SLICES = 4;
 __global__ void __launch_bounds__(256) accumulate_numerators_kernel(uint32_t n_rows,
                                                                     const uint32_t* const* __restrict__ cols,
                                                                     const NumEntry* __restrict__ entries,
                                                                     const uint32_t* __restrict__ batch_start,
                                                                     const qm31* __restrict__ batch_b,
                                                                     uint32_t n_batches, uint32_t* const* __restrict__ out) {
-  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n_rows) return;
+  __shared__ qm31 part[NUM_SLICES][64];
+  const uint32_t lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+  const uint32_t row = blockIdx.x * 64 + lane;
+  const bool active = row < n_rows;
   for (uint32_t b = 0; b < n_batches; ++b) {
     qacc acc = qacc_zero();
     uint32_t e_end = batch_start[b + 1];
-    for (uint32_t e = batch_start[b]; e < e_end; ++e) {
-      NumEntry en = entries[e];
-      qacc_mac(acc, en.c, cols[en.col][row]);
+    if (active)
+      for (uint32_t e = batch_start[b] + slice; e < e_end; e += NUM_SLICES) {
+        NumEntry en = entries[e];
+        qacc_mac(acc, en.c, cols[en.col][row]);
+      }
+    part[slice][lane] = qacc_reduce(acc);
+    __syncthreads();
+    if (slice == 0 && active) {
+      qm31 r = qm_add(qm_add(part[0][lane], part[1][lane]), qm_add(part[2][lane], part[3][lane]));
+      r = qm_sub(r, batch_b[b]);
+      out[4 * b][row] = r.a;
+      out[4 * b + 1][row] = r.b;
+      out[4 * b + 2][row] = r.c;
+      out[4 * b + 3][row] = r.d;
     }
-    qm31 r = qm_sub(qacc_reduce(acc), batch_b[b]);
-    out[4 * b][row] = r.a;
-    out[4 * b + 1][row] = r.b;
-    out[4 * b + 2][row] = r.c;
-    out[4 * b + 3][row] = r.d;
+    __syncthreads();
   }
 }
 
@@ -97,7 +134,7 @@ void launch_accumulate_numerators(uint32_t log_size, const uint32_t* const* cols
                                   const uint32_t* batch_start, const qm31* batch_b, uint32_t n_batches,
                                   uint32_t* const* out, hipStream_t stream) {
   uint32_t n = 1u << log_size;
-  hipLaunchKernelGGL(accumulate_numerators_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, cols, entries,
+  hipLaunchKernelGGL(accumulate_numerators_kernel, dim3((n + 63) / 64), dim3(256), 0, stream, n, cols, entries,
                      batch_start, batch_b, n_batches, out);
 }
 
@@ -154,61 +191,69 @@ void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCo
 }
 
 // ---- out-of-domain sampling on coefficients ------------------------------------------------
-// Chunk of 2^11 coefficients per workgroup: lane t owns 8 consecutive coefficients (index
-// t*8 + j). The basis weight of index lo within the chunk factors as W3[j] * base(t), where W3
-// uses the factors of bits 0..2 and base those of bits 3..10; the chunk weight uses bits >= 11.
+// f(P) = sum_n c_n prod_b f_b^(n_b), f_0 = y, f_1 = x, f_b = pi^(b-1)(x). Tasks that share a
+// point and a size share a weight table W[lo] = prod_{b < 11} f_b^(lo_b) (2^11 entries). A
+// workgroup takes a chunk of 2^11 coefficients; lane t owns the 8 coefficients t*8 + j, whose
+// weights are W[j] * W[8t]; the chunk weight uses the factors of bits >= 11.
 constexpr uint32_t OODS_CHUNK_LOG = 11;
 
+__global__ void __launch_bounds__(256) oods_weights_kernel(const qm31* __restrict__ factors,
+                                                           const uint32_t* __restrict__ group_log,
+                                                           qm31* __restrict__ W) {
+  const uint32_t g = blockIdx.x;
+  const uint32_t k = group_log[g];
+  const uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
+  const qm31* f = factors + (size_t)g * 32;
+  for (uint32_t lo = threadIdx.x; lo < (1u << OODS_CHUNK_LOG); lo += blockDim.x) {
+    qm31 w = qm_one();
+    for (uint32_t b = 0; b < m; ++b)
+      if ((lo >> b) & 1) w = qm_mul(w, f[b]);
+    W[((size_t)g << OODS_CHUNK_LOG) + lo] = w;
+  }
+}
+
 __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restrict__ tasks,
-                                                         const qm31* __restrict__ factors, qm31* __restrict__ partials,
+                                                         const uint32_t* __restrict__ task_group,
+                                                         const qm31* __restrict__ factors,
+                                                         const qm31* __restrict__ Wall, qm31* __restrict__ partials,
                                                          uint32_t max_chunks) {
-  __shared__ qm31 red[256];
+  __shared__ qm31 red[4];
   const OodsTask task = tasks[blockIdx.y];
+  const uint32_t grp = task_group[blockIdx.y];
   const uint32_t k = task.log_size;
   const uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
   const uint32_t n_chunks = 1u << (k - m);
   const uint32_t chunk = blockIdx.x;
-  const qm31* f = factors + (size_t)blockIdx.y * 32;
+  if (chunk >= n_chunks) return;  // uniform per block
+  const qm31* W = Wall + ((size_t)grp << OODS_CHUNK_LOG);
+  const uint32_t t = threadIdx.x;
+  const uint32_t* c = task.coeffs + ((size_t)chunk << m);
+  const uint32_t base_lo = t * 8;
   qm31 out = qm_zero();
-  if (chunk < n_chunks) {
-    const uint32_t t = threadIdx.x;
-    qm31 w3[8];
-    w3[0] = qm_one();
-#pragma unroll
-    for (uint32_t j = 1; j < 8; ++j) {
-      qm31 w = qm_one();
-      for (uint32_t b = 0; b < 3; ++b)
-        if (((j >> b) & 1) && b < m) w = qm_mul(w, f[b]);
-      w3[j] = w;
-    }
+  if (base_lo < (1u << m)) {
     qacc acc = qacc_zero();
-    const uint32_t* c = task.coeffs + ((size_t)chunk << m);
-    uint32_t base_lo = t * 8;
-    bool active = base_lo < (1u << m);
-    if (active) {
-      if (m >= 3) {
-        uint4 v0 = *reinterpret_cast<const uint4*>(c + base_lo);
-        uint4 v1 = *reinterpret_cast<const uint4*>(c + base_lo + 4);
-        uint32_t vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) qacc_mac(acc, w3[j], vv[j]);
-      } else {
-        for (uint32_t j = 0; j < (1u << m); ++j) qacc_mac(acc, w3[j], c[j]);
-      }
-      qm31 s = qacc_reduce(acc);
-      for (uint32_t b = 3; b < m; ++b)
-        if ((t >> (b - 3)) & 1) s = qm_mul(s, f[b]);
-      out = s;
+    if (m >= 3) {
+      uint4 v0 = *reinterpret_cast<const uint4*>(c + base_lo);
+      uint4 v1 = *reinterpret_cast<const uint4*>(c + base_lo + 4);
+      qacc_mac(acc, W[0], v0.x); qacc_mac(acc, W[1], v0.y); qacc_mac(acc, W[2], v0.z); qacc_mac(acc, W[3], v0.w);
+      qacc_mac(acc, W[4], v1.x); qacc_mac(acc, W[5], v1.y); qacc_mac(acc, W[6], v1.z); qacc_mac(acc, W[7], v1.w);
+      out = qm_mul(qacc_reduce(acc), W[base_lo]);
+    } else {
+      for (uint32_t j = 0; j < (1u << m); ++j) qacc_mac(acc, W[j], c[j]);
+      out = qacc_reduce(acc);
     }
   }
-  red[threadIdx.x] = out;
-  __syncthreads();
-  for (uint32_t w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = qm_add(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
+  // wave reduction by shuffles, then across the 4 waves
+  for (uint32_t off = 32; off > 0; off >>= 1) {
+    qm31 o = {(uint32_t)__shfl_down((int)out.a, off), (uint32_t)__shfl_down((int)out.b, off),
+              (uint32_t)__shfl_down((int)out.c, off), (uint32_t)__shfl_down((int)out.d, off)};
+    out = qm_add(out, o);
   }
-  if (threadIdx.x == 0 && chunk < n_chunks) {
-    qm31 s = red[0];
+  if ((t & 63) == 0) red[t >> 6] = out;
+  __syncthreads();
+  if (t == 0) {
+    qm31 s = qm_add(qm_add(red[0], red[1]), qm_add(red[2], red[3]));
+    const qm31* f = factors + (size_t)grp * 32;
     for (uint32_t b = m; b < k; ++b)
       if ((chunk >> (b - m)) & 1) s = qm_mul(s, f[b]);
     partials[(size_t)blockIdx.y * max_chunks + chunk] = s;
@@ -227,12 +272,14 @@ __global__ void oods_reduce_kernel(const OodsTask* __restrict__ tasks, uint32_t 
   out[tasks[tsk].out_index] = s;
 }
 
-void launch_oods(const OodsTask* tasks, uint32_t n_tasks, uint32_t max_log, const qm31* factors, qm31* partials,
-                 qm31* out, hipStream_t stream) {
+void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, uint32_t max_log,
+                 const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,
+                 qm31* partials, qm31* out, hipStream_t stream) {
   if (n_tasks == 0) return;
   uint32_t max_chunks = max_log > OODS_CHUNK_LOG ? 1u << (max_log - OODS_CHUNK_LOG) : 1u;
-  hipLaunchKernelGGL(oods_chunk_kernel, dim3(max_chunks, n_tasks), dim3(256), 0, stream, tasks, factors, partials,
-                     max_chunks);
+  hipLaunchKernelGGL(oods_weights_kernel, dim3(n_groups), dim3(256), 0, stream, group_factors, group_log, weights);
+  hipLaunchKernelGGL(oods_chunk_kernel, dim3(max_chunks, n_tasks), dim3(256), 0, stream, tasks, task_group,
+                     group_factors, weights, partials, max_chunks);
   hipLaunchKernelGGL(oods_reduce_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, stream, tasks, n_tasks, partials,
                      max_chunks, out);
 }

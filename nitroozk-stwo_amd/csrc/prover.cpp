@@ -139,12 +139,15 @@ std::vector<size_t> merkle_order(const std::vector<uint32_t>& logs) {
   return o;
 }
 
-// MerkleProver::decommit (prover/vcs/prover.rs:84-217): returns queried values and decommitment.
-std::vector<uint32_t> merkle_decommit(Ctx& ctx, const DevTree& t, const std::map<uint32_t, std::vector<size_t>>& qpl,
-                                      MerkleDec& dec) {
-  auto order = merkle_order(t.eval_logs);
-  Gather g;
+// MerkleProver::decommit (prover/vcs/prover.rs:84-217), in two phases so that every tree and FRI
+// layer of a proof is served by one gather: plan_merkle adds the needed words to the gather,
+// finish_merkle assembles the witness in the reference order.
+struct MerklePlan {
   std::vector<uint32_t> hw_idx, q_idx, w_idx;
+};
+MerklePlan plan_merkle(const DevTree& t, const std::map<uint32_t, std::vector<size_t>>& qpl, Gather& g) {
+  MerklePlan mp;
+  auto order = merkle_order(t.eval_logs);
   size_t ci = 0;
   std::vector<size_t> last;
   for (int ls = (int)t.layers.size() - 1; ls >= 0; --ls) {
@@ -162,20 +165,22 @@ std::vector<uint32_t> merkle_decommit(Ctx& ctx, const DevTree& t, const std::map
     for (size_t node : merged) {
       if (has_prev) {
         const uint32_t* prev = t.layers[ls + 1].p;
-        if (!prev_set.count(2 * node)) hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node)));
-        if (!prev_set.count(2 * node + 1)) hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node + 1)));
+        if (!prev_set.count(2 * node)) mp.hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node)));
+        if (!prev_set.count(2 * node + 1)) mp.hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node + 1)));
       }
       if (colq.count(node)) qn.push_back(node); else wn.push_back(node);
     }
-    for (size_t node : qn) for (size_t c : lc) q_idx.push_back(g.add(t.evals[c], (uint32_t)node));
-    for (size_t node : wn) for (size_t c : lc) w_idx.push_back(g.add(t.evals[c], (uint32_t)node));
+    for (size_t node : qn) for (size_t c : lc) mp.q_idx.push_back(g.add(t.evals[c], (uint32_t)node));
+    for (size_t node : wn) for (size_t c : lc) mp.w_idx.push_back(g.add(t.evals[c], (uint32_t)node));
     last.assign(merged.begin(), merged.end());
   }
-  auto vals = g.run(ctx);
-  for (auto i : hw_idx) dec.hash_witness.push_back(words_to_hash(vals, i));
-  for (auto i : w_idx) dec.column_witness.push_back(vals[i]);
+  return mp;
+}
+std::vector<uint32_t> finish_merkle(const MerklePlan& mp, const std::vector<uint32_t>& vals, MerkleDec& dec) {
+  for (auto i : mp.hw_idx) dec.hash_witness.push_back(words_to_hash(vals, i));
+  for (auto i : mp.w_idx) dec.column_witness.push_back(vals[i]);
   std::vector<uint32_t> queried;
-  for (auto i : q_idx) queried.push_back(vals[i]);
+  for (auto i : mp.q_idx) queried.push_back(vals[i]);
   return queried;
 }
 
@@ -332,19 +337,63 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
 
 }  // namespace
 
-void merkle_commit(Ctx& ctx, DevTree& t) {
+uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits) {
+  if (pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
+  uint32_t pw[8];
+  for (int i = 0; i < 8; ++i)
+    pw[i] = (uint32_t)prefix[4 * i] | ((uint32_t)prefix[4 * i + 1] << 8) | ((uint32_t)prefix[4 * i + 2] << 16) |
+            ((uint32_t)prefix[4 * i + 3] << 24);
+  const uint32_t* dpre = (const uint32_t*)ctx.upload(pw, 32);
+  DevBuf res(&ctx, 2);
+  unsigned long long found = ~0ull;
+  HIP_CHECK(hipMemcpyAsync(res.p, &found, 8, hipMemcpyHostToDevice, ctx.stream));
+  // the expected nonce is about 2^pow_bits: start there and double the range per launch
+  uint64_t count = (uint64_t)1 << (pow_bits + 3 < 12 ? 12 : (pow_bits + 3 > 24 ? 24 : pow_bits + 3));
+  const uint64_t limit = (uint64_t)0x7fffffff << 20;  // hi < P (simd/grind.rs:47-56)
+  for (uint64_t start = 0; found == ~0ull; start += count, count = count < (1u << 24) ? count * 2 : count) {
+    if (start >= limit) throw HipError(4, "Grind failed to find a solution.");
+    {
+      KScope ks(ctx, K_GRIND, 0.0);
+      launch_grind(dpre, pow_bits, start, (uint32_t)count, (unsigned long long*)res.p, ctx.stream);
+    }
+    ctx.d2h(&found, res.p, 8);
+  }
+  ctx.reset_arena();
+  return found;
+}
+
+void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
   t.layers.clear();
   if (t.evals.empty()) {
     t.layers.emplace_back(&ctx, 8);
     launch_merkle_layer(0, nullptr, nullptr, 0, t.layers[0].p, ctx.stream);
-    t.root = read_hash(ctx, t.layers[0].p);
+    if (read_root) t.root = read_hash(ctx, t.layers[0].p);
     return;
   }
   auto order = merkle_order(t.eval_logs);
   uint32_t max_log = t.eval_logs[order[0]];
-  std::vector<DevBuf> rev;
+  std::vector<DevBuf> rev;  // layers from max_log down to 0
+  std::vector<uint32_t> has_cols(max_log + 1, 0);
+  for (auto l : t.eval_logs) has_cols[l] = 1;
   size_t ci = 0;
-  for (int ls = (int)max_log; ls >= 0; --ls) {
+  int ls = (int)max_log;
+  while (ls >= 0) {
+    if (!rev.empty() && !has_cols[ls]) {
+      // run of levels without injected columns: fused, up to 8 levels per launch
+      uint32_t levels = 0;
+      while ((int)levels < 8 && ls - (int)levels >= 0 && !has_cols[ls - levels]) ++levels;
+      uint32_t log_child = (uint32_t)ls + 1;
+      std::vector<uint32_t*> outs;
+      for (uint32_t j = 0; j < levels; ++j) {
+        rev.emplace_back(&ctx, (size_t)8 << (ls - j));
+        outs.push_back(rev.back().p);
+      }
+      const uint32_t* child = rev[rev.size() - levels - 1].p;
+      KScope ks(ctx, K_MERKLE, 96.0 * (double)(1u << (log_child - 1)) * 2.0);
+      launch_merkle_subtree(log_child, child, levels, outs.data(), ctx.stream);
+      ls -= (int)levels;
+      continue;
+    }
     std::vector<const uint32_t*> lc;
     while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(t.evals[order[ci++]]);
     DevBuf layer(&ctx, (size_t)8 << ls);
@@ -353,9 +402,10 @@ void merkle_commit(Ctx& ctx, DevTree& t) {
     KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
     launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
     rev.push_back(std::move(layer));
+    --ls;
   }
   for (size_t i = rev.size(); i-- > 0;) t.layers.push_back(std::move(rev[i]));
-  t.root = read_hash(ctx, t.layers[0].p);
+  if (read_root) t.root = read_hash(ctx, t.layers[0].p);
 }
 
 void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, const std::vector<uint32_t>& logs,
@@ -460,16 +510,15 @@ struct FriCol {
   CCols4 cc() const { return {{c[0].p, c[1].p, c[2].p, c[3].p}}; }
   Cols4 mc() { return {{c[0].p, c[1].p, c[2].p, c[3].p}}; }
 };
-void fri_commit_col(Ctx& ctx, FriCol& f) {
-  f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
-  f.tree.eval_logs.assign(4, f.log);
-  merkle_commit(ctx, f.tree);
-  ctx.reset_arena();
-}
-FriLayer fri_decommit_layer(Ctx& ctx, const FriCol& f, const std::vector<size_t>& q, uint32_t fold_step) {
-  FriLayer L;
-  std::vector<size_t> pos;
+struct FriPlan {
+  std::vector<uint32_t> at;
   std::vector<bool> is_witness;
+  MerklePlan mp;
+  Hash commitment;
+};
+FriPlan plan_fri_layer(const FriCol& f, const std::vector<size_t>& q, uint32_t fold_step, Gather& g) {
+  FriPlan fp;
+  std::vector<size_t> pos;
   size_t i = 0;
   while (i < q.size()) {
     size_t j = i;
@@ -478,20 +527,23 @@ FriLayer fri_decommit_layer(Ctx& ctx, const FriCol& f, const std::vector<size_t>
     size_t k = i;
     for (size_t p = start; p < start + ((size_t)1 << fold_step); ++p) {
       pos.push_back(p);
-      if (k < j && q[k] == p) { ++k; is_witness.push_back(false); }
-      else is_witness.push_back(true);
+      if (k < j && q[k] == p) { ++k; fp.is_witness.push_back(false); }
+      else fp.is_witness.push_back(true);
     }
     i = j;
   }
-  Gather g;
-  std::vector<uint32_t> at;
-  for (size_t p : pos) for (int k = 0; k < 4; ++k) at.push_back(g.add(f.c[k].p, (uint32_t)p));
-  auto v = g.run(ctx);
-  for (size_t n = 0; n < pos.size(); ++n)
-    if (is_witness[n]) L.witness.push_back({v[at[4 * n]], v[at[4 * n + 1]], v[at[4 * n + 2]], v[at[4 * n + 3]]});
+  for (size_t p : pos) for (int k = 0; k < 4; ++k) fp.at.push_back(g.add(f.c[k].p, (uint32_t)p));
   std::map<uint32_t, std::vector<size_t>> qpl{{f.log, pos}};
-  merkle_decommit(ctx, f.tree, qpl, L.dec);
-  L.commitment = f.tree.root;
+  fp.mp = plan_merkle(f.tree, qpl, g);
+  fp.commitment = f.tree.root;
+  return fp;
+}
+FriLayer finish_fri_layer(const FriPlan& fp, const std::vector<uint32_t>& v) {
+  FriLayer L;
+  for (size_t n = 0; n < fp.is_witness.size(); ++n)
+    if (fp.is_witness[n]) L.witness.push_back({v[fp.at[4 * n]], v[fp.at[4 * n + 1]], v[fp.at[4 * n + 2]], v[fp.at[4 * n + 3]]});
+  finish_merkle(fp.mp, v, L.dec);
+  L.commitment = fp.commitment;
   return L;
 }
 
@@ -630,7 +682,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   for (auto& v : spts.back()) v = {oods};
   uint32_t Lc = trees.back().max_log();  // prove_values lifting size: the last tree
   std::vector<OodsTask> tasks;
-  std::vector<qm31> factors;
+  std::vector<uint32_t> task_group, group_log;
+  std::vector<qm31> group_factors;
+  std::vector<std::pair<uint32_t, cpqm31>> group_key;
   uint32_t max_coeff_log = 0;
   size_t n_vals = 0;
   for (size_t t = 0; t < trees.size(); ++t)
@@ -638,13 +692,19 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (auto& p : spts[t][c]) {
         cpqm31 q = cpq_repeated_double(p, Lc - trees[t].eval_logs[c]);
         uint32_t k = trees[t].coeff_logs[c];
-        OodsTask tk{trees[t].coeffs[c], k, (uint32_t)n_vals, {0, 0}};
-        tasks.push_back(tk);
-        std::vector<qm31> f(32, qm_zero());
-        f[0] = q.y;
-        qm31 x = q.x;
-        for (uint32_t b = 1; b < k; ++b) { f[b] = x; x = qm_double_x(x); }
-        factors.insert(factors.end(), f.begin(), f.end());
+        uint32_t g = 0;
+        while (g < group_key.size() && !(group_key[g].first == k && point_eq(group_key[g].second, q))) ++g;
+        if (g == group_key.size()) {
+          group_key.push_back({k, q});
+          group_log.push_back(k);
+          std::vector<qm31> f(32, qm_zero());
+          f[0] = q.y;
+          qm31 x = q.x;
+          for (uint32_t b = 1; b < k; ++b) { f[b] = x; x = qm_double_x(x); }
+          group_factors.insert(group_factors.end(), f.begin(), f.end());
+        }
+        tasks.push_back(OodsTask{trees[t].coeffs[c], k, (uint32_t)n_vals, {0, 0}});
+        task_group.push_back(g);
         max_coeff_log = std::max(max_coeff_log, k);
         ++n_vals;
       }
@@ -652,13 +712,17 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   if (n_vals) {
     uint32_t max_chunks = max_coeff_log > 11 ? 1u << (max_coeff_log - 11) : 1u;
     DevBuf partials(&ctx, (size_t)n_vals * max_chunks * 4), dvals(&ctx, n_vals * 4);
+    DevBuf weights(&ctx, group_key.size() * ((size_t)4 << 11));
     double obytes = 0;
     for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
     const OodsTask* dt = ctx.upload_vec(tasks);
-    const qm31* df = ctx.upload_vec(factors);
+    const uint32_t* dtg = ctx.upload_vec(task_group);
+    const qm31* df = ctx.upload_vec(group_factors);
+    const uint32_t* dgl = ctx.upload_vec(group_log);
     {
       KScope ks(ctx, K_OODS, obytes);
-      launch_oods(dt, (uint32_t)n_vals, max_coeff_log, df, (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
+      launch_oods(dt, dtg, (uint32_t)n_vals, max_coeff_log, df, dgl, (uint32_t)group_key.size(), (qm31*)weights.p,
+                  (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
     }
     ctx.d2h(values.data(), dvals.p, n_vals * 16);
     ctx.reset_arena();
@@ -801,21 +865,48 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   stage_ms[4] = tm.lap();
 
   // ---------------- FRI commit ----------------
+  // The channel runs on the device during the commit phase (one record of root + folding
+  // coefficient per layer), so that no layer waits for the host; the host replays the
+  // transcript from the recorded roots afterwards and checks every coefficient.
   ctx.ensure_twiddles(Lc);
-  fri_commit_col(ctx, first);
-  ch.mix_root(first.tree.root);
+  uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
+  if (Lc - 1 < last_log) throw HipError(3, "FRI: last layer larger than the first folded layer");
+  uint32_t n_layers = 1;
+  for (uint32_t l = Lc - 1; l > last_log;) {
+    uint32_t fs = l > last_log + cfg.fold_step ? cfg.fold_step : l - last_log;
+    ++n_layers;
+    l -= fs;
+  }
+  const size_t last_n = (size_t)1 << last_log;
+  DevBuf chan(&ctx, 16 + 12 * (size_t)n_layers + 4 * last_n);
+  uint32_t* dstate = chan.p;
+  uint32_t* drec = chan.p + 16;
+  {
+    uint32_t st[16] = {0};
+    for (int i = 0; i < 8; ++i)
+      st[i] = (uint32_t)ch.digest[4 * i] | (uint32_t)ch.digest[4 * i + 1] << 8 |
+              (uint32_t)ch.digest[4 * i + 2] << 16 | (uint32_t)ch.digest[4 * i + 3] << 24;
+    st[8] = ch.n_draws;
+    launch_copy_words(dstate, (const uint32_t*)ctx.upload(st, sizeof(st)), 16, ctx.stream);
+  }
+  auto mix_and_draw = [&](FriCol& f, uint32_t layer) {
+    f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
+    f.tree.eval_logs.assign(4, f.log);
+    merkle_commit(ctx, f.tree, false);
+    KScope ks(ctx, K_MISC, 0.0);
+    launch_channel_mix_root_draw(dstate, f.tree.layers[0].p, drec + 12 * layer, ctx.stream);
+    return (const qm31*)(drec + 12 * layer + 8);
+  };
+  const qm31* alpha0 = mix_and_draw(first, 0);
   std::vector<FriCol> inner;
   FriCol cur;
   cur.log = Lc - 1;
   for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log, true);
-  qm31 alpha = ch.draw_secure_felt();
   {
     KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 32.0 * (double)(1u << (Lc - 1)));
-    launch_fold_circle_into_line(Lc, first.cc(), cur.mc(), alpha, ctx.itw.p, ctx.tree_len(), ctx.stream);
+    launch_fold_circle_into_line_dev(Lc, first.cc(), cur.mc(), alpha0, ctx.itw.p, ctx.tree_len(), ctx.stream);
   }
-  uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
-  auto fold_layer = [&](FriCol& src, uint32_t fs, qm31 a) {
-    FriCol res;
+  auto fold_layer = [&](FriCol& src, uint32_t fs, const qm31* a) {
     FriCol* s = &src;
     FriCol tmp;
     for (uint32_t st = 0; st < fs; ++st) {
@@ -823,36 +914,41 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       nx.log = s->log - 1;
       for (int k = 0; k < 4; ++k) nx.c.emplace_back(&ctx, (size_t)1 << nx.log);
       KScope ks(ctx, K_FRI_FOLD, 24.0 * (double)(1u << s->log));
-      launch_fold_line(s->log, s->cc(), nx.mc(), a, ctx.itw.p, ctx.tree_len(), ctx.stream);
-      a = qm_sqr(a);
+      launch_fold_line_dev(s->log, s->cc(), nx.mc(), a, st, ctx.itw.p, ctx.tree_len(), ctx.stream);
       tmp = std::move(nx);
       s = &tmp;
     }
     return tmp;
   };
-  if (cur.log != last_log) {
-    while (cur.log > last_log + cfg.fold_step) {
-      cur.fold_step = cfg.fold_step;
-      fri_commit_col(ctx, cur);
-      ch.mix_root(cur.tree.root);
-      qm31 a = ch.draw_secure_felt();
-      FriCol nx = fold_layer(cur, cfg.fold_step, a);
-      inner.push_back(std::move(cur));
-      cur = std::move(nx);
-    }
-    uint32_t ls = cur.log - last_log;
-    cur.fold_step = ls;
-    fri_commit_col(ctx, cur);
-    ch.mix_root(cur.tree.root);
-    qm31 a = ch.draw_secure_felt();
-    FriCol nx = fold_layer(cur, ls, a);
+  uint32_t layer = 1;
+  while (cur.log != last_log) {
+    uint32_t fs = cur.log > last_log + cfg.fold_step ? cfg.fold_step : cur.log - last_log;
+    cur.fold_step = fs;
+    const qm31* a = mix_and_draw(cur, layer++);
+    FriCol nx = fold_layer(cur, fs, a);
     inner.push_back(std::move(cur));
     cur = std::move(nx);
   }
+  if (layer != n_layers) throw HipError(4, "FRI layer count mismatch");
   {
-    size_t n = (size_t)1 << cur.log;
-    std::vector<uint32_t> w(4 * n);
-    for (int k = 0; k < 4; ++k) ctx.d2h(w.data() + k * n, cur.c[k].p, n * 4);
+    // one read-back: the transcript records and the last layer's evaluations
+    uint32_t* dlast = drec + 12 * n_layers;
+    for (int k = 0; k < 4; ++k) launch_copy_words(dlast + k * last_n, cur.c[k].p, last_n, ctx.stream);
+    std::vector<uint32_t> hb(12 * (size_t)n_layers + 4 * last_n);
+    ctx.d2h(hb.data(), drec, hb.size() * 4);
+    ctx.reset_arena();
+    auto replay = [&](DevTree& t, uint32_t l) {
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 4; ++j) t.root[4 * i + j] = (uint8_t)(hb[12 * l + i] >> (8 * j));
+      ch.mix_root(t.root);
+      qm31 a = ch.draw_secure_felt();
+      const uint32_t* r = hb.data() + 12 * l + 8;
+      if (!qm_eq(a, qm31{r[0], r[1], r[2], r[3]})) throw HipError(4, "device channel diverged from the transcript");
+    };
+    replay(first.tree, 0);
+    for (uint32_t l = 1; l < n_layers; ++l) replay(inner[l - 1].tree, l);
+    const size_t n = last_n;
+    const uint32_t* w = hb.data() + 12 * (size_t)n_layers;
     std::vector<qm31> v(n);
     for (size_t i = 0; i < n; ++i) v[i] = {w[i], w[n + i], w[2 * n + i], w[3 * n + i]};
     HCoset dom = HCoset::half_odds(Lc - 1);
@@ -873,28 +969,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   stage_ms[5] = tm.lap();
 
   // ---------------- proof of work ----------------
-  {
-    if (cfg.pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
-    Hash pre = ch.pow_prefix(cfg.pow_bits);
-    uint32_t pw[8];
-    for (int i = 0; i < 8; ++i) pw[i] = (uint32_t)pre[4 * i] | ((uint32_t)pre[4 * i + 1] << 8) | ((uint32_t)pre[4 * i + 2] << 16) | ((uint32_t)pre[4 * i + 3] << 24);
-    const uint32_t* dpre = (const uint32_t*)ctx.upload(pw, 32);
-    DevBuf res(&ctx, 2);
-    unsigned long long none = ~0ull, found = ~0ull;
-    HIP_CHECK(hipMemcpyAsync(res.p, &none, 8, hipMemcpyHostToDevice, ctx.stream));
-    const uint32_t per = 16;
-    for (uint32_t hi = 0; found == ~0ull; hi += per) {
-      {
-        KScope ks(ctx, K_GRIND, 0.0);
-        launch_grind(dpre, cfg.pow_bits, hi, per, (unsigned long long*)res.p, ctx.stream);
-      }
-      ctx.d2h(&found, res.p, 8);
-      if (hi > 0x7fffffffu - per) throw HipError(4, "Grind failed to find a solution.");
-    }
-    out.pow = found;
-    ch.mix_u64(found);
-    ctx.reset_arena();
-  }
+  out.pow = device_grind(ctx, ch.pow_prefix(cfg.pow_bits), cfg.pow_bits);
+  ch.mix_u64(out.pow);
   stage_ms[6] = tm.lap();
 
   // ---------------- decommit ----------------
@@ -907,12 +983,15 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     }
   }
   auto qpos = sorted_unique(unsorted);
-  out.first = fri_decommit_layer(ctx, first, qpos, 1);
+  Gather g;
+  std::vector<FriPlan> fplans;
+  fplans.push_back(plan_fri_layer(first, qpos, 1, g));
   auto lq = fold_queries(qpos, 1);
   for (auto& l : inner) {
-    out.inner.push_back(fri_decommit_layer(ctx, l, lq, l.fold_step));
+    fplans.push_back(plan_fri_layer(l, lq, l.fold_step, g));
     lq = fold_queries(lq, l.fold_step);
   }
+  std::vector<MerklePlan> tplans;
   for (auto& t : trees) {
     auto q = tree_local_positions(qpos, Lc, t.max_log());
     std::map<uint32_t, std::vector<size_t>> qpl;
@@ -923,10 +1002,16 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (size_t p : q) v.push_back(((p >> (shift + 1)) << 1) + (p & 1));
       qpl[ls] = sorted_unique(v);
     }
+    tplans.push_back(plan_merkle(t, qpl, g));
+  }
+  auto vals = g.run(ctx);
+  out.first = finish_fri_layer(fplans[0], vals);
+  for (size_t i = 1; i < fplans.size(); ++i) out.inner.push_back(finish_fri_layer(fplans[i], vals));
+  for (size_t t = 0; t < trees.size(); ++t) {
     MerkleDec d;
-    out.queried.push_back(merkle_decommit(ctx, t, qpl, d));
+    out.queried.push_back(finish_merkle(tplans[t], vals, d));
     out.decs.push_back(d);
-    out.commitments.push_back(t.root);
+    out.commitments.push_back(trees[t].root);
   }
   ctx.reset_arena();
   stage_ms[7] = tm.lap();

@@ -52,8 +52,11 @@ struct ProveResult {
   double stage_ms[8] = {0};  // trace commit, interaction, composition, oods, quotients, fri, pow, decommit
 };
 
-// Builds the Merkle layers of a tree from its evaluation columns (MerkleProver::commit).
-void merkle_commit(Ctx& ctx, DevTree& t);
+// GrindOps::grind on the device: smallest nonce for the prefixed digest (simd/grind.rs order).
+uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits);
+// Builds the Merkle layers of a tree from its evaluation columns (MerkleProver::commit). With
+// read_root false the root stays on the device (layers[0]) and t.root is not set.
+void merkle_commit(Ctx& ctx, DevTree& t, bool read_root = true);
 // Interpolates columns (any sizes) into new coefficient buffers owned by `into`.
 void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, const std::vector<uint32_t>& logs,
                          DevTree& into);

@@ -87,6 +87,7 @@ void* Ctx::upload(const void* src, size_t bytes) {
     HIP_CHECK(hipMalloc((void**)&d, a));
     HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
+    uploads_in_flight = false;
     overflow.push_back(d);
     return d;
   }
@@ -94,12 +95,14 @@ void* Ctx::upload(const void* src, size_t bytes) {
   uint8_t* d = dev_arena + arena_off;
   memcpy(h, src, bytes);
   HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+  uploads_in_flight = true;
   pinned_off += a;
   arena_off += a;
   return d;
 }
 void Ctx::sync() {
   HIP_CHECK(hipStreamSynchronize(stream));
+  uploads_in_flight = false;
   if (!pending.empty()) collect_timing();
 }
 hipEvent_t Ctx::get_event() {
@@ -122,6 +125,8 @@ void Ctx::collect_timing() {
   pending.clear();
 }
 void Ctx::reset_arena() {
+  // staging still read by queued copies (no sync since the last upload) is kept until a later call
+  if (uploads_in_flight) return;
   for (auto* p : overflow) (void)hipFree(p);
   overflow.clear();
   pinned_off = 0;

@@ -80,6 +80,7 @@ struct Ctx {
   size_t pinned_cap = 0, pinned_off = 0;
   uint8_t* dev_arena = nullptr;
   size_t arena_cap = 0, arena_off = 0;
+  bool uploads_in_flight = false;
   std::vector<uint8_t*> overflow;  // arena overflow allocations, freed at reset
   // proof arena: HBM chunks kept across proofs; a proof bump-allocates its buffers from them
   std::vector<std::pair<uint8_t*, size_t>> bump_chunks;
@@ -94,7 +95,7 @@ struct Ctx {
   void* upload(const void* src, size_t bytes);
   template <class T> const T* upload_vec(const std::vector<T>& v) { return (const T*)upload(v.data(), v.size() * sizeof(T)); }
   void sync();
-  void reset_arena();  // after sync
+  void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
   void ensure_twiddles(uint32_t log_domain);
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
