@@ -235,7 +235,7 @@ int stwo_hip_accumulate_numerators(stwo_hip_ctx* h, uint32_t log_size, const uin
         qm31 aa = qm_sub(qm_conj(v), v), cc = qm_sub(qm_conj(p.y), p.y);
         qm31 bb = qm_sub(qm_mul(v, cc), qm_mul(aa, p.y));
         NumEntry e{};
-        e.col = bt.column_indices[j];
+        e.col = cols[bt.column_indices[j]];
         e.c = qm_mul(al, cc);
         ents.push_back(e);
         A = qm_add(A, qm_mul(al, aa));
@@ -245,11 +245,14 @@ int stwo_hip_accumulate_numerators(stwo_hip_ctx* h, uint32_t log_size, const uin
       if (first_linear_host) first_linear_host[b] = q(A);
     }
     starts.push_back((uint32_t)ents.size());
-    std::vector<const uint32_t*> cv(cols, cols + n_cols);
+    bool aligned = true;
+    for (uint32_t j = 0; j < n_cols; ++j) aligned &= ((uintptr_t)cols[j] & 15) == 0;
     std::vector<uint32_t*> ov(out, out + 4 * (size_t)n_batches);
-    launch_accumulate_numerators(log_size, (const uint32_t* const*)c.upload_vec(cv), c.upload_vec(ents),
-                                 c.upload_vec(starts), c.upload_vec(Bs), n_batches,
-                                 (uint32_t* const*)c.upload_vec(ov), c.stream);
+    const NumEntry* d_ents = c.stage_vec(ents);
+    const uint32_t* d_starts = c.stage_vec(starts);
+    const qm31* d_bs = c.stage_vec(Bs);
+    uint32_t* const* d_out = (uint32_t* const*)c.upload_vec(ov);
+    launch_accumulate_numerators(log_size, d_ents, d_starts, d_bs, n_batches, d_out, aligned, c.stream);
     c.sync();
     c.reset_arena();
   });
@@ -437,6 +440,7 @@ int stwo_hip_kernel_stats(stwo_hip_ctx* h, uint32_t cls, double* ms, double* byt
   return guarded(h, [&](Ctx& c) {
     if (cls >= K_NUM_CLASSES) throw HipError(1, "kernel class out of range");
     c.sync();
+    c.collect_timing();
     if (ms) *ms = c.kstats[cls].ms;
     if (bytes) *bytes = c.kstats[cls].bytes;
     if (launches) *launches = c.kstats[cls].launches;
@@ -445,6 +449,7 @@ int stwo_hip_kernel_stats(stwo_hip_ctx* h, uint32_t cls, double* ms, double* byt
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* h) {
   return guarded(h, [&](Ctx& c) {
     c.sync();
+    c.collect_timing();
     for (auto& k : c.kstats) k = KernelStat();
   });
 }

@@ -15,7 +15,11 @@ namespace sh {
 constexpr uint32_t P = 0x7fffffffu;
 
 SHD uint32_t m31_add(uint32_t a, uint32_t b) { uint32_t s = a + b; return s >= P ? s - P : s; }
-SHD uint32_t m31_sub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + P - b; }
+// a - b wraps above 2^31 exactly when a < b, and then a - b + P (mod 2^32) is the smaller value.
+SHD uint32_t m31_sub(uint32_t a, uint32_t b) {
+  uint32_t d = a - b, e = d + P;
+  return d < e ? d : e;
+}
 SHD uint32_t m31_neg(uint32_t a) { return a ? P - a : 0; }
 // Folds a value below 2^62 + 2^32 into [0, P).
 SHD uint32_t m31_reduce64(uint64_t x) {

@@ -18,6 +18,9 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
 // merkle.hip
 void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
                          uint32_t* out, hipStream_t stream);
+// The same for at most 16 columns, given as a host array of device addresses.
+void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const uint32_t* const* host_cols,
+                               uint32_t n_cols, uint32_t* out, hipStream_t stream);
 // `levels` (<= 8, <= log_child) column-free levels above a layer of 2^log_child nodes; outs[j]
 // receives level log_child - 1 - j.
 void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t levels, uint32_t* const* outs,
@@ -44,11 +47,12 @@ void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* a
 void launch_channel_mix_root_draw(uint32_t* state, const uint32_t* root, uint32_t* record, hipStream_t stream);
 
 // Fused accumulate_numerators for one size group. Entries of batch b are
-// [batch_start[b], batch_start[b+1]); out holds 4 coordinate pointers per batch.
-struct NumEntry { uint32_t col; uint32_t pad[3]; qm31 c; };
-void launch_accumulate_numerators(uint32_t log_size, const uint32_t* const* cols, const NumEntry* entries,
-                                  const uint32_t* batch_start, const qm31* batch_b, uint32_t n_batches,
-                                  uint32_t* const* out, hipStream_t stream);
+// [batch_start[b], batch_start[b+1]); out holds 4 coordinate pointers per batch. aligned16 says
+// that every column address is 16-byte aligned (enables the vectorised form).
+struct NumEntry { const uint32_t* col; uint32_t pad[2]; qm31 c; };
+void launch_accumulate_numerators(uint32_t log_size, const NumEntry* entries, const uint32_t* batch_start,
+                                  const qm31* batch_b, uint32_t n_batches, uint32_t* const* out, bool aligned16,
+                                  hipStream_t stream);
 
 struct QuotPoint { cm31 prx, pix, pry, piy; qm31 first_linear; uint32_t part_begin, part_end, pad[2]; };
 struct QuotPart { const uint32_t* c[4]; uint32_t log_size; uint32_t pad[3]; };

@@ -69,6 +69,52 @@ void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t
                      n_cols, out);
 }
 
+// The same node hash for at most 16 injected columns whose addresses travel as kernel
+// arguments (no pointer table in device memory).
+struct ColPtrs16 { const uint32_t* c[16]; };
+__global__ void __launch_bounds__(256) merkle_layer_small_kernel(uint32_t n_nodes, const uint32_t* __restrict__ prev,
+                                                                 ColPtrs16 cols, uint32_t n_cols,
+                                                                 uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  uint32_t h[8];
+  blake2s_init(h);
+  uint32_t m[16];
+  uint32_t t = 0;
+  if (prev) {
+    const uint4* p4 = reinterpret_cast<const uint4*>(prev + (size_t)i * 16);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint4 v = p4[k];
+      m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
+    }
+    t = 64;
+    blake2s_compress(h, m, t, n_cols == 0);
+  } else if (n_cols == 0) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = 0;
+    blake2s_compress(h, m, 0, true);
+  }
+  if (n_cols) {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) m[k] = k < n_cols ? cols.c[k][i] : 0u;
+    blake2s_compress(h, m, t + 4 * n_cols, true);
+  }
+  uint4* o4 = reinterpret_cast<uint4*>(out + (size_t)i * 8);
+  o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
+  o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
+void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const uint32_t* const* host_cols,
+                               uint32_t n_cols, uint32_t* out, hipStream_t stream) {
+  ColPtrs16 cp{};
+  for (uint32_t k = 0; k < n_cols && k < 16; ++k) cp.c[k] = host_cols[k];
+  uint32_t n = 1u << log_size;
+  uint32_t threads = n < 256 ? (n < 64 ? 64 : n) : 256;
+  hipLaunchKernelGGL(merkle_layer_small_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, n, prev,
+                     cp, n_cols, out);
+}
+
 // Up to 8 levels of a Merkle tree with no injected columns in one launch. Workgroup w reads the
 // 2^L children [w 2^L, (w+1) 2^L) of level log_child and writes its 2^(L-j) nodes of level
 // log_child - j (j = 1..L); the levels in between stay in LDS.

@@ -95,11 +95,76 @@ void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* a
 }
 
 // partial_b[row] = sum_{j in batch b} c_j * col_j[row] - sum_j b_j.
-// A block covers 64 rows; its 4 waves split each batch's columns (entry e goes to wave e mod 4)
-// so that a 2^16-row group still fills every CU, then the 4 partial sums are added in LDS.
+//
+// Wide form (2^k >= 256 rows, 16-byte aligned columns): a block of 8 waves covers 256 rows, lane l
+// owns rows 4l..4l+3 (one 16-byte load per column), and wave w takes every 8th entry of a batch,
+// 8 entries per step with their loads issued together; the 8 partial sums meet in LDS.
+constexpr uint32_t NUM_WAVES = 8;
+constexpr uint32_t NUM_UNROLL = 8;
+__device__ __forceinline__ void mac4(qacc* acc, qm31 c, uint4 v) {
+  qacc_mac(acc[0], c, v.x);
+  qacc_mac(acc[1], c, v.y);
+  qacc_mac(acc[2], c, v.z);
+  qacc_mac(acc[3], c, v.w);
+}
+__global__ void __launch_bounds__(512) accumulate_numerators_wide_kernel(const NumEntry* __restrict__ entries,
+                                                                         const uint32_t* __restrict__ batch_start,
+                                                                         const qm31* __restrict__ batch_b,
+                                                                         uint32_t n_batches,
+                                                                         uint32_t* const* __restrict__ out) {
+  __shared__ uint32_t part[NUM_WAVES][4][256];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t row0 = blockIdx.x * 256 + lane * 4;
+  for (uint32_t b = 0; b < n_batches; ++b) {
+    qacc acc[4] = {qacc_zero(), qacc_zero(), qacc_zero(), qacc_zero()};
+    const uint32_t e_end = batch_start[b + 1];
+    uint32_t e = batch_start[b] + wave;
+    for (; e + (NUM_UNROLL - 1) * NUM_WAVES < e_end; e += NUM_UNROLL * NUM_WAVES) {
+      const uint32_t* col[NUM_UNROLL];
+      qm31 c[NUM_UNROLL];
+      uint4 v[NUM_UNROLL];
+#pragma unroll
+      for (uint32_t k = 0; k < NUM_UNROLL; ++k) {
+        col[k] = entries[e + k * NUM_WAVES].col;
+        c[k] = entries[e + k * NUM_WAVES].c;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < NUM_UNROLL; ++k) v[k] = *reinterpret_cast<const uint4*>(col[k] + row0);
+#pragma unroll
+      for (uint32_t k = 0; k < NUM_UNROLL; ++k) mac4(acc, c[k], v[k]);
+    }
+    for (; e < e_end; e += NUM_WAVES) {
+      const NumEntry en = entries[e];
+      mac4(acc, en.c, *reinterpret_cast<const uint4*>(en.col + row0));
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r) {
+      qm31 q = qacc_reduce(acc[r]);
+      part[wave][0][lane * 4 + r] = q.a;
+      part[wave][1][lane * 4 + r] = q.b;
+      part[wave][2][lane * 4 + r] = q.c;
+      part[wave][3][lane * 4 + r] = q.d;
+    }
+    __syncthreads();
+    {
+      const uint32_t r = threadIdx.x & 255, c0 = (threadIdx.x >> 8) * 2;
+      const qm31 bb = batch_b[b];
+      const uint32_t bw[4] = {bb.a, bb.b, bb.c, bb.d};
+#pragma unroll
+      for (uint32_t cc = c0; cc < c0 + 2; ++cc) {
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NUM_WAVES; ++w) s = m31_add(s, part[w][cc][r]);
+        out[4 * b + cc][blockIdx.x * 256 + r] = m31_sub(s, bw[cc]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Any size: one row per lane, 4 waves splitting each batch's entries.
 constexpr uint32_t NUM_SLICES = 4;
 __global__ void __launch_bounds__(256) accumulate_numerators_kernel(uint32_t n_rows,
-                                                                    const uint32_t* const* __restrict__ cols,
                                                                     const NumEntry* __restrict__ entries,
                                                                     const uint32_t* __restrict__ batch_start,
                                                                     const qm31* __restrict__ batch_b,
@@ -114,7 +179,7 @@ __global__ void __launch_bounds__(256) accumulate_numerators_kernel(uint32_t n_r
     if (active)
       for (uint32_t e = batch_start[b] + slice; e < e_end; e += NUM_SLICES) {
         NumEntry en = entries[e];
-        qacc_mac(acc, en.c, cols[en.col][row]);
+        qacc_mac(acc, en.c, en.col[row]);
       }
     part[slice][lane] = qacc_reduce(acc);
     __syncthreads();
@@ -130,12 +195,16 @@ __global__ void __launch_bounds__(256) accumulate_numerators_kernel(uint32_t n_r
   }
 }
 
-void launch_accumulate_numerators(uint32_t log_size, const uint32_t* const* cols, const NumEntry* entries,
-                                  const uint32_t* batch_start, const qm31* batch_b, uint32_t n_batches,
-                                  uint32_t* const* out, hipStream_t stream) {
+void launch_accumulate_numerators(uint32_t log_size, const NumEntry* entries, const uint32_t* batch_start,
+                                  const qm31* batch_b, uint32_t n_batches, uint32_t* const* out, bool aligned16,
+                                  hipStream_t stream) {
   uint32_t n = 1u << log_size;
-  hipLaunchKernelGGL(accumulate_numerators_kernel, dim3((n + 63) / 64), dim3(256), 0, stream, n, cols, entries,
-                     batch_start, batch_b, n_batches, out);
+  if (aligned16 && n >= 256)
+    hipLaunchKernelGGL(accumulate_numerators_wide_kernel, dim3(n / 256), dim3(512), 0, stream, entries, batch_start,
+                       batch_b, n_batches, out);
+  else
+    hipLaunchKernelGGL(accumulate_numerators_kernel, dim3((n + 63) / 64), dim3(256), 0, stream, n, entries,
+                       batch_start, batch_b, n_batches, out);
 }
 
 // q[row] = sum_a (sum_parts lift(partial)[row] - first_linear_a * y) / den_a(row),

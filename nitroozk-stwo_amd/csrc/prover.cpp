@@ -16,6 +16,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <set>
 
 namespace sh {
@@ -32,6 +35,26 @@ struct Timer {
   }
 };
 
+// Host phase marks, printed to stderr after each proof when STWO_HIP_HOST_PHASES is set: the
+// host-side time between consecutive marks (including any waits on the device).
+struct HostMarks {
+  bool on = std::getenv("STWO_HIP_HOST_PHASES") != nullptr;
+  std::vector<std::pair<const char*, std::chrono::steady_clock::time_point>> v;
+  void mark(const char* name) {
+    if (on) v.push_back({name, std::chrono::steady_clock::now()});
+  }
+  void flush() {
+    if (!on || v.empty()) return;
+    for (size_t i = 1; i < v.size(); ++i)
+      std::fprintf(stderr, "%s=%.0f ", v[i].first,
+                   std::chrono::duration<double, std::micro>(v[i].second - v[i - 1].second).count());
+    std::fprintf(stderr, "\n");
+    v.clear();
+  }
+};
+HostMarks g_marks;
+#define HMARK(name) g_marks.mark(name)
+
 Hash read_hash(Ctx& ctx, const uint32_t* dev) {
   uint32_t w[8];
   ctx.d2h(w, dev, 32);
@@ -44,27 +67,28 @@ Hash read_hash(Ctx& ctx, const uint32_t* dev) {
 // Batched gather of device words: requests are collected first, then served by one kernel.
 struct Gather {
   std::vector<const uint32_t*> bases;
-  std::map<const uint32_t*, uint32_t> base_of;
   std::vector<uint32_t> bidx, offs;
-  uint32_t add(const uint32_t* base, uint32_t off) {
-    auto it = base_of.find(base);
-    uint32_t b;
-    if (it == base_of.end()) { b = (uint32_t)bases.size(); bases.push_back(base); base_of[base] = b; }
-    else b = it->second;
+  uint32_t base(const uint32_t* p) {  // a base may repeat: the table is only read by the gather
+    bases.push_back(p);
+    return (uint32_t)bases.size() - 1;
+  }
+  uint32_t add_at(uint32_t b, uint32_t off) {
     bidx.push_back(b);
     offs.push_back(off);
     return (uint32_t)offs.size() - 1;
   }
+  uint32_t add(const uint32_t* p, uint32_t off) { return add_at(base(p), off); }
   uint32_t add_hash(const uint32_t* layer, uint32_t node) {
-    uint32_t first = add(layer, node * 8);
-    for (uint32_t w = 1; w < 8; ++w) add(layer, node * 8 + w);
+    uint32_t b = base(layer);
+    uint32_t first = add_at(b, node * 8);
+    for (uint32_t w = 1; w < 8; ++w) add_at(b, node * 8 + w);
     return first;
   }
   std::vector<uint32_t> run(Ctx& ctx) {
     std::vector<uint32_t> out(offs.size());
     if (offs.empty()) return out;
-    const uint32_t* const* b = (const uint32_t* const*)ctx.upload(bases.data(), bases.size() * sizeof(void*));
-    const uint32_t* bi = ctx.upload_vec(bidx);
+    const uint32_t* const* b = (const uint32_t* const*)ctx.stage(bases.data(), bases.size() * sizeof(void*));
+    const uint32_t* bi = ctx.stage_vec(bidx);
     const uint32_t* of = ctx.upload_vec(offs);
     DevBuf o(&ctx, offs.size());
     launch_gather_words(b, bi, of, (uint32_t)offs.size(), o.p, ctx.stream);
@@ -82,10 +106,21 @@ Hash words_to_hash(const std::vector<uint32_t>& w, uint32_t at) {
 // ---- proof encoding (little endian, counts as u32) ---------------------------------------------
 struct Writer {
   std::vector<uint8_t> b;
-  void u32(uint32_t v) { for (int i = 0; i < 4; ++i) b.push_back((uint8_t)(v >> (8 * i))); }
+  size_t n = 0;
+  Writer() { b.resize(1 << 16); }
+  uint8_t* room(size_t k) {
+    if (n + k > b.size()) b.resize(std::max(2 * b.size(), n + k));
+    uint8_t* r = b.data() + n;
+    n += k;
+    return r;
+  }
+  void u32(uint32_t v) {  // the host is little endian, as is the encoding
+    std::memcpy(room(4), &v, 4);
+  }
   void u64(uint64_t v) { u32((uint32_t)v); u32((uint32_t)(v >> 32)); }
-  void hash(const Hash& h) { b.insert(b.end(), h.begin(), h.end()); }
-  void q(const qm31& x) { u32(x.a); u32(x.b); u32(x.c); u32(x.d); }
+  void hash(const Hash& h) { std::memcpy(room(32), h.data(), 32); }
+  void q(const qm31& x) { uint32_t w[4] = {x.a, x.b, x.c, x.d}; std::memcpy(room(16), w, 16); }
+  std::vector<uint8_t> take() { b.resize(n); return std::move(b); }
 };
 struct MerkleDec {
   std::vector<Hash> hash_witness;
@@ -146,33 +181,40 @@ struct MerklePlan {
   std::vector<uint32_t> hw_idx, q_idx, w_idx;
 };
 MerklePlan plan_merkle(const DevTree& t, const std::map<uint32_t, std::vector<size_t>>& qpl, Gather& g) {
+  // MerkleProver::decommit (vcs/prover.rs:84-217); query lists in qpl are sorted and unique
   MerklePlan mp;
   auto order = merkle_order(t.eval_logs);
   size_t ci = 0;
-  std::vector<size_t> last;
+  std::vector<size_t> last, merged, qn, wn;
+  std::vector<uint32_t> cb;
+  static const std::vector<size_t> none;
   for (int ls = (int)t.layers.size() - 1; ls >= 0; --ls) {
     bool has_prev = (size_t)ls + 1 < t.layers.size();
-    std::set<size_t> prev_set(last.begin(), last.end());
-    std::set<size_t> colq;
     auto it = qpl.find((uint32_t)ls);
-    if (it != qpl.end()) colq.insert(it->second.begin(), it->second.end());
-    std::set<size_t> merged;
-    for (size_t q : last) merged.insert(q / 2);
-    merged.insert(colq.begin(), colq.end());
-    std::vector<size_t> lc;
-    while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(order[ci++]);
-    std::vector<size_t> qn, wn;
+    const std::vector<size_t>& colq = it != qpl.end() ? it->second : none;
+    merged.clear();
+    for (size_t q : last) merged.push_back(q / 2);
+    merged.insert(merged.end(), colq.begin(), colq.end());
+    std::sort(merged.begin(), merged.end());
+    merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+    cb.clear();
+    while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) cb.push_back(g.base(t.evals[order[ci++]]));
+    qn.clear();
+    wn.clear();
+    uint32_t pb = has_prev ? g.base(t.layers[ls + 1].p) : 0;
     for (size_t node : merged) {
       if (has_prev) {
-        const uint32_t* prev = t.layers[ls + 1].p;
-        if (!prev_set.count(2 * node)) mp.hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node)));
-        if (!prev_set.count(2 * node + 1)) mp.hw_idx.push_back(g.add_hash(prev, (uint32_t)(2 * node + 1)));
+        for (size_t child = 2 * node; child < 2 * node + 2; ++child)
+          if (!std::binary_search(last.begin(), last.end(), child)) {
+            mp.hw_idx.push_back(g.add_at(pb, (uint32_t)(child * 8)));
+            for (uint32_t w = 1; w < 8; ++w) g.add_at(pb, (uint32_t)(child * 8 + w));
+          }
       }
-      if (colq.count(node)) qn.push_back(node); else wn.push_back(node);
+      if (std::binary_search(colq.begin(), colq.end(), node)) qn.push_back(node); else wn.push_back(node);
     }
-    for (size_t node : qn) for (size_t c : lc) mp.q_idx.push_back(g.add(t.evals[c], (uint32_t)node));
-    for (size_t node : wn) for (size_t c : lc) mp.w_idx.push_back(g.add(t.evals[c], (uint32_t)node));
-    last.assign(merged.begin(), merged.end());
+    for (size_t node : qn) for (uint32_t b : cb) mp.q_idx.push_back(g.add_at(b, (uint32_t)node));
+    for (size_t node : wn) for (uint32_t b : cb) mp.w_idx.push_back(g.add_at(b, (uint32_t)node));
+    last.swap(merged);
   }
   return mp;
 }
@@ -203,20 +245,35 @@ qm31 from_partial_evals(qm31 e0, qm31 e1, qm31 e2, qm31 e3) {
   return r;
 }
 
+// Sampled values of every committed column, flattened. Column k of tree t is global column
+// col_base[t] + k; its samples are [start[g], start[g + 1]) of pid (ids into pts) and val.
+struct SampleSet {
+  std::vector<uint32_t> col_base{0};
+  std::vector<uint32_t> start{0};
+  std::vector<uint32_t> pid;
+  std::vector<qm31> val;
+  std::vector<cpqm31> pts;
+  uint32_t n_trees() const { return (uint32_t)col_base.size() - 1; }
+  uint32_t n_cols(size_t t) const { return col_base[t + 1] - col_base[t]; }
+  uint32_t first(size_t t, size_t c) const { return start[col_base[t] + c]; }
+  uint32_t count(size_t t, size_t c) const { return start[col_base[t] + c + 1] - start[col_base[t] + c]; }
+  const qm31* vals(size_t t, size_t c) const { return val.data() + first(t, c); }
+};
+
 // ---- host point evaluation of the AIRs (sanity check of prove_ex) ------------------------------
 struct PointEval {
-  const std::vector<std::vector<std::vector<qm31>>>* sv;
+  const SampleSet* sv;
   const Component* c;
   size_t ti = 0, ii = 0;
   qm31* acc;
   qm31 rc, dinv;
-  qm31 next_trace() { return (*sv)[1][c->trace_start + ti++][0]; }
+  qm31 next_trace() { return sv->vals(1, c->trace_start + ti++)[0]; }
   void add(qm31 x) { *acc = qm_add(qm_mul(*acc, rc), qm_mul(dinv, x)); }
   std::vector<qm31> next_ext(size_t n_off) {
-    const std::vector<qm31>* m[4];
-    for (int k = 0; k < 4; ++k) m[k] = &(*sv)[2][c->inter_start + ii++];
+    const qm31* m[4];
+    for (int k = 0; k < 4; ++k) m[k] = sv->vals(2, c->inter_start + ii++);
     std::vector<qm31> r;
-    for (size_t o = 0; o < n_off; ++o) r.push_back(from_partial_evals((*m[0])[o], (*m[1])[o], (*m[2])[o], (*m[3])[o]));
+    for (size_t o = 0; o < n_off; ++o) r.push_back(from_partial_evals(m[0][o], m[1][o], m[2][o], m[3][o]));
     return r;
   }
 };
@@ -292,26 +349,16 @@ void eval_at_point(PointEval& e) {
 }
 
 // mask offsets per tree (0 preprocessed, 1 trace, 2 interaction), per column of the component
-std::vector<std::vector<std::vector<long>>> mask_offsets(const Component& c) {
-  std::vector<std::vector<std::vector<long>>> r(3);
-  r[1].assign(c.n_trace_cols(), {0});
-  if (c.kind == AIR_POSEIDON) {
-    for (uint32_t i = 0; i < 4 * (c.ipr - 1); ++i) r[2].push_back({0});
-    for (uint32_t i = 0; i < 4; ++i) r[2].push_back({-1, 0});
+// Mask offsets of column i of tree t (1 = trace, 2 = interaction) of a component: the rows,
+// relative to the current one, at which its constraints read the column. Returns the count.
+uint32_t mask_offsets(const Component& c, size_t t, size_t i, long* offs) {
+  if (t == 2 && c.kind == AIR_POSEIDON && i >= 4 * (size_t)(c.ipr - 1)) {
+    offs[0] = -1;  // the cumulative-sum columns also read the previous row
+    offs[1] = 0;
+    return 2;
   }
-  return r;
-}
-
-struct PointSample { cpqm31 point; qm31 value; qm31 rand; };
-
-// Line coefficients (a, b, c) * alpha (core/constraints.rs complex_conjugate_line_coeffs).
-void line_coeffs(const cpqm31& p, qm31 value, qm31 alpha, qm31& a, qm31& b, qm31& c) {
-  qm31 aa = qm_sub(qm_conj(value), value);
-  qm31 cc = qm_sub(qm_conj(p.y), p.y);
-  qm31 bb = qm_sub(qm_mul(value, cc), qm_mul(aa, p.y));
-  a = qm_mul(alpha, aa);
-  b = qm_mul(alpha, bb);
-  c = qm_mul(alpha, cc);
+  offs[0] = 0;
+  return 1;
 }
 
 void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const std::vector<uint32_t>& logs,
@@ -329,8 +376,10 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
       src.push_back(coeffs[i]);
       dst.push_back(out[i].p);
     }
+    const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
+    uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)));
-    launch_fft(false, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
+    launch_fft(false, d_src, d_dst,
                (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), ctx.stream);
   }
 }
@@ -398,9 +447,14 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
     while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(t.evals[order[ci++]]);
     DevBuf layer(&ctx, (size_t)8 << ls);
     const uint32_t* prev = rev.empty() ? nullptr : rev.back().p;
-    const uint32_t* const* cp = lc.empty() ? nullptr : (const uint32_t* const*)ctx.upload_vec(lc);
-    KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
-    launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
+    if (lc.size() <= 16) {
+      KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
+      launch_merkle_layer_small((uint32_t)ls, prev, lc.data(), (uint32_t)lc.size(), layer.p, ctx.stream);
+    } else {
+      const uint32_t* const* cp = (const uint32_t* const*)ctx.upload_vec(lc);
+      KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
+      launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
+    }
     rev.push_back(std::move(layer));
     --ls;
   }
@@ -427,8 +481,10 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
       into.coeffs[base + i] = into.owned.back().p;
       into.coeff_logs[base + i] = lg;
     }
+    const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
+    uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
     KScope ks(ctx, K_INTERPOLATE, (double)idx.size() * 8.0 * (1u << lg));
-    launch_fft(true, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
+    launch_fft(true, d_src, d_dst,
                (uint32_t)idx.size(), lg, lg, ctx.itw.p, ctx.tree_len(), ctx.stream);
   }
 }
@@ -450,8 +506,10 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
       t.evals[i] = t.owned.back().p;
       t.eval_logs[i] = K;
     }
+    const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
+    uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << K)));
-    launch_fft(false, (const uint32_t* const*)ctx.upload_vec(src), (uint32_t* const*)ctx.upload_vec(dst),
+    launch_fft(false, d_src, d_dst,
                (uint32_t)idx.size(), lg, K, ctx.tw.p, ctx.tree_len(), ctx.stream);
   }
 }
@@ -468,7 +526,7 @@ void commit_tree(Ctx& ctx, DevTree& t, const PcsCfg& cfg, Channel& ch) {
 
 struct ProofParts {
   std::vector<Hash> commitments;
-  std::vector<std::vector<std::vector<qm31>>> sampled;
+  SampleSet sampled;
   std::vector<MerkleDec> decs;
   std::vector<std::vector<uint32_t>> queried;
   uint64_t pow = 0;
@@ -483,10 +541,14 @@ std::vector<uint8_t> encode(const PcsCfg& cfg, const ProofParts& p) {
   w.u32(cfg.fold_step); w.u32(cfg.lifting_log_size);
   w.u32((uint32_t)p.commitments.size());
   for (auto& h : p.commitments) w.hash(h);
-  w.u32((uint32_t)p.sampled.size());
-  for (auto& t : p.sampled) {
-    w.u32((uint32_t)t.size());
-    for (auto& c : t) { w.u32((uint32_t)c.size()); for (auto& q : c) w.q(q); }
+  const SampleSet& S = p.sampled;
+  w.u32(S.n_trees());
+  for (uint32_t t = 0; t < S.n_trees(); ++t) {
+    w.u32(S.n_cols(t));
+    for (uint32_t c = 0; c < S.n_cols(t); ++c) {
+      w.u32(S.count(t, c));
+      for (uint32_t k = 0; k < S.count(t, c); ++k) w.q(S.vals(t, c)[k]);
+    }
   }
   w.u32((uint32_t)p.decs.size());
   for (auto& d : p.decs) write_dec(w, d);
@@ -498,7 +560,7 @@ std::vector<uint8_t> encode(const PcsCfg& cfg, const ProofParts& p) {
   for (auto& l : p.inner) write_layer(w, l);
   w.u32((uint32_t)p.last_poly.size());
   for (auto& q : p.last_poly) w.q(q);
-  return w.b;
+  return w.take();
 }
 
 // A FRI layer column (4 coordinate columns of 2^log) with its Merkle tree.
@@ -575,6 +637,7 @@ std::vector<qm31> line_interpolate(std::vector<qm31> v, HCoset dom) {
 int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vector<Component>& comps,
              const PcsCfg& cfg, ProofParts& out, double* stage_ms) {
   Timer tm;
+  HMARK("prove_ex");
   // ---------------- composition ----------------
   qm31 random_coeff = ch.draw_secure_felt();
   uint32_t max_log = 0;
@@ -612,11 +675,12 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     for (uint32_t i = 0; i < (1u << le); ++i) dinv[i] = m31_inv(coset_vanishing_m31(c.log_n, canonic_domain_at(E, i)));
     std::vector<uint32_t> dinv_br(dinv.size());
     for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
-    const uint32_t* d_dinv = ctx.upload_vec(dinv_br);
-    const qm31* d_pows = ctx.upload_vec(cp);
-    const uint32_t* const* d_t = (const uint32_t* const*)ctx.upload_vec(tcols);
+    const uint32_t* d_dinv = ctx.stage_vec(dinv_br);
+    const qm31* d_pows = ctx.stage_vec(cp);
+    const uint32_t* const* d_t = (const uint32_t* const*)ctx.stage_vec(tcols);
     double cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
     if (c.kind == AIR_WIDE_FIB) {
+      ctx.commit_staged();
       KScope ks(ctx, K_COMPOSITION, cbytes);
       launch_wide_fib_composition(E, c.log_n, d_t, c.n_cols, d_pows, d_dinv, a4, ctx.stream);
     } else {
@@ -626,13 +690,11 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (int i = 0; i < 16; ++i) pp.alpha_pow[i] = c.alpha_pow[i];
       pp.shift = c.cumsum_shift();
       pp.ipr = c.ipr;
-      const uint32_t* const* d_i = (const uint32_t* const*)ctx.upload_vec(icols);
+      const uint32_t* const* d_i = (const uint32_t* const*)ctx.stage_vec(icols);
       const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
       KScope ks(ctx, K_COMPOSITION, cbytes);
       launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, d_pows, d_dinv, a4, ctx.stream);
     }
-    ctx.sync();
-    ctx.reset_arena();
   }
   // finalize: lift smaller accumulations into the largest, then interpolate 4 coordinate columns
   auto& big = accs[max_log];
@@ -652,7 +714,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     for (auto& o : full.owned) comp.owned.push_back(std::move(o));
   }
   accs.clear();
+  HMARK("comp_launch");
   commit_tree(ctx, comp, cfg, ch);
+  HMARK("comp_commit");
   trees.push_back(std::move(comp));
   stage_ms[2] = tm.lap();
 
@@ -668,34 +732,61 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   uint32_t L = cfg.lifting_log_size ? cfg.lifting_log_size : split_log;
   if (L < split_log) throw HipError(3, "lifting log size below the composition size");
   uint32_t max_deg = L - cfg.log_blowup;
-  std::vector<std::vector<std::vector<cpqm31>>> spts(trees.size());
-  for (size_t t = 0; t < trees.size(); ++t) spts[t].resize(trees[t].evals.size());
-  cpm31 step = index_to_point(subgroup_gen(max_deg));
-  for (auto& c : comps) {
-    auto offs = mask_offsets(c);
+  // sample points: the mask offsets of every trace and interaction column, and the OODS point
+  // itself for the composition columns (prover/mod.rs:95-118, constraint-framework mask points)
+  SampleSet& S = out.sampled;
+  for (size_t t = 0; t < trees.size(); ++t) S.col_base.push_back(S.col_base.back() + (uint32_t)trees[t].evals.size());
+  const uint32_t NC = S.col_base.back();
+  const cpm31 step = index_to_point(subgroup_gen(max_deg));
+  std::vector<long> pt_off;
+  auto pid_of = [&](long o) -> uint32_t {
+    for (size_t i = 0; i < pt_off.size(); ++i)
+      if (pt_off[i] == o) return (uint32_t)i;
+    pt_off.push_back(o);
+    S.pts.push_back(cpq_add(oods, cpq_mul_signed(step, o)));
+    return (uint32_t)pt_off.size() - 1;
+  };
+  std::vector<uint8_t> ccount(NC, 0);
+  std::vector<std::array<uint32_t, 2>> cpid(NC);
+  for (auto& c : comps)
     for (size_t t = 1; t < 3 && t + 1 < trees.size(); ++t) {
       size_t start = t == 1 ? c.trace_start : c.inter_start;
-      for (size_t i = 0; i < offs[t].size(); ++i)
-        for (long o : offs[t][i]) spts[t][start + i].push_back(cpq_add(oods, cpq_mul_signed(step, o)));
+      size_t n = t == 1 ? c.n_trace_cols() : c.n_inter_cols();
+      for (size_t i = 0; i < n; ++i) {
+        long offs[2];
+        uint32_t k = mask_offsets(c, t, i, offs);
+        uint32_t g = S.col_base[t] + (uint32_t)(start + i);
+        if (ccount[g] + k > 2) throw HipError(3, "more than two mask points on a column");
+        for (uint32_t j = 0; j < k; ++j) cpid[g][ccount[g]++] = pid_of(offs[j]);
+      }
     }
-  }
-  for (auto& v : spts.back()) v = {oods};
+  for (uint32_t g = S.col_base[trees.size() - 1]; g < NC; ++g) { cpid[g][0] = pid_of(0); ccount[g] = 1; }
+  S.start.assign(NC + 1, 0);
+  for (uint32_t g = 0; g < NC; ++g) S.start[g + 1] = S.start[g] + ccount[g];
+  S.pid.resize(S.start[NC]);
+  for (uint32_t g = 0; g < NC; ++g)
+    for (uint32_t j = 0; j < ccount[g]; ++j) S.pid[S.start[g] + j] = cpid[g][j];
   uint32_t Lc = trees.back().max_log();  // prove_values lifting size: the last tree
+  // one OODS task per sample, grouped by (coefficient log size, point lifted to the column size)
   std::vector<OodsTask> tasks;
+  tasks.reserve(S.pid.size());
   std::vector<uint32_t> task_group, group_log;
+  task_group.reserve(S.pid.size());
   std::vector<qm31> group_factors;
-  std::vector<std::pair<uint32_t, cpqm31>> group_key;
+  struct GKey { uint32_t k, pid, shift; };
+  std::vector<GKey> group_key;
   uint32_t max_coeff_log = 0;
-  size_t n_vals = 0;
   for (size_t t = 0; t < trees.size(); ++t)
-    for (size_t c = 0; c < spts[t].size(); ++c)
-      for (auto& p : spts[t][c]) {
-        cpqm31 q = cpq_repeated_double(p, Lc - trees[t].eval_logs[c]);
-        uint32_t k = trees[t].coeff_logs[c];
+    for (size_t c = 0; c < trees[t].evals.size(); ++c)
+      for (uint32_t j = 0; j < S.count(t, c); ++j) {
+        const uint32_t pid = S.pid[S.first(t, c) + j];
+        const uint32_t shift = Lc - trees[t].eval_logs[c];
+        const uint32_t k = trees[t].coeff_logs[c];
         uint32_t g = 0;
-        while (g < group_key.size() && !(group_key[g].first == k && point_eq(group_key[g].second, q))) ++g;
+        while (g < group_key.size() && !(group_key[g].k == k && group_key[g].pid == pid && group_key[g].shift == shift)) ++g;
         if (g == group_key.size()) {
-          group_key.push_back({k, q});
+          cpqm31 q = cpq_repeated_double(S.pts[pid], shift);
+          group_key.push_back({k, pid, shift});
           group_log.push_back(k);
           std::vector<qm31> f(32, qm_zero());
           f[0] = q.y;
@@ -703,98 +794,121 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
           for (uint32_t b = 1; b < k; ++b) { f[b] = x; x = qm_double_x(x); }
           group_factors.insert(group_factors.end(), f.begin(), f.end());
         }
-        tasks.push_back(OodsTask{trees[t].coeffs[c], k, (uint32_t)n_vals, {0, 0}});
+        tasks.push_back(OodsTask{trees[t].coeffs[c], k, (uint32_t)tasks.size(), {0, 0}});
         task_group.push_back(g);
         max_coeff_log = std::max(max_coeff_log, k);
-        ++n_vals;
       }
-  std::vector<qm31> values(n_vals);
+  const size_t n_vals = tasks.size();
+  HMARK("oods_prep");
+  S.val.resize(n_vals);
   if (n_vals) {
     uint32_t max_chunks = max_coeff_log > 11 ? 1u << (max_coeff_log - 11) : 1u;
     DevBuf partials(&ctx, (size_t)n_vals * max_chunks * 4), dvals(&ctx, n_vals * 4);
     DevBuf weights(&ctx, group_key.size() * ((size_t)4 << 11));
     double obytes = 0;
     for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
-    const OodsTask* dt = ctx.upload_vec(tasks);
-    const uint32_t* dtg = ctx.upload_vec(task_group);
-    const qm31* df = ctx.upload_vec(group_factors);
+    const OodsTask* dt = ctx.stage_vec(tasks);
+    const uint32_t* dtg = ctx.stage_vec(task_group);
+    const qm31* df = ctx.stage_vec(group_factors);
     const uint32_t* dgl = ctx.upload_vec(group_log);
     {
       KScope ks(ctx, K_OODS, obytes);
       launch_oods(dt, dtg, (uint32_t)n_vals, max_coeff_log, df, dgl, (uint32_t)group_key.size(), (qm31*)weights.p,
                   (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
     }
-    ctx.d2h(values.data(), dvals.p, n_vals * 16);
+    ctx.d2h(S.val.data(), dvals.p, n_vals * 16);
     ctx.reset_arena();
   }
-  out.sampled.resize(trees.size());
-  std::vector<std::vector<std::vector<PointSample>>> samples(trees.size());
-  {
-    size_t vi = 0;
-    for (size_t t = 0; t < trees.size(); ++t) {
-      samples[t].resize(spts[t].size());
-      for (size_t c = 0; c < spts[t].size(); ++c) {
-        std::vector<qm31> v;
-        for (auto& p : spts[t][c]) { samples[t][c].push_back({p, values[vi], qm_zero()}); v.push_back(values[vi]); ++vi; }
-        out.sampled[t].push_back(v);
-      }
-    }
-  }
-  ch.mix_felts(values);
+  HMARK("oods_gpu");
+  ch.mix_felts(S.val);
+  HMARK("oods_post");
   stage_ms[3] = tm.lap();
 
   // ---------------- FRI quotients ----------------
+  // Quotient samples per column (core/pcs/quotients.rs:80-105,306-366): a column sampled at two
+  // points first gets a periodicity sample at its second point shifted by the generator of its
+  // evaluation domain, then its samples; consecutive powers of the random coefficient in order.
   qm31 qrc = ch.draw_secure_felt();
-  std::vector<std::vector<std::vector<PointSample>>> swr(trees.size());
+  std::vector<cpqm31> qsample_pts = S.pts;
+  std::vector<std::array<uint32_t, 3>> periodic;  // (point id, eval log, new point id)
+  std::vector<uint32_t> qstart(NC + 1, 0), qpid;
+  std::vector<qm31> qval, qrand;
   {
     qm31 pw = qm_one();
-    auto next = [&]() { qm31 r = pw; pw = qm_mul(pw, qrc); return r; };
-    cpm31 gen = index_to_point(subgroup_gen(Lc));
+    const cpm31 gen = index_to_point(subgroup_gen(Lc));
     for (size_t t = 0; t < trees.size(); ++t)
-      for (size_t c = 0; c < samples[t].size(); ++c) {
-        std::vector<PointSample> ns;
-        auto& sc = samples[t][c];
-        if (!sc.empty()) {
-          if (sc.size() == 2) {
+      for (size_t c = 0; c < trees[t].evals.size(); ++c) {
+        const uint32_t g = S.col_base[t] + (uint32_t)c, n = S.count(t, c);
+        const qm31* v = S.vals(t, c);
+        const uint32_t* pids = S.pid.data() + S.first(t, c);
+        qstart[g] = (uint32_t)qpid.size();
+        if (n == 2) {
+          const uint32_t el = trees[t].eval_logs[c];
+          uint32_t np = UINT32_MAX;
+          for (auto& e : periodic)
+            if (e[0] == pids[1] && e[1] == el) np = e[2];
+          if (np == UINT32_MAX) {
             cpm31 pg = gen;
-            for (uint32_t i = 0; i < trees[t].eval_logs[c]; ++i) pg = cp_double(pg);
-            ns.push_back({cpq_add(sc[1].point, cpq_from(pg)), sc[1].value, next()});
+            for (uint32_t i = 0; i < el; ++i) pg = cp_double(pg);
+            np = (uint32_t)qsample_pts.size();
+            qsample_pts.push_back(cpq_add(qsample_pts[pids[1]], cpq_from(pg)));
+            periodic.push_back({pids[1], el, np});
           }
-          for (auto s : sc) { s.rand = next(); ns.push_back(s); }
+          qpid.push_back(np);
+          qval.push_back(v[1]);
+          qrand.push_back(pw);
+          pw = qm_mul(pw, qrc);
         }
-        swr[t].push_back(ns);
+        for (uint32_t j = 0; j < n; ++j) {
+          qpid.push_back(pids[j]);
+          qval.push_back(v[j]);
+          qrand.push_back(pw);
+          pw = qm_mul(pw, qrc);
+        }
       }
+    qstart[NC] = (uint32_t)qpid.size();
   }
+  HMARK("q_samples");
   struct Item { size_t t, c; uint32_t ls; };
   std::vector<Item> items;
+  items.reserve(NC);
   for (size_t t = 0; t < trees.size(); ++t)
     for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
   std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
   struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; qm31 first_linear; };
   std::vector<Acc> acc_list;
   std::vector<DevBuf> partial_bufs;
+  std::vector<int> batch_of(qsample_pts.size(), -1);
   for (size_t i = 0; i < items.size();) {
     size_t j = i;
     while (j < items.size() && items[j].ls == items[i].ls) ++j;
     uint32_t ls = items[i].ls;
-    std::vector<const uint32_t*> gcols;
-    struct Batch { cpqm31 point; std::vector<NumEntry> entries; qm31 B, A; };
+    // ColumnSampleBatch per point; line coefficients (core/constraints.rs complex_conjugate_line_coeffs)
+    //   a = r (conj(v) - v), c = r (conj(p.y) - p.y), b = r (v (conj(p.y) - p.y) - (conj(v) - v) p.y)
+    // so that over a batch A = sum r (conj(v) - v) and B = cy sum r v - p.y A, with cy = conj(p.y) - p.y.
+    struct Batch { uint32_t pid; qm31 cy, sum_rv, sum_ra; std::vector<NumEntry> entries; };
     std::vector<Batch> batches;
+    std::vector<const uint32_t*> gcols;
+    gcols.reserve(j - i);
     for (size_t k = i; k < j; ++k) {
-      uint32_t gi = (uint32_t)gcols.size();
+      const uint32_t gi = (uint32_t)gcols.size();
       gcols.push_back(trees[items[k].t].evals[items[k].c]);
-      for (auto& s : swr[items[k].t][items[k].c]) {
-        Batch* b = nullptr;
-        for (auto& e : batches) if (point_eq(e.point, s.point)) { b = &e; break; }
-        if (!b) { batches.push_back({s.point, {}, qm_zero(), qm_zero()}); b = &batches.back(); }
-        qm31 la, lb, lc;
-        line_coeffs(s.point, s.value, s.rand, la, lb, lc);
+      const uint32_t g = S.col_base[items[k].t] + (uint32_t)items[k].c;
+      for (uint32_t q = qstart[g]; q < qstart[g + 1]; ++q) {
+        const uint32_t pid = qpid[q];
+        if (batch_of[pid] < 0) {
+          batch_of[pid] = (int)batches.size();
+          const qm31 py = qsample_pts[pid].y;
+          batches.push_back({pid, qm_sub(qm_conj(py), py), qm_zero(), qm_zero(), {}});
+        }
+        Batch& b = batches[batch_of[pid]];
+        const qm31 v = qval[q], r = qrand[q];
         NumEntry ne{};
-        ne.col = gi;
-        ne.c = lc;
-        b->entries.push_back(ne);
-        b->B = qm_add(b->B, lb);
-        b->A = qm_add(b->A, la);
+        ne.col = gcols[gi];
+        ne.c = qm_mul(r, b.cy);
+        b.entries.push_back(ne);
+        b.sum_rv = qm_add(b.sum_rv, qm_mul(r, v));
+        b.sum_ra = qm_add(b.sum_ra, qm_mul(r, qm_sub(qm_conj(v), v)));
       }
     }
     if (!batches.empty()) {
@@ -803,13 +917,14 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       std::vector<qm31> Bs;
       std::vector<uint32_t*> outs;
       for (auto& b : batches) {
+        batch_of[b.pid] = -1;
         starts.push_back((uint32_t)ents.size());
         ents.insert(ents.end(), b.entries.begin(), b.entries.end());
-        Bs.push_back(b.B);
+        Bs.push_back(qm_sub(qm_mul(b.cy, b.sum_rv), qm_mul(qsample_pts[b.pid].y, b.sum_ra)));
         Acc a;
-        a.point = b.point;
+        a.point = qsample_pts[b.pid];
         a.log = ls;
-        a.first_linear = b.A;
+        a.first_linear = b.sum_ra;
         for (int k = 0; k < 4; ++k) {
           partial_bufs.emplace_back(&ctx, (size_t)1 << ls);
           outs.push_back(partial_bufs.back().p);
@@ -818,13 +933,18 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         acc_list.push_back(a);
       }
       starts.push_back((uint32_t)ents.size());
+      bool aligned = true;
+      for (auto* p : gcols) aligned &= ((uintptr_t)p & 15) == 0;
+      const NumEntry* d_ents = ctx.stage_vec(ents);
+      const uint32_t* d_starts = ctx.stage_vec(starts);
+      const qm31* d_bs = ctx.stage_vec(Bs);
+      uint32_t* const* d_outs = (uint32_t* const*)ctx.upload_vec(outs);
       KScope ks(ctx, K_NUMERATORS, (4.0 * gcols.size() + 16.0 * batches.size()) * (double)(1u << ls));
-      launch_accumulate_numerators(ls, (const uint32_t* const*)ctx.upload_vec(gcols), ctx.upload_vec(ents),
-                                   ctx.upload_vec(starts), ctx.upload_vec(Bs), (uint32_t)batches.size(),
-                                   (uint32_t* const*)ctx.upload_vec(outs), ctx.stream);
+      launch_accumulate_numerators(ls, d_ents, d_starts, d_bs, (uint32_t)batches.size(), d_outs, aligned, ctx.stream);
     }
     i = j;
   }
+  HMARK("q_numer");
   std::stable_sort(acc_list.begin(), acc_list.end(), [](const Acc& a, const Acc& b) { return point_less(a.point, b.point); });
   std::vector<QuotPoint> qpts;
   std::vector<QuotPart> qparts;
@@ -853,15 +973,17 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   const uint32_t* dxy = ctx.domain_xy(Lc);
   double qbytes = 24.0 * (double)(1u << Lc);
   for (auto& pa : qparts) qbytes += 16.0 * (double)(1u << pa.log_size);
-  const QuotPoint* dq = ctx.upload_vec(qpts);
+  const QuotPoint* dq = ctx.stage_vec(qpts);
   const QuotPart* dp = ctx.upload_vec(qparts);
   {
     KScope ksq(ctx, K_QUOTIENTS, qbytes);
     launch_quotients(Lc, dxy, dq, (uint32_t)qpts.size(), dp, first.mc(), ctx.stream);
   }
+  HMARK("q_launch");
   ctx.sync();
   ctx.reset_arena();
   partial_bufs.clear();
+  HMARK("q_gpu");
   stage_ms[4] = tm.lap();
 
   // ---------------- FRI commit ----------------
@@ -930,6 +1052,29 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     cur = std::move(nx);
   }
   if (layer != n_layers) throw HipError(4, "FRI layer count mismatch");
+  HMARK("fri_launch");
+  // Sanity check (prover/mod.rs:127-144): the composition value at the OODS point against the
+  // constraints evaluated from the sampled values. It only sets the status, so it runs on the
+  // host while the FRI kernels are in flight.
+  int status;
+  {
+    const size_t lt = S.n_trees() - 1;
+    qm31 lft = from_partial_evals(S.vals(lt, 0)[0], S.vals(lt, 1)[0], S.vals(lt, 2)[0], S.vals(lt, 3)[0]);
+    qm31 rgt = from_partial_evals(S.vals(lt, 4)[0], S.vals(lt, 5)[0], S.vals(lt, 6)[0], S.vals(lt, 7)[0]);
+    qm31 committed = qm_add(lft, qm_mul(cpq_repeated_double(oods, max_deg - 1).x, rgt));
+    qm31 acc = qm_zero();
+    for (auto& c : comps) {
+      PointEval e;
+      e.sv = &out.sampled;
+      e.c = &c;
+      e.acc = &acc;
+      e.rc = random_coeff;
+      e.dinv = qm_inv(coset_vanishing_qm31(max_deg, oods));
+      eval_at_point(e);
+    }
+    status = qm_eq(committed, acc) ? 0 : 2;
+  }
+  HMARK("sanity");
   {
     // one read-back: the transcript records and the last layer's evaluations
     uint32_t* dlast = drec + 12 * n_layers;
@@ -937,6 +1082,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     std::vector<uint32_t> hb(12 * (size_t)n_layers + 4 * last_n);
     ctx.d2h(hb.data(), drec, hb.size() * 4);
     ctx.reset_arena();
+    HMARK("fri_wait");
     auto replay = [&](DevTree& t, uint32_t l) {
       for (int i = 0; i < 8; ++i)
         for (int j = 0; j < 4; ++j) t.root[4 * i + j] = (uint8_t)(hb[12 * l + i] >> (8 * j));
@@ -966,11 +1112,13 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     out.last_poly = ordered;
     ch.mix_felts(out.last_poly);
   }
+  HMARK("fri_host");
   stage_ms[5] = tm.lap();
 
   // ---------------- proof of work ----------------
   out.pow = device_grind(ctx, ch.pow_prefix(cfg.pow_bits), cfg.pow_bits);
   ch.mix_u64(out.pow);
+  HMARK("pow");
   stage_ms[6] = tm.lap();
 
   // ---------------- decommit ----------------
@@ -1004,7 +1152,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     }
     tplans.push_back(plan_merkle(t, qpl, g));
   }
+  HMARK("dec_plan");
   auto vals = g.run(ctx);
+  HMARK("dec_gather");
   out.first = finish_fri_layer(fplans[0], vals);
   for (size_t i = 1; i < fplans.size(); ++i) out.inner.push_back(finish_fri_layer(fplans[i], vals));
   for (size_t t = 0; t < trees.size(); ++t) {
@@ -1014,24 +1164,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     out.commitments.push_back(trees[t].root);
   }
   ctx.reset_arena();
+  HMARK("dec_finish");
   stage_ms[7] = tm.lap();
 
-  // ---------------- sanity check (prover/mod.rs:127-144) ----------------
-  auto& last = out.sampled.back();
-  qm31 lft = from_partial_evals(last[0][0], last[1][0], last[2][0], last[3][0]);
-  qm31 rgt = from_partial_evals(last[4][0], last[5][0], last[6][0], last[7][0]);
-  qm31 committed = qm_add(lft, qm_mul(cpq_repeated_double(oods, max_deg - 1).x, rgt));
-  qm31 acc = qm_zero();
-  for (auto& c : comps) {
-    PointEval e;
-    e.sv = &out.sampled;
-    e.c = &c;
-    e.acc = &acc;
-    e.rc = random_coeff;
-    e.dinv = qm_inv(coset_vanishing_qm31(max_deg, oods));
-    eval_at_point(e);
-  }
-  return qm_eq(committed, acc) ? 0 : 2;
+  return status;
 }
 
 }  // namespace
@@ -1065,11 +1201,15 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   ProveResult r;
   Timer tm;
   Channel ch;
+  HMARK("begin");
   std::vector<DevTree> trees(2);
   ctx.ensure_twiddles(log_n + 2 + cfg.log_blowup);
   commit_tree(ctx, trees[0], cfg, ch);
+  HMARK("pre_tree");
   interpolate_columns(ctx, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
+  HMARK("tr_interp");
   commit_tree(ctx, trees[1], cfg, ch);
+  HMARK("tr_commit");
   r.stage_ms[0] = tm.lap();
   Component c;
   c.kind = AIR_POSEIDON;
@@ -1091,8 +1231,8 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     std::vector<uint32_t*> ip;
     for (auto& b : icols) ip.push_back(b.p);
     DevBuf sums(&ctx, 8, true);
-    const uint32_t* const* d_tr = (const uint32_t* const*)ctx.upload_vec(trace);
-    uint32_t* const* d_ip = (uint32_t* const*)ctx.upload_vec(ip);
+    const uint32_t* const* d_tr = (const uint32_t* const*)ctx.stage_vec(trace);
+    uint32_t* const* d_ip = (uint32_t* const*)ctx.stage_vec(ip);
     const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
     {
       KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
@@ -1100,6 +1240,7 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     }
     unsigned long long s[4];
     ctx.d2h(s, sums.p, 32);
+    HMARK("logup");
     c.claimed_sum = {m31_reduce64(s[0] % P), m31_reduce64(s[1] % P), m31_reduce64(s[2] % P), m31_reduce64(s[3] % P)};
     qm31 shift = c.cumsum_shift();
     std::vector<uint32_t*> last4(ip.end() - 4, ip.end());
@@ -1116,12 +1257,15 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     commit_tree(ctx, trees[2], cfg, ch);
   }
   icols.clear();
+  HMARK("inter_commit");
   r.stage_ms[1] = tm.lap();
   r.claimed_sum = c.claimed_sum;
   ProofParts parts;
   r.status = prove_ex(ctx, ch, trees, {c}, cfg, parts, r.stage_ms);
   r.proof = encode(cfg, parts);
   r.digest = ch.digest;
+  HMARK("encode");
+  g_marks.flush();
   return r;
 }
 

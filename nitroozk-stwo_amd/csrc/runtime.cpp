@@ -1,5 +1,6 @@
 // Host runtime of libstwo_hip (see runtime.h).
 #include "runtime.h"
+#include <cstring>
 
 #include <algorithm>
 
@@ -78,11 +79,12 @@ Ctx::~Ctx() {
   (void)hipStreamDestroy(stream);
 }
 
-void* Ctx::upload(const void* src, size_t bytes) {
+void* Ctx::stage(const void* src, size_t bytes) {
   size_t a = (bytes + 255) & ~(size_t)255;
   if (a == 0) a = 256;
   if (pinned_off + a > pinned_cap || arena_off + a > arena_cap) {
-    // rare large upload: dedicated allocation, synchronous copy
+    // rare large upload: queue what is staged, then a dedicated allocation and synchronous copy
+    commit_staged();
     uint8_t* d;
     HIP_CHECK(hipMalloc((void**)&d, a));
     HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream));
@@ -93,17 +95,30 @@ void* Ctx::upload(const void* src, size_t bytes) {
   }
   uint8_t* h = pinned + pinned_off;
   uint8_t* d = dev_arena + arena_off;
-  memcpy(h, src, bytes);
-  HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
-  uploads_in_flight = true;
+  std::memcpy(h, src, bytes);
   pinned_off += a;
   arena_off += a;
   return d;
 }
+void Ctx::commit_staged() {
+  if (pinned_off > staged_from) {
+    HIP_CHECK(hipMemcpyAsync(dev_arena + staged_from, pinned + staged_from, pinned_off - staged_from,
+                             hipMemcpyHostToDevice, stream));
+    uploads_in_flight = true;
+  }
+  staged_from = pinned_off;
+}
+void* Ctx::upload(const void* src, size_t bytes) {
+  void* d = stage(src, bytes);
+  commit_staged();
+  return d;
+}
 void Ctx::sync() {
+  commit_staged();
   HIP_CHECK(hipStreamSynchronize(stream));
   uploads_in_flight = false;
-  if (!pending.empty()) collect_timing();
+  // event pairs are folded into kstats in bulk (each elapsed-time query costs host time)
+  if (pending.size() >= 16384) collect_timing();
 }
 hipEvent_t Ctx::get_event() {
   if (!event_pool.empty()) { hipEvent_t e = event_pool.back(); event_pool.pop_back(); return e; }
@@ -127,10 +142,13 @@ void Ctx::collect_timing() {
 void Ctx::reset_arena() {
   // staging still read by queued copies (no sync since the last upload) is kept until a later call
   if (uploads_in_flight) return;
+  commit_staged();
+  if (uploads_in_flight) return;
   for (auto* p : overflow) (void)hipFree(p);
   overflow.clear();
   pinned_off = 0;
   arena_off = 0;
+  staged_from = 0;
 }
 void Ctx::d2h(void* dst, const void* src, size_t bytes) {
   HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));

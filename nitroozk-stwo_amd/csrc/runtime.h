@@ -68,7 +68,7 @@ struct Ctx {
   std::vector<hipEvent_t> event_pool;
   KernelStat kstats[K_NUM_CLASSES];
   hipEvent_t get_event();
-  void collect_timing();  // after a sync: folds finished events into kstats
+  void collect_timing();  // after a sync: folds finished event pairs into kstats
   hipStream_t stream = nullptr;
   std::string last_error;
   // twiddle tree for the half coset of CanonicCoset(tw_log): 2^(tw_log-1) words each
@@ -81,6 +81,7 @@ struct Ctx {
   uint8_t* dev_arena = nullptr;
   size_t arena_cap = 0, arena_off = 0;
   bool uploads_in_flight = false;
+  size_t staged_from = 0;  // start of staged bytes whose transfer is not queued yet
   std::vector<uint8_t*> overflow;  // arena overflow allocations, freed at reset
   // proof arena: HBM chunks kept across proofs; a proof bump-allocates its buffers from them
   std::vector<std::pair<uint8_t*, size_t>> bump_chunks;
@@ -94,6 +95,11 @@ struct Ctx {
   // Copies host bytes to device through pinned staging; valid on the stream until reset_arena().
   void* upload(const void* src, size_t bytes);
   template <class T> const T* upload_vec(const std::vector<T>& v) { return (const T*)upload(v.data(), v.size() * sizeof(T)); }
+  // Staging without the copy: the bytes land in pinned memory and the returned device address is
+  // valid once commit_staged() (or the next upload) has queued the transfer.
+  void* stage(const void* src, size_t bytes);
+  void commit_staged();
+  template <class T> const T* stage_vec(const std::vector<T>& v) { return (const T*)stage(v.data(), v.size() * sizeof(T)); }
   void sync();
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
   void ensure_twiddles(uint32_t log_domain);
@@ -132,12 +138,24 @@ struct KScope {
     if (c.timing) { a = c.get_event(); (void)hipEventRecord(a, c.stream); }
   }
   ~KScope() {
-    if (c.timing) {
+    if (c.timing && a) {
       hipEvent_t b = c.get_event();
       (void)hipEventRecord(b, c.stream);
       c.pending.push_back({cls, bytes, a, b});
     }
   }
+};
+
+// Several host arrays sent to the device in one transfer: add() stages each array and returns its
+// device address; commit() queues the copy and must precede the first launch that reads them.
+struct UploadBatch {
+  Ctx& c;
+  bool committed = false;
+  explicit UploadBatch(Ctx& ctx) : c(ctx) { c.commit_staged(); }
+  template <class T> const T* add(const std::vector<T>& v) { return (const T*)c.stage(v.data(), v.size() * sizeof(T)); }
+  const void* add_bytes(const void* p, size_t n) { return c.stage(p, n); }
+  void commit() { c.commit_staged(); committed = true; }
+  ~UploadBatch() { if (!committed) c.commit_staged(); }
 };
 
 // ---- host Blake2s and channel ------------------------------------------------------------
