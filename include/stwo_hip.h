@@ -188,7 +188,7 @@ int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* ctx, uint32_t* const* cols, uint32
  * of each class are bracketed with HIP events on the context stream; stats accumulate the event
  * time, the algorithmic HBM bytes and the launch count. Classes: 0 interpolate, 1 evaluate,
  * 2 merkle, 3 composition, 4 accumulate_numerators, 5 quotients, 6 fri_fold, 7 oods, 8 logup,
- * 9 grind, 10 misc. */
+ * 9 grind, 10 misc, 11 fri_tail (FRI layers of at most 2^12 values: Merkle, channel and fold). */
 int stwo_hip_set_kernel_timing(stwo_hip_ctx* ctx, int enable);
 int stwo_hip_kernel_stats(stwo_hip_ctx* ctx, uint32_t kernel_class, double* ms, double* bytes, uint64_t* launches);
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* ctx);

@@ -99,95 +99,158 @@ __device__ __forceinline__ uint32_t offset_prev_row(uint32_t row, uint32_t trace
   return bitrev(prev, eval_log);
 }
 
-// One Poseidon2 instance: rounds with their 142 constraints; returns the final state in s.
-__device__ __forceinline__ void poseidon_rounds_constraints(uint32_t* s, const uint32_t* const* __restrict__ cols,
-                                                            uint32_t row, const qm31* __restrict__ pows, qacc& acc) {
-  uint32_t ci = 16, k = 0;
-  for (uint32_t r = 0; r < 4; ++r) {
+// Column access: a table of column addresses (any layout), or columns at a constant stride from
+// a base (one allocation; no address loads in the kernel).
+struct ColTable {
+  const uint32_t* const* p;
+  __device__ __forceinline__ const uint32_t* col(uint32_t c) const { return p[c]; }
+};
+struct ColStrided {
+  const uint32_t* base;
+  size_t stride;
+  __device__ __forceinline__ const uint32_t* col(uint32_t c) const { return base + c * stride; }
+};
+
+// One Poseidon2 instance (columns c0 .. c0 + 157): rounds with their 142 constraints; s holds the
+// initial state (columns 0..15) on entry and the final state on return. The column values of the
+// next round are loaded while the current round is evaluated.
+__device__ __forceinline__ void psd_full_round(uint32_t* s, const uint32_t* m, const qm31* __restrict__ pows,
+                                               qacc& acc) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
-    external_matrix(s);
+  for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
+  external_matrix(s);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      uint32_t m = cols[ci + i][row];
-      qacc_mac(acc, pows[k + i], m31_sub(pow5(s[i]), m));
-      s[i] = m;
-    }
-    ci += 16;
-    k += 16;
+  for (int i = 0; i < 16; ++i) {
+    qacc_mac(acc, pows[i], m31_sub(pow5(s[i]), m[i]));
+    s[i] = m[i];
   }
-  for (uint32_t r = 0; r < 14; ++r) {
-    s[0] = m31_add(s[0], PSD_ROUND_CONST);
-    internal_matrix(s);
-    uint32_t m = cols[ci][row];
-    qacc_mac(acc, pows[k], m31_sub(pow5(s[0]), m));
-    s[0] = m;
-    ++ci;
-    ++k;
-  }
+}
+template <class Cols>
+__device__ __forceinline__ void poseidon_rounds_constraints(uint32_t* s, const Cols& cols, uint32_t c0, uint32_t row,
+                                                            const qm31* __restrict__ pows, qacc& acc) {
+  uint32_t m[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) m[i] = cols.col(c0 + 16 + i)[row];
+  // full rounds 0..3 read columns 16 + 16 r ..; the last one prefetches the partial-round columns 80..93
+#pragma unroll 1
   for (uint32_t r = 0; r < 4; ++r) {
+    uint32_t nx[16];
+    const uint32_t base = c0 + 32 + 16 * r;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s[i] = m31_add(s[i], PSD_ROUND_CONST);
-    external_matrix(s);
+    for (uint32_t i = 0; i < 16; ++i) nx[i] = (r < 3 || i < 14) ? cols.col(base + i)[row] : 0u;
+    psd_full_round(s, m, pows + 16 * r, acc);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      uint32_t m = cols[ci + i][row];
-      qacc_mac(acc, pows[k + i], m31_sub(pow5(s[i]), m));
-      s[i] = m;
+    for (int i = 0; i < 16; ++i) m[i] = nx[i];
+  }
+  // 14 partial rounds (columns 80..93), prefetching columns 94..109
+  {
+    uint32_t nx[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) nx[i] = cols.col(c0 + 94 + i)[row];
+#pragma unroll
+    for (int r = 0; r < 14; ++r) {
+      s[0] = m31_add(s[0], PSD_ROUND_CONST);
+      internal_matrix(s);
+      qacc_mac(acc, pows[64 + r], m31_sub(pow5(s[0]), m[r]));
+      s[0] = m[r];
     }
-    ci += 16;
-    k += 16;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = nx[i];
+  }
+  // full rounds 4..7 read columns 94 + 16 (r - 4) ..
+#pragma unroll 1
+  for (uint32_t r = 0; r < 4; ++r) {
+    uint32_t nx[16];
+    const uint32_t base = c0 + 110 + 16 * r;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) nx[i] = r < 3 ? cols.col(base + i)[row] : 0u;
+    psd_full_round(s, m, pows + 78 + 16 * r, acc);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = nx[i];
   }
 }
 
-__global__ void __launch_bounds__(256) poseidon_comp_kernel(uint32_t eval_log, uint32_t trace_log,
-                                                            const uint32_t* const* __restrict__ trace_cols,
-                                                            const uint32_t* const* __restrict__ inter_cols,
-                                                            const PoseidonParams* __restrict__ params,
+// A block covers 64 rows; wave w evaluates instances w, w + W, ... (W waves) of those rows, and
+// the W partial sums meet in LDS. Each instance contributes its 142 round constraints and its
+// LogUp constraint (finalize_logup_in_pairs: cumulative column rep minus column rep - 1, or for the
+// last instance minus the previous row's last column, plus the claimed-sum shift).
+constexpr uint32_t PSD_MAX_WAVES = 8;
+template <class Cols>
+__global__ void __launch_bounds__(512) poseidon_comp_kernel(uint32_t eval_log, uint32_t trace_log, Cols trace,
+                                                            Cols inter, const PoseidonParams* __restrict__ params,
                                                             const qm31* __restrict__ pows,
                                                             const uint32_t* __restrict__ denom_inv, Cols4 acc_out) {
-  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= (1u << eval_log)) return;
+  __shared__ uint32_t part[PSD_MAX_WAVES][4][64];
+  // the wave index is uniform across the wave: readfirstlane lets the compiler keep the instance
+  // loop and the constraint coefficients in scalar registers
+  const uint32_t lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), W = blockDim.x >> 6;
+  const uint32_t row = blockIdx.x * 64 + lane;
+  const bool active = row < (1u << eval_log);
   const PoseidonParams& p = *params;
   const uint32_t ipr = p.ipr;
   qacc acc = qacc_zero();
-  qm31 prev_col = qm_zero();
-  for (uint32_t rep = 0; rep < ipr; ++rep) {
-    const uint32_t* const* cols = trace_cols + rep * PSD_COLS_PER_REP;
-    uint32_t s[16];
+  if (active) {
+    for (uint32_t rep = w; rep < ipr; rep += W) {
+      const uint32_t c0 = rep * PSD_COLS_PER_REP;
+      uint32_t s[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s[i] = cols[i][row];
-    qm31 d0 = combine16(p, s);
-    poseidon_rounds_constraints(s, cols, row, pows + rep * 142, acc);
-    qm31 d1 = combine16(p, s);
-    // pair fraction 1/d0 - 1/d1 = (d1 - d0) / (d0 d1)
-    qm31 num = qm_sub(d1, d0), den = qm_mul(d0, d1);
-    const uint32_t* const* ic = inter_cols + 4 * rep;
-    qm31 cur = {ic[0][row], ic[1][row], ic[2][row], ic[3][row]};
-    qm31 diff;
-    if (rep + 1 < ipr) {
-      diff = qm_sub(cur, prev_col);
-      prev_col = cur;
-    } else {
-      uint32_t pr = offset_prev_row(row, trace_log, eval_log);
-      qm31 prow = {ic[0][pr], ic[1][pr], ic[2][pr], ic[3][pr]};
-      diff = qm_add(qm_sub(qm_sub(cur, prow), prev_col), p.shift);
+      for (int i = 0; i < 16; ++i) s[i] = trace.col(c0 + i)[row];
+      const qm31 d0 = combine16(p, s);
+      poseidon_rounds_constraints(s, trace, c0, row, pows + rep * 142, acc);
+      const qm31 d1 = combine16(p, s);
+      // pair fraction 1/d0 - 1/d1 = (d1 - d0) / (d0 d1)
+      const qm31 num = qm_sub(d1, d0), den = qm_mul(d0, d1);
+      const uint32_t i0 = 4 * rep;
+      const qm31 cur = {inter.col(i0)[row], inter.col(i0 + 1)[row], inter.col(i0 + 2)[row], inter.col(i0 + 3)[row]};
+      qm31 prev_col = qm_zero();
+      if (rep > 0)
+        prev_col = {inter.col(i0 - 4)[row], inter.col(i0 - 3)[row], inter.col(i0 - 2)[row], inter.col(i0 - 1)[row]};
+      qm31 diff;
+      if (rep + 1 < ipr) {
+        diff = qm_sub(cur, prev_col);
+      } else {
+        const uint32_t pr = offset_prev_row(row, trace_log, eval_log);
+        const qm31 prow = {inter.col(i0)[pr], inter.col(i0 + 1)[pr], inter.col(i0 + 2)[pr], inter.col(i0 + 3)[pr]};
+        diff = qm_add(qm_sub(qm_sub(cur, prow), prev_col), p.shift);
+      }
+      const qm31 cons = qm_sub(qm_mul(diff, den), num);
+      acc_add_qm(acc, qm_mul(pows[ipr * 142 + rep], cons));
     }
-    qm31 cons = qm_sub(qm_mul(diff, den), num);
-    acc_add_qm(acc, qm_mul(pows[ipr * 142 + rep], cons));
   }
-  qm31 r = qm_mul_m31(qacc_reduce(acc), denom_inv[row >> trace_log]);
-  qm31 prev = {acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]};
-  r = qm_add(prev, r);
-  acc_out.c[0][row] = r.a; acc_out.c[1][row] = r.b; acc_out.c[2][row] = r.c; acc_out.c[3][row] = r.d;
+  const qm31 r = qacc_reduce(acc);
+  part[w][0][lane] = r.a;
+  part[w][1][lane] = r.b;
+  part[w][2][lane] = r.c;
+  part[w][3][lane] = r.d;
+  __syncthreads();
+  if (w == 0 && active) {
+    qm31 tot = qm_zero();
+    for (uint32_t v = 0; v < W; ++v)
+      tot = qm_add(tot, qm31{part[v][0][lane], part[v][1][lane], part[v][2][lane], part[v][3][lane]});
+    tot = qm_mul_m31(tot, denom_inv[row >> trace_log]);
+    const qm31 prev = {acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]};
+    const qm31 res = qm_add(prev, tot);
+    acc_out.c[0][row] = res.a; acc_out.c[1][row] = res.b; acc_out.c[2][row] = res.c; acc_out.c[3][row] = res.d;
+  }
 }
 
 void launch_poseidon_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* trace_cols,
-                                 const uint32_t* const* inter_cols, const PoseidonParams* params, const qm31* pows,
-                                 const uint32_t* denom_inv, Cols4 acc, hipStream_t stream) {
+                                 const uint32_t* const* inter_cols, const PoseidonParams* params, uint32_t ipr,
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream) {
   uint32_t n = 1u << eval_log;
-  hipLaunchKernelGGL(poseidon_comp_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, eval_log, trace_log,
-                     trace_cols, inter_cols, params, pows, denom_inv, acc);
+  uint32_t waves = ipr < PSD_MAX_WAVES ? ipr : PSD_MAX_WAVES;
+  hipLaunchKernelGGL(poseidon_comp_kernel<ColTable>, dim3((n + 63) / 64), dim3(64 * waves), 0, stream, eval_log,
+                     trace_log, ColTable{trace_cols}, ColTable{inter_cols}, params, pows, denom_inv, acc);
+}
+void launch_poseidon_composition_strided(uint32_t eval_log, uint32_t trace_log, const uint32_t* trace_base,
+                                         const uint32_t* inter_base, size_t stride, const PoseidonParams* params,
+                                         uint32_t ipr, const qm31* pows, const uint32_t* denom_inv, Cols4 acc,
+                                         hipStream_t stream) {
+  uint32_t n = 1u << eval_log;
+  uint32_t waves = ipr < PSD_MAX_WAVES ? ipr : PSD_MAX_WAVES;
+  hipLaunchKernelGGL(poseidon_comp_kernel<ColStrided>, dim3((n + 63) / 64), dim3(64 * waves), 0, stream, eval_log,
+                     trace_log, ColStrided{trace_base, stride}, ColStrided{inter_base, stride}, params, pows,
+                     denom_inv, acc);
 }
 
 // ---- trace generation -----------------------------------------------------------------------

@@ -359,7 +359,7 @@ int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* 
       std::vector<const uint32_t*> iv(interaction, interaction + n_interaction);
       launch_poseidon_composition(eval_log, trace_log, (const uint32_t* const*)c.upload_vec(tv),
                                   (const uint32_t* const*)c.upload_vec(iv), (const PoseidonParams*)c.upload(&pp, sizeof(pp)),
-                                  c.upload_vec(pows), c.upload_vec(dinv), a4, c.stream);
+                                  pp.ipr, c.upload_vec(pows), c.upload_vec(dinv), a4, c.stream);
     } else {
       throw HipError(5, "no device kernel for this component");
     }

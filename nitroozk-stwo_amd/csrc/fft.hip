@@ -122,6 +122,125 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
   }
 }
 
+// ---- low stages of a 2^S-element block in one pass (11 <= S <= 15) ---------------------------
+// A workgroup of 2^(S-5) threads owns one block of 2^S consecutive elements of one column: it
+// loads the block into LDS, runs stages [0, S) as rounds of up to 5 stages (each thread holds 32
+// elements in registers per round), and stores the block. LDS word e lives at e + e/32, which
+// keeps every round's accesses free of bank conflicts.
+__device__ __forceinline__ uint32_t lds_pad(uint32_t e) { return e + (e >> 5); }
+
+template <bool INV, int S, int A, int R>
+__device__ __forceinline__ void low_round(uint32_t* lds, uint32_t gbase, const uint32_t* __restrict__ tree,
+                                          uint32_t tree_len, uint32_t log_n) {
+  constexpr uint32_t T = 1u << (S - 5), G = 1u << (5 - R), Q = 1u << R;
+  const uint32_t t = threadIdx.x;
+  uint32_t v[32];
+#pragma unroll
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t rho = t + g * T;
+    const uint32_t e0 = ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1));
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) v[g * Q + q] = lds[lds_pad(e0 | (q << A))];
+  }
+#pragma unroll
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t rho = t + g * T;
+    const uint32_t idx_base = gbase | ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1));
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+      const int k = INV ? kk : R - 1 - kk;
+      const uint32_t s = A + k;
+#pragma unroll
+      for (uint32_t u = 0; u < (Q >> (k + 1)); ++u) {
+        const uint32_t tw = stage_twiddle(tree, tree_len, log_n, s, idx_base | ((u << (k + 1)) << A));
+#pragma unroll
+        for (uint32_t l = 0; l < (1u << k); ++l) {
+          const uint32_t q = (u << (k + 1)) | l;
+          bfly<INV>(v[g * Q + q], v[g * Q + (q | (1u << k))], tw);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t rho = t + g * T;
+    const uint32_t e0 = ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1));
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) lds[lds_pad(e0 | (q << A))] = v[g * Q + q];
+  }
+  __syncthreads();
+}
+
+template <bool INV, int S>
+__global__ void __launch_bounds__(1u << (S - 5)) fft_low_kernel(const uint32_t* const* __restrict__ src,
+                                                                uint32_t* const* __restrict__ dst, uint32_t log_n,
+                                                                uint32_t in_mask, const uint32_t* __restrict__ tree,
+                                                                uint32_t tree_len, uint32_t scale) {
+  constexpr uint32_t N = 1u << S, T = 1u << (S - 5);
+  __shared__ uint32_t lds[N + (N >> 5)];
+  const uint32_t* __restrict__ in = src[blockIdx.y];
+  uint32_t* __restrict__ out = dst[blockIdx.y];
+  const uint32_t gbase = blockIdx.x << S;
+  const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  if (vec) {
+    // 8 independent 16-byte loads per thread, all in flight before the first LDS write
+    uint4 w[8];
+#pragma unroll
+    for (uint32_t m = 0; m < 8; ++m) {
+      const uint32_t i = 4 * (threadIdx.x + m * T);
+      w[m] = *reinterpret_cast<const uint4*>(in + ((gbase | i) & in_mask));
+    }
+#pragma unroll
+    for (uint32_t m = 0; m < 8; ++m) {
+      const uint32_t i = 4 * (threadIdx.x + m * T);
+      lds[lds_pad(i)] = w[m].x;
+      lds[lds_pad(i + 1)] = w[m].y;
+      lds[lds_pad(i + 2)] = w[m].z;
+      lds[lds_pad(i + 3)] = w[m].w;
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < N; i += T) lds[lds_pad(i)] = in[(gbase | i) & in_mask];
+  }
+  __syncthreads();
+  if (INV) {
+    low_round<INV, S, 0, 5>(lds, gbase, tree, tree_len, log_n);
+    low_round<INV, S, 5, 5>(lds, gbase, tree, tree_len, log_n);
+    low_round<INV, S, 10, S - 10>(lds, gbase, tree, tree_len, log_n);
+  } else {
+    low_round<INV, S, 10, S - 10>(lds, gbase, tree, tree_len, log_n);
+    low_round<INV, S, 5, 5>(lds, gbase, tree, tree_len, log_n);
+    low_round<INV, S, 0, 5>(lds, gbase, tree, tree_len, log_n);
+  }
+  if (vec) {
+#pragma unroll
+    for (uint32_t m = 0; m < 8; ++m) {
+      const uint32_t i = 4 * (threadIdx.x + m * T);
+      uint4 w = make_uint4(lds[lds_pad(i)], lds[lds_pad(i + 1)], lds[lds_pad(i + 2)], lds[lds_pad(i + 3)]);
+      if (scale) w = make_uint4(m31_mul(w.x, scale), m31_mul(w.y, scale), m31_mul(w.z, scale), m31_mul(w.w, scale));
+      *reinterpret_cast<uint4*>(out + (gbase | i)) = w;
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < N; i += T) {
+      const uint32_t v = lds[lds_pad(i)];
+      out[gbase | i] = scale ? m31_mul(v, scale) : v;
+    }
+  }
+}
+
+template <bool INV>
+static void launch_low(uint32_t S, uint32_t n_cols, const uint32_t* const* src, uint32_t* const* dst, uint32_t log_n,
+                       uint32_t in_mask, const uint32_t* tree, uint32_t tree_len, uint32_t scale,
+                       hipStream_t stream) {
+  const dim3 grid(1u << (log_n - S), n_cols), block(1u << (S - 5));
+  switch (S) {
+    case 11: hipLaunchKernelGGL((fft_low_kernel<INV, 11>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
+    case 12: hipLaunchKernelGGL((fft_low_kernel<INV, 12>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
+    case 13: hipLaunchKernelGGL((fft_low_kernel<INV, 13>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
+    case 14: hipLaunchKernelGGL((fft_low_kernel<INV, 14>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
+    default: hipLaunchKernelGGL((fft_low_kernel<INV, 15>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
+  }
+}
+
 // Host-side pass planning shared by interpolate and evaluate.
 struct FftPass { uint32_t e, S, c; };
 static std::vector<FftPass> plan_passes(uint32_t k, uint32_t K) {
@@ -147,30 +266,50 @@ static std::vector<FftPass> plan_passes(uint32_t k, uint32_t K) {
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
                 uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream) {
   if (n_cols == 0) return;
-  uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
-  auto passes = plan_passes(k, log_n);
-  uint32_t inv_n = inverse ? m31_inv(1u << log_n) : 0;
-  if (inverse) {
-    for (size_t i = 0; i < passes.size(); ++i) {
-      auto& ps = passes[i];
-      uint32_t tiles = 1u << (log_n - ps.S - ps.c);
-      uint32_t tile_n = 1u << (ps.S + ps.c);
-      uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
-      bool last = i + 1 == passes.size();
-      hipLaunchKernelGGL(fft_pass_kernel<true>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream,
-                         i == 0 ? src : (const uint32_t* const*)dst, dst, log_n, (1u << log_n) - 1, ps.e, ps.S, ps.c,
-                         tree, tree_len, last ? inv_n : 0u);
+  const uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
+  const uint32_t inv_n = inverse ? m31_inv(1u << log_n) : 0;
+  auto run_pass = [&](const FftPass& ps, const uint32_t* const* from, uint32_t mask, uint32_t scale) {
+    const uint32_t tiles = 1u << (log_n - ps.S - ps.c), tile_n = 1u << (ps.S + ps.c);
+    const uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
+    if (inverse)
+      hipLaunchKernelGGL(fft_pass_kernel<true>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream, from, dst,
+                         log_n, mask, ps.e, ps.S, ps.c, tree, tree_len, scale);
+    else
+      hipLaunchKernelGGL(fft_pass_kernel<false>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream, from, dst,
+                         log_n, mask, ps.e, ps.S, ps.c, tree, tree_len, scale);
+  };
+  const uint32_t full = (1u << log_n) - 1, in_mask = inverse ? full : (1u << log_in) - 1;
+  if (k >= 11) {
+    // stages [0, S) in one pass of 2^S-element blocks; stages [S, k) in strided passes
+    const uint32_t S = k < 15 ? k : 15;
+    std::vector<FftPass> high;
+    for (uint32_t e = S; e < k;) {
+      uint32_t st = k - e < 8 ? k - e : 8, c = 12 - st;
+      if (c > e) c = e;
+      high.push_back({e, st, c});
+      e += st;
     }
+    if (inverse) {
+      launch_low<true>(S, n_cols, src, dst, log_n, full, tree, tree_len, high.empty() ? inv_n : 0u, stream);
+      for (size_t i = 0; i < high.size(); ++i)
+        run_pass(high[i], (const uint32_t* const*)dst, full, i + 1 == high.size() ? inv_n : 0u);
+    } else {
+      for (size_t ii = high.size(); ii-- > 0;)
+        run_pass(high[ii], ii + 1 == high.size() ? src : (const uint32_t* const*)dst,
+                 ii + 1 == high.size() ? in_mask : full, 0u);
+      launch_low<false>(S, n_cols, high.empty() ? src : (const uint32_t* const*)dst, dst, log_n,
+                        high.empty() ? in_mask : full, tree, tree_len, 0u, stream);
+    }
+    return;
+  }
+  auto passes = plan_passes(k, log_n);
+  if (inverse) {
+    for (size_t i = 0; i < passes.size(); ++i)
+      run_pass(passes[i], i == 0 ? src : (const uint32_t* const*)dst, full, i + 1 == passes.size() ? inv_n : 0u);
   } else {
     for (size_t ii = passes.size(); ii-- > 0;) {
-      auto& ps = passes[ii];
-      uint32_t tiles = 1u << (log_n - ps.S - ps.c);
-      uint32_t tile_n = 1u << (ps.S + ps.c);
-      uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
-      bool first = ii + 1 == passes.size();
-      hipLaunchKernelGGL(fft_pass_kernel<false>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream,
-                         first ? src : (const uint32_t* const*)dst, dst, log_n,
-                         first ? (1u << log_in) - 1 : (1u << log_n) - 1, ps.e, ps.S, ps.c, tree, tree_len, 0u);
+      const bool first = ii + 1 == passes.size();
+      run_pass(passes[ii], first ? src : (const uint32_t* const*)dst, first ? in_mask : full, 0u);
     }
   }
 }

@@ -30,6 +30,11 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
 void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream);
 
+// FRI layers of at most 2^12 values (l_start <= 12) in one workgroup: Merkle trees, device
+// channel and fold_line per layer; see merkle.hip for the layouts of vals, nodes and rec.
+void launch_fri_tail(uint32_t l_start, uint32_t last_log, uint32_t* vals, uint32_t* nodes, uint32_t* state,
+                     uint32_t* rec, const uint32_t* itree, uint32_t tree_len, hipStream_t stream);
+
 // ops.hip
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                   uint32_t tree_len, hipStream_t stream);
@@ -76,9 +81,16 @@ void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx,
 void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* cols, uint32_t n_cols,
                                  const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
 struct PoseidonParams { qm31 z; qm31 alpha_pow[16]; qm31 shift; uint32_t ipr; uint32_t pad[3]; };
+// ipr (instances per row, also in params) sets the launch shape: min(ipr, 8) waves per 64 rows.
 void launch_poseidon_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* trace_cols,
-                                 const uint32_t* const* inter_cols, const PoseidonParams* params, const qm31* pows,
-                                 const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
+                                 const uint32_t* const* inter_cols, const PoseidonParams* params, uint32_t ipr,
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
+// The same with trace column c at trace_base + c stride and interaction column c at
+// inter_base + c stride (column blocks of one allocation each).
+void launch_poseidon_composition_strided(uint32_t eval_log, uint32_t trace_log, const uint32_t* trace_base,
+                                         const uint32_t* inter_base, size_t stride, const PoseidonParams* params,
+                                         uint32_t ipr, const qm31* pows, const uint32_t* denom_inv, Cols4 acc,
+                                         hipStream_t stream);
 
 // Trace generation and the Poseidon LogUp interaction trace.
 void launch_gen_wide_fib(uint32_t log_n, uint32_t* const* cols, uint32_t n_cols, uint32_t mode, uint64_t seed,

@@ -361,26 +361,42 @@ uint32_t mask_offsets(const Component& c, size_t t, size_t i, long* offs) {
   return 1;
 }
 
+// Whether column k is at v[0] + k stride for every k (one allocation, no address table needed).
+bool is_strided(const std::vector<const uint32_t*>& v, size_t stride) {
+  for (size_t k = 1; k < v.size(); ++k)
+    if (v[k] != v[0] + k * stride) return false;
+  return !v.empty();
+}
+
 void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const std::vector<uint32_t>& logs,
-                   uint32_t target_log, std::vector<DevBuf>& out) {
-  // evaluates every column onto CanonicCoset(target_log), grouping equal sizes into one batch
+                   uint32_t target_log, std::vector<DevBuf>& out, std::vector<DevBuf>& keep, bool on_side = false) {
+  // evaluates every column onto CanonicCoset(target_log), grouping equal sizes into one batch; the
+  // columns of a batch are consecutive in one allocation (kept in `keep`; out holds views)
   std::map<uint32_t, std::vector<size_t>> by;
   for (size_t i = 0; i < coeffs.size(); ++i) by[logs[i]].push_back(i);
   out.resize(coeffs.size());
   ctx.ensure_twiddles(target_log);
+  const size_t n = (size_t)1 << target_log;
   for (auto& [lg, idx] : by) {
     std::vector<const uint32_t*> src;
     std::vector<uint32_t*> dst;
-    for (size_t i : idx) {
-      out[i] = DevBuf(&ctx, (size_t)1 << target_log);
+    keep.emplace_back(&ctx, idx.size() * n);
+    uint32_t* blk = keep.back().p;
+    for (size_t j = 0; j < idx.size(); ++j) {
+      const size_t i = idx[j];
+      out[i] = DevBuf::view(&ctx, blk + j * n, n);
       src.push_back(coeffs[i]);
       dst.push_back(out[i].p);
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
-    KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)));
-    launch_fft(false, d_src, d_dst,
-               (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), ctx.stream);
+    hipStream_t st = ctx.stream;
+    if (on_side) {
+      ctx.fork();  // the side stream starts after the upload and everything queued before it
+      st = ctx.side;
+    }
+    KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)), st);
+    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), st);
   }
 }
 
@@ -657,15 +673,21 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     if (acc.empty()) for (int k = 0; k < 4; ++k) acc.emplace_back(&ctx, (size_t)1 << E, true);
     Cols4 a4 = {{acc[0].p, acc[1].p, acc[2].p, acc[3].p}};
     // columns on the evaluation domain: committed evaluations when sizes match, else extended
-    std::vector<DevBuf> ext_t, ext_i;
+    std::vector<DevBuf> ext_t, ext_i, ext_keep;
+    bool join_side = false;
     auto cols_on_domain = [&](const DevTree& t, size_t start, size_t n, std::vector<DevBuf>& ext) {
       std::vector<const uint32_t*> r(n);
       bool need = false;
       for (size_t i = 0; i < n; ++i) need |= t.eval_logs[start + i] != E;
       if (!need) { for (size_t i = 0; i < n; ++i) r[i] = t.evals[start + i]; return r; }
+      if (t.ext_log == E && t.ext.size() >= start + n) {
+        for (size_t i = 0; i < n; ++i) r[i] = t.ext[start + i].p;
+        join_side = true;
+        return r;
+      }
       std::vector<const uint32_t*> cf(t.coeffs.begin() + start, t.coeffs.begin() + start + n);
       std::vector<uint32_t> lg(t.coeff_logs.begin() + start, t.coeff_logs.begin() + start + n);
-      evaluate_into(ctx, cf, lg, E, ext);
+      evaluate_into(ctx, cf, lg, E, ext, ext_keep);
       for (size_t i = 0; i < n; ++i) r[i] = ext[i].p;
       return r;
     };
@@ -679,6 +701,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     const qm31* d_pows = ctx.stage_vec(cp);
     const uint32_t* const* d_t = (const uint32_t* const*)ctx.stage_vec(tcols);
     double cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
+    if (join_side) ctx.join();
     if (c.kind == AIR_WIDE_FIB) {
       ctx.commit_staged();
       KScope ks(ctx, K_COMPOSITION, cbytes);
@@ -693,7 +716,12 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       const uint32_t* const* d_i = (const uint32_t* const*)ctx.stage_vec(icols);
       const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
       KScope ks(ctx, K_COMPOSITION, cbytes);
-      launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, d_pows, d_dinv, a4, ctx.stream);
+      const size_t stride = (size_t)1 << E;
+      if (is_strided(tcols, stride) && is_strided(icols, stride))
+        launch_poseidon_composition_strided(E, c.log_n, tcols[0], icols[0], stride, d_pp, c.ipr, d_pows, d_dinv, a4,
+                                            ctx.stream);
+      else
+        launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, c.ipr, d_pows, d_dinv, a4, ctx.stream);
     }
   }
   // finalize: lift smaller accumulations into the largest, then interpolate 4 coordinate columns
@@ -1019,11 +1047,40 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     launch_channel_mix_root_draw(dstate, f.tree.layers[0].p, drec + 12 * layer, ctx.stream);
     return (const qm31*)(drec + 12 * layer + 8);
   };
+  // Layers of at most 2^12 values (fold step 1) are committed and folded by one workgroup
+  // (launch_fri_tail); their evaluations and Merkle trees live in one block with the layout
+  // that kernel documents.
+  const uint32_t FRI_TAIL_MAX_LOG = 12;
+  const uint32_t l_start = std::min(FRI_TAIL_MAX_LOG, Lc - 1);
+  const bool tail = cfg.fold_step == 1 && l_start > last_log;
+  size_t tv_words = 0, tn_words = 0;
+  if (tail) {
+    tv_words = 4 * (((size_t)2 << l_start) - ((size_t)1 << last_log));
+    for (uint32_t l = last_log + 1; l <= l_start; ++l) tn_words += 8 * (((size_t)2 << l) - 1);
+  }
+  DevBuf tail_buf(&ctx, tail ? tv_words + tn_words : 0);
+  auto tail_vals = [&](uint32_t l) { return tail_buf.p + 4 * (((size_t)2 << l_start) - ((size_t)2 << l)); };
+  auto tail_nodes = [&](uint32_t l) {
+    size_t off = tv_words;
+    for (uint32_t m = l + 1; m <= l_start; ++m) off += 8 * (((size_t)2 << m) - 1);
+    return tail_buf.p + off;
+  };
+  auto tail_col = [&](uint32_t l) {
+    FriCol f;
+    f.log = l;
+    for (int k = 0; k < 4; ++k) f.c.push_back(DevBuf::view(&ctx, tail_vals(l) + ((size_t)k << l), (size_t)1 << l));
+    return f;
+  };
   const qm31* alpha0 = mix_and_draw(first, 0);
   std::vector<FriCol> inner;
   FriCol cur;
-  cur.log = Lc - 1;
-  for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log, true);
+  if (tail && l_start == Lc - 1) {
+    cur = tail_col(Lc - 1);
+    for (int k = 0; k < 4; ++k) launch_fill_zero(cur.c[k].p, cur.c[k].n, ctx.stream);
+  } else {
+    cur.log = Lc - 1;
+    for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log, true);
+  }
   {
     KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 32.0 * (double)(1u << (Lc - 1)));
     launch_fold_circle_into_line_dev(Lc, first.cc(), cur.mc(), alpha0, ctx.itw.p, ctx.tree_len(), ctx.stream);
@@ -1033,8 +1090,12 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     FriCol tmp;
     for (uint32_t st = 0; st < fs; ++st) {
       FriCol nx;
-      nx.log = s->log - 1;
-      for (int k = 0; k < 4; ++k) nx.c.emplace_back(&ctx, (size_t)1 << nx.log);
+      if (tail && s->log - 1 == l_start) {
+        nx = tail_col(l_start);
+      } else {
+        nx.log = s->log - 1;
+        for (int k = 0; k < 4; ++k) nx.c.emplace_back(&ctx, (size_t)1 << nx.log);
+      }
       KScope ks(ctx, K_FRI_FOLD, 24.0 * (double)(1u << s->log));
       launch_fold_line_dev(s->log, s->cc(), nx.mc(), a, st, ctx.itw.p, ctx.tree_len(), ctx.stream);
       tmp = std::move(nx);
@@ -1043,13 +1104,32 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     return tmp;
   };
   uint32_t layer = 1;
-  while (cur.log != last_log) {
+  while (cur.log != last_log && !(tail && cur.log == l_start)) {
     uint32_t fs = cur.log > last_log + cfg.fold_step ? cfg.fold_step : cur.log - last_log;
     cur.fold_step = fs;
     const qm31* a = mix_and_draw(cur, layer++);
     FriCol nx = fold_layer(cur, fs, a);
     inner.push_back(std::move(cur));
     cur = std::move(nx);
+  }
+  if (tail) {
+    {
+      double tb = 0;
+      for (uint32_t l = l_start; l > last_log; --l) tb += 16.0 * (1u << l) + 32.0 * (2u << l) + 8.0 * (1u << l);
+      KScope ks(ctx, K_FRI_TAIL, tb);
+      launch_fri_tail(l_start, last_log, tail_vals(l_start), tail_nodes(l_start), dstate, drec + 12 * layer,
+                      ctx.itw.p, ctx.tree_len(), ctx.stream);
+    }
+    for (uint32_t l = l_start; l > last_log; --l, ++layer) {
+      FriCol f = tail_col(l);
+      f.fold_step = 1;
+      f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
+      f.tree.eval_logs.assign(4, l);
+      for (uint32_t j = 0; j <= l; ++j)
+        f.tree.layers.push_back(DevBuf::view(&ctx, tail_nodes(l) + 8 * (((size_t)1 << j) - 1), (size_t)8 << j));
+      inner.push_back(std::move(f));
+    }
+    cur = tail_col(last_log);
   }
   if (layer != n_layers) throw HipError(4, "FRI layer count mismatch");
   HMARK("fri_launch");
@@ -1210,6 +1290,13 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   HMARK("tr_interp");
   commit_tree(ctx, trees[1], cfg, ch);
   HMARK("tr_commit");
+  // The extension of the trace to the composition domain needs only its coefficients: it runs on
+  // the side stream, overlapping the latency-bound LogUp interaction phase.
+  {
+    const uint32_t E = log_n + 2;
+    evaluate_into(ctx, trees[1].coeffs, trees[1].coeff_logs, E, trees[1].ext, trees[1].owned, true);
+    trees[1].ext_log = E;
+  }
   r.stage_ms[0] = tm.lap();
   Component c;
   c.kind = AIR_POSEIDON;

@@ -41,6 +41,10 @@ struct DevTree {
   std::vector<DevBuf> owned;
   std::vector<DevBuf> layers;  // layers[k]: 2^k nodes of 8 words; layers[0] is the root
   Hash root{};
+  // Evaluations of every column on CanonicCoset(ext_log), computed on the side stream ahead of
+  // the composition (empty when not precomputed); readers join the side stream first.
+  std::vector<DevBuf> ext;
+  uint32_t ext_log = 0;
   uint32_t max_log() const { return (uint32_t)layers.size() - 1; }
 };
 

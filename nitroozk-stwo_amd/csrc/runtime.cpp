@@ -55,6 +55,9 @@ DevBuf::~DevBuf() { reset(); }
 Ctx::Ctx(int dev) : device(dev) {
   HIP_CHECK(hipSetDevice(dev));
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   hipMemPool_t pool;
   HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
   uint64_t thr = UINT64_MAX;  // keep freed blocks cached in the pool
@@ -65,6 +68,7 @@ Ctx::Ctx(int dev) : device(dev) {
   HIP_CHECK(hipMalloc((void**)&dev_arena, arena_cap));
 }
 Ctx::~Ctx() {
+  (void)hipStreamSynchronize(side);
   (void)hipStreamSynchronize(stream);
   for (auto& p : pending) { event_pool.push_back(p.a); event_pool.push_back(p.b); }
   for (auto e : event_pool) (void)hipEventDestroy(e);
@@ -76,6 +80,9 @@ Ctx::~Ctx() {
   for (auto& ch : bump_chunks) (void)hipFree(ch.first);
   if (dev_arena) (void)hipFree(dev_arena);
   if (pinned) (void)hipHostFree(pinned);
+  (void)hipEventDestroy(ev_fork);
+  (void)hipEventDestroy(ev_join);
+  (void)hipStreamDestroy(side);
   (void)hipStreamDestroy(stream);
 }
 
@@ -113,8 +120,25 @@ void* Ctx::upload(const void* src, size_t bytes) {
   commit_staged();
   return d;
 }
+void Ctx::fork() {
+  commit_staged();
+  side_busy = true;
+  HIP_CHECK(hipEventRecord(ev_fork, stream));
+  HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+}
+void Ctx::join() {
+  HIP_CHECK(hipEventRecord(ev_join, side));
+  HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
+}
+void Ctx::sync_main() {
+  commit_staged();
+  HIP_CHECK(hipStreamSynchronize(stream));
+  uploads_in_flight = false;
+}
 void Ctx::sync() {
   commit_staged();
+  HIP_CHECK(hipStreamSynchronize(side));
+  side_busy = false;
   HIP_CHECK(hipStreamSynchronize(stream));
   uploads_in_flight = false;
   // event pairs are folded into kstats in bulk (each elapsed-time query costs host time)
@@ -141,7 +165,8 @@ void Ctx::collect_timing() {
 }
 void Ctx::reset_arena() {
   // staging still read by queued copies (no sync since the last upload) is kept until a later call
-  if (uploads_in_flight) return;
+  // (and, while forked work may still read the arena, until a full sync)
+  if (uploads_in_flight || side_busy) return;
   commit_staged();
   if (uploads_in_flight) return;
   for (auto* p : overflow) (void)hipFree(p);
@@ -152,7 +177,7 @@ void Ctx::reset_arena() {
 }
 void Ctx::d2h(void* dst, const void* src, size_t bytes) {
   HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
-  sync();
+  sync_main();
 }
 
 // Twiddle tree of the half coset of CanonicCoset(log_domain) (prover/backend/cpu/circle.rs:274-291):

@@ -41,6 +41,15 @@ struct DevBuf {
   DevBuf& operator=(DevBuf&& o) noexcept;
   ~DevBuf();
   void reset();
+  // A non-owning view of device memory owned elsewhere (a proof arena block).
+  static DevBuf view(Ctx* c, uint32_t* p, size_t words) {
+    DevBuf b;
+    b.p = p;
+    b.n = words;
+    b.ctx = c;
+    b.owned = false;
+    return b;
+  }
 };
 
 using Hash = std::array<uint8_t, 32>;
@@ -52,7 +61,7 @@ struct StageTimes {
 // Kernel classes timed with HIP events when timing is enabled (stwo_hip_set_kernel_timing).
 enum KernelClass {
   K_INTERPOLATE = 0, K_EVALUATE, K_MERKLE, K_COMPOSITION, K_NUMERATORS, K_QUOTIENTS, K_FRI_FOLD, K_OODS,
-  K_LOGUP, K_GRIND, K_MISC, K_NUM_CLASSES
+  K_LOGUP, K_GRIND, K_MISC, K_FRI_TAIL, K_NUM_CLASSES
 };
 struct KernelStat {
   double ms = 0;
@@ -70,6 +79,13 @@ struct Ctx {
   hipEvent_t get_event();
   void collect_timing();  // after a sync: folds finished event pairs into kstats
   hipStream_t stream = nullptr;
+  // Second stream for independent work that overlaps the main stream: fork() orders it after
+  // everything queued on the main stream so far, join() orders the main stream after it.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void fork();
+  void join();
+  bool side_busy = false;  // work forked to the side stream since the last full sync()
   std::string last_error;
   // twiddle tree for the half coset of CanonicCoset(tw_log): 2^(tw_log-1) words each
   uint32_t tw_log = 0;
@@ -100,7 +116,8 @@ struct Ctx {
   void* stage(const void* src, size_t bytes);
   void commit_staged();
   template <class T> const T* stage_vec(const std::vector<T>& v) { return (const T*)stage(v.data(), v.size() * sizeof(T)); }
-  void sync();
+  void sync();       // both streams
+  void sync_main();  // the main stream only
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
   void ensure_twiddles(uint32_t log_domain);
   const uint32_t* domain_xy(uint32_t log);
@@ -116,6 +133,7 @@ struct ProofArenaScope {
   ~ProofArenaScope() {
     if (--c.bump_depth == 0) {
       (void)hipStreamSynchronize(c.stream);
+      (void)hipStreamSynchronize(c.side);
       c.bump_reset();
     }
   }
@@ -128,19 +146,21 @@ struct PersistentAllocScope {
   ~PersistentAllocScope() { c.bump_depth = saved; }
 };
 
-// Times the kernels launched during its lifetime on the context stream (class, algorithmic bytes).
+// Times the kernels launched during its lifetime on a stream of the context (class, algorithmic
+// bytes); the main stream unless another is given.
 struct KScope {
   Ctx& c;
   int cls;
   double bytes;
+  hipStream_t st;
   hipEvent_t a = nullptr;
-  KScope(Ctx& ctx, int k, double b) : c(ctx), cls(k), bytes(b) {
-    if (c.timing) { a = c.get_event(); (void)hipEventRecord(a, c.stream); }
+  KScope(Ctx& ctx, int k, double b, hipStream_t s = nullptr) : c(ctx), cls(k), bytes(b), st(s ? s : ctx.stream) {
+    if (c.timing) { a = c.get_event(); (void)hipEventRecord(a, st); }
   }
   ~KScope() {
     if (c.timing && a) {
       hipEvent_t b = c.get_event();
-      (void)hipEventRecord(b, c.stream);
+      (void)hipEventRecord(b, st);
       c.pending.push_back({cls, bytes, a, b});
     }
   }

@@ -79,7 +79,7 @@ EXPORTS = [
 ]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
-                  "fri_fold", "oods", "logup", "grind", "misc"]
+                  "fri_fold", "oods", "logup", "grind", "misc", "fri_tail"]
 
 _lib = None
 
