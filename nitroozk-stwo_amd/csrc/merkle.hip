@@ -144,21 +144,42 @@ __global__ void __launch_bounds__(128) merkle_subtree_kernel(const uint32_t* __r
   __syncthreads();
   for (uint32_t j = 1; j < L; ++j) {
     n >>= 1;
-    uint32_t h[8];
-    if (i < n) {
-      uint32_t m[16];
+    if (n > blockDim.x / 4) {
+      uint32_t h[8];
+      if (i < n) {
+        uint32_t m[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) m[k] = lds[2 * i * 8 + k];
-      blake2s_init(h);
-      blake2s_compress(h, m, 64, true);
-    }
-    __syncthreads();
-    if (i < n) {
+        for (int k = 0; k < 16; ++k) m[k] = lds[2 * i * 8 + k];
+        blake2s_init(h);
+        blake2s_compress(h, m, 64, true);
+      }
+      __syncthreads();
+      if (i < n) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) lds[i * 8 + k] = h[k];
-      uint4* o4 = reinterpret_cast<uint4*>(outs.p[j] + ((size_t)w * n + i) * 8);
-      o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
-      o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
+        for (int k = 0; k < 8; ++k) lds[i * 8 + k] = h[k];
+        uint4* o4 = reinterpret_cast<uint4*>(outs.p[j] + ((size_t)w * n + i) * 8);
+        o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
+        o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
+      }
+    } else {
+      // narrow level: one quad of lanes per node
+      const uint32_t nd = i >> 2, q = i & 3;
+      uint32_t hA = 0, hB = 0;
+      if (nd < n) {
+        uint32_t m[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m[k] = lds[2 * nd * 8 + k];
+        blake2s_init_quad(q, hA, hB);
+        blake2s_compress_quad(hA, hB, m, 64, true, q);
+      }
+      __syncthreads();
+      if (nd < n) {
+        lds[nd * 8 + q] = hA;
+        lds[nd * 8 + 4 + q] = hB;
+        uint32_t* o = outs.p[j] + ((size_t)w * n + nd) * 8;
+        o[q] = hA;
+        o[4 + q] = hB;
+      }
     }
     __syncthreads();
   }
@@ -325,7 +346,7 @@ __global__ void __launch_bounds__(1024) fri_tail_kernel(uint32_t l_start, uint32
         for (int k = 0; k < 8; ++k) Nd[8 * i + k] = p[k];
       }
       top = l - 1;
-    } else {
+    } else if (n > T / 4) {
       for (uint32_t i = t; i < n; i += T) {
         uint32_t h[8];
         leaf4(V, i, h);
@@ -334,17 +355,51 @@ __global__ void __launch_bounds__(1024) fri_tail_kernel(uint32_t l_start, uint32
         for (int k = 0; k < 8; ++k) Nd[8 * i + k] = h[k];
       }
       top = l;
+    } else {
+      // few leaves: one quad of lanes per leaf (latency-bound)
+      const uint32_t i = t >> 2, q = t & 3;
+      if (i < n) {
+        uint32_t msg[16] = {V[0][i], V[1][i], V[2][i], V[3][i], 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t hA, hB;
+        blake2s_init_quad(q, hA, hB);
+        blake2s_compress_quad(hA, hB, msg, 16, true, q);
+        tree[8 * ((n - 1) + i) + q] = hA;
+        tree[8 * ((n - 1) + i) + 4 + q] = hB;
+        Nd[8 * i + q] = hA;
+        Nd[8 * i + 4 + q] = hB;
+      }
+      top = l;
     }
     __syncthreads();
     for (uint32_t j = top; j > 0; --j) {
       const uint32_t m = 1u << (j - 1);  // parents at level j - 1 (m <= 1024)
-      uint32_t h[8];
-      if (t < m) node2(Nd + 16 * t, Nd + 16 * t + 8, h);
-      __syncthreads();
-      if (t < m) {
+      if (m > T / 4) {
+        uint32_t h[8];
+        if (t < m) node2(Nd + 16 * t, Nd + 16 * t + 8, h);
+        __syncthreads();
+        if (t < m) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) Nd[8 * t + k] = h[k];
-        store8(tree + 8 * ((m - 1) + t), h);
+          for (int k = 0; k < 8; ++k) Nd[8 * t + k] = h[k];
+          store8(tree + 8 * ((m - 1) + t), h);
+        }
+      } else {
+        // one quad of lanes per parent: the level is latency-bound
+        const uint32_t i = t >> 2, q = t & 3;
+        uint32_t hA = 0, hB = 0;
+        if (i < m) {
+          uint32_t msg[16];
+#pragma unroll
+          for (int k = 0; k < 16; ++k) msg[k] = Nd[16 * i + k];
+          blake2s_init_quad(q, hA, hB);
+          blake2s_compress_quad(hA, hB, msg, 64, true, q);
+        }
+        __syncthreads();
+        if (i < m) {
+          Nd[8 * i + q] = hA;
+          Nd[8 * i + 4 + q] = hB;
+          tree[8 * ((m - 1) + i) + q] = hA;
+          tree[8 * ((m - 1) + i) + 4 + q] = hB;
+        }
       }
       __syncthreads();
     }

@@ -87,7 +87,7 @@ struct Gather {
   std::vector<uint32_t> run(Ctx& ctx) {
     std::vector<uint32_t> out(offs.size());
     if (offs.empty()) return out;
-    const uint32_t* const* b = (const uint32_t* const*)ctx.stage(bases.data(), bases.size() * sizeof(void*));
+    const uint32_t* const* b = (const uint32_t* const*)ctx.upload_table(bases.data(), bases.size() * sizeof(void*));
     const uint32_t* bi = ctx.stage_vec(bidx);
     const uint32_t* of = ctx.upload_vec(offs);
     DevBuf o(&ctx, offs.size());
@@ -388,8 +388,8 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
       src.push_back(coeffs[i]);
       dst.push_back(out[i].p);
     }
-    const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
-    uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
+    const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
+    uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
     hipStream_t st = ctx.stream;
     if (on_side) {
       ctx.fork();  // the side stream starts after the upload and everything queued before it
@@ -467,7 +467,7 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
       launch_merkle_layer_small((uint32_t)ls, prev, lc.data(), (uint32_t)lc.size(), layer.p, ctx.stream);
     } else {
-      const uint32_t* const* cp = (const uint32_t* const*)ctx.upload_vec(lc);
+      const uint32_t* const* cp = (const uint32_t* const*)ctx.table_vec(lc);
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
       launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
     }
@@ -497,8 +497,8 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
       into.coeffs[base + i] = into.owned.back().p;
       into.coeff_logs[base + i] = lg;
     }
-    const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
-    uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
+    const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
+    uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
     KScope ks(ctx, K_INTERPOLATE, (double)idx.size() * 8.0 * (1u << lg));
     launch_fft(true, d_src, d_dst,
                (uint32_t)idx.size(), lg, lg, ctx.itw.p, ctx.tree_len(), ctx.stream);
@@ -522,8 +522,8 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
       t.evals[i] = t.owned.back().p;
       t.eval_logs[i] = K;
     }
-    const uint32_t* const* d_src = (const uint32_t* const*)ctx.stage_vec(src);
-    uint32_t* const* d_dst = (uint32_t* const*)ctx.upload_vec(dst);
+    const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
+    uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << K)));
     launch_fft(false, d_src, d_dst,
                (uint32_t)idx.size(), lg, K, ctx.tw.p, ctx.tree_len(), ctx.stream);
@@ -697,9 +697,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     for (uint32_t i = 0; i < (1u << le); ++i) dinv[i] = m31_inv(coset_vanishing_m31(c.log_n, canonic_domain_at(E, i)));
     std::vector<uint32_t> dinv_br(dinv.size());
     for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
-    const uint32_t* d_dinv = ctx.stage_vec(dinv_br);
+    const uint32_t* d_dinv = ctx.table_vec(dinv_br);
     const qm31* d_pows = ctx.stage_vec(cp);
-    const uint32_t* const* d_t = (const uint32_t* const*)ctx.stage_vec(tcols);
+    const uint32_t* const* d_t = (const uint32_t* const*)ctx.table_vec(tcols);
     double cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
     if (join_side) ctx.join();
     if (c.kind == AIR_WIDE_FIB) {
@@ -713,7 +713,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (int i = 0; i < 16; ++i) pp.alpha_pow[i] = c.alpha_pow[i];
       pp.shift = c.cumsum_shift();
       pp.ipr = c.ipr;
-      const uint32_t* const* d_i = (const uint32_t* const*)ctx.stage_vec(icols);
+      const uint32_t* const* d_i = (const uint32_t* const*)ctx.table_vec(icols);
       const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
       KScope ks(ctx, K_COMPOSITION, cbytes);
       const size_t stride = (size_t)1 << E;
@@ -835,10 +835,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     DevBuf weights(&ctx, group_key.size() * ((size_t)4 << 11));
     double obytes = 0;
     for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
-    const OodsTask* dt = ctx.stage_vec(tasks);
-    const uint32_t* dtg = ctx.stage_vec(task_group);
-    const qm31* df = ctx.stage_vec(group_factors);
-    const uint32_t* dgl = ctx.upload_vec(group_log);
+    const OodsTask* dt = ctx.table_vec(tasks);
+    const uint32_t* dtg = ctx.table_vec(task_group);
+    const uint32_t* dgl = ctx.table_vec(group_log);
+    const qm31* df = ctx.upload_vec(group_factors);
     {
       KScope ks(ctx, K_OODS, obytes);
       launch_oods(dt, dtg, (uint32_t)n_vals, max_coeff_log, df, dgl, (uint32_t)group_key.size(), (qm31*)weights.p,
@@ -964,9 +964,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       bool aligned = true;
       for (auto* p : gcols) aligned &= ((uintptr_t)p & 15) == 0;
       const NumEntry* d_ents = ctx.stage_vec(ents);
-      const uint32_t* d_starts = ctx.stage_vec(starts);
-      const qm31* d_bs = ctx.stage_vec(Bs);
-      uint32_t* const* d_outs = (uint32_t* const*)ctx.upload_vec(outs);
+      const uint32_t* d_starts = ctx.table_vec(starts);
+      uint32_t* const* d_outs = (uint32_t* const*)ctx.table_vec(outs);
+      const qm31* d_bs = ctx.upload_vec(Bs);
       KScope ks(ctx, K_NUMERATORS, (4.0 * gcols.size() + 16.0 * batches.size()) * (double)(1u << ls));
       launch_accumulate_numerators(ls, d_ents, d_starts, d_bs, (uint32_t)batches.size(), d_outs, aligned, ctx.stream);
     }
@@ -1001,8 +1001,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   const uint32_t* dxy = ctx.domain_xy(Lc);
   double qbytes = 24.0 * (double)(1u << Lc);
   for (auto& pa : qparts) qbytes += 16.0 * (double)(1u << pa.log_size);
-  const QuotPoint* dq = ctx.stage_vec(qpts);
-  const QuotPart* dp = ctx.upload_vec(qparts);
+  const QuotPart* dp = ctx.table_vec(qparts);
+  const QuotPoint* dq = ctx.upload_vec(qpts);
   {
     KScope ksq(ctx, K_QUOTIENTS, qbytes);
     launch_quotients(Lc, dxy, dq, (uint32_t)qpts.size(), dp, first.mc(), ctx.stream);
@@ -1318,8 +1318,8 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     std::vector<uint32_t*> ip;
     for (auto& b : icols) ip.push_back(b.p);
     DevBuf sums(&ctx, 8, true);
-    const uint32_t* const* d_tr = (const uint32_t* const*)ctx.stage_vec(trace);
-    uint32_t* const* d_ip = (uint32_t* const*)ctx.stage_vec(ip);
+    const uint32_t* const* d_tr = (const uint32_t* const*)ctx.table_vec(trace);
+    uint32_t* const* d_ip = (uint32_t* const*)ctx.table_vec(ip);
     const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
     {
       KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
@@ -1335,7 +1335,7 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     DevBuf scratch(&ctx, 4 * nb);
     {
       KScope ks2(ctx, K_LOGUP, 32.0 * (double)(1u << log_n));
-      launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.upload_vec(last4), shift, scratch.p, ctx.stream);
+      launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.table_vec(last4), shift, scratch.p, ctx.stream);
     }
     std::vector<const uint32_t*> ev;
     for (auto& b : icols) ev.push_back(b.p);

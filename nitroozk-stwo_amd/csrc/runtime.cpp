@@ -80,6 +80,8 @@ Ctx::~Ctx() {
   for (auto& ch : bump_chunks) (void)hipFree(ch.first);
   if (dev_arena) (void)hipFree(dev_arena);
   if (pinned) (void)hipHostFree(pinned);
+  if (table_dev) (void)hipFree(table_dev);
+  if (table_pinned) (void)hipHostFree(table_pinned);
   (void)hipEventDestroy(ev_fork);
   (void)hipEventDestroy(ev_join);
   (void)hipStreamDestroy(side);
@@ -114,6 +116,25 @@ void Ctx::commit_staged() {
     uploads_in_flight = true;
   }
   staged_from = pinned_off;
+}
+const void* Ctx::upload_table(const void* src, size_t bytes) {
+  std::string key((const char*)src, bytes);
+  auto it = table_cache.find(key);
+  if (it != table_cache.end()) return it->second;
+  const size_t a = (bytes + 255) & ~(size_t)255;
+  if (!table_dev) {
+    table_cap = 16u << 20;
+    HIP_CHECK(hipMalloc((void**)&table_dev, table_cap));
+    HIP_CHECK(hipHostMalloc((void**)&table_pinned, table_cap, hipHostMallocDefault));
+  }
+  if (a == 0 || table_off + a > table_cap) return upload(src, bytes);
+  // the pinned copy is never overwritten, so the queued transfer may read it at any time
+  std::memcpy(table_pinned + table_off, src, bytes);
+  HIP_CHECK(hipMemcpyAsync(table_dev + table_off, table_pinned + table_off, bytes, hipMemcpyHostToDevice, stream));
+  const void* d = table_dev + table_off;
+  table_off += a;
+  table_cache.emplace(std::move(key), d);
+  return d;
 }
 void* Ctx::upload(const void* src, size_t bytes) {
   void* d = stage(src, bytes);

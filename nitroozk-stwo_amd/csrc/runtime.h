@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <array>
+#include <unordered_map>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -116,6 +117,15 @@ struct Ctx {
   void* stage(const void* src, size_t bytes);
   void commit_staged();
   template <class T> const T* stage_vec(const std::vector<T>& v) { return (const T*)stage(v.data(), v.size() * sizeof(T)); }
+  // Tables that repeat from call to call (column address tables, per-shape constants) are kept in
+  // device memory keyed by their bytes: a table seen before costs no transfer. Falls back to
+  // upload() once the table store is full.
+  const void* upload_table(const void* src, size_t bytes);
+  template <class T> const T* table_vec(const std::vector<T>& v) { return (const T*)upload_table(v.data(), v.size() * sizeof(T)); }
+  std::unordered_map<std::string, const void*> table_cache;
+  uint8_t* table_dev = nullptr;
+  uint8_t* table_pinned = nullptr;
+  size_t table_cap = 0, table_off = 0;
   void sync();       // both streams
   void sync_main();  // the main stream only
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
