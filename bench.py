@@ -45,12 +45,192 @@ def device_sync(hb):
     hb.synchronize()
 
 
+def kernel_batch(args, m):
+    """Kernel batch: n columns of 2^k seeded uniform values through interpolate, evaluate with blowup 2,
+    the Merkle tree of the extended columns (commit_on_layer per layer) and fold_circle_into_line of
+    the extended columns taken as n/4 QM31 columns (coordinates 4j..4j+3). One step runs all four
+    stages through the C ABI; per-stage device time comes from HIP events on the library stream."""
+    import numpy as np
+    P = (1 << 31) - 1
+    n_cols, k = args.kb_cols, args.kb_log
+    if n_cols % 4:
+        raise SystemExit("--kb-cols must be a multiple of 4")
+    hb = m.HipBackend(0)
+    device_sync(hb)
+    rng = np.random.default_rng(args.seed)
+    host = [rng.integers(0, P, 1 << k, dtype=np.uint32) for _ in range(n_cols)]
+    evals = [hb.from_cpu(h) for h in host]
+    hb.precompute_twiddles(k + 1)
+    coeffs = [hb.column(1 << k) for _ in range(n_cols)]
+    ext = [hb.column(2 << k) for _ in range(n_cols)]
+    layers = [hb.column(8 << j) for j in range(k + 2)]
+    dst = [[hb.column(1 << k, zeroed=True) for _ in range(4)] for _ in range(n_cols // 4)]
+    alpha = tuple(int(x) for x in rng.integers(0, P, 4))
+
+    def step():
+        hb.interpolate_columns(evals, k, out=coeffs)
+        hb.evaluate_polynomials(coeffs, k, 1, out=ext)
+        hb.commit_on_layer(k + 1, None, ext, out=layers[k + 1])
+        for j in range(k, -1, -1):
+            hb.commit_on_layer(j, layers[j + 1], [], out=layers[j])
+        for j in range(n_cols // 4):
+            hb.fold_circle_into_line(dst[j], ext[4 * j:4 * j + 4], alpha, k + 1)
+
+    step()  # first step: parity spot check against the oracle on one column / one QM31 column
+    parity = None
+    if not args.no_cpu_baseline:
+        import oracle_lib as O
+        c0 = O.interpolate(host[0])
+        e0 = O.evaluate(c0, k + 1)
+        src4 = [O.evaluate(O.interpolate(host[i]), k + 1) for i in range(4)]
+        f0 = O.fold_circle_into_line(src4, [np.zeros(1 << k, np.uint32)] * 4, alpha)
+        parity = bool(np.array_equal(coeffs[0].to_cpu(), c0) and np.array_equal(ext[0].to_cpu(), e0)
+                      and all(np.array_equal(dst[0][i].to_cpu(), f0[i]) for i in range(4)))
+    for _ in range(max(0, args.warmup - 1)):
+        step()
+    hb.reset_kernel_stats()
+    hb.set_kernel_timing(True)
+    device_sync(hb)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    device_sync(hb)
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    hb.set_kernel_timing(False)
+    stats = hb.kernel_stats()
+    names = {"interpolate": "interpolate", "evaluate": "evaluate", "merkle": "merkle", "fri_fold": "fold_circle_into_line"}
+    stages = {}
+    tot_b = tot_ms = 0.0
+    for cls, label in names.items():
+        ms, by, n = stats.get(cls, (0.0, 0.0, 0))
+        ms /= args.steps
+        by /= args.steps
+        tot_b += by
+        tot_ms += ms
+        stages[label] = {"ms": round(ms, 3), "GB": round(by / 1e9, 3), "GB_per_s": round(by / (ms / 1e3) / 1e9, 1) if ms else None}
+    dom = max(stages, key=lambda s: stages[s]["ms"])
+    ach = stages[dom]["GB_per_s"]
+    out = {
+        "metric": "kernel batch GB/s (%d x 2^%d: interpolate, evaluate x2, Merkle, fold)" % (n_cols, k),
+        "value": round(tot_b / (tot_ms / 1e3) / 1e9, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(tot_ms, 3), "wall_ms_per_step": round(wall_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (M31 field, exact)",
+        "data": "synthetic: seeded uniform [0,P) columns (numpy PCG64, seed %d)" % args.seed,
+        "config": {"workload": "kernel_batch", "columns": n_cols, "log_size": k, "log_blowup": 1,
+                   "fold_columns": n_cols // 4},
+        "stages": stages, "parity_spot_check": parity,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None, "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
+    hb.close()
+
+
+def serve(args, m):
+    """Serving: a seeded stream of requests, 70% wide-Fibonacci (2^n rows, n in 12..20, 32..100
+    columns) and 30% Poseidon2 (2^12..2^18 instances, 8 per row), each generating its trace on the
+    device and proving it. Requests arrive all at once (throughput) or as a Poisson stream of
+    --rate per second. "plain" serves them one after another on one context; "batched" runs
+    --contexts contexts from as many host threads, so that the latency-bound phases of one proof
+    overlap the bandwidth-bound phases of others. Latency = completion - arrival."""
+    import queue
+    import random
+    import threading
+    rnd = random.Random(args.seed)
+    reqs = []
+    t = 0.0
+    for i in range(args.requests):
+        if rnd.random() < 0.7:
+            reqs.append(("wide_fib", rnd.randint(12, 20), rnd.randint(32, 100)))
+        else:
+            li = rnd.randint(12, 18)
+            reqs.append(("poseidon", li - 3, 8))
+        if args.rate > 0:
+            t += rnd.expovariate(args.rate)
+        reqs[-1] = reqs[-1] + (t,)
+    cfg = m.PcsConfig.make(args.pow_bits, 1, 0, 3, 1, 0)
+
+    def run(n_ctx):
+        backends = [m.HipBackend(0) for _ in range(n_ctx)]
+        q = queue.Queue()
+        for r in reqs:
+            q.put(r)
+        lat, prove_ms, errors = [], [], []
+        lock = threading.Lock()
+        t0 = time.perf_counter()
+
+        def worker(hb):
+            while True:
+                try:
+                    kind, log_rows, width, arrival = q.get_nowait()
+                except queue.Empty:
+                    return
+                now = time.perf_counter() - t0
+                if arrival > now:
+                    time.sleep(arrival - now)
+                try:
+                    s0 = time.perf_counter()
+                    if kind == "wide_fib":
+                        tr = hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed)
+                        pr = hb.prove_wide_fibonacci(tr, log_rows, cfg)
+                    else:
+                        tr = hb.gen_trace_poseidon(width, log_rows, 1, args.seed)
+                        pr = hb.prove_poseidon(tr, width, log_rows, cfg)
+                    _ = pr.bytes
+                    done = time.perf_counter()
+                    for c in tr:
+                        c.free()
+                    with lock:
+                        prove_ms.append((done - s0) * 1e3)
+                        lat.append((done - t0 - arrival) * 1e3)
+                except Exception as e:  # reported in the JSON line
+                    with lock:
+                        errors.append(str(e))
+
+        # one warm-up proof per context (twiddles, arenas)
+        for hb in backends:
+            tr = hb.gen_trace_poseidon(8, 9, 1, args.seed)
+            hb.prove_poseidon(tr, 8, 9, cfg)
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=worker, args=(hb,)) for hb in backends]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        wall = time.perf_counter() - t0
+        for hb in backends:
+            hb.close()
+        lat.sort()
+        pct = lambda p: round(lat[min(len(lat) - 1, int(round(p * len(lat))) - 1)], 3) if lat else None
+        return {"contexts": n_ctx, "proofs": len(lat), "proofs_per_s": round(len(lat) / wall, 2),
+                "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_prove_ms": round(statistics.mean(prove_ms), 3),
+                "errors": errors[:3]}
+
+    plain = run(1)
+    batched = run(args.contexts)
+    out = {
+        "metric": "proofs/s (mixed serving: 70%% wide-Fibonacci, 30%% Poseidon2; %d contexts)" % args.contexts,
+        "value": batched["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": args.requests, "warmup": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (M31 field, exact)",
+        "data": "synthetic: seeded request mix and seeded uniform traces generated on the device (seed %d)" % args.seed,
+        "config": {"workload": "serve", "requests": args.requests, "rate_per_s": args.rate, "pow_bits": args.pow_bits,
+                   "log_blowup": 1, "n_queries": 3},
+        "plain": plain, "batched": batched,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["poseidon", "wide_fib"], default="poseidon")
+    ap.add_argument("--workload", choices=["poseidon", "wide_fib", "kernels", "serve"], default="poseidon")
+    ap.add_argument("--requests", type=int, default=200, help="serve: number of requests")
+    ap.add_argument("--contexts", type=int, default=4, help="serve: concurrent contexts (batched mode)")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrivals per second (0 = all at once)")
+    ap.add_argument("--kb-cols", type=int, default=256, help="kernel batch: columns")
+    ap.add_argument("--kb-log", type=int, default=20, help="kernel batch: log column size")
     ap.add_argument("--log-instances", type=int, default=18)
     ap.add_argument("--ipr", type=int, default=8, help="Poseidon instances per row")
     ap.add_argument("--n-cols", type=int, default=100, help="wide-fib columns")
@@ -64,6 +244,10 @@ def main():
         sys.exit(2)
 
     m = load_pkg()
+    if args.workload == "kernels":
+        return kernel_batch(args, m)
+    if args.workload == "serve":
+        return serve(args, m)
     hb = m.HipBackend(0)
     cfg_d = dict(pow_bits=args.pow_bits, log_blowup=1, log_last_layer=0, n_queries=3, fold_step=1)
     cfg = m.PcsConfig.make(args.pow_bits, 1, 0, 3, 1, 0)

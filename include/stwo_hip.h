@@ -79,6 +79,16 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* ctx, const uint32_t* const* coef
 int stwo_hip_batch_eval_at_point(stwo_hip_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
                                  stwo_circle_point_qm31 point, stwo_qm31* out_host);
 
+/* barycentric_weights (ops.rs:61-76; barycentric_* in bindings.rs:141-158; cpu/circle.rs:101-132):
+ * weights[0..3] (coordinate columns of 2^log_size) receive w_i = V_n(p) / (S_i V_i(p)) over the
+ * bit-reversed circle domain of CanonicCoset(log_size). Synchronous. */
+int stwo_hip_barycentric_weights(stwo_hip_ctx* ctx, uint32_t log_size, stwo_circle_point_qm31 point,
+                                 uint32_t* const weights[4]);
+/* barycentric_eval_at_point (ops.rs:61-76; cpu/circle.rs:134-141) for n_cols columns at once:
+ * out_host[c] = sum_i evals[c][i] * w_i with the weights above. Synchronous. */
+int stwo_hip_batch_barycentric_eval(stwo_hip_ctx* ctx, const uint32_t* const* evals, uint32_t n_cols, uint32_t log_size,
+                                    const uint32_t* const weights[4], stwo_qm31* out_host);
+
 /* ---------------------------------------------------------------- MerkleOps ------------ */
 /* MerkleOps<Blake2sMerkleHasher>::commit_on_layer (prover/vcs/ops.rs:8-29; commit_on_first_layer /
  * commit_on_layer_with_previous, bindings.rs:262-275): out (2^log_size nodes x 8 words) =

@@ -21,6 +21,8 @@ template <class F>
 int guarded(stwo_hip_ctx* h, F&& f) {
   if (!h || !h->ctx) return 1;
   try {
+    // the calling thread may differ from the one that created the context: select its device
+    HIP_CHECK(hipSetDevice(h->ctx->device));
     f(*h->ctx);
     return 0;
   } catch (const HipError& e) {
@@ -139,8 +141,12 @@ int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, 
     c.ensure_twiddles(log_size);
     std::vector<const uint32_t*> s(evals, evals + n_cols);
     std::vector<uint32_t*> d(coeffs, coeffs + n_cols);
-    launch_fft(true, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
-               log_size, c.itw.p, c.tree_len(), c.stream);
+    const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
+    uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
+    {
+      KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
+      launch_fft(true, ds, dd, n_cols, log_size, log_size, c.itw.p, c.tree_len(), c.stream);
+    }
     c.sync();
     c.reset_arena();
   });
@@ -152,8 +158,12 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs
     c.ensure_twiddles(log_size + log_blowup);
     std::vector<const uint32_t*> s(coeffs, coeffs + n_cols);
     std::vector<uint32_t*> d(evals, evals + n_cols);
-    launch_fft(false, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
-               log_size + log_blowup, c.tw.p, c.tree_len(), c.stream);
+    const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
+    uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
+    {
+      KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_size) + 4.0 * (double)(1u << (log_size + log_blowup))));
+      launch_fft(false, ds, dd, n_cols, log_size, log_size + log_blowup, c.tw.p, c.tree_len(), c.stream);
+    }
     c.sync();
     c.reset_arena();
   });
@@ -181,13 +191,54 @@ int stwo_hip_batch_eval_at_point(stwo_hip_ctx* h, const uint32_t* const* coeffs,
   });
 }
 
+int stwo_hip_barycentric_weights(stwo_hip_ctx* h, uint32_t log_size, stwo_circle_point_qm31 point,
+                                 uint32_t* const weights[4]) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size < 1 || log_size > 30) throw HipError(3, "log_size out of range");
+    check_canonical(point.x);
+    check_canonical(point.y);
+    for (int k = 0; k < 4; ++k)
+      if (!weights[k]) throw HipError(1, "null weight column");
+    cpqm31 p = cp(point);
+    std::vector<cpm31> shifts;
+    for (uint32_t j = 1; j < log_size; ++j) {
+      HCoset hc = HCoset::odds(j);
+      shifts.push_back(cp_add(cp_conj(index_to_point(hc.initial_index)), index_to_point(hc.step_size >> 1)));
+    }
+    const qm31 vn = coset_vanishing_qm31(log_size, p);
+    launch_barycentric_weights(log_size, p, vn, shifts.data(), {{weights[0], weights[1], weights[2], weights[3]}},
+                               c.stream);
+    c.sync();
+  });
+}
+int stwo_hip_batch_barycentric_eval(stwo_hip_ctx* h, const uint32_t* const* evals, uint32_t n_cols, uint32_t log_size,
+                                    const uint32_t* const weights[4], stwo_qm31* out_host) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size > 30) throw HipError(3, "log_size out of range");
+    if (n_cols == 0) return;
+    for (uint32_t i = 0; i < n_cols; ++i)
+      if (!evals[i]) throw HipError(1, "null evaluation column");
+    std::vector<const uint32_t*> ev(evals, evals + n_cols);
+    const uint32_t blocks = ((1u << log_size) + 2047) / 2048;
+    DevBuf partial(&c, (size_t)n_cols * blocks * 4), vals(&c, (size_t)n_cols * 4);
+    launch_barycentric_eval(log_size, (const uint32_t* const*)c.upload_vec(ev), n_cols,
+                            {{weights[0], weights[1], weights[2], weights[3]}}, (qm31*)partial.p, (qm31*)vals.p,
+                            c.stream);
+    c.d2h(out_host, vals.p, (size_t)n_cols * 16);
+    c.reset_arena();
+  });
+}
+
 int stwo_hip_commit_on_layer(stwo_hip_ctx* h, uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols,
                              uint32_t n_cols, uint32_t* out) {
   return guarded(h, [&](Ctx& c) {
     if (log_size > 30) throw HipError(3, "log_size out of range");
     std::vector<const uint32_t*> v(cols, cols + n_cols);
     const uint32_t* const* d = n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
-    launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
+    {
+      KScope ks(c, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * n_cols + 32.0) * (double)(1u << log_size));
+      launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
+    }
     c.sync();
     c.reset_arena();
   });
@@ -199,8 +250,11 @@ int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint
     if (log_src < 3 || log_src > 30) throw HipError(3, "log_src must be in [3, 30]");
     check_canonical(alpha);
     c.ensure_twiddles(log_src);
-    launch_fold_circle_into_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
-                                 q(alpha), c.itw.p, c.tree_len(), c.stream);
+    {
+      KScope ks(c, K_FRI_FOLD, 16.0 * (double)(1u << log_src) + 32.0 * (double)(1u << (log_src - 1)));
+      launch_fold_circle_into_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
+                                   q(alpha), c.itw.p, c.tree_len(), c.stream);
+    }
     c.sync();
   });
 }
@@ -210,8 +264,11 @@ int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const 
     if (log_src < 1 || log_src > 29) throw HipError(3, "log_src must be in [1, 29]");
     check_canonical(alpha);
     c.ensure_twiddles(log_src + 1);
-    launch_fold_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}}, q(alpha), c.itw.p,
-                     c.tree_len(), c.stream);
+    {
+      KScope ks(c, K_FRI_FOLD, 24.0 * (double)(1u << log_src));
+      launch_fold_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}}, q(alpha), c.itw.p,
+                       c.tree_len(), c.stream);
+    }
     c.sync();
   });
 }

@@ -74,6 +74,14 @@ void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_t
                  const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,
                  qm31* partials, qm31* out, hipStream_t stream);
 
+// Barycentric weights of CanonicCoset(log_size) for p (4 coordinate columns) and the batched
+// sums sum_i evals_c[i] w_i; shifts[j-1] (j = 1 .. log_size-1) as in barycentric_weights_kernel.
+// partial holds n_cols * ceil(2^log_size / 2048) QM31 (scratch).
+void launch_barycentric_weights(uint32_t log_size, cpqm31 p, qm31 vn, const cpm31* shifts, Cols4 out,
+                                hipStream_t stream);
+void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, uint32_t n_cols, CCols4 weights,
+                             qm31* partial, qm31* out, hipStream_t stream);
+
 void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
                          uint32_t* out, hipStream_t stream);
 

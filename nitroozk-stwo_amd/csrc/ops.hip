@@ -259,6 +259,77 @@ void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCo
                      src);
 }
 
+// ---- barycentric evaluation (PolyOps::barycentric_weights / barycentric_eval_at_point) --------
+// w_i = V_n(p) / (S_i V_i(p)) over the bit-reversed CanonicCoset(k) circle domain
+// (prover/backend/cpu/circle.rs:101-141): S_i = -2 y_i 4^(k-1) prod_{j=1}^{k-1} V_{CanonicCoset(j)}(x_i)
+// (coset_vanishing_derivative), V_i(p) = h.y / (1 + h.x) with h = p - x_i (point_vanishing).
+// shift[j-1] = half step of CanonicCoset(j) minus its initial point, so that
+// V_{CanonicCoset(j)}(q) = x-coordinate of q + shift[j-1] doubled j - 1 times.
+struct BaryShifts { cpm31 s[31]; };
+__global__ void __launch_bounds__(256) barycentric_weights_kernel(uint32_t k, cpqm31 p, qm31 vn, BaryShifts sh,
+                                                                  uint32_t four_pow, Cols4 out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1u << k)) return;
+  // domain point bitrev(i) of CanonicCoset(k).circle_domain(): half_odds(k - 1) and its negation
+  const uint32_t idx = bitrev(i, k), half = 1u << (k - 1);
+  const uint32_t init = 1u << (31 - (k + 1)), step = 1u << (31 - (k - 1));
+  uint32_t pi = idx < half ? (init + idx * step) : (0x80000000u - (init + (idx - half) * step));
+  const cpm31 x = index_to_point(pi & 0x7fffffffu);
+  uint32_t van = 1;
+  for (uint32_t j = 1; j < k; ++j) {
+    uint32_t v = cp_add(x, sh.s[j - 1]).x;
+    for (uint32_t r = 1; r < j; ++r) v = m31_double_x(v);
+    van = m31_mul(van, v);
+  }
+  const uint32_t si = m31_mul(m31_mul(m31_neg(m31_add(x.y, x.y)), four_pow), van);
+  const cpqm31 h = cpq_add(p, cpq_from(cp_conj(x)));
+  const qm31 w = qm_mul(qm_mul(vn, qm_add(qm_one(), h.x)), qm_inv(qm_mul_m31(h.y, si)));
+  store_q(out.c, i, w);
+}
+
+void launch_barycentric_weights(uint32_t log_size, cpqm31 p, qm31 vn, const cpm31* shifts, Cols4 out,
+                                hipStream_t stream) {
+  BaryShifts sh{};
+  for (uint32_t j = 1; j < log_size; ++j) sh.s[j - 1] = shifts[j - 1];
+  uint32_t four_pow = 1;
+  for (uint32_t j = 1; j < log_size; ++j) four_pow = m31_mul(four_pow, 4);
+  const uint32_t n = 1u << log_size;
+  hipLaunchKernelGGL(barycentric_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, log_size, p, vn, sh,
+                     four_pow, out);
+}
+
+// partial[c * gridDim.x + b] = sum over the 2048 rows of block b of evals_c[i] * w_i.
+__global__ void __launch_bounds__(256) barycentric_dot_kernel(uint32_t n, const uint32_t* const* __restrict__ evals,
+                                                              CCols4 w, qm31* __restrict__ partial) {
+  __shared__ qm31 red[256];
+  const uint32_t c = blockIdx.y;
+  const uint32_t* __restrict__ e = evals[c];
+  qacc acc = qacc_zero();
+  for (uint32_t i = blockIdx.x * 2048 + threadIdx.x; i < n && i < (blockIdx.x + 1) * 2048; i += 256)
+    qacc_mac(acc, load_q(w.c, i), e[i]);
+  red[threadIdx.x] = qacc_reduce(acc);
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = qm_add(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[c * gridDim.x + blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(64) sum_partials_kernel(const qm31* __restrict__ partial, uint32_t n_parts,
+                                                          qm31* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  qm31 s = qm_zero();
+  for (uint32_t b = 0; b < n_parts; ++b) s = qm_add(s, partial[blockIdx.x * n_parts + b]);
+  out[blockIdx.x] = s;
+}
+
+void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, uint32_t n_cols, CCols4 weights,
+                             qm31* partial, qm31* out, hipStream_t stream) {
+  const uint32_t n = 1u << log_size, blocks = (n + 2047) / 2048;
+  hipLaunchKernelGGL(barycentric_dot_kernel, dim3(blocks, n_cols), dim3(256), 0, stream, n, evals, weights, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(n_cols), dim3(64), 0, stream, partial, blocks, out);
+}
+
 // ---- out-of-domain sampling on coefficients ------------------------------------------------
 // f(P) = sum_n c_n prod_b f_b^(n_b), f_0 = y, f_1 = x, f_b = pi^(b-1)(x). Tasks that share a
 // point and a size share a weight table W[lo] = prod_{b < 11} f_b^(lo_b) (2^11 entries). A

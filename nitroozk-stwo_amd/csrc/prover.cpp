@@ -52,7 +52,7 @@ struct HostMarks {
     v.clear();
   }
 };
-HostMarks g_marks;
+thread_local HostMarks g_marks;
 #define HMARK(name) g_marks.mark(name)
 
 Hash read_hash(Ctx& ctx, const uint32_t* dev) {

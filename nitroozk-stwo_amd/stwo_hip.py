@@ -69,7 +69,8 @@ EXPORTS = [
     "stwo_hip_create", "stwo_hip_destroy", "stwo_hip_synchronize", "stwo_hip_last_error", "stwo_hip_alloc",
     "stwo_hip_free", "stwo_hip_copy_h2d", "stwo_hip_copy_d2h", "stwo_hip_batch_at_multi",
     "stwo_hip_precompute_twiddles", "stwo_hip_interpolate_columns", "stwo_hip_evaluate_polynomials",
-    "stwo_hip_batch_eval_at_point", "stwo_hip_commit_on_layer", "stwo_hip_fold_circle_into_line",
+    "stwo_hip_batch_eval_at_point", "stwo_hip_barycentric_weights", "stwo_hip_batch_barycentric_eval",
+    "stwo_hip_commit_on_layer", "stwo_hip_fold_circle_into_line",
     "stwo_hip_fold_line", "stwo_hip_accumulate_numerators", "stwo_hip_compute_quotients_and_combine",
     "stwo_hip_accumulate", "stwo_hip_lift_and_accumulate", "stwo_hip_generate_secure_powers", "stwo_hip_grind",
     "stwo_hip_evaluate_constraint_quotients_on_domain", "stwo_hip_prove_wide_fibonacci", "stwo_hip_prove_poseidon",
@@ -109,6 +110,8 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_evaluate_polynomials.argtypes = [c.c_void_p, _VPP, _VPP, c.c_uint32, c.c_uint32, c.c_uint32]
     L.stwo_hip_batch_eval_at_point.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, CirclePointQM31,
                                                c.POINTER(QM31)]
+    L.stwo_hip_barycentric_weights.argtypes = [c.c_void_p, c.c_uint32, CirclePointQM31, _VPP]
+    L.stwo_hip_batch_barycentric_eval.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, _VPP, c.POINTER(QM31)]
     L.stwo_hip_commit_on_layer.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32, c.c_void_p]
     L.stwo_hip_fold_circle_into_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
     L.stwo_hip_fold_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
@@ -263,8 +266,8 @@ class HipBackend:
         self._check(self.lib.stwo_hip_interpolate_columns(self.ctx, _ptrs(evals), _ptrs(out), len(evals), log_size))
         return out
 
-    def evaluate_polynomials(self, coeffs, log_size, log_blowup):
-        out = [self.column(1 << (log_size + log_blowup)) for _ in coeffs]
+    def evaluate_polynomials(self, coeffs, log_size, log_blowup, out=None):
+        out = out if out is not None else [self.column(1 << (log_size + log_blowup)) for _ in coeffs]
         self._check(self.lib.stwo_hip_evaluate_polynomials(self.ctx, _ptrs(coeffs), _ptrs(out), len(coeffs), log_size,
                                                            log_blowup))
         return out
@@ -275,9 +278,23 @@ class HipBackend:
         self._check(self.lib.stwo_hip_batch_eval_at_point(self.ctx, _ptrs(coeffs), len(coeffs), log_size, pt, out))
         return [o.tuple() for o in out]
 
+    def barycentric_weights(self, log_size, point):
+        """PolyOps::barycentric_weights: 4 device coordinate columns of 2^log_size."""
+        w = [self.column(1 << log_size) for _ in range(4)]
+        pt = CirclePointQM31(QM31.of(point[:4]), QM31.of(point[4:]))
+        self._check(self.lib.stwo_hip_barycentric_weights(self.ctx, log_size, pt, _ptrs(w)))
+        return w
+
+    def batch_barycentric_eval(self, evals, log_size, weights):
+        """PolyOps::barycentric_eval_at_point for several columns: list of QM31 tuples."""
+        out = (QM31 * len(evals))()
+        self._check(self.lib.stwo_hip_batch_barycentric_eval(self.ctx, _ptrs(evals), len(evals), log_size,
+                                                             _ptrs(weights), out))
+        return [o.tuple() for o in out]
+
     # ---- MerkleOps ----
-    def commit_on_layer(self, log_size, prev, cols):
-        out = self.column(8 << log_size)
+    def commit_on_layer(self, log_size, prev, cols, out=None):
+        out = out if out is not None else self.column(8 << log_size)
         self._check(self.lib.stwo_hip_commit_on_layer(self.ctx, log_size, prev.ptr if prev is not None else None,
                                                       _ptrs(cols), len(cols), out.ptr))
         return out

@@ -946,6 +946,13 @@ void oracle_barycentric_eval(uint32_t log_size, const uint32_t* evals, const uin
   for (size_t i = 0; i < w.size(); ++i) acc = acc + w[i] * M31(evals[i]);
   for (int k = 0; k < 4; ++k) out[k] = acc.coord(k).v;
 }
+// Barycentric weights, coordinate-major: out[k * 2^log_size + i] = coordinate k of w_i.
+void oracle_barycentric_weights(uint32_t log_size, const uint32_t point[8], uint32_t* out) {
+  CirclePoint<QM31> p{QM31::from_u32(point[0], point[1], point[2], point[3]), QM31::from_u32(point[4], point[5], point[6], point[7])};
+  auto w = barycentric_weights(log_size, p);
+  for (size_t i = 0; i < w.size(); ++i)
+    for (int k = 0; k < 4; ++k) out[(size_t)k * w.size() + i] = w[i].coord(k).v;
+}
 // One Merkle layer: 2^log_size hashes, prev may be null; columns are 2^log_size words each.
 void oracle_commit_on_layer(uint32_t log_size, const uint8_t* prev, uint32_t n_cols, const uint32_t* const* cols,
                             uint8_t* out) {

@@ -151,6 +151,13 @@ def eval_at_point(coeffs, point):
     return out
 
 
+def barycentric_weights(log_n, point):
+    p = np.asarray(point, dtype=np.uint32)
+    out = np.zeros(4 << log_n, dtype=np.uint32)
+    lib().oracle_barycentric_weights(ctypes.c_uint32(log_n), u32ptr(p), u32ptr(out))
+    return out.reshape(4, 1 << log_n)
+
+
 def barycentric_eval(evals, point):
     e = np.ascontiguousarray(evals, np.uint32)
     p = np.asarray(point, np.uint32).copy()

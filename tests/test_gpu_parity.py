@@ -130,6 +130,20 @@ def test_batch_eval_at_point(hb, log_n):
         assert tuple(g) == tuple(map(int, O.eval_at_point(c, pt)))
 
 
+@pytest.mark.parametrize("log_n", [4, 9, 13])
+def test_barycentric_weights_and_eval(hb, log_n):
+    say("barycentric", log_n)
+    rng = np.random.default_rng(300 + log_n)
+    pt = random_point(log_n + 1)
+    w = hb.barycentric_weights(log_n, pt)
+    got_w = np.stack([x.to_cpu() for x in w])
+    assert np.array_equal(got_w, O.barycentric_weights(log_n, pt))
+    cols = rand_cols(rng, 3, log_n)
+    got = hb.batch_barycentric_eval([hb.from_cpu(c) for c in cols], log_n, w)
+    for c, g in zip(cols, got):
+        assert tuple(g) == tuple(map(int, O.barycentric_eval(c, pt)))
+
+
 def test_grind_matches_oracle(hb):
     for bits in (1, 5, 10, 14):
         digest = hashlib.blake2s(b"pow" + bytes([bits]), digest_size=32).digest()
