@@ -126,6 +126,50 @@ def kernel_batch(args, m):
     hb.close()
 
 
+def mixed(args, m):
+    """Mixed components: --components wide-Fibonacci components in one proof, 2^n rows with
+    n = 10 + min(12, G), G geometric with p = 0.3 (so n is in 10..22, mostly small), and 8..160
+    columns, all seeded; one trace tree of mixed sizes, blowup 1."""
+    import random
+    rnd = random.Random(args.seed)
+    logs, ncols = [], []
+    for _ in range(args.components):
+        g = 0
+        while g < 12 and rnd.random() > 0.3:
+            g += 1
+        logs.append(10 + g)
+        ncols.append(rnd.randint(8, 160))
+    hb = m.HipBackend(0)
+    device_sync(hb)
+    cfg = m.PcsConfig.make(args.pow_bits, 1, 0, 3, 1, 0)
+    traces = [hb.gen_trace_wide_fibonacci(nc, lg, 1, args.seed + i) for i, (lg, nc) in enumerate(zip(logs, ncols))]
+    prove = lambda: hb.prove_wide_fibonacci_multi(traces, logs, cfg)
+    for _ in range(args.warmup):
+        prove()
+    device_sync(hb)
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        pr = prove()
+        times.append((time.perf_counter() - s0) * 1e3)
+    device_sync(hb)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    cells = sum(nc << lg for lg, nc in zip(logs, ncols))
+    out = {
+        "metric": "prove ms (%d mixed wide-Fibonacci components)" % args.components, "value": round(ms, 3), "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 (M31 field, exact)",
+        "data": "synthetic: seeded component sizes and seeded uniform traces generated on the device (seed %d)" % args.seed,
+        "config": {"workload": "mixed", "components": args.components, "log_n_rows": logs, "columns": ncols,
+                   "trace_cells": cells, "log_blowup": 1, "n_queries": 3, "pow_bits": args.pow_bits},
+        "p50_ms": round(statistics.median(times), 3), "proof_status": pr.status, "proof_bytes": len(pr.bytes),
+        "trace_cells_per_s": round(cells / (ms / 1e3), 1),
+    }
+    print(json.dumps(out), flush=True)
+    hb.close()
+
+
 def serve(args, m):
     """Serving: a seeded stream of requests, 70% wide-Fibonacci (2^n rows, n in 12..20, 32..100
     columns) and 30% Poseidon2 (2^12..2^18 instances, 8 per row), each generating its trace on the
@@ -225,7 +269,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["poseidon", "wide_fib", "kernels", "serve"], default="poseidon")
+    ap.add_argument("--workload", choices=["poseidon", "wide_fib", "kernels", "serve", "mixed"], default="poseidon")
+    ap.add_argument("--components", type=int, default=48, help="mixed: number of wide-Fibonacci components")
     ap.add_argument("--requests", type=int, default=200, help="serve: number of requests")
     ap.add_argument("--contexts", type=int, default=4, help="serve: concurrent contexts (batched mode)")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrivals per second (0 = all at once)")
@@ -237,17 +282,28 @@ def main():
     ap.add_argument("--pow-bits", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device-to-device once (a kernel of known traffic, for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this pool has one MI355X; --gpus %d is not run "
               "(see DESIGN.md for how the prover shards)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     m = load_pkg()
     if args.workload == "kernels":
         return kernel_batch(args, m)
     if args.workload == "serve":
         return serve(args, m)
+    if args.workload == "mixed":
+        return mixed(args, m)
     hb = m.HipBackend(0)
     cfg_d = dict(pow_bits=args.pow_bits, log_blowup=1, log_last_layer=0, n_queries=3, fold_step=1)
     cfg = m.PcsConfig.make(args.pow_bits, 1, 0, 3, 1, 0)

@@ -176,6 +176,13 @@ int stwo_hip_prove_wide_fibonacci(stwo_hip_ctx* ctx, const uint32_t* const* trac
                                   uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
 int stwo_hip_prove_poseidon(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t instances_per_row,
                             uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
+/* prove() over several wide-Fibonacci components of mixed sizes (prover/mod.rs:28-150 with a
+ * component list; trace columns allocated in component order, as TraceLocationAllocator does):
+ * component i has 2^log_n_rows[i] rows and n_cols[i] columns; trace holds all components'
+ * columns in order. Synchronous. */
+int stwo_hip_prove_wide_fibonacci_multi(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t n_components,
+                                        const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                        const stwo_pcs_config* config, stwo_proof** out);
 int stwo_proof_status(const stwo_proof* p);
 size_t stwo_proof_size(const stwo_proof* p);
 const uint8_t* stwo_proof_bytes(const stwo_proof* p);

@@ -33,7 +33,8 @@ __global__ void __launch_bounds__(256) wide_fib_comp_kernel(uint32_t n_rows, uin
   for (uint32_t k = 2; k < n_cols; ++k) {
     uint32_t c = cols[k][row];
     uint32_t s = m31_reduce64((uint64_t)a * a + (uint64_t)b * b);
-    qacc_mac(acc, pows[k - 2], m31_sub(c, s));
+    qacc_mad(acc, pows[k - 2], m31_sub(c, s));
+    if (k % 3 == 1) qacc_fold(acc);  // at most 3 unreduced products between folds
     a = b;
     b = c;
   }
@@ -81,7 +82,7 @@ __device__ __forceinline__ void internal_matrix(uint32_t* s) {
   for (int i = 0; i < 16; ++i) sum += s[i];
   uint32_t sm = m31_reduce64(sum);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) s[i] = m31_add(m31_mul(s[i], 1u << (i + 1)), sm);
+  for (int i = 0; i < 16; ++i) s[i] = m31_add(m31_rotl(s[i], i + 1), sm);
 }
 __device__ __forceinline__ qm31 combine16(const PoseidonParams& p, const uint32_t* v) {
   qacc acc = qacc_zero();
@@ -121,8 +122,9 @@ __device__ __forceinline__ void psd_full_round(uint32_t* s, const uint32_t* m, c
   external_matrix(s);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    qacc_mac(acc, pows[i], m31_sub(pow5(s[i]), m[i]));
+    qacc_mad(acc, pows[i], m31_sub(pow5(s[i]), m[i]));
     s[i] = m[i];
+    if (i % 3 == 2 || i == 15) qacc_fold(acc);
   }
 }
 template <class Cols>
@@ -151,8 +153,9 @@ __device__ __forceinline__ void poseidon_rounds_constraints(uint32_t* s, const C
     for (int r = 0; r < 14; ++r) {
       s[0] = m31_add(s[0], PSD_ROUND_CONST);
       internal_matrix(s);
-      qacc_mac(acc, pows[64 + r], m31_sub(pow5(s[0]), m[r]));
+      qacc_mad(acc, pows[64 + r], m31_sub(pow5(s[0]), m[r]));
       s[0] = m[r];
+      if (r % 3 == 2 || r == 13) qacc_fold(acc);
     }
 #pragma unroll
     for (int i = 0; i < 16; ++i) m[i] = nx[i];

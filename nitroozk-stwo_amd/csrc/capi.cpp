@@ -86,7 +86,7 @@ const char* stwo_hip_last_error(stwo_hip_ctx* h) { return h ? h->err.c_str() : "
 int stwo_hip_alloc(stwo_hip_ctx* h, size_t n, int zeroed, uint32_t** out) {
   return guarded(h, [&](Ctx& c) {
     if (!out) throw HipError(1, "null out");
-    HIP_CHECK(hipMallocAsync((void**)out, n ? n * 4 : 4, c.stream));
+    HIP_CHECK(hipMallocFromPoolAsync((void**)out, n ? n * 4 : 4, c.pool, c.stream));
     if (zeroed) launch_fill_zero(*out, n, c.stream);
   });
 }
@@ -436,6 +436,31 @@ int stwo_hip_prove_wide_fibonacci(stwo_hip_ctx* h, const uint32_t* const* trace,
     auto* p = new stwo_proof();
     try {
       p->r = prove_wide_fib(c, t, log_n_rows, n_cols, cfg);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+int stwo_hip_prove_wide_fibonacci_multi(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
+                                        const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                        const stwo_pcs_config* config, stwo_proof** out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!out || !log_n_rows || !n_cols) throw HipError(1, "null argument");
+    if (n_components == 0) throw HipError(1, "no components");
+    std::vector<uint32_t> logs(log_n_rows, log_n_rows + n_components), nc(n_cols, n_cols + n_components);
+    size_t total = 0;
+    for (uint32_t i = 0; i < n_components; ++i) {
+      if (nc[i] < 3) throw HipError(1, "wide-fib needs at least 3 columns per component");
+      if (logs[i] < 3 || logs[i] > 26) throw HipError(3, "log_n_rows must be in [3, 26]");
+      total += nc[i];
+    }
+    PcsCfg cfg = cfg_of(config);
+    std::vector<const uint32_t*> t(trace, trace + total);
+    auto* p = new stwo_proof();
+    try {
+      p->r = prove_wide_fib_multi(c, t, logs, nc, cfg);
     } catch (...) {
       delete p;
       throw;

@@ -34,6 +34,8 @@ SHD uint32_t m31_mul(uint32_t a, uint32_t b) {
   return s >= P ? s - P : s;                          // lo + hi < 2P, so one subtraction
 }
 SHD uint32_t m31_sqr(uint32_t a) { return m31_mul(a, a); }
+// a * 2^k for 0 < k < 31: multiplying by 2 modulo 2^31 - 1 rotates the 31-bit word.
+SHD uint32_t m31_rotl(uint32_t a, uint32_t k) { return ((a << k) & P) | (a >> (31 - k)); }
 SHD uint32_t m31_pow(uint32_t a, uint32_t e) {
   uint32_t r = 1;
   while (e) { if (e & 1) r = m31_mul(r, a); a = m31_mul(a, a); e >>= 1; }
@@ -124,6 +126,20 @@ SHD void qacc_mac(qacc& acc, qm31 c, uint32_t v) {
   acc.b += fold31((uint64_t)c.b * v);
   acc.c += fold31((uint64_t)c.c * v);
   acc.d += fold31((uint64_t)c.d * v);
+}
+// acc += c * v with no reduction: products are below 2^62, so at most 3 of them may follow a
+// qacc_fold (which leaves each coordinate below 2^33) before the next fold.
+SHD void qacc_mad(qacc& acc, qm31 c, uint32_t v) {
+  acc.a += (uint64_t)c.a * v;
+  acc.b += (uint64_t)c.b * v;
+  acc.c += (uint64_t)c.c * v;
+  acc.d += (uint64_t)c.d * v;
+}
+SHD void qacc_fold(qacc& acc) {
+  acc.a = fold31(acc.a);
+  acc.b = fold31(acc.b);
+  acc.c = fold31(acc.c);
+  acc.d = fold31(acc.d);
 }
 SHD qm31 qacc_reduce(const qacc& acc) {
   return {m31_reduce64(fold31(acc.a)), m31_reduce64(fold31(acc.b)), m31_reduce64(fold31(acc.c)), m31_reduce64(fold31(acc.d))};

@@ -1275,6 +1275,41 @@ ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   return r;
 }
 
+ProveResult prove_wide_fib_multi(Ctx& ctx, const std::vector<const uint32_t*>& trace, const std::vector<uint32_t>& logs,
+                                 const std::vector<uint32_t>& ncols, const PcsCfg& cfg) {
+  ProofArenaScope arena(ctx);
+  ProveResult r;
+  Timer tm;
+  Channel ch;
+  std::vector<DevTree> trees(2);
+  uint32_t max_n = 0;
+  std::vector<uint32_t> col_logs;
+  std::vector<Component> comps;
+  size_t start = 0;
+  for (size_t i = 0; i < logs.size(); ++i) {
+    max_n = std::max(max_n, logs[i]);
+    Component c;
+    c.kind = AIR_WIDE_FIB;
+    c.log_n = logs[i];
+    c.n_cols = ncols[i];
+    c.trace_start = start;
+    start += ncols[i];
+    col_logs.insert(col_logs.end(), ncols[i], logs[i]);
+    comps.push_back(c);
+  }
+  if (start != trace.size()) throw HipError(1, "trace column count does not match the components");
+  ctx.ensure_twiddles(max_n + 1 + cfg.log_blowup);
+  commit_tree(ctx, trees[0], cfg, ch);  // empty preprocessed tree
+  interpolate_columns(ctx, trace, col_logs, trees[1]);
+  commit_tree(ctx, trees[1], cfg, ch);
+  r.stage_ms[0] = tm.lap();
+  ProofParts parts;
+  r.status = prove_ex(ctx, ch, trees, comps, cfg, parts, r.stage_ms);
+  r.proof = encode(cfg, parts);
+  r.digest = ch.digest;
+  return r;
+}
+
 ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
                            const PcsCfg& cfg) {
   ProofArenaScope arena(ctx);

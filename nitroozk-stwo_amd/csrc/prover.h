@@ -69,6 +69,10 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup);
 
 ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t n_cols,
                            const PcsCfg& cfg);
+// Several wide-Fibonacci components (2^logs[i] rows, ncols[i] columns each) in one proof; trace
+// holds the components' columns in order.
+ProveResult prove_wide_fib_multi(Ctx& ctx, const std::vector<const uint32_t*>& trace, const std::vector<uint32_t>& logs,
+                                 const std::vector<uint32_t>& ncols, const PcsCfg& cfg);
 ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
                            const PcsCfg& cfg);
 

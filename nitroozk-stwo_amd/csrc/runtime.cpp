@@ -12,7 +12,7 @@ DevBuf::DevBuf(Ctx* c, size_t words, bool zero) : n(words), ctx(c) {
       p = (uint32_t*)c->bump_alloc(words * 4);
       owned = false;
     } else {
-      HIP_CHECK(hipMallocAsync((void**)&p, words * 4, c->stream));
+      HIP_CHECK(hipMallocFromPoolAsync((void**)&p, words * 4, c->pool, c->stream));
     }
     if (zero) launch_fill_zero(p, words, c->stream);
   }
@@ -55,11 +55,15 @@ DevBuf::~DevBuf() { reset(); }
 Ctx::Ctx(int dev) : device(dev) {
   HIP_CHECK(hipSetDevice(dev));
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
   HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-  hipMemPool_t pool;
-  HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+  // A pool of its own: blocks freed by this context are never handed to another context's stream,
+  // so contexts used side by side do not wait on each other's frees.
+  hipMemPoolProps props{};
+  props.allocType = hipMemAllocationTypePinned;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = dev;
+  HIP_CHECK(hipMemPoolCreate(&pool, &props));
   uint64_t thr = UINT64_MAX;  // keep freed blocks cached in the pool
   HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
   pinned_cap = 64u << 20;
@@ -68,7 +72,7 @@ Ctx::Ctx(int dev) : device(dev) {
   HIP_CHECK(hipMalloc((void**)&dev_arena, arena_cap));
 }
 Ctx::~Ctx() {
-  (void)hipStreamSynchronize(side);
+  if (side) (void)hipStreamSynchronize(side);
   (void)hipStreamSynchronize(stream);
   for (auto& p : pending) { event_pool.push_back(p.a); event_pool.push_back(p.b); }
   for (auto e : event_pool) (void)hipEventDestroy(e);
@@ -84,7 +88,9 @@ Ctx::~Ctx() {
   if (table_pinned) (void)hipHostFree(table_pinned);
   (void)hipEventDestroy(ev_fork);
   (void)hipEventDestroy(ev_join);
-  (void)hipStreamDestroy(side);
+  if (side) (void)hipStreamDestroy(side);
+  (void)hipStreamSynchronize(stream);
+  if (pool) (void)hipMemPoolDestroy(pool);
   (void)hipStreamDestroy(stream);
 }
 
@@ -142,12 +148,14 @@ void* Ctx::upload(const void* src, size_t bytes) {
   return d;
 }
 void Ctx::fork() {
+  if (!side) HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority
   commit_staged();
   side_busy = true;
   HIP_CHECK(hipEventRecord(ev_fork, stream));
   HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
 }
 void Ctx::join() {
+  if (!side) return;
   HIP_CHECK(hipEventRecord(ev_join, side));
   HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
 }
@@ -158,7 +166,7 @@ void Ctx::sync_main() {
 }
 void Ctx::sync() {
   commit_staged();
-  HIP_CHECK(hipStreamSynchronize(side));
+  if (side) HIP_CHECK(hipStreamSynchronize(side));
   side_busy = false;
   HIP_CHECK(hipStreamSynchronize(stream));
   uploads_in_flight = false;

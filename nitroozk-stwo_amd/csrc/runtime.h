@@ -82,7 +82,8 @@ struct Ctx {
   hipStream_t stream = nullptr;
   // Second stream for independent work that overlaps the main stream: fork() orders it after
   // everything queued on the main stream so far, join() orders the main stream after it.
-  hipStream_t side = nullptr;
+  hipStream_t side = nullptr;  // created on the first fork()
+  hipMemPool_t pool = nullptr;  // this context's device memory pool
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void fork();
   void join();
@@ -143,7 +144,7 @@ struct ProofArenaScope {
   ~ProofArenaScope() {
     if (--c.bump_depth == 0) {
       (void)hipStreamSynchronize(c.stream);
-      (void)hipStreamSynchronize(c.side);
+      if (c.side) (void)hipStreamSynchronize(c.side);
       c.bump_reset();
     }
   }

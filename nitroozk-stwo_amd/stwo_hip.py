@@ -74,6 +74,7 @@ EXPORTS = [
     "stwo_hip_fold_line", "stwo_hip_accumulate_numerators", "stwo_hip_compute_quotients_and_combine",
     "stwo_hip_accumulate", "stwo_hip_lift_and_accumulate", "stwo_hip_generate_secure_powers", "stwo_hip_grind",
     "stwo_hip_evaluate_constraint_quotients_on_domain", "stwo_hip_prove_wide_fibonacci", "stwo_hip_prove_poseidon",
+    "stwo_hip_prove_wide_fibonacci_multi",
     "stwo_proof_status", "stwo_proof_size", "stwo_proof_bytes", "stwo_proof_digest", "stwo_proof_claimed_sum",
     "stwo_proof_stage_ms", "stwo_proof_free", "stwo_hip_gen_trace_wide_fibonacci", "stwo_hip_gen_trace_poseidon",
     "stwo_hip_set_kernel_timing", "stwo_hip_kernel_stats", "stwo_hip_reset_kernel_stats",
@@ -130,6 +131,8 @@ def load_library(path=LIB_PATH):
                                                 c.POINTER(c.c_void_p)]
     L.stwo_hip_prove_poseidon.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.POINTER(PcsConfig),
                                           c.POINTER(c.c_void_p)]
+    L.stwo_hip_prove_wide_fibonacci_multi.argtypes = [c.c_void_p, _VPP, c.c_uint32, _U32P, _U32P,
+                                                      c.POINTER(PcsConfig), c.POINTER(c.c_void_p)]
     L.stwo_proof_status.argtypes = [c.c_void_p]
     L.stwo_proof_size.argtypes = [c.c_void_p]
     L.stwo_proof_size.restype = c.c_size_t
@@ -412,6 +415,17 @@ class HipBackend:
         h = ctypes.c_void_p()
         self._check(self.lib.stwo_hip_prove_wide_fibonacci(self.ctx, _ptrs(trace), len(trace), log_n_rows,
                                                            ctypes.byref(config), ctypes.byref(h)))
+        return Proof(self, h.value)
+
+    def prove_wide_fibonacci_multi(self, traces, log_n_rows, config):
+        """traces[i]: the columns of component i (2^log_n_rows[i] rows each)."""
+        flat = [col for t in traces for col in t]
+        logs = np.array(log_n_rows, np.uint32)
+        ncols = np.array([len(t) for t in traces], np.uint32)
+        h = ctypes.c_void_p()
+        self._check(self.lib.stwo_hip_prove_wide_fibonacci_multi(self.ctx, _ptrs(flat), len(traces),
+                                                                 logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
+                                                                 ctypes.byref(config), ctypes.byref(h)))
         return Proof(self, h.value)
 
     def prove_poseidon(self, trace, instances_per_row, log_n_rows, config):

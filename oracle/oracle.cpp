@@ -908,6 +908,91 @@ int oracle_verify(const OracleProveArgs* a, const uint8_t* proof, size_t len, co
   return e.empty() ? 0 : 1;
 }
 
+// Several wide-Fibonacci components in one proof (prove() over a component list, S/prover/mod.rs:28-150):
+// component i has 2^logs[i] rows and ncols[i] columns, its trace generated with seed + i; the
+// trace tree holds the components' columns in order (TraceLocationAllocator order).
+static PcsConfig cfg_of(const OracleProveArgs* a) {
+  PcsConfig cfg;
+  cfg.pow_bits = a->pow_bits;
+  cfg.fri.log_blowup_factor = a->log_blowup;
+  cfg.fri.log_last_layer_degree_bound = a->log_last_layer;
+  cfg.fri.n_queries = a->n_queries;
+  cfg.fri.line_fold_step = a->fold_step;
+  cfg.lifting_log_size = a->lifting_log_size;
+  return cfg;
+}
+int oracle_prove_wide_fib_multi(const OracleProveArgs* a, uint32_t n_comp, const uint32_t* logs, const uint32_t* ncols,
+                                uint8_t** proof_out, size_t* proof_len, uint8_t digest_out[32], double* prove_ms) {
+  try {
+    if (n_comp == 0) throw std::runtime_error("no components");
+    PcsConfig cfg = cfg_of(a);
+    uint32_t max_n = 0;
+    for (uint32_t i = 0; i < n_comp; ++i) max_n = std::max(max_n, logs[i]);
+    TwiddleTree tw = precompute_twiddles(canonic_domain(max_n + 1 + a->log_blowup).half_coset);
+    Channel ch;
+    SchemeProver sp{cfg, &tw, {}};
+    ComponentSet cs;
+    std::vector<std::vector<M31>> trace;
+    for (uint32_t i = 0; i < n_comp; ++i) {
+      Component c;
+      c.kind = WIDE_FIB;
+      c.log_n_rows = logs[i];
+      c.n_cols = ncols[i];
+      c.trace_col_start = trace.size();
+      auto t = gen_wide_fib(logs[i], ncols[i], a->input_mode, a->seed + i);
+      for (auto& col : t) trace.push_back(std::move(col));
+      cs.comps.push_back(c);
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    sp.commit({}, ch);
+    sp.commit(sp.interpolate_columns(std::move(trace)), ch);
+    StarkProof proof;
+    int st = prove_ex(cs, ch, sp, proof);
+    auto t1 = std::chrono::steady_clock::now();
+    if (prove_ms) *prove_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    auto bytes = serialize(proof);
+    *proof_out = (uint8_t*)malloc(bytes.size());
+    memcpy(*proof_out, bytes.data(), bytes.size());
+    *proof_len = bytes.size();
+    memcpy(digest_out, ch.digest.data(), 32);
+    return st;
+  } catch (const std::exception& e) {
+    fprintf(stderr, "oracle_prove_wide_fib_multi: %s\n", e.what());
+    return -1;
+  }
+}
+int oracle_verify_wide_fib_multi(const OracleProveArgs* a, uint32_t n_comp, const uint32_t* logs, const uint32_t* ncols,
+                                 const uint8_t* proof, size_t len, char* err, size_t errlen) {
+  std::string e;
+  try {
+    (void)a;
+    StarkProof p = deserialize(proof, len);
+    if (p.commitments.size() < 3) throw std::runtime_error("InvalidStructure");
+    uint32_t blow = p.config.fri.log_blowup_factor;
+    ComponentSet cs;
+    Channel ch;
+    std::vector<std::vector<uint32_t>> tlogs(2);
+    ch.mix_root(p.commitments[0]);
+    ch.mix_root(p.commitments[1]);
+    size_t start = 0;
+    for (uint32_t i = 0; i < n_comp; ++i) {
+      Component c;
+      c.kind = WIDE_FIB;
+      c.log_n_rows = logs[i];
+      c.n_cols = ncols[i];
+      c.trace_col_start = start;
+      start += ncols[i];
+      for (uint32_t k = 0; k < ncols[i]; ++k) tlogs[1].push_back(logs[i] + blow);
+      cs.comps.push_back(c);
+    }
+    e = verify(cs, ch, tlogs, p);
+  } catch (const std::exception& ex) {
+    e = ex.what();
+  }
+  if (err && errlen) { snprintf(err, errlen, "%s", e.c_str()); }
+  return e.empty() ? 0 : 1;
+}
+
 // ---- primitives, for kernel-level parity tests ----
 void oracle_blake2s(const uint8_t* data, size_t len, uint8_t out[32]) {
   auto h = Blake2s::hash(data, len);

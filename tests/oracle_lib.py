@@ -73,6 +73,11 @@ def lib():
         L.oracle_offset_bit_reversed_index.argtypes = [ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                                        ctypes.c_long]
         L.oracle_blake2s.argtypes = [ctypes.c_char_p, ctypes.c_size_t, u8p]
+        L.oracle_prove_wide_fib_multi.argtypes = [ctypes.POINTER(ProveArgs), ctypes.c_uint32, u32p, u32p,
+                                                  ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t), u8p,
+                                                  ctypes.POINTER(ctypes.c_double)]
+        L.oracle_verify_wide_fib_multi.argtypes = [ctypes.POINTER(ProveArgs), ctypes.c_uint32, u32p, u32p, u8p,
+                                                   ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -119,6 +124,38 @@ def verify(proof, claimed_sum=None, **kw):
     cs = np.zeros(4, np.uint32) if claimed_sum is None else np.asarray(claimed_sum, np.uint32).copy()
     err = ctypes.create_string_buffer(256)
     L.oracle_verify(ctypes.byref(a), u8ptr(buf), len(proof), u32ptr(cs), err, 256)
+    return err.value.decode()
+
+
+def prove_multi(logs, ncols, **kw):
+    """Several wide-Fibonacci components (component i seeded with seed + i) in one proof.
+    Returns (status, proof_bytes, final_digest, prove_ms)."""
+    L = lib()
+    a = make_args(**kw)
+    lg = np.asarray(logs, np.uint32).copy()
+    nc = np.asarray(ncols, np.uint32).copy()
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    dig = np.zeros(32, np.uint8)
+    ms = ctypes.c_double()
+    st = L.oracle_prove_wide_fib_multi(ctypes.byref(a), len(lg), u32ptr(lg), u32ptr(nc), ctypes.byref(out),
+                                       ctypes.byref(n), u8ptr(dig), ctypes.byref(ms))
+    if st < 0:
+        raise RuntimeError("oracle_prove_wide_fib_multi failed")
+    data = ctypes.string_at(out, n.value)
+    L.oracle_free(out)
+    return st, data, dig.tobytes(), ms.value
+
+
+def verify_multi(proof, logs, ncols, **kw):
+    """Returns '' when the multi-component proof is accepted, else the rejection reason."""
+    L = lib()
+    a = make_args(**kw)
+    lg = np.asarray(logs, np.uint32).copy()
+    nc = np.asarray(ncols, np.uint32).copy()
+    buf = np.frombuffer(proof, np.uint8).copy()
+    err = ctypes.create_string_buffer(256)
+    L.oracle_verify_wide_fib_multi(ctypes.byref(a), len(lg), u32ptr(lg), u32ptr(nc), u8ptr(buf), len(proof), err, 256)
     return err.value.decode()
 
 

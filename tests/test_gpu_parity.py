@@ -222,6 +222,22 @@ def test_wide_fib_other_configs(hb, cfg):
     assert O.verify(pr.bytes, air="wide_fib", log_n_rows=9, n_cols=20, **cfg) == ""
 
 
+@pytest.mark.parametrize("logs,ncols,cfg", [
+    ([5, 7, 4], [16, 9, 30], WF),
+    ([6, 6], [12, 20], dict(WF, log_blowup=2)),
+    ([8, 5, 6, 7], [10, 40, 8, 17], dict(WF, pow_bits=3, n_queries=5)),
+    ([12, 9, 10, 11, 9, 10], [24, 100, 8, 60, 33, 12], WF),
+])
+def test_wide_fib_multi_component_byte_identical(hb, logs, ncols, cfg):
+    say("wide-fib components", logs, ncols)
+    traces = [hb.gen_trace_wide_fibonacci(nc, lg, 1, 7 + i) for i, (lg, nc) in enumerate(zip(logs, ncols))]
+    pr = hb.prove_wide_fibonacci_multi(traces, logs, cfg_of(cfg))
+    st, proof, dig, _ = O.prove_multi(logs, ncols, input_mode=1, seed=7, **cfg)
+    assert pr.status == st == 0
+    assert pr.bytes == proof and pr.digest == dig
+    assert O.verify_multi(pr.bytes, logs, ncols, **cfg) == ""
+
+
 @pytest.mark.parametrize("log_n,ipr,mode", [(5, 1, 0), (6, 2, 1), (7, 8, 1), (10, 1, 1)])
 def test_poseidon_proof_byte_identical(hb, log_n, ipr, mode):
     say("poseidon proof", log_n, ipr, mode)

@@ -282,3 +282,21 @@ def test_poseidon_prove_reports_lifted_protocol_limit(ipr):
     assert O.verify(proof, claimed_sum=cs, log_n_rows=5, ipr=ipr, **PO) == "OodsNotMatching"
     st2, proof2, dig2, _, _ = O.prove(log_n_rows=5, ipr=ipr, input_mode=1, seed=5, **PO)
     assert proof2 == proof and dig2 == dig
+
+
+# ---- several components in one proof (the mixed-size workload) ----
+@pytest.mark.parametrize("logs,ncols,kw", [
+    ([5, 7, 4], [16, 9, 30], {}),
+    ([6, 6], [12, 20], dict(log_blowup=2)),
+    ([8, 5, 6, 7], [10, 40, 8, 17], dict(pow_bits=3, n_queries=5)),
+])
+def test_multi_component_prove_verify(logs, ncols, kw):
+    st, pf, _, _ = O.prove_multi(logs, ncols, input_mode=1, seed=7, **kw)
+    assert st == 0
+    assert O.verify_multi(pf, logs, ncols, **kw) == ""
+    # a different component split of the same columns is a different statement
+    assert O.verify_multi(pf, logs[::-1], ncols[::-1], **kw) != ""
+    for off in (40, len(pf) // 2, len(pf) - 20):
+        bad = bytearray(pf)
+        bad[off] ^= 0x10
+        assert O.verify_multi(bytes(bad), logs, ncols, **kw) != ""
