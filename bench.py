@@ -199,7 +199,7 @@ def serve(args, m):
         q = queue.Queue()
         for r in reqs:
             q.put(r)
-        lat, prove_ms, errors = [], [], []
+        lat, prove_ms, gen_ms, free_ms, errors = [], [], [], [], []
         lock = threading.Lock()
         t0 = time.perf_counter()
 
@@ -216,16 +216,21 @@ def serve(args, m):
                     s0 = time.perf_counter()
                     if kind == "wide_fib":
                         tr = hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed)
+                        s1 = time.perf_counter()
                         pr = hb.prove_wide_fibonacci(tr, log_rows, cfg)
                     else:
                         tr = hb.gen_trace_poseidon(width, log_rows, 1, args.seed)
+                        s1 = time.perf_counter()
                         pr = hb.prove_poseidon(tr, width, log_rows, cfg)
                     _ = pr.bytes
                     done = time.perf_counter()
                     for c in tr:
                         c.free()
+                    freed = time.perf_counter()
                     with lock:
-                        prove_ms.append((done - s0) * 1e3)
+                        gen_ms.append((s1 - s0) * 1e3)
+                        prove_ms.append((done - s1) * 1e3)
+                        free_ms.append((freed - done) * 1e3)
                         lat.append((done - t0 - arrival) * 1e3)
                 except Exception as e:  # reported in the JSON line
                     with lock:
@@ -248,6 +253,7 @@ def serve(args, m):
         pct = lambda p: round(lat[min(len(lat) - 1, int(round(p * len(lat))) - 1)], 3) if lat else None
         return {"contexts": n_ctx, "proofs": len(lat), "proofs_per_s": round(len(lat) / wall, 2),
                 "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_prove_ms": round(statistics.mean(prove_ms), 3),
+                "mean_trace_gen_ms": round(statistics.mean(gen_ms), 3), "mean_free_ms": round(statistics.mean(free_ms), 3),
                 "errors": errors[:3]}
 
     plain = run(1)

@@ -67,8 +67,10 @@ void launch_quotients(uint32_t lifting_log, const uint32_t* domain_xy, const Quo
 void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCols4 src, hipStream_t stream);
 
 // OODS: value of FFT-basis coefficient columns at QM31 points. Tasks are grouped by (point, size):
-// group_factors[g*32 + b] is the folding factor of bit b (y, x, pi(x), ...) and weights holds
-// 2^11 QM31 per group (scratch).
+// group_factors[g*32 + b] is the folding factor of bit b (y, x, pi(x), ...). Scratch: weights holds
+// 2^OODS_TABLE_LOG QM31 per group, partials oods_max_chunks(max_log) QM31 per task.
+constexpr uint32_t OODS_TABLE_LOG = 16;
+inline uint32_t oods_max_chunks(uint32_t max_log) { return max_log > 13 ? 1u << (max_log - 13) : 1u; }
 struct OodsTask { const uint32_t* coeffs; uint32_t log_size; uint32_t out_index; uint32_t pad[2]; };
 void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, uint32_t max_log,
                  const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,

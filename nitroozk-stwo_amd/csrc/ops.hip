@@ -332,26 +332,28 @@ void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, ui
 
 // ---- out-of-domain sampling on coefficients ------------------------------------------------
 // f(P) = sum_n c_n prod_b f_b^(n_b), f_0 = y, f_1 = x, f_b = pi^(b-1)(x). Tasks that share a
-// point and a size share a weight table W[lo] = prod_{b < 11} f_b^(lo_b) (2^11 entries). A
-// workgroup takes a chunk of 2^11 coefficients; lane t owns the 8 coefficients t*8 + j, whose
-// weights are W[j] * W[8t]; the chunk weight uses the factors of bits >= 11.
-constexpr uint32_t OODS_CHUNK_LOG = 11;
-
+// point and a size share a weight table W[i] = prod_{b < T} f_b^(i_b) over the low
+// T = min(k, 16) index bits (1 MiB per group, read through L2 by every column of the group). A
+// block takes 8192 coefficients of one column: each lane accumulates W[i] c_i for its 32
+// coefficients with a fold after every 2 products, the block sums its lanes,
+// and for columns longer than 2^16 the block sum is multiplied by the factors of its index bits
+// >= 16.
 __global__ void __launch_bounds__(256) oods_weights_kernel(const qm31* __restrict__ factors,
                                                            const uint32_t* __restrict__ group_log,
                                                            qm31* __restrict__ W) {
-  const uint32_t g = blockIdx.x;
+  const uint32_t g = blockIdx.y;
   const uint32_t k = group_log[g];
-  const uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
+  const uint32_t T = k < OODS_TABLE_LOG ? k : OODS_TABLE_LOG;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1u << T)) return;
   const qm31* f = factors + (size_t)g * 32;
-  for (uint32_t lo = threadIdx.x; lo < (1u << OODS_CHUNK_LOG); lo += blockDim.x) {
-    qm31 w = qm_one();
-    for (uint32_t b = 0; b < m; ++b)
-      if ((lo >> b) & 1) w = qm_mul(w, f[b]);
-    W[((size_t)g << OODS_CHUNK_LOG) + lo] = w;
-  }
+  qm31 w = qm_one();
+  for (uint32_t b = 0; b < T; ++b)
+    if ((i >> b) & 1) w = qm_mul(w, f[b]);
+  W[((size_t)g << OODS_TABLE_LOG) + i] = w;
 }
 
+constexpr uint32_t OODS_BLOCK_LOG = 13;  // coefficients per block
 __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restrict__ tasks,
                                                          const uint32_t* __restrict__ task_group,
                                                          const qm31* __restrict__ factors,
@@ -361,28 +363,34 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
   const OodsTask task = tasks[blockIdx.y];
   const uint32_t grp = task_group[blockIdx.y];
   const uint32_t k = task.log_size;
-  const uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
-  const uint32_t n_chunks = 1u << (k - m);
-  const uint32_t chunk = blockIdx.x;
-  if (chunk >= n_chunks) return;  // uniform per block
-  const qm31* W = Wall + ((size_t)grp << OODS_CHUNK_LOG);
+  const uint32_t n = 1u << k;
+  const uint32_t blk = blockIdx.x;
+  if (((size_t)blk << OODS_BLOCK_LOG) >= n) return;  // uniform per block
+  const uint32_t T = k < OODS_TABLE_LOG ? k : OODS_TABLE_LOG;
+  const uint32_t tmask = (1u << T) - 1;
+  const qm31* W = Wall + ((size_t)grp << OODS_TABLE_LOG);
   const uint32_t t = threadIdx.x;
-  const uint32_t* c = task.coeffs + ((size_t)chunk << m);
-  const uint32_t base_lo = t * 8;
-  qm31 out = qm_zero();
-  if (base_lo < (1u << m)) {
-    qacc acc = qacc_zero();
-    if (m >= 3) {
-      uint4 v0 = *reinterpret_cast<const uint4*>(c + base_lo);
-      uint4 v1 = *reinterpret_cast<const uint4*>(c + base_lo + 4);
-      qacc_mac(acc, W[0], v0.x); qacc_mac(acc, W[1], v0.y); qacc_mac(acc, W[2], v0.z); qacc_mac(acc, W[3], v0.w);
-      qacc_mac(acc, W[4], v1.x); qacc_mac(acc, W[5], v1.y); qacc_mac(acc, W[6], v1.z); qacc_mac(acc, W[7], v1.w);
-      out = qm_mul(qacc_reduce(acc), W[base_lo]);
-    } else {
-      for (uint32_t j = 0; j < (1u << m); ++j) qacc_mac(acc, W[j], c[j]);
-      out = qacc_reduce(acc);
+  const uint32_t base = blk << OODS_BLOCK_LOG;
+  qacc acc = qacc_zero();
+  if (n >= 4) {
+#pragma unroll 2
+    for (uint32_t r = 0; r < 8; ++r) {
+      const uint32_t i = base + 4 * (t + 256 * r);
+      if (i < n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(task.coeffs + i);
+        const uint32_t wi = i & tmask;
+        qacc_mad(acc, W[wi], v.x);
+        qacc_mad(acc, W[wi + 1], v.y);
+        qacc_fold(acc);
+        qacc_mad(acc, W[wi + 2], v.z);
+        qacc_mad(acc, W[wi + 3], v.w);
+        qacc_fold(acc);
+      }
     }
+  } else if (t < n) {
+    qacc_mad(acc, W[t], task.coeffs[t]);
   }
+  qm31 out = qacc_reduce(acc);
   // wave reduction by shuffles, then across the 4 waves
   for (uint32_t off = 32; off > 0; off >>= 1) {
     qm31 o = {(uint32_t)__shfl_down((int)out.a, off), (uint32_t)__shfl_down((int)out.b, off),
@@ -394,9 +402,9 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
   if (t == 0) {
     qm31 s = qm_add(qm_add(red[0], red[1]), qm_add(red[2], red[3]));
     const qm31* f = factors + (size_t)grp * 32;
-    for (uint32_t b = m; b < k; ++b)
-      if ((chunk >> (b - m)) & 1) s = qm_mul(s, f[b]);
-    partials[(size_t)blockIdx.y * max_chunks + chunk] = s;
+    for (uint32_t b = OODS_TABLE_LOG; b < k; ++b)
+      if ((base >> b) & 1) s = qm_mul(s, f[b]);
+    partials[(size_t)blockIdx.y * max_chunks + blk] = s;
   }
 }
 
@@ -405,8 +413,7 @@ __global__ void oods_reduce_kernel(const OodsTask* __restrict__ tasks, uint32_t 
   uint32_t tsk = blockIdx.x * blockDim.x + threadIdx.x;
   if (tsk >= n_tasks) return;
   uint32_t k = tasks[tsk].log_size;
-  uint32_t m = k < OODS_CHUNK_LOG ? k : OODS_CHUNK_LOG;
-  uint32_t n_chunks = 1u << (k - m);
+  uint32_t n_chunks = k > OODS_BLOCK_LOG ? 1u << (k - OODS_BLOCK_LOG) : 1u;
   qm31 s = qm_zero();
   for (uint32_t i = 0; i < n_chunks; ++i) s = qm_add(s, partials[(size_t)tsk * max_chunks + i]);
   out[tasks[tsk].out_index] = s;
@@ -416,8 +423,10 @@ void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_t
                  const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,
                  qm31* partials, qm31* out, hipStream_t stream) {
   if (n_tasks == 0) return;
-  uint32_t max_chunks = max_log > OODS_CHUNK_LOG ? 1u << (max_log - OODS_CHUNK_LOG) : 1u;
-  hipLaunchKernelGGL(oods_weights_kernel, dim3(n_groups), dim3(256), 0, stream, group_factors, group_log, weights);
+  const uint32_t max_chunks = oods_max_chunks(max_log);
+  const uint32_t tab = max_log < OODS_TABLE_LOG ? max_log : OODS_TABLE_LOG;
+  hipLaunchKernelGGL(oods_weights_kernel, dim3(((1u << tab) + 255) / 256, n_groups), dim3(256), 0, stream,
+                     group_factors, group_log, weights);
   hipLaunchKernelGGL(oods_chunk_kernel, dim3(max_chunks, n_tasks), dim3(256), 0, stream, tasks, task_group,
                      group_factors, weights, partials, max_chunks);
   hipLaunchKernelGGL(oods_reduce_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, stream, tasks, n_tasks, partials,

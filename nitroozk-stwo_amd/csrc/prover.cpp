@@ -830,9 +830,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   HMARK("oods_prep");
   S.val.resize(n_vals);
   if (n_vals) {
-    uint32_t max_chunks = max_coeff_log > 11 ? 1u << (max_coeff_log - 11) : 1u;
+    uint32_t max_chunks = oods_max_chunks(max_coeff_log);
     DevBuf partials(&ctx, (size_t)n_vals * max_chunks * 4), dvals(&ctx, n_vals * 4);
-    DevBuf weights(&ctx, group_key.size() * ((size_t)4 << 11));
+    DevBuf weights(&ctx, group_key.size() * ((size_t)4 << OODS_TABLE_LOG));
     double obytes = 0;
     for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
     const OodsTask* dt = ctx.table_vec(tasks);

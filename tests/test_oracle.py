@@ -300,3 +300,21 @@ def test_multi_component_prove_verify(logs, ncols, kw):
         bad = bytearray(pf)
         bad[off] ^= 0x10
         assert O.verify_multi(bytes(bad), logs, ncols, **kw) != ""
+
+
+@pytest.mark.parametrize("log_n", [2, 4, 7])
+def test_barycentric_weights_reproduce_low_degree_functions(log_n):
+    """sum_i w_i f(x_i) = f(p) for f = 1, x and y (all in the span of the FFT basis for log_n >= 2),
+    with x_i the bit-reversed CanonicCoset(log_n) domain: a closed form the weights must satisfy."""
+    dig = np.frombuffer(hashlib.blake2s(b"bary" + bytes([log_n]), digest_size=32).digest(), np.uint8).copy()
+    pt = np.zeros(8, np.uint32)
+    O.lib().oracle_random_point(O.u8ptr(dig), O.u32ptr(pt))
+    w = O.barycentric_weights(log_n, pt).astype(object)  # (4, 2^log_n)
+    n = 1 << log_n
+    pts = [pyref.canonic_domain_point(log_n, pyref.bitrev(i, log_n)) for i in range(n)]
+    one = tuple(int(sum(w[k][i] for i in range(n)) % P) for k in range(4))
+    sx = tuple(int(sum(w[k][i] * pts[i][0] for i in range(n)) % P) for k in range(4))
+    sy = tuple(int(sum(w[k][i] * pts[i][1] for i in range(n)) % P) for k in range(4))
+    assert one == (1, 0, 0, 0)
+    assert sx == tuple(map(int, pt[:4]))
+    assert sy == tuple(map(int, pt[4:]))
