@@ -51,7 +51,9 @@ int stwo_hip_synchronize(stwo_hip_ctx* ctx);
 const char* stwo_hip_last_error(stwo_hip_ctx* ctx);
 
 /* cuda_malloc_uint32_t / cuda_alloc_zeroes_uint32_t / cuda_free_memory and the copies
- * (bindings.rs:84-131): Column::zeros / uninitialized / to_cpu / from_iter. */
+ * (bindings.rs:84-131): Column::zeros / uninitialized / to_cpu / from_iter. stwo_hip_free takes
+ * only addresses stwo_hip_alloc returned on the same context (status 1 otherwise); freed blocks
+ * are kept by the context and reused, in stream order, by later allocations of the same size. */
 int stwo_hip_alloc(stwo_hip_ctx* ctx, size_t n_words, int zeroed, uint32_t** out);
 int stwo_hip_free(stwo_hip_ctx* ctx, uint32_t* ptr);
 int stwo_hip_copy_h2d(stwo_hip_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t n_words);

@@ -25,7 +25,8 @@ __device__ __forceinline__ void acc_add_qm(qacc& acc, qm31 v) {
 __global__ void __launch_bounds__(256) wide_fib_comp_kernel(uint32_t n_rows, uint32_t trace_log,
                                                             const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                             const qm31* __restrict__ pows,
-                                                            const uint32_t* __restrict__ denom_inv, Cols4 acc_out) {
+                                                            const uint32_t* __restrict__ denom_inv, Cols4 acc_out,
+                                                            uint32_t accumulate) {
   uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n_rows) return;
   uint32_t a = cols[0][row], b = cols[1][row];
@@ -39,16 +40,16 @@ __global__ void __launch_bounds__(256) wide_fib_comp_kernel(uint32_t n_rows, uin
     b = c;
   }
   qm31 r = qm_mul_m31(qacc_reduce(acc), denom_inv[row >> trace_log]);
-  qm31 prev = {acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]};
-  r = qm_add(prev, r);
+  if (accumulate) r = qm_add(qm31{acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]}, r);
   acc_out.c[0][row] = r.a; acc_out.c[1][row] = r.b; acc_out.c[2][row] = r.c; acc_out.c[3][row] = r.d;
 }
 
 void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* cols, uint32_t n_cols,
-                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream) {
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, bool accumulate,
+                                 hipStream_t stream) {
   uint32_t n = 1u << eval_log;
   hipLaunchKernelGGL(wide_fib_comp_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, trace_log, cols, n_cols,
-                     pows, denom_inv, acc);
+                     pows, denom_inv, acc, (uint32_t)accumulate);
 }
 
 // ---- Poseidon2 -------------------------------------------------------------------------------
@@ -182,7 +183,8 @@ template <class Cols>
 __global__ void __launch_bounds__(512) poseidon_comp_kernel(uint32_t eval_log, uint32_t trace_log, Cols trace,
                                                             Cols inter, const PoseidonParams* __restrict__ params,
                                                             const qm31* __restrict__ pows,
-                                                            const uint32_t* __restrict__ denom_inv, Cols4 acc_out) {
+                                                            const uint32_t* __restrict__ denom_inv, Cols4 acc_out,
+                                                            uint32_t accumulate) {
   __shared__ uint32_t part[PSD_MAX_WAVES][4][64];
   // the wave index is uniform across the wave: readfirstlane lets the compiler keep the instance
   // loop and the constraint coefficients in scalar registers
@@ -231,29 +233,31 @@ __global__ void __launch_bounds__(512) poseidon_comp_kernel(uint32_t eval_log, u
     for (uint32_t v = 0; v < W; ++v)
       tot = qm_add(tot, qm31{part[v][0][lane], part[v][1][lane], part[v][2][lane], part[v][3][lane]});
     tot = qm_mul_m31(tot, denom_inv[row >> trace_log]);
-    const qm31 prev = {acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]};
-    const qm31 res = qm_add(prev, tot);
-    acc_out.c[0][row] = res.a; acc_out.c[1][row] = res.b; acc_out.c[2][row] = res.c; acc_out.c[3][row] = res.d;
+    if (accumulate)
+      tot = qm_add(qm31{acc_out.c[0][row], acc_out.c[1][row], acc_out.c[2][row], acc_out.c[3][row]}, tot);
+    acc_out.c[0][row] = tot.a; acc_out.c[1][row] = tot.b; acc_out.c[2][row] = tot.c; acc_out.c[3][row] = tot.d;
   }
 }
 
 void launch_poseidon_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* trace_cols,
                                  const uint32_t* const* inter_cols, const PoseidonParams* params, uint32_t ipr,
-                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream) {
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, bool accumulate,
+                                 hipStream_t stream) {
   uint32_t n = 1u << eval_log;
   uint32_t waves = ipr < PSD_MAX_WAVES ? ipr : PSD_MAX_WAVES;
   hipLaunchKernelGGL(poseidon_comp_kernel<ColTable>, dim3((n + 63) / 64), dim3(64 * waves), 0, stream, eval_log,
-                     trace_log, ColTable{trace_cols}, ColTable{inter_cols}, params, pows, denom_inv, acc);
+                     trace_log, ColTable{trace_cols}, ColTable{inter_cols}, params, pows, denom_inv, acc,
+                     (uint32_t)accumulate);
 }
 void launch_poseidon_composition_strided(uint32_t eval_log, uint32_t trace_log, const uint32_t* trace_base,
                                          const uint32_t* inter_base, size_t stride, const PoseidonParams* params,
                                          uint32_t ipr, const qm31* pows, const uint32_t* denom_inv, Cols4 acc,
-                                         hipStream_t stream) {
+                                         bool accumulate, hipStream_t stream) {
   uint32_t n = 1u << eval_log;
   uint32_t waves = ipr < PSD_MAX_WAVES ? ipr : PSD_MAX_WAVES;
   hipLaunchKernelGGL(poseidon_comp_kernel<ColStrided>, dim3((n + 63) / 64), dim3(64 * waves), 0, stream, eval_log,
                      trace_log, ColStrided{trace_base, stride}, ColStrided{inter_base, stride}, params, pows,
-                     denom_inv, acc);
+                     denom_inv, acc, (uint32_t)accumulate);
 }
 
 // ---- trace generation -----------------------------------------------------------------------

@@ -86,12 +86,12 @@ const char* stwo_hip_last_error(stwo_hip_ctx* h) { return h ? h->err.c_str() : "
 int stwo_hip_alloc(stwo_hip_ctx* h, size_t n, int zeroed, uint32_t** out) {
   return guarded(h, [&](Ctx& c) {
     if (!out) throw HipError(1, "null out");
-    HIP_CHECK(hipMallocFromPoolAsync((void**)out, n ? n * 4 : 4, c.pool, c.stream));
+    *out = (uint32_t*)c.col_alloc(n ? n * 4 : 4);
     if (zeroed) launch_fill_zero(*out, n, c.stream);
   });
 }
 int stwo_hip_free(stwo_hip_ctx* h, uint32_t* p) {
-  return guarded(h, [&](Ctx& c) { if (p) HIP_CHECK(hipFreeAsync(p, c.stream)); });
+  return guarded(h, [&](Ctx& c) { if (p) c.col_free(p); });
 }
 int stwo_hip_copy_h2d(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_t n) {
   return guarded(h, [&](Ctx& c) {
@@ -399,7 +399,7 @@ int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* 
     if (id == fnv1a("fibonacci_example")) {
       if (n_trace < 2 || n_constraints != n_trace - 2) throw HipError(1, "wide-fib: n_constraints must be n_trace - 2");
       launch_wide_fib_composition(eval_log, trace_log, (const uint32_t* const*)c.upload_vec(tv), n_trace,
-                                  c.upload_vec(pows), c.upload_vec(dinv), a4, c.stream);
+                                  c.upload_vec(pows), c.upload_vec(dinv), a4, true, c.stream);
     } else if (id == fnv1a("poseidon_example")) {
       if (blob_len < 4 + 4 + 32) throw HipError(1, "poseidon eval_blob too short");
       PoseidonParams pp{};
@@ -416,7 +416,7 @@ int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* 
       std::vector<const uint32_t*> iv(interaction, interaction + n_interaction);
       launch_poseidon_composition(eval_log, trace_log, (const uint32_t* const*)c.upload_vec(tv),
                                   (const uint32_t* const*)c.upload_vec(iv), (const PoseidonParams*)c.upload(&pp, sizeof(pp)),
-                                  pp.ipr, c.upload_vec(pows), c.upload_vec(dinv), a4, c.stream);
+                                  pp.ipr, c.upload_vec(pows), c.upload_vec(dinv), a4, true, c.stream);
     } else {
       throw HipError(5, "no device kernel for this component");
     }

@@ -42,8 +42,9 @@ void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const
                       hipStream_t stream);
 
 // Variants that read the folding coefficient from device memory (drawn by the device channel);
-// fold_line squares it n_squarings times first (fold steps > 1).
-void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev,
+// fold_line squares it n_squarings times first (fold steps > 1). With dst_zero the circle fold
+// treats dst as zero (writes without reading it).
+void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev, bool dst_zero,
                                       const uint32_t* itree, uint32_t tree_len, hipStream_t stream);
 void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev, uint32_t n_squarings,
                           const uint32_t* itree, uint32_t tree_len, hipStream_t stream);
@@ -87,20 +88,23 @@ void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, ui
 void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
                          uint32_t* out, hipStream_t stream);
 
-// Composition (constraint quotients on the evaluation domain), accumulated into acc.
+// Composition (constraint quotients on the evaluation domain), added into acc (accumulate) or
+// written to it.
 void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* cols, uint32_t n_cols,
-                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, bool accumulate,
+                                 hipStream_t stream);
 struct PoseidonParams { qm31 z; qm31 alpha_pow[16]; qm31 shift; uint32_t ipr; uint32_t pad[3]; };
 // ipr (instances per row, also in params) sets the launch shape: min(ipr, 8) waves per 64 rows.
 void launch_poseidon_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* trace_cols,
                                  const uint32_t* const* inter_cols, const PoseidonParams* params, uint32_t ipr,
-                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, hipStream_t stream);
+                                 const qm31* pows, const uint32_t* denom_inv, Cols4 acc, bool accumulate,
+                                 hipStream_t stream);
 // The same with trace column c at trace_base + c stride and interaction column c at
 // inter_base + c stride (column blocks of one allocation each).
 void launch_poseidon_composition_strided(uint32_t eval_log, uint32_t trace_log, const uint32_t* trace_base,
                                          const uint32_t* inter_base, size_t stride, const PoseidonParams* params,
                                          uint32_t ipr, const qm31* pows, const uint32_t* denom_inv, Cols4 acc,
-                                         hipStream_t stream);
+                                         bool accumulate, hipStream_t stream);
 
 // Trace generation and the Poseidon LogUp interaction trace.
 void launch_gen_wide_fib(uint32_t log_n, uint32_t* const* cols, uint32_t n_cols, uint32_t mode, uint64_t seed,

@@ -117,69 +117,40 @@ void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const ui
 
 // Up to 8 levels of a Merkle tree with no injected columns in one launch. Workgroup w reads the
 // 2^L children [w 2^L, (w+1) 2^L) of level log_child and writes its 2^(L-j) nodes of level
-// log_child - j (j = 1..L); the levels in between stay in LDS.
+// log_child - j (j = 1..L); the levels in between stay in LDS. The levels are latency-bound (each
+// waits for the one below), so every node is hashed by a quad of lanes (blake2s_compress_quad):
+// 4 * 2^(L-1) threads, lane q of a quad holding chaining words q and 4 + q.
 struct LayerPtrs { uint32_t* p[8]; };
-__global__ void __launch_bounds__(128) merkle_subtree_kernel(const uint32_t* __restrict__ child, uint32_t L,
+__global__ void __launch_bounds__(512) merkle_subtree_kernel(const uint32_t* __restrict__ child, uint32_t L,
                                                              LayerPtrs outs) {
   __shared__ uint32_t lds[128 * 8];
-  const uint32_t w = blockIdx.x, i = threadIdx.x;
+  const uint32_t w = blockIdx.x, nd = threadIdx.x >> 2, q = threadIdx.x & 3;
   uint32_t n = 1u << (L - 1);  // nodes of the first level computed by this workgroup
-  if (i < n) {
-    const uint4* p4 = reinterpret_cast<const uint4*>(child + ((size_t)w * 2 * n + 2 * i) * 8);
-    uint32_t m[16];
+  for (uint32_t j = 0; j < L; ++j, n >>= 1) {
+    uint32_t hA = 0, hB = 0;
+    if (nd < n) {
+      uint32_t m[16];
+      if (j == 0) {
+        const uint4* p4 = reinterpret_cast<const uint4*>(child + ((size_t)w * 2 * n + 2 * nd) * 8);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      uint4 v = p4[k];
-      m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
-    }
-    uint32_t h[8];
-    blake2s_init(h);
-    blake2s_compress(h, m, 64, true);
-    uint4* o4 = reinterpret_cast<uint4*>(outs.p[0] + ((size_t)w * n + i) * 8);
-    o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
-    o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) lds[i * 8 + k] = h[k];
-  }
-  __syncthreads();
-  for (uint32_t j = 1; j < L; ++j) {
-    n >>= 1;
-    if (n > blockDim.x / 4) {
-      uint32_t h[8];
-      if (i < n) {
-        uint32_t m[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) m[k] = lds[2 * i * 8 + k];
-        blake2s_init(h);
-        blake2s_compress(h, m, 64, true);
-      }
-      __syncthreads();
-      if (i < n) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) lds[i * 8 + k] = h[k];
-        uint4* o4 = reinterpret_cast<uint4*>(outs.p[j] + ((size_t)w * n + i) * 8);
-        o4[0] = make_uint4(h[0], h[1], h[2], h[3]);
-        o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
-      }
-    } else {
-      // narrow level: one quad of lanes per node
-      const uint32_t nd = i >> 2, q = i & 3;
-      uint32_t hA = 0, hB = 0;
-      if (nd < n) {
-        uint32_t m[16];
+        for (int k = 0; k < 4; ++k) {
+          const uint4 v = p4[k];
+          m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
+        }
+      } else {
 #pragma unroll
         for (int k = 0; k < 16; ++k) m[k] = lds[2 * nd * 8 + k];
-        blake2s_init_quad(q, hA, hB);
-        blake2s_compress_quad(hA, hB, m, 64, true, q);
       }
-      __syncthreads();
-      if (nd < n) {
-        lds[nd * 8 + q] = hA;
-        lds[nd * 8 + 4 + q] = hB;
-        uint32_t* o = outs.p[j] + ((size_t)w * n + nd) * 8;
-        o[q] = hA;
-        o[4 + q] = hB;
-      }
+      blake2s_init_quad(q, hA, hB);
+      blake2s_compress_quad(hA, hB, m, 64, true, q);
+    }
+    __syncthreads();
+    if (nd < n) {
+      lds[nd * 8 + q] = hA;
+      lds[nd * 8 + 4 + q] = hB;
+      uint32_t* o = outs.p[j] + ((size_t)w * n + nd) * 8;
+      o[q] = hA;
+      o[4 + q] = hB;
     }
     __syncthreads();
   }
@@ -191,7 +162,7 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
   LayerPtrs lp{};
   for (uint32_t j = 0; j < levels; ++j) lp.p[j] = outs[j];
   uint32_t wgs = 1u << (log_child - levels);
-  uint32_t threads = 1u << (levels - 1);
+  uint32_t threads = 4u << (levels - 1);
   if (threads < 64) threads = 64;
   hipLaunchKernelGGL(merkle_subtree_kernel, dim3(wgs), dim3(threads), 0, stream, child, levels, lp);
 }

@@ -32,7 +32,7 @@ __device__ __forceinline__ uint32_t circle_itw(const uint32_t* __restrict__ lt0,
 
 __global__ void __launch_bounds__(256) fold_circle_kernel(uint32_t n_out, CCols4 src, Cols4 dst, qm31 alpha,
                                                           qm31 alpha2, const uint32_t* __restrict__ lt0,
-                                                          const qm31* __restrict__ alpha_dev) {
+                                                          const qm31* __restrict__ alpha_dev, uint32_t dst_zero) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
   if (alpha_dev) {  // folding coefficient drawn on the device
@@ -44,8 +44,8 @@ __global__ void __launch_bounds__(256) fold_circle_kernel(uint32_t n_out, CCols4
   qm31 f0 = qm_add(fp, fn);
   qm31 f1 = qm_mul_m31(qm_sub(fp, fn), t);
   qm31 fr = qm_add(f0, qm_mul(alpha, f1));
-  qm31 d = load_q((const uint32_t* const*)dst.c, i);
-  store_q(dst.c, i, qm_add(qm_mul(d, alpha2), fr));
+  if (!dst_zero) fr = qm_add(qm_mul(load_q((const uint32_t* const*)dst.c, i), alpha2), fr);
+  store_q(dst.c, i, fr);
 }
 
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
@@ -53,14 +53,14 @@ void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 
   uint32_t n = 1u << (log_src - 1);
   const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
   hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha,
-                     qm_mul(alpha, alpha), lt0, (const qm31*)nullptr);
+                     qm_mul(alpha, alpha), lt0, (const qm31*)nullptr, 0u);
 }
-void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev,
+void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev, bool dst_zero,
                                       const uint32_t* itree, uint32_t tree_len, hipStream_t stream) {
   uint32_t n = 1u << (log_src - 1);
   const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
   hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, qm_zero(),
-                     qm_zero(), lt0, alpha_dev);
+                     qm_zero(), lt0, alpha_dev, (uint32_t)dst_zero);
 }
 
 __global__ void __launch_bounds__(256) fold_line_kernel(uint32_t n_out, CCols4 src, Cols4 dst, qm31 alpha,

@@ -670,7 +670,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     uint32_t E = c.eval_log();
     ctx.ensure_twiddles(E);
     auto& acc = accs[E];
-    if (acc.empty()) for (int k = 0; k < 4; ++k) acc.emplace_back(&ctx, (size_t)1 << E, true);
+    // the first component of an evaluation size writes the accumulator, later ones add to it
+    const bool accumulate = !acc.empty();
+    if (acc.empty()) for (int k = 0; k < 4; ++k) acc.emplace_back(&ctx, (size_t)1 << E);
     Cols4 a4 = {{acc[0].p, acc[1].p, acc[2].p, acc[3].p}};
     // columns on the evaluation domain: committed evaluations when sizes match, else extended
     std::vector<DevBuf> ext_t, ext_i, ext_keep;
@@ -705,7 +707,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     if (c.kind == AIR_WIDE_FIB) {
       ctx.commit_staged();
       KScope ks(ctx, K_COMPOSITION, cbytes);
-      launch_wide_fib_composition(E, c.log_n, d_t, c.n_cols, d_pows, d_dinv, a4, ctx.stream);
+      launch_wide_fib_composition(E, c.log_n, d_t, c.n_cols, d_pows, d_dinv, a4, accumulate, ctx.stream);
     } else {
       auto icols = cols_on_domain(trees[2], c.inter_start, c.n_inter_cols(), ext_i);
       PoseidonParams pp{};
@@ -719,9 +721,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       const size_t stride = (size_t)1 << E;
       if (is_strided(tcols, stride) && is_strided(icols, stride))
         launch_poseidon_composition_strided(E, c.log_n, tcols[0], icols[0], stride, d_pp, c.ipr, d_pows, d_dinv, a4,
-                                            ctx.stream);
+                                            accumulate, ctx.stream);
       else
-        launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, c.ipr, d_pows, d_dinv, a4, ctx.stream);
+        launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, c.ipr, d_pows, d_dinv, a4, accumulate, ctx.stream);
     }
   }
   // finalize: lift smaller accumulations into the largest, then interpolate 4 coordinate columns
@@ -1074,16 +1076,16 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   const qm31* alpha0 = mix_and_draw(first, 0);
   std::vector<FriCol> inner;
   FriCol cur;
+  // the first line layer starts at zero (FriProver::commit): the fold writes it without reading
   if (tail && l_start == Lc - 1) {
     cur = tail_col(Lc - 1);
-    for (int k = 0; k < 4; ++k) launch_fill_zero(cur.c[k].p, cur.c[k].n, ctx.stream);
   } else {
     cur.log = Lc - 1;
-    for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log, true);
+    for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log);
   }
   {
-    KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 32.0 * (double)(1u << (Lc - 1)));
-    launch_fold_circle_into_line_dev(Lc, first.cc(), cur.mc(), alpha0, ctx.itw.p, ctx.tree_len(), ctx.stream);
+    KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 16.0 * (double)(1u << (Lc - 1)));
+    launch_fold_circle_into_line_dev(Lc, first.cc(), cur.mc(), alpha0, true, ctx.itw.p, ctx.tree_len(), ctx.stream);
   }
   auto fold_layer = [&](FriCol& src, uint32_t fs, const qm31* a) {
     FriCol* s = &src;

@@ -79,6 +79,9 @@ Ctx::~Ctx() {
   tw.reset();
   itw.reset();
   domain_points.clear();
+  for (auto& [bytes, v] : col_cache)
+    for (void* p : v) (void)hipFreeAsync(p, stream);
+  for (auto& [p, bytes] : col_live) (void)hipFreeAsync(p, stream);
   for (auto* p : overflow) (void)hipFree(p);
   (void)hipStreamSynchronize(stream);
   for (auto& ch : bump_chunks) (void)hipFree(ch.first);
@@ -122,6 +125,32 @@ void Ctx::commit_staged() {
     uploads_in_flight = true;
   }
   staged_from = pinned_off;
+}
+void* Ctx::col_alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  void* p = nullptr;
+  auto it = col_cache.find(bytes);
+  if (it != col_cache.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+    col_cached_bytes -= bytes;
+  } else {
+    HIP_CHECK(hipMallocFromPoolAsync(&p, bytes, pool, stream));
+  }
+  col_live[p] = bytes;
+  return p;
+}
+void Ctx::col_free(void* p) {
+  auto it = col_live.find(p);
+  if (it == col_live.end()) throw HipError(1, "free of an address not allocated by this context");
+  const size_t bytes = it->second;
+  col_live.erase(it);
+  if (col_cached_bytes + bytes > COL_CACHE_CAP) {
+    HIP_CHECK(hipFreeAsync(p, stream));
+    return;
+  }
+  col_cache[bytes].push_back(p);  // reused by later work on this context's stream, in order
+  col_cached_bytes += bytes;
 }
 const void* Ctx::upload_table(const void* src, size_t bytes) {
   std::string key((const char*)src, bytes);

@@ -84,6 +84,15 @@ struct Ctx {
   // everything queued on the main stream so far, join() orders the main stream after it.
   hipStream_t side = nullptr;  // created on the first fork()
   hipMemPool_t pool = nullptr;  // this context's device memory pool
+  // Column allocations handed to callers (stwo_hip_alloc): released blocks are kept per size and
+  // handed out again in stream order, so a serving loop that allocates and frees the same trace
+  // shapes makes no allocator calls after its first request. Cached bytes are capped.
+  void* col_alloc(size_t bytes);
+  void col_free(void* p);
+  std::unordered_map<void*, size_t> col_live;
+  std::unordered_map<size_t, std::vector<void*>> col_cache;
+  size_t col_cached_bytes = 0;
+  static constexpr size_t COL_CACHE_CAP = (size_t)32 << 30;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void fork();
   void join();
