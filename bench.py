@@ -236,10 +236,21 @@ def serve(args, m):
                     with lock:
                         errors.append(str(e))
 
-        # one warm-up proof per context (twiddles, arenas)
+        # warm-up per context, untimed: the largest proof of each kind (twiddles, proof arena) and
+        # one trace of every column size at its widest (column cache)
+        widest = {}
+        for kind, log_rows, width, _ in reqs:
+            widest[(kind, log_rows)] = max(width, widest.get((kind, log_rows), 0))
+        big = {k: max((lr for (kd, lr) in widest if kd == k), default=None) for k in ("wide_fib", "poseidon")}
         for hb in backends:
-            tr = hb.gen_trace_poseidon(8, 9, 1, args.seed)
-            hb.prove_poseidon(tr, 8, 9, cfg)
+            for (kind, log_rows), width in widest.items():
+                tr = (hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed) if kind == "wide_fib"
+                      else hb.gen_trace_poseidon(width, log_rows, 1, args.seed))
+                if big[kind] == log_rows:
+                    (hb.prove_wide_fibonacci(tr, log_rows, cfg) if kind == "wide_fib"
+                     else hb.prove_poseidon(tr, width, log_rows, cfg))
+                for c in tr:
+                    c.free()
         t0 = time.perf_counter()
         threads = [threading.Thread(target=worker, args=(hb,)) for hb in backends]
         for th in threads:
