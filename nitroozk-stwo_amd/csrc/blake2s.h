@@ -104,9 +104,17 @@ __device__ __forceinline__ uint32_t quad_rot(uint32_t x) {
     c = quad_rot<B2Q_ROT2>(c);                                                              \
     d = quad_rot<B2Q_ROT1>(d);                                                              \
   } while (0)
-__device__ __forceinline__ void blake2s_compress_quad(uint32_t& hA, uint32_t& hB, const uint32_t m[16], uint32_t t,
+__device__ __forceinline__ void blake2s_compress_quad(uint32_t& hA, uint32_t& hB, const uint32_t msg[16], uint32_t t,
                                                       bool last, uint32_t j) {
   const bool j1 = (j & 1) != 0, j2 = (j & 2) != 0;
+  // The message words pass through an empty asm so that each lane's per-round choice among them
+  // stays a register select: folded into an indexed load, the array would go to scratch memory.
+  uint32_t m[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    m[k] = msg[k];
+    asm volatile("" : "+v"(m[k]));
+  }
   uint32_t a = hA, b = hB;
   uint32_t c = B2Q_SEL(B2_IV0, B2_IV1, B2_IV2, B2_IV3);
   uint32_t d = B2Q_SEL(B2_IV4 ^ t, B2_IV5, last ? ~B2_IV6 : B2_IV6, B2_IV7);

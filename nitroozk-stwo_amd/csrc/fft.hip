@@ -178,9 +178,16 @@ __global__ void __launch_bounds__(1u << (S - 5)) fft_low_kernel(const uint32_t* 
                                                                 uint32_t tree_len, uint32_t scale) {
   constexpr uint32_t N = 1u << S, T = 1u << (S - 5);
   __shared__ uint32_t lds[N + (N >> 5)];
-  const uint32_t* __restrict__ in = src[blockIdx.y];
-  uint32_t* __restrict__ out = dst[blockIdx.y];
-  const uint32_t gbase = blockIdx.x << S;
+  // One block per (column, 2^S-block of the column) on a 1-D grid. Workgroups go to the 8 XCDs in
+  // turn (workgroup n to XCD n mod 8); renumbering them so that each XCD takes a contiguous range
+  // keeps the blocks of one column on one XCD and its L2, which matters when evaluation onto a
+  // larger domain makes every block of a column read the same coefficients.
+  const uint32_t n_wg = gridDim.x, hw = blockIdx.x;
+  const uint32_t wg = (n_wg & 7) == 0 ? (hw & 7) * (n_wg >> 3) + (hw >> 3) : hw;
+  const uint32_t col = wg >> (log_n - S);
+  const uint32_t* __restrict__ in = src[col];
+  uint32_t* __restrict__ out = dst[col];
+  const uint32_t gbase = (wg & ((1u << (log_n - S)) - 1)) << S;
   const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
   if (vec) {
     // 8 independent 16-byte loads per thread, all in flight before the first LDS write
@@ -231,7 +238,7 @@ template <bool INV>
 static void launch_low(uint32_t S, uint32_t n_cols, const uint32_t* const* src, uint32_t* const* dst, uint32_t log_n,
                        uint32_t in_mask, const uint32_t* tree, uint32_t tree_len, uint32_t scale,
                        hipStream_t stream) {
-  const dim3 grid(1u << (log_n - S), n_cols), block(1u << (S - 5));
+  const dim3 grid(n_cols << (log_n - S)), block(1u << (S - 5));
   switch (S) {
     case 11: hipLaunchKernelGGL((fft_low_kernel<INV, 11>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
     case 12: hipLaunchKernelGGL((fft_low_kernel<INV, 12>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;

@@ -30,11 +30,6 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
 void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream);
 
-// FRI layers of at most 2^12 values (l_start <= 12) in one workgroup: Merkle trees, device
-// channel and fold_line per layer; see merkle.hip for the layouts of vals, nodes and rec.
-void launch_fri_tail(uint32_t l_start, uint32_t last_log, uint32_t* vals, uint32_t* nodes, uint32_t* state,
-                     uint32_t* rec, const uint32_t* itree, uint32_t tree_len, hipStream_t stream);
-
 // ops.hip
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                   uint32_t tree_len, hipStream_t stream);
@@ -51,6 +46,29 @@ void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* a
 // Device channel (state = 8 digest words + n_draws): mix a root, draw one QM31. record[0..8) gets
 // the root, record[8..12) the drawn QM31.
 void launch_channel_mix_root_draw(uint32_t* state, const uint32_t* root, uint32_t* record, hipStream_t stream);
+
+// fri.hip: one FRI layer per step, fused: fold from the layer above, Merkle tree of the layer's 4
+// coordinate columns, channel step (mix the root, draw the next folding coefficient).
+enum FriMode : uint32_t { FRI_LEAVES = 0, FRI_FOLD_CIRCLE = 1, FRI_FOLD_LINE = 2 };
+struct FriLayerDesc {
+  uint32_t log;     // the layer holds 2^log values
+  uint32_t mode;    // FRI_LEAVES: the layer is src itself; folds: src is the layer above (2^(log+1) values)
+  uint32_t commit;  // 0 for the last layer: folded only
+  uint32_t pad;
+  CCols4 src;
+  Cols4 dst;                // folds: the layer's 4 coordinate columns
+  const uint32_t* tw;       // FRI_FOLD_CIRCLE: line layer 0 of the inverse twiddle tree of the layer above
+                            // (itree + tree_len - 2^log); FRI_FOLD_LINE: itree + tree_len - 2^(log+1)
+  const uint32_t* alpha;    // folds: the 4 words of the folding coefficient drawn for the layer above
+  uint32_t* nodes;          // Merkle heap: level j (2^j nodes of 8 words) at 8 (2^j - 1) words, 16-byte aligned
+  uint32_t* rec;            // 12 words: root, then the coefficient drawn after it
+};
+// One layer, 9 <= log <= 18, over 2^(log - c) workgroups (c = max(8, log - 8)); *ticket is zero
+// before the launch. state: channel digest (8 words) and n_draws.
+void launch_fri_layer(const FriLayerDesc& d, uint32_t* state, uint32_t* ticket, hipStream_t stream);
+// Layers of at most 2^8 values in one workgroup, in order (descs in device memory); every entry
+// after the first is a fold of the one before it. Only the last entry may have commit == 0.
+void launch_fri_small(const FriLayerDesc* descs_dev, uint32_t n, uint32_t* state, hipStream_t stream);
 
 // Fused accumulate_numerators for one size group. Entries of batch b are
 // [batch_start[b], batch_start[b+1]); out holds 4 coordinate pointers per batch. aligned16 says

@@ -252,183 +252,4 @@ void launch_channel_mix_root_draw(uint32_t* state, const uint32_t* root, uint32_
   hipLaunchKernelGGL(channel_mix_root_draw_kernel, dim3(1), dim3(64), 0, stream, state, root, record);
 }
 
-// ---- FRI tail: every layer of at most 2^12 values in one workgroup -----------------------------
-// Layer l (l = l_start down to last_log + 1) holds 2^l QM31 values as 4 coordinate columns. Per
-// layer: the Merkle tree of its 4 columns (leaf = H(c0[i] || c1[i] || c2[i] || c3[i])), the root
-// mixed into the channel and a folding coefficient drawn (FriProver::commit_inner_layers,
-// prover/fri.rs:100-272), then fold_line into layer l - 1 (core/fri.rs:702-725). Layouts in global
-// memory, shared with the host:
-//   vals:  layer l at 4 (2^(l_start+1) - 2^(l+1)) words, coordinate k at + k 2^l
-//   nodes: tree of layer l at sum_{m = l+1}^{l_start} 8 (2^(m+1) - 1) words; level j of it
-//          (2^j nodes of 8 words) at + 8 (2^j - 1)
-//   rec:   12 words per layer (root, drawn coefficient), layer l_start first
-constexpr uint32_t FRI_TAIL_MAX_LOG = 12;
-
-__device__ __forceinline__ void leaf4(const uint32_t (*V)[1 << FRI_TAIL_MAX_LOG], uint32_t i, uint32_t h[8]) {
-  uint32_t m[16] = {V[0][i], V[1][i], V[2][i], V[3][i], 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  blake2s_init(h);
-  blake2s_compress(h, m, 16, true);
-}
-__device__ __forceinline__ void node2(const uint32_t* l, const uint32_t* r, uint32_t h[8]) {
-  uint32_t m[16];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) { m[k] = l[k]; m[8 + k] = r[k]; }
-  blake2s_init(h);
-  blake2s_compress(h, m, 64, true);
-}
-__device__ __forceinline__ void store8(uint32_t* dst, const uint32_t h[8]) {
-  uint4* o = reinterpret_cast<uint4*>(dst);
-  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
-  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
-}
-
-__global__ void __launch_bounds__(1024) fri_tail_kernel(uint32_t l_start, uint32_t last_log, uint32_t* __restrict__ vals,
-                                                        uint32_t* __restrict__ nodes, uint32_t* __restrict__ state,
-                                                        uint32_t* __restrict__ rec, const uint32_t* __restrict__ itree,
-                                                        uint32_t tree_len) {
-  __shared__ uint32_t V[4][1 << FRI_TAIL_MAX_LOG];
-  __shared__ uint32_t Nd[(1 << (FRI_TAIL_MAX_LOG - 1)) * 8];
-  __shared__ uint32_t alpha_s[4];
-  const uint32_t t = threadIdx.x, T = blockDim.x;
-  size_t voff = 0, noff = 0;
-  {
-    const uint32_t n = 1u << l_start;
-    for (uint32_t i = t; i < n; i += T)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) V[k][i] = vals[(size_t)k * n + i];
-  }
-  __syncthreads();
-  uint32_t layer = 0;
-  for (uint32_t l = l_start; l > last_log; --l, ++layer) {
-    const uint32_t n = 1u << l;
-    uint32_t* tree = nodes + noff;
-    uint32_t top;  // level held in Nd after the leaf step
-    if (l == FRI_TAIL_MAX_LOG) {
-      // 2^12 leaves do not fit Nd: each thread hashes two leaves and their parent
-      for (uint32_t i = t; i < n / 2; i += T) {
-        uint32_t h0[8], h1[8], p[8];
-        leaf4(V, 2 * i, h0);
-        leaf4(V, 2 * i + 1, h1);
-        store8(tree + 8 * ((n - 1) + 2 * i), h0);
-        store8(tree + 8 * ((n - 1) + 2 * i + 1), h1);
-        node2(h0, h1, p);
-        store8(tree + 8 * ((n / 2 - 1) + i), p);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) Nd[8 * i + k] = p[k];
-      }
-      top = l - 1;
-    } else if (n > T / 4) {
-      for (uint32_t i = t; i < n; i += T) {
-        uint32_t h[8];
-        leaf4(V, i, h);
-        store8(tree + 8 * ((n - 1) + i), h);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) Nd[8 * i + k] = h[k];
-      }
-      top = l;
-    } else {
-      // few leaves: one quad of lanes per leaf (latency-bound)
-      const uint32_t i = t >> 2, q = t & 3;
-      if (i < n) {
-        uint32_t msg[16] = {V[0][i], V[1][i], V[2][i], V[3][i], 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t hA, hB;
-        blake2s_init_quad(q, hA, hB);
-        blake2s_compress_quad(hA, hB, msg, 16, true, q);
-        tree[8 * ((n - 1) + i) + q] = hA;
-        tree[8 * ((n - 1) + i) + 4 + q] = hB;
-        Nd[8 * i + q] = hA;
-        Nd[8 * i + 4 + q] = hB;
-      }
-      top = l;
-    }
-    __syncthreads();
-    for (uint32_t j = top; j > 0; --j) {
-      const uint32_t m = 1u << (j - 1);  // parents at level j - 1 (m <= 1024)
-      if (m > T / 4) {
-        uint32_t h[8];
-        if (t < m) node2(Nd + 16 * t, Nd + 16 * t + 8, h);
-        __syncthreads();
-        if (t < m) {
-#pragma unroll
-          for (int k = 0; k < 8; ++k) Nd[8 * t + k] = h[k];
-          store8(tree + 8 * ((m - 1) + t), h);
-        }
-      } else {
-        // one quad of lanes per parent: the level is latency-bound
-        const uint32_t i = t >> 2, q = t & 3;
-        uint32_t hA = 0, hB = 0;
-        if (i < m) {
-          uint32_t msg[16];
-#pragma unroll
-          for (int k = 0; k < 16; ++k) msg[k] = Nd[16 * i + k];
-          blake2s_init_quad(q, hA, hB);
-          blake2s_compress_quad(hA, hB, msg, 64, true, q);
-        }
-        __syncthreads();
-        if (i < m) {
-          Nd[8 * i + q] = hA;
-          Nd[8 * i + 4 + q] = hB;
-          tree[8 * ((m - 1) + i) + q] = hA;
-          tree[8 * ((m - 1) + i) + 4 + q] = hB;
-        }
-      }
-      __syncthreads();
-    }
-    if (t == 0) {
-      // mix_root, then draw_secure_felt (redraw until all 8 words are below 2P)
-      uint32_t m[16], h[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) { m[k] = state[k]; m[8 + k] = Nd[k]; rec[12 * layer + k] = Nd[k]; }
-      blake2s_init(h);
-      blake2s_compress(h, m, 64, true);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) state[k] = h[k];
-      state[8] = 0;
-      uint32_t a[4] = {P, P, P, P};  // not canonical if no draw is accepted: the host replay reports it
-      for (int attempt = 0; attempt < 64; ++attempt) {
-        uint32_t u[8];
-        dev_channel_draw_u32s(state, u);
-        bool ok = true;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) ok &= u[k] < 2 * P;
-        if (ok) {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) a[k] = u[k] >= P ? u[k] - P : u[k];
-          break;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) { rec[12 * layer + 8 + k] = a[k]; alpha_s[k] = a[k]; }
-    }
-    __syncthreads();
-    const qm31 alpha = {alpha_s[0], alpha_s[1], alpha_s[2], alpha_s[3]};
-    // fold_line: f'[i] = f0 + alpha f1, (f0, f1) = ibutterfly(f[2i], f[2i+1], 1/x_i)
-    const uint32_t nn = n / 2;
-    const uint32_t* lt = itree + tree_len - n;
-    uint32_t* nv = vals + voff + 4 * (size_t)n;
-    qm31 r[2];
-    for (uint32_t i = t, s = 0; i < nn; i += T, ++s) {
-      qm31 fx = {V[0][2 * i], V[1][2 * i], V[2][2 * i], V[3][2 * i]};
-      qm31 fnx = {V[0][2 * i + 1], V[1][2 * i + 1], V[2][2 * i + 1], V[3][2 * i + 1]};
-      qm31 f0 = qm_add(fx, fnx);
-      qm31 f1 = qm_mul_m31(qm_sub(fx, fnx), lt[i]);
-      r[s] = qm_add(f0, qm_mul(alpha, f1));
-    }
-    __syncthreads();
-    for (uint32_t i = t, s = 0; i < nn; i += T, ++s) {
-      V[0][i] = r[s].a; V[1][i] = r[s].b; V[2][i] = r[s].c; V[3][i] = r[s].d;
-      nv[i] = r[s].a; nv[nn + i] = r[s].b; nv[2 * nn + i] = r[s].c; nv[3 * nn + i] = r[s].d;
-    }
-    __syncthreads();
-    voff += 4 * (size_t)n;
-    noff += 8 * (size_t)(2 * n - 1);
-  }
-}
-
-void launch_fri_tail(uint32_t l_start, uint32_t last_log, uint32_t* vals, uint32_t* nodes, uint32_t* state,
-                     uint32_t* rec, const uint32_t* itree, uint32_t tree_len, hipStream_t stream) {
-  hipLaunchKernelGGL(fri_tail_kernel, dim3(1), dim3(1024), 0, stream, l_start, last_log, vals, nodes, state, rec,
-                     itree, tree_len);
-}
-
 }  // namespace sh

@@ -1030,16 +1030,20 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     l -= fs;
   }
   const size_t last_n = (size_t)1 << last_log;
-  DevBuf chan(&ctx, 16 + 12 * (size_t)n_layers + 4 * last_n);
+  // device block: channel state (16 words), one ticket counter per layer (zero), then the
+  // transcript records (12 words per layer) and the last layer's values
+  const size_t n_tickets = ((size_t)n_layers + 3) & ~(size_t)3;
+  DevBuf chan(&ctx, 16 + n_tickets + 12 * (size_t)n_layers + 4 * last_n);
   uint32_t* dstate = chan.p;
-  uint32_t* drec = chan.p + 16;
+  uint32_t* dtickets = chan.p + 16;
+  uint32_t* drec = dtickets + n_tickets;
   {
-    uint32_t st[16] = {0};
+    std::vector<uint32_t> st(16 + n_tickets, 0);
     for (int i = 0; i < 8; ++i)
       st[i] = (uint32_t)ch.digest[4 * i] | (uint32_t)ch.digest[4 * i + 1] << 8 |
               (uint32_t)ch.digest[4 * i + 2] << 16 | (uint32_t)ch.digest[4 * i + 3] << 24;
     st[8] = ch.n_draws;
-    launch_copy_words(dstate, (const uint32_t*)ctx.upload(st, sizeof(st)), 16, ctx.stream);
+    launch_copy_words(dstate, ctx.upload_vec(st), st.size(), ctx.stream);
   }
   auto mix_and_draw = [&](FriCol& f, uint32_t layer) {
     f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
@@ -1049,89 +1053,131 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     launch_channel_mix_root_draw(dstate, f.tree.layers[0].p, drec + 12 * layer, ctx.stream);
     return (const qm31*)(drec + 12 * layer + 8);
   };
-  // Layers of at most 2^12 values (fold step 1) are committed and folded by one workgroup
-  // (launch_fri_tail); their evaluations and Merkle trees live in one block with the layout
-  // that kernel documents.
-  const uint32_t FRI_TAIL_MAX_LOG = 12;
-  const uint32_t l_start = std::min(FRI_TAIL_MAX_LOG, Lc - 1);
-  const bool tail = cfg.fold_step == 1 && l_start > last_log;
-  size_t tv_words = 0, tn_words = 0;
-  if (tail) {
-    tv_words = 4 * (((size_t)2 << l_start) - ((size_t)1 << last_log));
-    for (uint32_t l = last_log + 1; l <= l_start; ++l) tn_words += 8 * (((size_t)2 << l) - 1);
-  }
-  DevBuf tail_buf(&ctx, tail ? tv_words + tn_words : 0);
-  auto tail_vals = [&](uint32_t l) { return tail_buf.p + 4 * (((size_t)2 << l_start) - ((size_t)2 << l)); };
-  auto tail_nodes = [&](uint32_t l) {
-    size_t off = tv_words;
-    for (uint32_t m = l + 1; m <= l_start; ++m) off += 8 * (((size_t)2 << m) - 1);
-    return tail_buf.p + off;
-  };
-  auto tail_col = [&](uint32_t l) {
-    FriCol f;
-    f.log = l;
-    for (int k = 0; k < 4; ++k) f.c.push_back(DevBuf::view(&ctx, tail_vals(l) + ((size_t)k << l), (size_t)1 << l));
-    return f;
-  };
-  const qm31* alpha0 = mix_and_draw(first, 0);
   std::vector<FriCol> inner;
+  inner.reserve(n_layers);  // FriCol addresses stay valid while layers are appended
   FriCol cur;
-  // the first line layer starts at zero (FriProver::commit): the fold writes it without reading
-  if (tail && l_start == Lc - 1) {
-    cur = tail_col(Lc - 1);
+  uint32_t layer = 1;
+  if (cfg.fold_step == 1) {
+    // Fused layers (fri.hip): a layer of 2^9 .. 2^18 values is one launch of launch_fri_layer;
+    // layers of at most 2^8 values go to one launch_fri_small at the end; larger layers (and the
+    // fold into a last layer above 2^8) use the separate fold, Merkle and channel kernels.
+    constexpr uint32_t FAST_MAX = 18, SMALL_MAX = 8;
+    // Line layers of at most 2^FAST_MAX values share one block: the values of layer l (l_start
+    // down to last_log) at 4 (2^(l_start+1) - 2^(l+1)) words, coordinate k at + k 2^l, then the
+    // Merkle heaps of the committed ones, then the first layer's heap when it is fused too.
+    const uint32_t l_start = std::min(FAST_MAX, Lc - 1);
+    const size_t tv_words = 4 * (((size_t)2 << l_start) - ((size_t)1 << last_log));
+    std::vector<size_t> heap_off(l_start + 1, 0);
+    size_t tn_words = 0;
+    for (uint32_t l = l_start; l > last_log; --l) {
+      heap_off[l] = tv_words + tn_words;
+      tn_words += 8 * (((size_t)2 << l) - 1);
+    }
+    const bool first_fused = Lc <= FAST_MAX;
+    const size_t fh_words = first_fused ? 8 * (((size_t)2 << Lc) - 1) : 0;
+    DevBuf fri_buf(&ctx, tv_words + tn_words + fh_words);
+    auto heap_views = [&](FriCol& f, uint32_t* heap) {
+      f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
+      f.tree.eval_logs.assign(4, f.log);
+      f.tree.layers.clear();
+      for (uint32_t j = 0; j <= f.log; ++j)
+        f.tree.layers.push_back(DevBuf::view(&ctx, heap + 8 * (((size_t)1 << j) - 1), (size_t)8 << j));
+    };
+    auto line_col = [&](uint32_t l) {
+      FriCol f;
+      f.log = l;
+      f.fold_step = 1;
+      if (l <= l_start) {
+        uint32_t* v = fri_buf.p + 4 * (((size_t)2 << l_start) - ((size_t)2 << l));
+        for (int k = 0; k < 4; ++k) f.c.push_back(DevBuf::view(&ctx, v + ((size_t)k << l), (size_t)1 << l));
+      } else {
+        for (int k = 0; k < 4; ++k) f.c.emplace_back(&ctx, (size_t)1 << l);
+      }
+      return f;
+    };
+    std::vector<FriLayerDesc> small;
+    // one step per layer: li = 0 is the circle layer (the quotients), li = n_layers the last layer
+    FriCol* prev = &first;
+    for (uint32_t li = 0; li <= n_layers; ++li) {
+      const uint32_t l = Lc - li;
+      const bool commit = li < n_layers;
+      FriCol f;
+      if (li > 0) f = line_col(l);
+      FriCol& cf = li == 0 ? first : f;
+      FriLayerDesc d{};
+      d.log = l;
+      d.mode = li == 0 ? FRI_LEAVES : li == 1 ? FRI_FOLD_CIRCLE : FRI_FOLD_LINE;
+      d.commit = commit ? 1u : 0u;
+      d.src = li == 0 ? first.cc() : prev->cc();
+      d.dst = cf.mc();
+      d.tw = li == 1 ? ctx.itw.p + ctx.tree_len() - ((size_t)1 << l) : ctx.itw.p + ctx.tree_len() - ((size_t)2 << l);
+      d.alpha = li > 0 ? drec + 12 * (li - 1) + 8 : nullptr;
+      d.rec = commit ? drec + 12 * li : nullptr;
+      if (commit && l <= FAST_MAX) {
+        d.nodes = li == 0 ? fri_buf.p + tv_words + tn_words : fri_buf.p + heap_off[l];
+        heap_views(cf, d.nodes);
+      }
+      if (l <= SMALL_MAX) {
+        small.push_back(d);
+      } else if (commit && l <= FAST_MAX) {
+        KScope ks(ctx, K_FRI_FOLD, (li == 0 ? 16.0 : 48.0) * (double)(1u << l) + 64.0 * (double)(1u << l));
+        launch_fri_layer(d, dstate, dtickets + li, ctx.stream);
+      } else {
+        if (li > 0) {
+          KScope ks(ctx, K_FRI_FOLD, 48.0 * (double)(1u << l));
+          const qm31* a = (const qm31*)d.alpha;
+          if (li == 1) launch_fold_circle_into_line_dev(Lc, first.cc(), cf.mc(), a, true, ctx.itw.p, ctx.tree_len(), ctx.stream);
+          else launch_fold_line_dev(l + 1, prev->cc(), cf.mc(), a, 0, ctx.itw.p, ctx.tree_len(), ctx.stream);
+        }
+        if (commit) mix_and_draw(cf, li);
+      }
+      if (li > 0) {
+        if (commit) {
+          inner.push_back(std::move(f));
+          prev = &inner.back();
+        } else {
+          cur = std::move(f);
+        }
+      }
+    }
+    if (!small.empty()) {
+      double tb = 0;
+      for (auto& d : small) tb += 64.0 * (double)(1u << d.log);
+      KScope ks(ctx, K_FRI_TAIL, tb);
+      launch_fri_small(ctx.upload_vec(small), (uint32_t)small.size(), dstate, ctx.stream);
+    }
+    layer = n_layers;
   } else {
+    const qm31* alpha0 = mix_and_draw(first, 0);
     cur.log = Lc - 1;
     for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log);
-  }
-  {
-    KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 16.0 * (double)(1u << (Lc - 1)));
-    launch_fold_circle_into_line_dev(Lc, first.cc(), cur.mc(), alpha0, true, ctx.itw.p, ctx.tree_len(), ctx.stream);
-  }
-  auto fold_layer = [&](FriCol& src, uint32_t fs, const qm31* a) {
-    FriCol* s = &src;
-    FriCol tmp;
-    for (uint32_t st = 0; st < fs; ++st) {
-      FriCol nx;
-      if (tail && s->log - 1 == l_start) {
-        nx = tail_col(l_start);
-      } else {
+    {
+      // the first line layer starts at zero (FriProver::commit): the fold writes it without reading
+      KScope ks(ctx, K_FRI_FOLD, 16.0 * (double)(1u << Lc) + 16.0 * (double)(1u << (Lc - 1)));
+      launch_fold_circle_into_line_dev(Lc, first.cc(), cur.mc(), alpha0, true, ctx.itw.p, ctx.tree_len(), ctx.stream);
+    }
+    auto fold_layer = [&](FriCol& src, uint32_t fs, const qm31* a) {
+      FriCol* s = &src;
+      FriCol tmp;
+      for (uint32_t st = 0; st < fs; ++st) {
+        FriCol nx;
         nx.log = s->log - 1;
         for (int k = 0; k < 4; ++k) nx.c.emplace_back(&ctx, (size_t)1 << nx.log);
+        KScope ks(ctx, K_FRI_FOLD, 24.0 * (double)(1u << s->log));
+        launch_fold_line_dev(s->log, s->cc(), nx.mc(), a, st, ctx.itw.p, ctx.tree_len(), ctx.stream);
+        tmp = std::move(nx);
+        s = &tmp;
       }
-      KScope ks(ctx, K_FRI_FOLD, 24.0 * (double)(1u << s->log));
-      launch_fold_line_dev(s->log, s->cc(), nx.mc(), a, st, ctx.itw.p, ctx.tree_len(), ctx.stream);
-      tmp = std::move(nx);
-      s = &tmp;
+      return tmp;
+    };
+    while (cur.log != last_log) {
+      uint32_t fs = cur.log > last_log + cfg.fold_step ? cfg.fold_step : cur.log - last_log;
+      cur.fold_step = fs;
+      const qm31* a = mix_and_draw(cur, layer++);
+      FriCol nx = fold_layer(cur, fs, a);
+      inner.push_back(std::move(cur));
+      cur = std::move(nx);
     }
-    return tmp;
-  };
-  uint32_t layer = 1;
-  while (cur.log != last_log && !(tail && cur.log == l_start)) {
-    uint32_t fs = cur.log > last_log + cfg.fold_step ? cfg.fold_step : cur.log - last_log;
-    cur.fold_step = fs;
-    const qm31* a = mix_and_draw(cur, layer++);
-    FriCol nx = fold_layer(cur, fs, a);
-    inner.push_back(std::move(cur));
-    cur = std::move(nx);
-  }
-  if (tail) {
-    {
-      double tb = 0;
-      for (uint32_t l = l_start; l > last_log; --l) tb += 16.0 * (1u << l) + 32.0 * (2u << l) + 8.0 * (1u << l);
-      KScope ks(ctx, K_FRI_TAIL, tb);
-      launch_fri_tail(l_start, last_log, tail_vals(l_start), tail_nodes(l_start), dstate, drec + 12 * layer,
-                      ctx.itw.p, ctx.tree_len(), ctx.stream);
-    }
-    for (uint32_t l = l_start; l > last_log; --l, ++layer) {
-      FriCol f = tail_col(l);
-      f.fold_step = 1;
-      f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
-      f.tree.eval_logs.assign(4, l);
-      for (uint32_t j = 0; j <= l; ++j)
-        f.tree.layers.push_back(DevBuf::view(&ctx, tail_nodes(l) + 8 * (((size_t)1 << j) - 1), (size_t)8 << j));
-      inner.push_back(std::move(f));
-    }
-    cur = tail_col(last_log);
   }
   if (layer != n_layers) throw HipError(4, "FRI layer count mismatch");
   HMARK("fri_launch");

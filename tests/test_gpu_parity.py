@@ -199,7 +199,8 @@ def cfg_of(d):
     return SH.PcsConfig.make(d["pow_bits"], d["log_blowup"], d["log_last_layer"], d["n_queries"], d["fold_step"], 0)
 
 
-@pytest.mark.parametrize("log_n,n_cols,mode", [(3, 16, 0), (5, 16, 1), (8, 100, 1), (10, 37, 0), (12, 100, 1)])
+@pytest.mark.parametrize("log_n,n_cols,mode", [(3, 16, 0), (5, 16, 1), (8, 100, 1), (10, 37, 0), (12, 100, 1),
+                                               (18, 4, 1)])
 def test_wide_fib_proof_byte_identical(hb, log_n, n_cols, mode):
     say("wide-fib proof", log_n, n_cols, mode)
     trace = hb.gen_trace_wide_fibonacci(n_cols, log_n, mode, 21)
@@ -212,7 +213,8 @@ def test_wide_fib_proof_byte_identical(hb, log_n, n_cols, mode):
 
 
 @pytest.mark.parametrize("cfg", [dict(pow_bits=8, log_blowup=2, log_last_layer=2, n_queries=7, fold_step=1),
-                                 dict(pow_bits=3, log_blowup=1, log_last_layer=1, n_queries=5, fold_step=2)])
+                                 dict(pow_bits=3, log_blowup=1, log_last_layer=1, n_queries=5, fold_step=2),
+                                 dict(pow_bits=4, log_blowup=1, log_last_layer=8, n_queries=4, fold_step=1)])
 def test_wide_fib_other_configs(hb, cfg):
     say("wide-fib config", cfg)
     trace = hb.gen_trace_wide_fibonacci(20, 9, 1, 5)
