@@ -123,25 +123,105 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
 }
 
 // ---- low stages of a 2^S-element block in one pass (11 <= S <= 15) ---------------------------
-// A workgroup of 2^(S-5) threads owns one block of 2^S consecutive elements of one column: it
-// loads the block into LDS, runs stages [0, S) as rounds of up to 5 stages (each thread holds 32
-// elements in registers per round), and stores the block. LDS word e lives at e + e/32, which
-// keeps every round's accesses free of bank conflicts.
+// A workgroup of 2^(S-5) threads owns one block (tile) of 2^S consecutive elements of one column
+// and runs stages [0, S) on it as three rounds (bits 0-4, 5-9, 10..S-1); in a round each thread
+// holds 32 elements in registers and runs up to 5 stages on them. The first round reads its
+// elements straight from global memory and the last round writes its results straight back; only
+// the hand-overs between rounds go through LDS, where word e lives at e + e/32 so that every
+// round's accesses are free of bank conflicts.
+//
+// Workgroups go to the 8 XCDs in turn (workgroup n to XCD n mod 8). Renumbering them so that each
+// XCD takes a contiguous range of tiles keeps the tiles of one column on one XCD and its L2: when
+// evaluation onto a larger domain makes every tile of a column read the same coefficients, only
+// the first read reaches HBM.
 __device__ __forceinline__ uint32_t lds_pad(uint32_t e) { return e + (e >> 5); }
 
-template <bool INV, int S, int A, int R>
-__device__ __forceinline__ void low_round(uint32_t* lds, uint32_t gbase, const uint32_t* __restrict__ tree,
-                                          uint32_t tree_len, uint32_t log_n) {
-  constexpr uint32_t T = 1u << (S - 5), G = 1u << (5 - R), Q = 1u << R;
-  const uint32_t t = threadIdx.x;
-  uint32_t v[32];
+// Threads per workgroup and element registers per thread for a tile of 2^S elements.
+template <int S> struct LowShape {
+  static constexpr int X = 0;                       // 2^X groups of 32 elements per thread
+  static constexpr uint32_t T = 1u << (S - 5 - X);
+  static constexpr uint32_t V = 32u << X;
+};
+
+// Element (within the tile) held in register slot g * 2^R + q by thread t in the round at bit A.
+template <int S, int A, int R>
+__device__ __forceinline__ uint32_t round_elem(uint32_t t, uint32_t g, uint32_t q) {
+  constexpr uint32_t T = LowShape<S>::T;
+  const uint32_t rho = t + g * T;
+  return ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1)) | (q << A);
+}
+
+template <int S, int A, int R>
+__device__ __forceinline__ void round_from_lds(uint32_t* v, const uint32_t* lds) {
+  constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
 #pragma unroll
-  for (uint32_t g = 0; g < G; ++g) {
-    const uint32_t rho = t + g * T;
-    const uint32_t e0 = ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1));
+  for (uint32_t g = 0; g < G; ++g)
 #pragma unroll
-    for (uint32_t q = 0; q < Q; ++q) v[g * Q + q] = lds[lds_pad(e0 | (q << A))];
+    for (uint32_t q = 0; q < Q; ++q) v[g * Q + q] = lds[lds_pad(round_elem<S, A, R>(threadIdx.x, g, q))];
+}
+template <int S, int A, int R>
+__device__ __forceinline__ void round_to_lds(const uint32_t* v, uint32_t* lds) {
+  constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+#pragma unroll
+  for (uint32_t g = 0; g < G; ++g)
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) lds[lds_pad(round_elem<S, A, R>(threadIdx.x, g, q))] = v[g * Q + q];
+}
+// Global elements gbase | e (index masked with in_mask) of the round's slots. At A = 0 a thread's
+// 32 elements are consecutive: 16-byte loads when the column allows them.
+template <int S, int A, int R>
+__device__ __forceinline__ void round_from_global(uint32_t* v, const uint32_t* __restrict__ in, uint32_t gbase,
+                                                  uint32_t in_mask, bool vec) {
+  constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+  if (A == 0 && R == 5 && vec) {
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) {
+      const uint32_t e0 = 32 * (threadIdx.x + g * LowShape<S>::T);
+#pragma unroll
+      for (uint32_t m = 0; m < 8; ++m) {
+        const uint4 w = *reinterpret_cast<const uint4*>(in + ((gbase | (e0 + 4 * m)) & in_mask));
+        v[32 * g + 4 * m] = w.x; v[32 * g + 4 * m + 1] = w.y; v[32 * g + 4 * m + 2] = w.z; v[32 * g + 4 * m + 3] = w.w;
+      }
+    }
+    return;
   }
+#pragma unroll
+  for (uint32_t g = 0; g < G; ++g)
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) v[g * Q + q] = in[(gbase | round_elem<S, A, R>(threadIdx.x, g, q)) & in_mask];
+}
+template <int S, int A, int R>
+__device__ __forceinline__ void round_to_global(uint32_t* v, uint32_t* __restrict__ out, uint32_t gbase,
+                                                uint32_t scale, bool vec) {
+  constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+  if (scale) {
+#pragma unroll
+    for (uint32_t i = 0; i < LowShape<S>::V; ++i) v[i] = m31_mul(v[i], scale);
+  }
+  if (A == 0 && R == 5 && vec) {
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) {
+      const uint32_t e0 = 32 * (threadIdx.x + g * LowShape<S>::T);
+#pragma unroll
+      for (uint32_t m = 0; m < 8; ++m)
+        *reinterpret_cast<uint4*>(out + (gbase | (e0 + 4 * m))) =
+            make_uint4(v[32 * g + 4 * m], v[32 * g + 4 * m + 1], v[32 * g + 4 * m + 2], v[32 * g + 4 * m + 3]);
+    }
+    return;
+  }
+#pragma unroll
+  for (uint32_t g = 0; g < G; ++g)
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) out[gbase | round_elem<S, A, R>(threadIdx.x, g, q)] = v[g * Q + q];
+}
+
+// The R stages of the round at bit A on the registers (2^(R-1) * R butterflies per group, one
+// twiddle per distinct butterfly block).
+template <bool INV, int S, int A, int R>
+__device__ __forceinline__ void round_butterflies(uint32_t* v, uint32_t gbase, const uint32_t* __restrict__ tree,
+                                                  uint32_t tree_len, uint32_t log_n) {
+  constexpr uint32_t T = LowShape<S>::T, G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+  const uint32_t t = threadIdx.x;
 #pragma unroll
   for (uint32_t g = 0; g < G; ++g) {
     const uint32_t rho = t + g * T;
@@ -161,77 +241,36 @@ __device__ __forceinline__ void low_round(uint32_t* lds, uint32_t gbase, const u
       }
     }
   }
-#pragma unroll
-  for (uint32_t g = 0; g < G; ++g) {
-    const uint32_t rho = t + g * T;
-    const uint32_t e0 = ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1));
-#pragma unroll
-    for (uint32_t q = 0; q < Q; ++q) lds[lds_pad(e0 | (q << A))] = v[g * Q + q];
-  }
-  __syncthreads();
 }
 
 template <bool INV, int S>
-__global__ void __launch_bounds__(1u << (S - 5)) fft_low_kernel(const uint32_t* const* __restrict__ src,
-                                                                uint32_t* const* __restrict__ dst, uint32_t log_n,
-                                                                uint32_t in_mask, const uint32_t* __restrict__ tree,
-                                                                uint32_t tree_len, uint32_t scale) {
-  constexpr uint32_t N = 1u << S, T = 1u << (S - 5);
+__global__ void __launch_bounds__(LowShape<S>::T) fft_low_kernel(const uint32_t* const* __restrict__ src,
+                                                                 uint32_t* const* __restrict__ dst, uint32_t log_n,
+                                                                 uint32_t in_mask, const uint32_t* __restrict__ tree,
+                                                                 uint32_t tree_len, uint32_t scale) {
+  constexpr uint32_t N = 1u << S;
+  // rounds in execution order: the inverse runs bits 0-4, 5-9, 10..S-1; the forward the reverse
+  constexpr int A1 = INV ? 0 : 10, R1 = INV ? 5 : S - 10;
+  constexpr int A3 = INV ? 10 : 0, R3 = INV ? S - 10 : 5;
   __shared__ uint32_t lds[N + (N >> 5)];
-  // One block per (column, 2^S-block of the column) on a 1-D grid. Workgroups go to the 8 XCDs in
-  // turn (workgroup n to XCD n mod 8); renumbering them so that each XCD takes a contiguous range
-  // keeps the blocks of one column on one XCD and its L2, which matters when evaluation onto a
-  // larger domain makes every block of a column read the same coefficients.
   const uint32_t n_wg = gridDim.x, hw = blockIdx.x;
-  const uint32_t wg = (n_wg & 7) == 0 ? (hw & 7) * (n_wg >> 3) + (hw >> 3) : hw;
-  const uint32_t col = wg >> (log_n - S);
+  const uint32_t tile = (n_wg & 7) == 0 ? (hw & 7) * (n_wg >> 3) + (hw >> 3) : hw;
+  const uint32_t blk_log = log_n - S;
+  const uint32_t col = tile >> blk_log, gbase = (tile & ((1u << blk_log) - 1)) << S;
   const uint32_t* __restrict__ in = src[col];
   uint32_t* __restrict__ out = dst[col];
-  const uint32_t gbase = (wg & ((1u << (log_n - S)) - 1)) << S;
-  const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
-  if (vec) {
-    // 8 independent 16-byte loads per thread, all in flight before the first LDS write
-    uint4 w[8];
-#pragma unroll
-    for (uint32_t m = 0; m < 8; ++m) {
-      const uint32_t i = 4 * (threadIdx.x + m * T);
-      w[m] = *reinterpret_cast<const uint4*>(in + ((gbase | i) & in_mask));
-    }
-#pragma unroll
-    for (uint32_t m = 0; m < 8; ++m) {
-      const uint32_t i = 4 * (threadIdx.x + m * T);
-      lds[lds_pad(i)] = w[m].x;
-      lds[lds_pad(i + 1)] = w[m].y;
-      lds[lds_pad(i + 2)] = w[m].z;
-      lds[lds_pad(i + 3)] = w[m].w;
-    }
-  } else {
-    for (uint32_t i = threadIdx.x; i < N; i += T) lds[lds_pad(i)] = in[(gbase | i) & in_mask];
-  }
+  uint32_t v[LowShape<S>::V];
+  round_from_global<S, A1, R1>(v, in, gbase, in_mask, (reinterpret_cast<uintptr_t>(in) & 15) == 0);
+  round_butterflies<INV, S, A1, R1>(v, gbase, tree, tree_len, log_n);
+  round_to_lds<S, A1, R1>(v, lds);
   __syncthreads();
-  if (INV) {
-    low_round<INV, S, 0, 5>(lds, gbase, tree, tree_len, log_n);
-    low_round<INV, S, 5, 5>(lds, gbase, tree, tree_len, log_n);
-    low_round<INV, S, 10, S - 10>(lds, gbase, tree, tree_len, log_n);
-  } else {
-    low_round<INV, S, 10, S - 10>(lds, gbase, tree, tree_len, log_n);
-    low_round<INV, S, 5, 5>(lds, gbase, tree, tree_len, log_n);
-    low_round<INV, S, 0, 5>(lds, gbase, tree, tree_len, log_n);
-  }
-  if (vec) {
-#pragma unroll
-    for (uint32_t m = 0; m < 8; ++m) {
-      const uint32_t i = 4 * (threadIdx.x + m * T);
-      uint4 w = make_uint4(lds[lds_pad(i)], lds[lds_pad(i + 1)], lds[lds_pad(i + 2)], lds[lds_pad(i + 3)]);
-      if (scale) w = make_uint4(m31_mul(w.x, scale), m31_mul(w.y, scale), m31_mul(w.z, scale), m31_mul(w.w, scale));
-      *reinterpret_cast<uint4*>(out + (gbase | i)) = w;
-    }
-  } else {
-    for (uint32_t i = threadIdx.x; i < N; i += T) {
-      const uint32_t v = lds[lds_pad(i)];
-      out[gbase | i] = scale ? m31_mul(v, scale) : v;
-    }
-  }
+  round_from_lds<S, 5, 5>(v, lds);
+  round_butterflies<INV, S, 5, 5>(v, gbase, tree, tree_len, log_n);
+  round_to_lds<S, 5, 5>(v, lds);
+  __syncthreads();
+  round_from_lds<S, A3, R3>(v, lds);
+  round_butterflies<INV, S, A3, R3>(v, gbase, tree, tree_len, log_n);
+  round_to_global<S, A3, R3>(v, out, gbase, scale, (reinterpret_cast<uintptr_t>(out) & 15) == 0);
 }
 
 template <bool INV>
