@@ -13,6 +13,7 @@
 // One lane evaluates one evaluation-domain row; trace columns are column-major, so the 64 lanes
 // of a wave read 256 contiguous bytes of each column. Constraint coefficients are wave-uniform
 // and load through the scalar cache.
+#include <stdexcept>
 #include "field.h"
 #include "kernels.h"
 
@@ -340,45 +341,69 @@ void launch_gen_poseidon(uint32_t log_n, uint32_t* const* cols, uint32_t ipr, ui
 
 // LogupTraceGenerator for the Poseidon relation: column rep holds sum_{r' <= rep} frac_{r'} per row;
 // the last column's coordinate sums (over all rows) are accumulated into sums[0..4].
-__global__ void __launch_bounds__(256) poseidon_logup_kernel(uint32_t n_rows,
+// One thread per (row, instance): in a block of 256 threads, thread rep * RB + j owns row
+// blockIdx.x * RB + j and instance rep (RB = 2^log_rb = 256 / ipr rounded up to a power of two),
+// so the field inversions of a row's instances run side by side; the running sums over the
+// instances of a row meet in LDS.
+__global__ void __launch_bounds__(256) poseidon_logup_kernel(uint32_t n_rows, uint32_t log_rb,
                                                              const uint32_t* const* __restrict__ trace_cols,
                                                              uint32_t* const* __restrict__ inter_cols,
                                                              const PoseidonParams* __restrict__ params,
                                                              unsigned long long* __restrict__ sums) {
+  CRITICAL_PATH_WAVE();
+  __shared__ uint32_t fr[4][256];
   __shared__ unsigned long long red[4][256];
-  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   const PoseidonParams& p = *params;
-  qm31 cum = qm_zero();
-  if (r < n_rows) {
-    for (uint32_t rep = 0; rep < p.ipr; ++rep) {
-      const uint32_t* const* c = trace_cols + rep * PSD_COLS_PER_REP;
-      uint32_t s[16];
-      for (int i = 0; i < 16; ++i) s[i] = c[i][r];
-      qm31 d0 = combine16(p, s);
-      for (int i = 0; i < 16; ++i) s[i] = c[PSD_COLS_PER_REP - 16 + i][r];
-      qm31 d1 = combine16(p, s);
-      qm31 frac = qm_mul(qm_sub(d1, d0), qm_inv(qm_mul(d0, d1)));
-      cum = qm_add(cum, frac);
-      uint32_t* const* ic = inter_cols + 4 * rep;
-      ic[0][r] = cum.a; ic[1][r] = cum.b; ic[2][r] = cum.c; ic[3][r] = cum.d;
+  const uint32_t t = threadIdx.x, rb = 1u << log_rb;
+  const uint32_t rep = t >> log_rb, j = t & (rb - 1);
+  const uint32_t r = blockIdx.x * rb + j;
+  const bool live = r < n_rows && rep < p.ipr;
+  qm31 frac = qm_zero();
+  if (live) {
+    const uint32_t* const* c = trace_cols + rep * PSD_COLS_PER_REP;
+    uint32_t s0[16], s1[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      s0[i] = gmem(c[i])[r];
+      s1[i] = gmem(c[PSD_COLS_PER_REP - 16 + i])[r];
     }
+    const qm31 d0 = combine16(p, s0), d1 = combine16(p, s1);
+    frac = qm_mul(qm_sub(d1, d0), qm_inv(qm_mul(d0, d1)));
   }
-  red[0][threadIdx.x] = r < n_rows ? cum.a : 0;
-  red[1][threadIdx.x] = r < n_rows ? cum.b : 0;
-  red[2][threadIdx.x] = r < n_rows ? cum.c : 0;
-  red[3][threadIdx.x] = r < n_rows ? cum.d : 0;
+  fr[0][t] = frac.a; fr[1][t] = frac.b; fr[2][t] = frac.c; fr[3][t] = frac.d;
+  __syncthreads();
+  qm31 cum = qm_zero();
+  for (uint32_t q = 0; q <= rep && q < p.ipr; ++q) {
+    const uint32_t u = (q << log_rb) + j;
+    cum = qm_add(cum, qm31{fr[0][u], fr[1][u], fr[2][u], fr[3][u]});
+  }
+  if (live) {
+    uint32_t* const* ic = inter_cols + 4 * rep;
+    gmem(ic[0])[r] = cum.a; gmem(ic[1])[r] = cum.b; gmem(ic[2])[r] = cum.c; gmem(ic[3])[r] = cum.d;
+  }
+  const bool last = live && rep + 1 == p.ipr;
+  red[0][t] = last ? cum.a : 0;
+  red[1][t] = last ? cum.b : 0;
+  red[2][t] = last ? cum.c : 0;
+  red[3][t] = last ? cum.d : 0;
   __syncthreads();
   for (uint32_t w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    if (t < w)
+      for (int k = 0; k < 4; ++k) red[k][t] += red[k][t + w];
     __syncthreads();
   }
-  if (threadIdx.x < 4) atomicAdd(&sums[threadIdx.x], red[threadIdx.x][0]);
+  if (t < 4) atomicAdd(&sums[t], red[t][0]);
 }
-void launch_poseidon_logup_fracs(uint32_t log_n, const uint32_t* const* trace_cols, uint32_t* const* inter_cols,
-                                 const PoseidonParams* params, unsigned long long* sums, hipStream_t stream) {
-  uint32_t n = 1u << log_n;
-  hipLaunchKernelGGL(poseidon_logup_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, trace_cols, inter_cols,
+
+void launch_poseidon_logup_fracs(uint32_t log_n, uint32_t ipr, const uint32_t* const* trace_cols,
+                                 uint32_t* const* inter_cols, const PoseidonParams* params, unsigned long long* sums,
+                                 hipStream_t stream) {
+  uint32_t log_ipr = 0;  // instance slots per row: the next power of two
+  while ((1u << log_ipr) < ipr) ++log_ipr;
+  if (log_ipr > 8) throw std::invalid_argument("LogUp: at most 256 instances per row");
+  const uint32_t log_rb = 8 - log_ipr, n = 1u << log_n;
+  const uint32_t blocks = (n + (1u << log_rb) - 1) >> log_rb;
+  hipLaunchKernelGGL(poseidon_logup_kernel, dim3(blocks), dim3(256), 0, stream, n, log_rb, trace_cols, inter_cols,
                      params, sums);
 }
 

@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
                                                        uint32_t scale) {
   extern __shared__ uint32_t lds[];
   const uint32_t col = blockIdx.y;
-  const uint32_t* __restrict__ in = src[col];
-  uint32_t* __restrict__ out = dst[col];
+  const GLOBAL_AS uint32_t* __restrict__ in = gmem(src[col]);
+  GLOBAL_AS uint32_t* __restrict__ out = gmem(dst[col]);
   const uint32_t C = 1u << c;
   const uint32_t tile = blockIdx.x;
   const uint32_t lo_blk = tile & ((1u << (e - c)) - 1);
@@ -170,7 +170,7 @@ __device__ __forceinline__ void round_to_lds(const uint32_t* v, uint32_t* lds) {
 // Global elements gbase | e (index masked with in_mask) of the round's slots. At A = 0 a thread's
 // 32 elements are consecutive: 16-byte loads when the column allows them.
 template <int S, int A, int R>
-__device__ __forceinline__ void round_from_global(uint32_t* v, const uint32_t* __restrict__ in, uint32_t gbase,
+__device__ __forceinline__ void round_from_global(uint32_t* v, const GLOBAL_AS uint32_t* __restrict__ in, uint32_t gbase,
                                                   uint32_t in_mask, bool vec) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
   if (A == 0 && R == 5 && vec) {
@@ -179,7 +179,7 @@ __device__ __forceinline__ void round_from_global(uint32_t* v, const uint32_t* _
       const uint32_t e0 = 32 * (threadIdx.x + g * LowShape<S>::T);
 #pragma unroll
       for (uint32_t m = 0; m < 8; ++m) {
-        const uint4 w = *reinterpret_cast<const uint4*>(in + ((gbase | (e0 + 4 * m)) & in_mask));
+        const u32x4 w = *reinterpret_cast<const GLOBAL_AS u32x4*>(in + ((gbase | (e0 + 4 * m)) & in_mask));
         v[32 * g + 4 * m] = w.x; v[32 * g + 4 * m + 1] = w.y; v[32 * g + 4 * m + 2] = w.z; v[32 * g + 4 * m + 3] = w.w;
       }
     }
@@ -191,7 +191,7 @@ __device__ __forceinline__ void round_from_global(uint32_t* v, const uint32_t* _
     for (uint32_t q = 0; q < Q; ++q) v[g * Q + q] = in[(gbase | round_elem<S, A, R>(threadIdx.x, g, q)) & in_mask];
 }
 template <int S, int A, int R>
-__device__ __forceinline__ void round_to_global(uint32_t* v, uint32_t* __restrict__ out, uint32_t gbase,
+__device__ __forceinline__ void round_to_global(uint32_t* v, GLOBAL_AS uint32_t* __restrict__ out, uint32_t gbase,
                                                 uint32_t scale, bool vec) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
   if (scale) {
@@ -204,8 +204,8 @@ __device__ __forceinline__ void round_to_global(uint32_t* v, uint32_t* __restric
       const uint32_t e0 = 32 * (threadIdx.x + g * LowShape<S>::T);
 #pragma unroll
       for (uint32_t m = 0; m < 8; ++m)
-        *reinterpret_cast<uint4*>(out + (gbase | (e0 + 4 * m))) =
-            make_uint4(v[32 * g + 4 * m], v[32 * g + 4 * m + 1], v[32 * g + 4 * m + 2], v[32 * g + 4 * m + 3]);
+        *reinterpret_cast<GLOBAL_AS u32x4*>(out + (gbase | (e0 + 4 * m))) =
+            u32x4{v[32 * g + 4 * m], v[32 * g + 4 * m + 1], v[32 * g + 4 * m + 2], v[32 * g + 4 * m + 3]};
     }
     return;
   }
@@ -257,8 +257,8 @@ __global__ void __launch_bounds__(LowShape<S>::T) fft_low_kernel(const uint32_t*
   const uint32_t tile = (n_wg & 7) == 0 ? (hw & 7) * (n_wg >> 3) + (hw >> 3) : hw;
   const uint32_t blk_log = log_n - S;
   const uint32_t col = tile >> blk_log, gbase = (tile & ((1u << blk_log) - 1)) << S;
-  const uint32_t* __restrict__ in = src[col];
-  uint32_t* __restrict__ out = dst[col];
+  const GLOBAL_AS uint32_t* __restrict__ in = gmem(src[col]);
+  GLOBAL_AS uint32_t* __restrict__ out = gmem(dst[col]);
   uint32_t v[LowShape<S>::V];
   round_from_global<S, A1, R1>(v, in, gbase, in_mask, (reinterpret_cast<uintptr_t>(in) & 15) == 0);
   round_butterflies<INV, S, A1, R1>(v, gbase, tree, tree_len, log_n);

@@ -10,6 +10,25 @@
 
 #define SHD __host__ __device__ __forceinline__
 
+// Raises the issue priority of the calling wave above default-priority waves of kernels running
+// beside it on another stream (the trace extension overlaps the commitment phase): the kernels on
+// the proof's critical path call it first.
+#define CRITICAL_PATH_WAVE() __builtin_amdgcn_s_setprio(1)
+
+// Global-memory view of a pointer. Pointers read from memory (column tables, descriptors) are
+// generic to the compiler, which then accesses through them with FLAT instructions; those also
+// count against the LDS and scalar-load wait counter, so that every LDS wait waits for them too.
+// Accesses through gmem() pointers compile to global loads and stores.
+#define GLOBAL_AS __attribute__((address_space(1)))
+// Native vectors for wide loads and stores through gmem() pointers (HIP's uint2/uint4 classes have
+// no constructors from address-space-qualified objects).
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T* gmem(T* p) {
+  return (GLOBAL_AS T*)p;
+}
+
 namespace sh {
 
 constexpr uint32_t P = 0x7fffffffu;

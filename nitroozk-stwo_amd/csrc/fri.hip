@@ -16,6 +16,7 @@
 // Wide levels hash one node per lane (Blake2s is issue-bound: one wave per SIMD already reaches
 // peak throughput); levels of at most 64 parents hash one node per quad of lanes, which has about
 // half the latency of one lane.
+#include <type_traits>
 #include "blake2s.h"
 #include "field.h"
 #include "kernels.h"
@@ -44,14 +45,14 @@ __device__ __forceinline__ uint32_t pin(uint32_t x) {
 }
 
 __device__ __forceinline__ void store8(uint32_t* dst, const uint32_t h[8]) {
-  uint4* o = reinterpret_cast<uint4*>(dst);
-  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
-  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+  GLOBAL_AS u32x4* o = reinterpret_cast<GLOBAL_AS u32x4*>(gmem(dst));
+  o[0] = u32x4{h[0], h[1], h[2], h[3]};
+  o[1] = u32x4{h[4], h[5], h[6], h[7]};
 }
 
 // Inverse circle twiddle of output h: line layer 0 pairs [x, y] -> [y, -y, -x, x].
 __device__ __forceinline__ uint32_t fri_circle_itw(const uint32_t* __restrict__ lt0, uint32_t h) {
-  uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
+  uint32_t x = gmem(lt0)[(h >> 2) * 2], y = gmem(lt0)[(h >> 2) * 2 + 1];
   switch (h & 3) {
     case 0: return y;
     case 1: return m31_neg(y);
@@ -67,18 +68,18 @@ __device__ __forceinline__ qm31 fri_fold(qm31 fp, qm31 fn, uint32_t tw, qm31 alp
   return qm_add(f0, qm_mul(alpha, f1));
 }
 __device__ __forceinline__ uint32_t fri_twiddle(const FriLayerDesc& d, uint32_t i) {
-  return d.mode == FRI_FOLD_CIRCLE ? fri_circle_itw(d.tw, i) : d.tw[i];
+  return d.mode == FRI_FOLD_CIRCLE ? fri_circle_itw(d.tw, i) : gmem(d.tw)[i];
 }
 
 // Value i of the layer: read (FRI_LEAVES) or folded from the layer above in global memory, which
 // an earlier launch wrote; a folded value is also stored to the layer's columns.
 __device__ __forceinline__ qm31 fri_value_global(const FriLayerDesc& d, uint32_t i, qm31 alpha) {
-  if (d.mode == FRI_LEAVES) return {d.src.c[0][i], d.src.c[1][i], d.src.c[2][i], d.src.c[3][i]};
-  uint2 w[4];
+  if (d.mode == FRI_LEAVES) return {gmem(d.src.c[0])[i], gmem(d.src.c[1])[i], gmem(d.src.c[2])[i], gmem(d.src.c[3])[i]};
+  u32x2 w[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) w[k] = *reinterpret_cast<const uint2*>(d.src.c[k] + 2 * i);
+  for (int k = 0; k < 4; ++k) w[k] = *reinterpret_cast<const GLOBAL_AS u32x2*>(gmem(d.src.c[k]) + 2 * i);
   qm31 v = fri_fold({w[0].x, w[1].x, w[2].x, w[3].x}, {w[0].y, w[1].y, w[2].y, w[3].y}, fri_twiddle(d, i), alpha);
-  d.dst.c[0][i] = v.a; d.dst.c[1][i] = v.b; d.dst.c[2][i] = v.c; d.dst.c[3][i] = v.d;
+  gmem(d.dst.c[0])[i] = v.a; gmem(d.dst.c[1])[i] = v.b; gmem(d.dst.c[2])[i] = v.c; gmem(d.dst.c[3])[i] = v.d;
   return v;
 }
 
@@ -123,8 +124,8 @@ __device__ __forceinline__ void fl_level(const uint32_t* in, uint32_t* out, uint
       blake2s_compress_quad(hA, hB, msg, 64, true, q);
       out[q * m + p] = hA;
       out[(4 + q) * m + p] = hB;
-      g[8 * p + q] = hA;
-      g[8 * p + 4 + q] = hB;
+      gmem(g)[8 * p + q] = hA;
+      gmem(g)[8 * p + 4 + q] = hB;
     }
   }
   __syncthreads();
@@ -190,22 +191,23 @@ __device__ __forceinline__ void fl_channel(uint32_t st[9], const uint32_t root[8
 // Lane q of the quad writes words q and q + 4 of the record (root, then alpha in words 8..11) and of
 // the channel state (n_draws from lane 0) to rec and st_out, and alpha word q to alpha_out when
 // given. Selects instead of indexing by q keep the arrays in registers.
+template <class StPtr, class AlphaPtr>
 __device__ __forceinline__ void fl_publish(uint32_t q, const uint32_t st[9], const uint32_t root[8],
-                                           const uint32_t alpha[4], uint32_t* __restrict__ rec, uint32_t* st_out,
-                                           uint32_t* alpha_out) {
+                                           const uint32_t alpha[4], uint32_t* __restrict__ rec, StPtr st_out,
+                                           AlphaPtr alpha_out) {
   uint32_t r_lo = pin(root[0]), r_hi = pin(root[4]), a = pin(alpha[0]), s_lo = pin(st[0]), s_hi = pin(st[4]);
 #pragma unroll
   for (uint32_t k = 1; k < 4; ++k) {
     const uint32_t rl = pin(root[k]), rh = pin(root[4 + k]), ak = pin(alpha[k]), sl = pin(st[k]), sh = pin(st[4 + k]);
     if (q == k) { r_lo = rl; r_hi = rh; a = ak; s_lo = sl; s_hi = sh; }
   }
-  rec[q] = r_lo;
-  rec[4 + q] = r_hi;
-  rec[8 + q] = a;
+  gmem(rec)[q] = r_lo;
+  gmem(rec)[4 + q] = r_hi;
+  gmem(rec)[8 + q] = a;
   st_out[q] = s_lo;
   st_out[4 + q] = s_hi;
   if (q == 0) st_out[8] = st[8];
-  if (alpha_out) alpha_out[q] = a;
+  if constexpr (!std::is_same_v<AlphaPtr, std::nullptr_t>) alpha_out[q] = a;
 }
 
 }  // namespace
@@ -220,7 +222,7 @@ __global__ void __launch_bounds__(256) fri_layer_kernel(FriLayerDesc d, uint32_t
   const uint32_t t = threadIdx.x, w = blockIdx.x;
   const uint32_t l = d.log, C = 1u << c;
   qm31 alpha = qm_zero();
-  if (d.mode != FRI_LEAVES) alpha = {d.alpha[0], d.alpha[1], d.alpha[2], d.alpha[3]};
+  if (d.mode != FRI_LEAVES) alpha = {gmem(d.alpha)[0], gmem(d.alpha)[1], gmem(d.alpha)[2], gmem(d.alpha)[3]};
   for (uint32_t r = t; r < C; r += FL_THREADS) {
     const uint32_t i = w * C + r;
     const qm31 v = fri_value_global(d, i, alpha);
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(256) fri_small_kernel(const FriLayerDesc* __re
     const uint32_t N = 1u << d.log;
     qm31 alpha = qm_zero();
     if (d.mode != FRI_LEAVES)
-      alpha = e == 0 ? qm31{d.alpha[0], d.alpha[1], d.alpha[2], d.alpha[3]}
+      alpha = e == 0 ? qm31{gmem(d.alpha)[0], gmem(d.alpha)[1], gmem(d.alpha)[2], gmem(d.alpha)[3]}
                      : qm31{alpha_s[0], alpha_s[1], alpha_s[2], alpha_s[3]};
     qm31 v = qm_zero();
     if (t < N) {
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(256) fri_small_kernel(const FriLayerDesc* __re
         const qm31 fp = {V[0][2 * t], V[1][2 * t], V[2][2 * t], V[3][2 * t]};
         const qm31 fn = {V[0][2 * t + 1], V[1][2 * t + 1], V[2][2 * t + 1], V[3][2 * t + 1]};
         v = fri_fold(fp, fn, fri_twiddle(d, t), alpha);
-        d.dst.c[0][t] = v.a; d.dst.c[1][t] = v.b; d.dst.c[2][t] = v.c; d.dst.c[3][t] = v.d;
+        gmem(d.dst.c[0])[t] = v.a; gmem(d.dst.c[1])[t] = v.b; gmem(d.dst.c[2])[t] = v.c; gmem(d.dst.c[3])[t] = v.d;
       }
     }
     __syncthreads();  // every read of V is done
@@ -315,8 +317,8 @@ __global__ void __launch_bounds__(256) fri_small_kernel(const FriLayerDesc* __re
         blake2s_compress_quad(hA, hB, msg, 16, true, q);
         bufA[q * N + p] = hA;
         bufA[(4 + q) * N + p] = hB;
-        d.nodes[8 * (N - 1 + p) + q] = hA;
-        d.nodes[8 * (N - 1 + p) + 4 + q] = hB;
+        gmem(d.nodes)[8 * (N - 1 + p) + q] = hA;
+        gmem(d.nodes)[8 * (N - 1 + p) + 4 + q] = hB;
       }
     }
     __syncthreads();

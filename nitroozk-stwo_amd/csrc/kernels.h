@@ -129,8 +129,10 @@ void launch_gen_wide_fib(uint32_t log_n, uint32_t* const* cols, uint32_t n_cols,
                          hipStream_t stream);
 void launch_gen_poseidon(uint32_t log_n, uint32_t* const* cols, uint32_t ipr, uint32_t mode, uint64_t seed,
                          hipStream_t stream);
-void launch_poseidon_logup_fracs(uint32_t log_n, const uint32_t* const* trace_cols, uint32_t* const* inter_cols,
-                                 const PoseidonParams* params, unsigned long long* sums, hipStream_t stream);
+// ipr: instances per row, at most 256 (also in params).
+void launch_poseidon_logup_fracs(uint32_t log_n, uint32_t ipr, const uint32_t* const* trace_cols,
+                                 uint32_t* const* inter_cols, const PoseidonParams* params, unsigned long long* sums,
+                                 hipStream_t stream);
 void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, uint32_t* scratch,
                              hipStream_t stream);
 

@@ -18,6 +18,7 @@ namespace sh {
 __global__ void __launch_bounds__(256) merkle_layer_kernel(uint32_t n_nodes, const uint32_t* __restrict__ prev,
                                                            const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                            uint32_t* __restrict__ out) {
+  CRITICAL_PATH_WAVE();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_nodes) return;
   uint32_t h[8];
@@ -39,21 +40,23 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(uint32_t n_nodes, con
     blake2s_compress(h, m, 0, true);
   }
   if (n_cols) {
-    // software pipeline: the next block's 16 column words are loaded while this block compresses
-    uint32_t nw = n_cols < 16 ? n_cols : 16;
+    // software pipeline: the column words of the next two blocks are in flight while a block
+    // compresses (one wave per SIMD when a tree has few leaves, so nothing else hides the loads)
+    auto words = [&](uint32_t cs) { return cs < n_cols ? (n_cols - cs < 16 ? n_cols - cs : 16) : 0u; };
+    uint32_t nx1[16], nx2[16];
+    const uint32_t w0 = words(0), w1 = words(16);
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) m[k] = k < nw ? cols[k][i] : 0u;
+    for (uint32_t k = 0; k < 16; ++k) m[k] = k < w0 ? gmem(cols[k])[i] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) nx1[k] = k < w1 ? gmem(cols[16 + k])[i] : 0u;
     for (uint32_t cs = 0; cs < n_cols; cs += 16) {
-      uint32_t nx[16];
-      uint32_t ncs = cs + 16;
-      uint32_t nnw = ncs < n_cols ? (n_cols - ncs < 16 ? n_cols - ncs : 16) : 0;
+      const uint32_t w2 = words(cs + 32);
 #pragma unroll
-      for (uint32_t k = 0; k < 16; ++k) nx[k] = k < nnw ? cols[ncs + k][i] : 0u;
-      t += 4 * nw;
-      blake2s_compress(h, m, t, ncs >= n_cols);
+      for (uint32_t k = 0; k < 16; ++k) nx2[k] = k < w2 ? gmem(cols[cs + 32 + k])[i] : 0u;
+      t += 4 * words(cs);
+      blake2s_compress(h, m, t, cs + 16 >= n_cols);
 #pragma unroll
-      for (uint32_t k = 0; k < 16; ++k) m[k] = nx[k];
-      nw = nnw;
+      for (uint32_t k = 0; k < 16; ++k) { m[k] = nx1[k]; nx1[k] = nx2[k]; }
     }
   }
   uint4* o4 = reinterpret_cast<uint4*>(out + (size_t)i * 8);
@@ -75,6 +78,7 @@ struct ColPtrs16 { const uint32_t* c[16]; };
 __global__ void __launch_bounds__(256) merkle_layer_small_kernel(uint32_t n_nodes, const uint32_t* __restrict__ prev,
                                                                  ColPtrs16 cols, uint32_t n_cols,
                                                                  uint32_t* __restrict__ out) {
+  CRITICAL_PATH_WAVE();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_nodes) return;
   uint32_t h[8];
@@ -123,6 +127,7 @@ void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const ui
 struct LayerPtrs { uint32_t* p[8]; };
 __global__ void __launch_bounds__(512) merkle_subtree_kernel(const uint32_t* __restrict__ child, uint32_t L,
                                                              LayerPtrs outs) {
+  CRITICAL_PATH_WAVE();
   __shared__ uint32_t lds[128 * 8];
   const uint32_t w = blockIdx.x, nd = threadIdx.x >> 2, q = threadIdx.x & 3;
   uint32_t n = 1u << (L - 1);  // nodes of the first level computed by this workgroup

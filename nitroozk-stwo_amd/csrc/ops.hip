@@ -13,10 +13,10 @@
 namespace sh {
 
 __device__ __forceinline__ qm31 load_q(const uint32_t* const c[4], uint32_t i) {
-  return {c[0][i], c[1][i], c[2][i], c[3][i]};
+  return {gmem(c[0])[i], gmem(c[1])[i], gmem(c[2])[i], gmem(c[3])[i]};
 }
 __device__ __forceinline__ void store_q(uint32_t* const c[4], uint32_t i, qm31 v) {
-  c[0][i] = v.a; c[1][i] = v.b; c[2][i] = v.c; c[3][i] = v.d;
+  gmem(c[0])[i] = v.a; gmem(c[1])[i] = v.b; gmem(c[2])[i] = v.c; gmem(c[3])[i] = v.d;
 }
 
 // Inverse circle twiddle of pair h: line layer 0 pairs [x, y] -> [y, -y, -x, x].
@@ -101,7 +101,7 @@ void launch_fold_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* a
 // 8 entries per step with their loads issued together; the 8 partial sums meet in LDS.
 constexpr uint32_t NUM_WAVES = 8;
 constexpr uint32_t NUM_UNROLL = 8;
-__device__ __forceinline__ void mac4(qacc* acc, qm31 c, uint4 v) {
+__device__ __forceinline__ void mac4(qacc* acc, qm31 c, u32x4 v) {
   qacc_mac(acc[0], c, v.x);
   qacc_mac(acc[1], c, v.y);
   qacc_mac(acc[2], c, v.z);
@@ -122,20 +122,20 @@ __global__ void __launch_bounds__(512) accumulate_numerators_wide_kernel(const N
     for (; e + (NUM_UNROLL - 1) * NUM_WAVES < e_end; e += NUM_UNROLL * NUM_WAVES) {
       const uint32_t* col[NUM_UNROLL];
       qm31 c[NUM_UNROLL];
-      uint4 v[NUM_UNROLL];
+      u32x4 v[NUM_UNROLL];
 #pragma unroll
       for (uint32_t k = 0; k < NUM_UNROLL; ++k) {
         col[k] = entries[e + k * NUM_WAVES].col;
         c[k] = entries[e + k * NUM_WAVES].c;
       }
 #pragma unroll
-      for (uint32_t k = 0; k < NUM_UNROLL; ++k) v[k] = *reinterpret_cast<const uint4*>(col[k] + row0);
+      for (uint32_t k = 0; k < NUM_UNROLL; ++k) v[k] = *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(col[k]) + row0);
 #pragma unroll
       for (uint32_t k = 0; k < NUM_UNROLL; ++k) mac4(acc, c[k], v[k]);
     }
     for (; e < e_end; e += NUM_WAVES) {
       const NumEntry en = entries[e];
-      mac4(acc, en.c, *reinterpret_cast<const uint4*>(en.col + row0));
+      mac4(acc, en.c, *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(en.col) + row0));
     }
 #pragma unroll
     for (uint32_t r = 0; r < 4; ++r) {
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(512) accumulate_numerators_wide_kernel(const N
         uint32_t s = 0;
 #pragma unroll
         for (uint32_t w = 0; w < NUM_WAVES; ++w) s = m31_add(s, part[w][cc][r]);
-        out[4 * b + cc][blockIdx.x * 256 + r] = m31_sub(s, bw[cc]);
+        gmem(out[4 * b + cc])[blockIdx.x * 256 + r] = m31_sub(s, bw[cc]);
       }
     }
     __syncthreads();
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
     for (uint32_t r = 0; r < 8; ++r) {
       const uint32_t i = base + 4 * (t + 256 * r);
       if (i < n) {
-        const uint4 v = *reinterpret_cast<const uint4*>(task.coeffs + i);
+        const u32x4 v = *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(task.coeffs) + i);
         const uint32_t wi = i & tmask;
         qacc_mad(acc, W[wi], v.x);
         qacc_mad(acc, W[wi + 1], v.y);
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
       }
     }
   } else if (t < n) {
-    qacc_mad(acc, W[t], task.coeffs[t]);
+    qacc_mad(acc, W[t], gmem(task.coeffs)[t]);
   }
   qm31 out = qacc_reduce(acc);
   // wave reduction by shuffles, then across the 4 waves

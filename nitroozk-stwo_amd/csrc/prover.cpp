@@ -430,9 +430,14 @@ uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits) {
 void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
   t.layers.clear();
   if (t.evals.empty()) {
+    // a tree without columns is one node, Blake2s of the empty string: known on the host
+    t.root = blake2s_hash(nullptr, 0);
+    uint32_t w[8];
+    for (int i = 0; i < 8; ++i)
+      w[i] = (uint32_t)t.root[4 * i] | (uint32_t)t.root[4 * i + 1] << 8 | (uint32_t)t.root[4 * i + 2] << 16 |
+             (uint32_t)t.root[4 * i + 3] << 24;
     t.layers.emplace_back(&ctx, 8);
-    launch_merkle_layer(0, nullptr, nullptr, 0, t.layers[0].p, ctx.stream);
-    if (read_root) t.root = read_hash(ctx, t.layers[0].p);
+    launch_copy_words(t.layers[0].p, (const uint32_t*)ctx.upload(w, sizeof(w)), 8, ctx.stream);
     return;
   }
   auto order = merkle_order(t.eval_logs);
@@ -1010,8 +1015,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     launch_quotients(Lc, dxy, dq, (uint32_t)qpts.size(), dp, first.mc(), ctx.stream);
   }
   HMARK("q_launch");
-  ctx.sync();
-  ctx.reset_arena();
+  // no wait here: the FRI kernels queue behind the quotients while the host prepares them
   partial_bufs.clear();
   HMARK("q_gpu");
   stage_ms[4] = tm.lap();
@@ -1371,15 +1375,19 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   HMARK("pre_tree");
   interpolate_columns(ctx, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
   HMARK("tr_interp");
-  commit_tree(ctx, trees[1], cfg, ch);
-  HMARK("tr_commit");
+  evaluate_polynomials(ctx, trees[1], cfg.log_blowup);
   // The extension of the trace to the composition domain needs only its coefficients: it runs on
-  // the side stream, overlapping the latency-bound LogUp interaction phase.
+  // the side stream, beside the trace's Merkle tree and the LogUp interaction phase, whose
+  // kernels take issue priority over it.
   {
     const uint32_t E = log_n + 2;
     evaluate_into(ctx, trees[1].coeffs, trees[1].coeff_logs, E, trees[1].ext, trees[1].owned, true);
     trees[1].ext_log = E;
   }
+  merkle_commit(ctx, trees[1]);
+  ch.mix_root(trees[1].root);
+  ctx.reset_arena();
+  HMARK("tr_commit");
   r.stage_ms[0] = tm.lap();
   Component c;
   c.kind = AIR_POSEIDON;
@@ -1406,7 +1414,7 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
     {
       KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
-      launch_poseidon_logup_fracs(log_n, d_tr, d_ip, d_pp, (unsigned long long*)sums.p, ctx.stream);
+      launch_poseidon_logup_fracs(log_n, ipr, d_tr, d_ip, d_pp, (unsigned long long*)sums.p, ctx.stream);
     }
     unsigned long long s[4];
     ctx.d2h(s, sums.p, 32);
