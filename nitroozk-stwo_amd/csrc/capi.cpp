@@ -234,10 +234,14 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* h, uint32_t log_size, const uint32_t*
   return guarded(h, [&](Ctx& c) {
     if (log_size > 30) throw HipError(3, "log_size out of range");
     std::vector<const uint32_t*> v(cols, cols + n_cols);
-    const uint32_t* const* d = n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
+    // columns laid out at a fixed stride of one column need no address table
+    bool strided = n_cols > 1;
+    for (uint32_t k = 1; strided && k < n_cols; ++k) strided = v[k] == v[0] + ((size_t)k << log_size);
+    const uint32_t* const* d = !strided && n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
     {
       KScope ks(c, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * n_cols + 32.0) * (double)(1u << log_size));
-      launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
+      if (strided) launch_merkle_layer_strided(log_size, prev, v[0], (size_t)1 << log_size, n_cols, out, c.stream);
+      else launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
     }
     c.sync();
     c.reset_arena();

@@ -88,7 +88,8 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
                                                        uint32_t* const* __restrict__ dst, uint32_t log_n,
                                                        uint32_t in_mask, uint32_t e, uint32_t S, uint32_t c,
                                                        const uint32_t* __restrict__ tree, uint32_t tree_len,
-                                                       uint32_t scale) {
+                                                       uint32_t scale, uint32_t prio) {
+  if (prio) CRITICAL_PATH_WAVE();
   extern __shared__ uint32_t lds[];
   const uint32_t col = blockIdx.y;
   const GLOBAL_AS uint32_t* __restrict__ in = gmem(src[col]);
@@ -247,7 +248,8 @@ template <bool INV, int S>
 __global__ void __launch_bounds__(LowShape<S>::T) fft_low_kernel(const uint32_t* const* __restrict__ src,
                                                                  uint32_t* const* __restrict__ dst, uint32_t log_n,
                                                                  uint32_t in_mask, const uint32_t* __restrict__ tree,
-                                                                 uint32_t tree_len, uint32_t scale) {
+                                                                 uint32_t tree_len, uint32_t scale, uint32_t prio) {
+  if (prio) CRITICAL_PATH_WAVE();
   constexpr uint32_t N = 1u << S;
   // rounds in execution order: the inverse runs bits 0-4, 5-9, 10..S-1; the forward the reverse
   constexpr int A1 = INV ? 0 : 10, R1 = INV ? 5 : S - 10;
@@ -275,15 +277,15 @@ __global__ void __launch_bounds__(LowShape<S>::T) fft_low_kernel(const uint32_t*
 
 template <bool INV>
 static void launch_low(uint32_t S, uint32_t n_cols, const uint32_t* const* src, uint32_t* const* dst, uint32_t log_n,
-                       uint32_t in_mask, const uint32_t* tree, uint32_t tree_len, uint32_t scale,
+                       uint32_t in_mask, const uint32_t* tree, uint32_t tree_len, uint32_t scale, uint32_t prio,
                        hipStream_t stream) {
   const dim3 grid(n_cols << (log_n - S)), block(1u << (S - 5));
   switch (S) {
-    case 11: hipLaunchKernelGGL((fft_low_kernel<INV, 11>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
-    case 12: hipLaunchKernelGGL((fft_low_kernel<INV, 12>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
-    case 13: hipLaunchKernelGGL((fft_low_kernel<INV, 13>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
-    case 14: hipLaunchKernelGGL((fft_low_kernel<INV, 14>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
-    default: hipLaunchKernelGGL((fft_low_kernel<INV, 15>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale); break;
+    case 11: hipLaunchKernelGGL((fft_low_kernel<INV, 11>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale, prio); break;
+    case 12: hipLaunchKernelGGL((fft_low_kernel<INV, 12>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale, prio); break;
+    case 13: hipLaunchKernelGGL((fft_low_kernel<INV, 13>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale, prio); break;
+    case 14: hipLaunchKernelGGL((fft_low_kernel<INV, 14>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale, prio); break;
+    default: hipLaunchKernelGGL((fft_low_kernel<INV, 15>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale, prio); break;
   }
 }
 
@@ -310,8 +312,9 @@ static std::vector<FftPass> plan_passes(uint32_t k, uint32_t K) {
 }
 
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
-                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream) {
+                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background) {
   if (n_cols == 0) return;
+  const uint32_t prio = background ? 0u : 1u;
   const uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
   const uint32_t inv_n = inverse ? m31_inv(1u << log_n) : 0;
   auto run_pass = [&](const FftPass& ps, const uint32_t* const* from, uint32_t mask, uint32_t scale) {
@@ -319,10 +322,10 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
     const uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
     if (inverse)
       hipLaunchKernelGGL(fft_pass_kernel<true>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream, from, dst,
-                         log_n, mask, ps.e, ps.S, ps.c, tree, tree_len, scale);
+                         log_n, mask, ps.e, ps.S, ps.c, tree, tree_len, scale, prio);
     else
       hipLaunchKernelGGL(fft_pass_kernel<false>, dim3(tiles, n_cols), dim3(threads), tile_n * 4, stream, from, dst,
-                         log_n, mask, ps.e, ps.S, ps.c, tree, tree_len, scale);
+                         log_n, mask, ps.e, ps.S, ps.c, tree, tree_len, scale, prio);
   };
   const uint32_t full = (1u << log_n) - 1, in_mask = inverse ? full : (1u << log_in) - 1;
   if (k >= 11) {
@@ -336,7 +339,7 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
       e += st;
     }
     if (inverse) {
-      launch_low<true>(S, n_cols, src, dst, log_n, full, tree, tree_len, high.empty() ? inv_n : 0u, stream);
+      launch_low<true>(S, n_cols, src, dst, log_n, full, tree, tree_len, high.empty() ? inv_n : 0u, prio, stream);
       for (size_t i = 0; i < high.size(); ++i)
         run_pass(high[i], (const uint32_t* const*)dst, full, i + 1 == high.size() ? inv_n : 0u);
     } else {
@@ -344,7 +347,7 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
         run_pass(high[ii], ii + 1 == high.size() ? src : (const uint32_t* const*)dst,
                  ii + 1 == high.size() ? in_mask : full, 0u);
       launch_low<false>(S, n_cols, high.empty() ? src : (const uint32_t* const*)dst, dst, log_n,
-                        high.empty() ? in_mask : full, tree, tree_len, 0u, stream);
+                        high.empty() ? in_mask : full, tree, tree_len, 0u, prio, stream);
     }
     return;
   }

@@ -12,12 +12,17 @@ struct Cols4 { uint32_t* c[4]; };
 struct CCols4 { const uint32_t* c[4]; };
 
 // fft.hip
+// background: the transform runs beside critical-path kernels on another stream and leaves them
+// the issue priority.
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
-                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream);
+                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background = false);
 
 // merkle.hip
 void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
                          uint32_t* out, hipStream_t stream);
+// The same with column c at base + c * stride (columns of one allocation): no address table.
+void launch_merkle_layer_strided(uint32_t log_size, const uint32_t* prev, const uint32_t* base, size_t stride,
+                                 uint32_t n_cols, uint32_t* out, hipStream_t stream);
 // The same for at most 16 columns, given as a host array of device addresses.
 void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const uint32_t* const* host_cols,
                                uint32_t n_cols, uint32_t* out, hipStream_t stream);

@@ -15,8 +15,17 @@
 
 namespace sh {
 
+// Column addresses: a table, or (STRIDED) column c at base + c * stride, which needs no address
+// loads at all.
+struct LeafCols {
+  const uint32_t* const* table;
+  const uint32_t* base;
+  size_t stride;
+};
+
+template <bool STRIDED>
 __global__ void __launch_bounds__(256) merkle_layer_kernel(uint32_t n_nodes, const uint32_t* __restrict__ prev,
-                                                           const uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                                           LeafCols cols, uint32_t n_cols,
                                                            uint32_t* __restrict__ out) {
   CRITICAL_PATH_WAVE();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,21 +49,31 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(uint32_t n_nodes, con
     blake2s_compress(h, m, 0, true);
   }
   if (n_cols) {
-    // software pipeline: the column words of the next two blocks are in flight while a block
-    // compresses (one wave per SIMD when a tree has few leaves, so nothing else hides the loads)
-    auto words = [&](uint32_t cs) { return cs < n_cols ? (n_cols - cs < 16 ? n_cols - cs : 16) : 0u; };
+    // Software pipeline: the column words of the next two blocks are in flight while a block
+    // compresses (one wave per SIMD when a tree has few leaves, so nothing else hides the loads).
+    // Loads are issued unconditionally and two blocks ahead (a column index past the end is
+    // clamped to the last column), and only the last block, which may be partial, is masked when it
+    // is consumed: masking every loaded word would make each block wait for all loads in flight.
+    auto raw = [&](uint32_t c) {
+      c = min(c, n_cols - 1);
+      return gmem(STRIDED ? cols.base + (size_t)c * cols.stride : cols.table[c])[i];
+    };
+    const uint32_t rem = n_cols & 15, n_blocks = (n_cols + 15) >> 4;
     uint32_t nx1[16], nx2[16];
-    const uint32_t w0 = words(0), w1 = words(16);
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) m[k] = k < w0 ? gmem(cols[k])[i] : 0u;
+    for (uint32_t k = 0; k < 16; ++k) m[k] = raw(k);
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) nx1[k] = k < w1 ? gmem(cols[16 + k])[i] : 0u;
-    for (uint32_t cs = 0; cs < n_cols; cs += 16) {
-      const uint32_t w2 = words(cs + 32);
+    for (uint32_t k = 0; k < 16; ++k) nx1[k] = raw(16 + k);
+    for (uint32_t b = 0; b < n_blocks; ++b) {
 #pragma unroll
-      for (uint32_t k = 0; k < 16; ++k) nx2[k] = k < w2 ? gmem(cols[cs + 32 + k])[i] : 0u;
-      t += 4 * words(cs);
-      blake2s_compress(h, m, t, cs + 16 >= n_cols);
+      for (uint32_t k = 0; k < 16; ++k) nx2[k] = raw(16 * (b + 2) + k);
+      const bool last = b + 1 == n_blocks;
+      if (last && rem) {
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) m[k] = k < rem ? m[k] : 0u;
+      }
+      t += last && rem ? 4 * rem : 64;
+      blake2s_compress(h, m, t, last);
 #pragma unroll
       for (uint32_t k = 0; k < 16; ++k) { m[k] = nx1[k]; nx1[k] = nx2[k]; }
     }
@@ -64,12 +83,26 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(uint32_t n_nodes, con
   o4[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
 
-void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
-                         uint32_t* out, hipStream_t stream) {
+static void launch_leaf_layer(uint32_t log_size, const uint32_t* prev, LeafCols cols, uint32_t n_cols, uint32_t* out,
+                              hipStream_t stream) {
   uint32_t n = 1u << log_size;
   uint32_t threads = n < 256 ? (n < 64 ? 64 : n) : 256;
-  hipLaunchKernelGGL(merkle_layer_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, n, prev, cols,
-                     n_cols, out);
+  if (cols.base)
+    hipLaunchKernelGGL(merkle_layer_kernel<true>, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, n, prev,
+                       cols, n_cols, out);
+  else
+    hipLaunchKernelGGL(merkle_layer_kernel<false>, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, n,
+                       prev, cols, n_cols, out);
+}
+
+void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
+                         uint32_t* out, hipStream_t stream) {
+  launch_leaf_layer(log_size, prev, LeafCols{cols, nullptr, 0}, n_cols, out, stream);
+}
+
+void launch_merkle_layer_strided(uint32_t log_size, const uint32_t* prev, const uint32_t* base, size_t stride,
+                                 uint32_t n_cols, uint32_t* out, hipStream_t stream) {
+  launch_leaf_layer(log_size, prev, LeafCols{nullptr, base, stride}, n_cols, out, stream);
 }
 
 // The same node hash for at most 16 injected columns whose addresses travel as kernel

@@ -396,7 +396,7 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
       st = ctx.side;
     }
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)), st);
-    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), st);
+    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), st, on_side);
   }
 }
 
@@ -471,6 +471,9 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
     if (lc.size() <= 16) {
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
       launch_merkle_layer_small((uint32_t)ls, prev, lc.data(), (uint32_t)lc.size(), layer.p, ctx.stream);
+    } else if (is_strided(lc, (size_t)1 << ls)) {
+      KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
+      launch_merkle_layer_strided((uint32_t)ls, prev, lc[0], (size_t)1 << ls, (uint32_t)lc.size(), layer.p, ctx.stream);
     } else {
       const uint32_t* const* cp = (const uint32_t* const*)ctx.table_vec(lc);
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
@@ -1375,10 +1378,11 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   HMARK("pre_tree");
   interpolate_columns(ctx, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
   HMARK("tr_interp");
+  // The extension of the trace to the composition domain needs only its coefficients. It runs on
+  // the side stream once the commitment's transforms are queued (two transforms side by side
+  // would split the CUs, whose LDS holds one transform block each), beside the trace's Merkle
+  // tree and the LogUp interaction phase, whose kernels take issue priority.
   evaluate_polynomials(ctx, trees[1], cfg.log_blowup);
-  // The extension of the trace to the composition domain needs only its coefficients: it runs on
-  // the side stream, beside the trace's Merkle tree and the LogUp interaction phase, whose
-  // kernels take issue priority over it.
   {
     const uint32_t E = log_n + 2;
     evaluate_into(ctx, trees[1].coeffs, trees[1].coeff_logs, E, trees[1].ext, trees[1].owned, true);
@@ -1432,7 +1436,14 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     for (auto& b : icols) ev.push_back(b.p);
     trees.emplace_back();
     interpolate_columns(ctx, ev, std::vector<uint32_t>(ev.size(), log_n), trees[2]);
-    commit_tree(ctx, trees[2], cfg, ch);
+    evaluate_polynomials(ctx, trees[2], cfg.log_blowup);
+    // the interaction columns' extension follows the trace's on the side stream
+    const uint32_t E = log_n + 2;
+    evaluate_into(ctx, trees[2].coeffs, trees[2].coeff_logs, E, trees[2].ext, trees[2].owned, true);
+    trees[2].ext_log = E;
+    merkle_commit(ctx, trees[2]);
+    ch.mix_root(trees[2].root);
+    ctx.reset_arena();
   }
   icols.clear();
   HMARK("inter_commit");

@@ -152,10 +152,29 @@ void Ctx::col_free(void* p) {
   col_cache[bytes].push_back(p);  // reused by later work on this context's stream, in order
   col_cached_bytes += bytes;
 }
+// 64-bit content hash of a table (8 bytes at a time, multiply-xorshift mixing).
+static uint64_t table_hash(const void* src, size_t bytes) {
+  const uint8_t* p = (const uint8_t*)src;
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ bytes;
+  size_t i = 0;
+  for (; i + 8 <= bytes; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + i, 8);
+    h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+  }
+  uint64_t w = 0;
+  std::memcpy(&w, p + i, bytes - i);
+  h = (h ^ w) * 0x94D049BB133111EBull;
+  return h ^ (h >> 29);
+}
 const void* Ctx::upload_table(const void* src, size_t bytes) {
-  std::string key((const char*)src, bytes);
-  auto it = table_cache.find(key);
-  if (it != table_cache.end()) return it->second;
+  const uint64_t key = table_hash(src, bytes);
+  // a hit is confirmed against the pinned copy of the table's bytes
+  auto range = table_cache.equal_range(key);
+  for (auto it = range.first; it != range.second; ++it)
+    if (it->second.bytes == bytes && std::memcmp(table_pinned + it->second.off, src, bytes) == 0)
+      return table_dev + it->second.off;
   const size_t a = (bytes + 255) & ~(size_t)255;
   if (!table_dev) {
     table_cap = 16u << 20;
@@ -167,8 +186,8 @@ const void* Ctx::upload_table(const void* src, size_t bytes) {
   std::memcpy(table_pinned + table_off, src, bytes);
   HIP_CHECK(hipMemcpyAsync(table_dev + table_off, table_pinned + table_off, bytes, hipMemcpyHostToDevice, stream));
   const void* d = table_dev + table_off;
+  table_cache.emplace(key, TableEntry{table_off, bytes});
   table_off += a;
-  table_cache.emplace(std::move(key), d);
   return d;
 }
 void* Ctx::upload(const void* src, size_t bytes) {

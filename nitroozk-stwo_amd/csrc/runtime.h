@@ -132,7 +132,8 @@ struct Ctx {
   // upload() once the table store is full.
   const void* upload_table(const void* src, size_t bytes);
   template <class T> const T* table_vec(const std::vector<T>& v) { return (const T*)upload_table(v.data(), v.size() * sizeof(T)); }
-  std::unordered_map<std::string, const void*> table_cache;
+  struct TableEntry { size_t off, bytes; };
+  std::unordered_multimap<uint64_t, TableEntry> table_cache;  // content hash -> table in table_dev
   uint8_t* table_dev = nullptr;
   uint8_t* table_pinned = nullptr;
   size_t table_cap = 0, table_off = 0;

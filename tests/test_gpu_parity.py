@@ -93,6 +93,21 @@ def test_commit_on_layer(hb, log_n, n_cols, with_prev):
     assert out.view(np.uint8).tobytes() == exp.tobytes()
 
 
+@pytest.mark.parametrize("log_n,n_cols,with_prev", [(6, 17, False), (8, 31, True), (11, 100, False), (12, 16, False)])
+def test_commit_on_layer_strided_columns(hb, log_n, n_cols, with_prev):
+    """Columns at a fixed stride of one column in one allocation take the address-free kernel."""
+    say("merkle layer strided", log_n, n_cols, with_prev)
+    rng = np.random.default_rng(log_n * 7 + n_cols)
+    cols = rand_cols(rng, n_cols, log_n)
+    big = hb.from_cpu(np.concatenate(cols))
+    views = [big.ptr + 4 * (c << log_n) for c in range(n_cols)]
+    prev = rng.integers(0, 256, (2 << log_n) * 32, dtype=np.uint8) if with_prev else None
+    dprev = hb.from_cpu(prev.view(np.uint32)) if with_prev else None
+    out = hb.commit_on_layer(log_n, dprev, views).to_cpu()
+    exp = O.commit_on_layer(log_n, prev, cols)
+    assert out.view(np.uint8).tobytes() == exp.tobytes()
+
+
 @pytest.mark.parametrize("log_src", [3, 5, 10, 16])
 def test_fold_circle_into_line(hb, log_src):
     say("fold circle", log_src)
