@@ -25,6 +25,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per proof of the roofline kernel class, from two rocprofv3 counter passes (FETCH_SIZE,
+# WRITE_SIZE) over this bench, each scaled by the counter's reading for a 256 MiB device copy
+# (profiles/evaluate_traffic_poseidon2_2e18.md). Keyed by (workload, log instances, instances per
+# row, kernel class); other workloads report null.
+MEASURED_TRAFFIC = {("poseidon2", 18, 8, "evaluate"): 1413.9e6}
 
 
 def load_pkg():
@@ -375,6 +380,9 @@ def main():
     dom = max((k for k in stats if k != "grind"), key=lambda k: stats[k][0])
     ms, by, n = stats[dom]
     achieved = by / (ms / 1e3) / 1e9
+    traffic = None
+    if args.workload == "poseidon":
+        traffic = MEASURED_TRAFFIC.get(("poseidon2", args.log_instances, args.ipr, dom))
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[2] // args.steps,
                    "GB_per_s": round(v[1] / (v[0] / 1e3) / 1e9, 1) if v[0] > 0 and v[1] > 0 else None}
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
@@ -415,7 +423,9 @@ def main():
         "byte_identical_to_oracle": parity,
         "stage_ms": stage,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_unit": "HBM bytes per proof over the class's launches" if traffic else None,
+                     "traffic_over_algorithmic": round(traffic / (by / args.steps), 3) if traffic else None},
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -51,7 +51,8 @@ def test_interpolate_columns(hb, log_n):
         assert np.array_equal(o.to_cpu(), O.interpolate(c))
 
 
-@pytest.mark.parametrize("log_n,blow", [(3, 1), (4, 2), (7, 1), (10, 3), (11, 1), (12, 1), (13, 1), (15, 2), (17, 1)])
+@pytest.mark.parametrize("log_n,blow", [(3, 1), (4, 2), (7, 1), (10, 3), (11, 1), (12, 1), (13, 1), (15, 1), (15, 2),
+                                        (16, 1), (17, 1)])
 def test_evaluate_polynomials(hb, log_n, blow):
     say("evaluate log", log_n, "blowup", blow)
     rng = np.random.default_rng(100 + log_n)
