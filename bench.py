@@ -229,8 +229,7 @@ def serve(args, m):
                         pr = hb.prove_poseidon(tr, width, log_rows, cfg)
                     _ = pr.bytes
                     done = time.perf_counter()
-                    for c in tr:
-                        c.free()
+                    hb.free_columns(tr)
                     freed = time.perf_counter()
                     with lock:
                         gen_ms.append((s1 - s0) * 1e3)
@@ -254,8 +253,7 @@ def serve(args, m):
                 if big[kind] == log_rows:
                     (hb.prove_wide_fibonacci(tr, log_rows, cfg) if kind == "wide_fib"
                      else hb.prove_poseidon(tr, width, log_rows, cfg))
-                for c in tr:
-                    c.free()
+                hb.free_columns(tr)
         t0 = time.perf_counter()
         threads = [threading.Thread(target=worker, args=(hb,)) for hb in backends]
         for th in threads:

@@ -56,6 +56,11 @@ const char* stwo_hip_last_error(stwo_hip_ctx* ctx);
  * are kept by the context and reused, in stream order, by later allocations of the same size. */
 int stwo_hip_alloc(stwo_hip_ctx* ctx, size_t n_words, int zeroed, uint32_t** out);
 int stwo_hip_free(stwo_hip_ctx* ctx, uint32_t* ptr);
+/* The same for n_cols columns of n_words each in one call (a trace's columns: one library call
+ * instead of one per column, which is what a host binding pays per call). out receives n_cols
+ * addresses; the free stops at the first address the context did not hand out (status 1). */
+int stwo_hip_alloc_columns(stwo_hip_ctx* ctx, uint32_t n_cols, size_t n_words, int zeroed, uint32_t** out);
+int stwo_hip_free_columns(stwo_hip_ctx* ctx, uint32_t* const* ptrs, uint32_t n_cols);
 int stwo_hip_copy_h2d(stwo_hip_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t n_words);
 int stwo_hip_copy_d2h(stwo_hip_ctx* ctx, uint32_t* dst, const uint32_t* src, size_t n_words); /* synchronous */
 /* Column::batch_at_multi (crates/stwo/src/prover/backend/mod.rs:67-89; batch_gather_multi_uint32,

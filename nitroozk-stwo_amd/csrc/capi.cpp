@@ -93,6 +93,21 @@ int stwo_hip_alloc(stwo_hip_ctx* h, size_t n, int zeroed, uint32_t** out) {
 int stwo_hip_free(stwo_hip_ctx* h, uint32_t* p) {
   return guarded(h, [&](Ctx& c) { if (p) c.col_free(p); });
 }
+int stwo_hip_alloc_columns(stwo_hip_ctx* h, uint32_t n_cols, size_t n, int zeroed, uint32_t** out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!out && n_cols) throw HipError(1, "null out");
+    for (uint32_t k = 0; k < n_cols; ++k) {
+      out[k] = (uint32_t*)c.col_alloc(n ? n * 4 : 4);
+      if (zeroed) launch_fill_zero(out[k], n, c.stream);
+    }
+  });
+}
+int stwo_hip_free_columns(stwo_hip_ctx* h, uint32_t* const* ptrs, uint32_t n_cols) {
+  return guarded(h, [&](Ctx& c) {
+    for (uint32_t k = 0; k < n_cols; ++k)
+      if (ptrs[k]) c.col_free(ptrs[k]);
+  });
+}
 int stwo_hip_copy_h2d(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_t n) {
   return guarded(h, [&](Ctx& c) {
     HIP_CHECK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, c.stream));

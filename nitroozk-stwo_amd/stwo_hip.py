@@ -67,7 +67,7 @@ _VPP = ctypes.POINTER(ctypes.c_void_p)
 
 EXPORTS = [
     "stwo_hip_create", "stwo_hip_destroy", "stwo_hip_synchronize", "stwo_hip_last_error", "stwo_hip_alloc",
-    "stwo_hip_free", "stwo_hip_copy_h2d", "stwo_hip_copy_d2h", "stwo_hip_batch_at_multi",
+    "stwo_hip_free", "stwo_hip_alloc_columns", "stwo_hip_free_columns", "stwo_hip_copy_h2d", "stwo_hip_copy_d2h", "stwo_hip_batch_at_multi",
     "stwo_hip_precompute_twiddles", "stwo_hip_interpolate_columns", "stwo_hip_evaluate_polynomials",
     "stwo_hip_batch_eval_at_point", "stwo_hip_barycentric_weights", "stwo_hip_batch_barycentric_eval",
     "stwo_hip_commit_on_layer", "stwo_hip_fold_circle_into_line",
@@ -103,6 +103,8 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_last_error.restype = c.c_char_p
     L.stwo_hip_alloc.argtypes = [c.c_void_p, c.c_size_t, c.c_int, c.POINTER(c.c_void_p)]
     L.stwo_hip_free.argtypes = [c.c_void_p, c.c_void_p]
+    L.stwo_hip_alloc_columns.argtypes = [c.c_void_p, c.c_uint32, c.c_size_t, c.c_int, _VPP]
+    L.stwo_hip_free_columns.argtypes = [c.c_void_p, _VPP, c.c_uint32]
     L.stwo_hip_copy_h2d.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_size_t]
     L.stwo_hip_copy_d2h.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_size_t]
     L.stwo_hip_batch_at_multi.argtypes = [c.c_void_p, _VPP, c.c_uint32, _U32P, c.c_uint32, _U32P]
@@ -167,9 +169,12 @@ def _ptrs(cols):
 class DeviceColumn:
     """A device-resident column of u32 words (M31 values), freed with its backend."""
 
-    def __init__(self, backend, n_words, zeroed=False):
+    def __init__(self, backend, n_words, zeroed=False, ptr=None):
         self.backend = backend
         self.n = int(n_words)
+        if ptr is not None:  # an address the backend already allocated (columns())
+            self.ptr = ptr
+            return
         p = ctypes.c_void_p()
         backend._check(backend.lib.stwo_hip_alloc(backend.ctx, self.n, int(zeroed), ctypes.byref(p)))
         self.ptr = p.value
@@ -238,6 +243,22 @@ class HipBackend:
     # ---- columns ----
     def column(self, n_words, zeroed=False):
         return DeviceColumn(self, n_words, zeroed)
+
+    def columns(self, n_cols, n_words, zeroed=False):
+        """n_cols columns of n_words each, allocated in one library call."""
+        arr = (ctypes.c_void_p * max(1, n_cols))()
+        self._check(self.lib.stwo_hip_alloc_columns(self.ctx, n_cols, n_words, int(zeroed), arr))
+        return [DeviceColumn(self, n_words, ptr=arr[k]) for k in range(n_cols)]
+
+    def free_columns(self, cols):
+        """Frees columns in one library call."""
+        live = [col for col in cols if col.ptr]
+        if not live:
+            return
+        st = self.lib.stwo_hip_free_columns(self.ctx, _ptrs(live), len(live))
+        for col in live:
+            col.ptr = None
+        self._check(st)
 
     def from_cpu(self, arr):
         a = np.ascontiguousarray(arr, np.uint32)
@@ -400,13 +421,13 @@ class HipBackend:
 
     # ---- trace generation and whole proofs ----
     def gen_trace_wide_fibonacci(self, n_cols, log_n_rows, mode=1, seed=0):
-        cols = [self.column(1 << log_n_rows) for _ in range(n_cols)]
+        cols = self.columns(n_cols, 1 << log_n_rows)
         self._check(self.lib.stwo_hip_gen_trace_wide_fibonacci(self.ctx, _ptrs(cols), n_cols, log_n_rows, mode,
                                                                seed))
         return cols
 
     def gen_trace_poseidon(self, instances_per_row, log_n_rows, mode=1, seed=0):
-        cols = [self.column(1 << log_n_rows) for _ in range(158 * instances_per_row)]
+        cols = self.columns(158 * instances_per_row, 1 << log_n_rows)
         self._check(self.lib.stwo_hip_gen_trace_poseidon(self.ctx, _ptrs(cols), instances_per_row, log_n_rows, mode,
                                                          seed))
         return cols

@@ -75,6 +75,20 @@ def test_fft_round_trip_large(hb):
     assert np.array_equal(back[: 1 << log_n], c) and not back[1 << log_n:].any()
 
 
+def test_alloc_and_free_columns(hb):
+    say("alloc columns")
+    cols = hb.columns(5, 1 << 12, zeroed=True)
+    assert len({c.ptr for c in cols}) == 5
+    assert all(not c.to_cpu().any() for c in cols)
+    data = np.arange(1 << 12, dtype=np.uint32)
+    hb.lib.stwo_hip_copy_h2d(hb.ctx, cols[2].ptr, data.ctypes.data, data.size)
+    assert np.array_equal(cols[2].to_cpu(), data)
+    hb.free_columns(cols)
+    assert all(c.ptr is None for c in cols)
+    again = hb.columns(5, 1 << 12)  # served from the context's cache of freed blocks
+    hb.free_columns(again)
+
+
 def test_twiddles_match_oracle(hb):
     tw, itw = hb.precompute_twiddles(10)
     otw, oitw = O.twiddles(10)
