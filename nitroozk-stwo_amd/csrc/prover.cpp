@@ -669,6 +669,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   for (auto& c : comps) { max_log = std::max(max_log, c.eval_log()); T += c.n_constraints(); }
   std::vector<qm31> powers(T);
   { qm31 p = qm_one(); for (size_t i = 0; i < T; ++i) { powers[i] = p; p = qm_mul(p, random_coeff); } }
+  HMARK("x_pows");
   std::map<uint32_t, std::vector<DevBuf>> accs;
   for (auto& c : comps) {
     size_t nc = c.n_constraints();
@@ -710,6 +711,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     const uint32_t* d_dinv = ctx.table_vec(dinv_br);
     const qm31* d_pows = ctx.stage_vec(cp);
     const uint32_t* const* d_t = (const uint32_t* const*)ctx.table_vec(tcols);
+    HMARK("x_tables");
     double cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
     if (join_side) ctx.join();
     if (c.kind == AIR_WIDE_FIB) {
@@ -739,6 +741,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   Cols4 b4 = {{big[0].p, big[1].p, big[2].p, big[3].p}};
   for (auto& [lg, a] : accs)
     if (lg != max_log) launch_accumulate_lifted(max_log, b4, lg, {{a[0].p, a[1].p, a[2].p, a[3].p}}, ctx.stream);
+  HMARK("x_compkernel");
   DevTree comp;
   {
     std::vector<const uint32_t*> ev = {big[0].p, big[1].p, big[2].p, big[3].p};
@@ -913,6 +916,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   for (size_t t = 0; t < trees.size(); ++t)
     for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
   std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
+  HMARK("x_qitems");
   struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; qm31 first_linear; };
   std::vector<Acc> acc_list;
   std::vector<DevBuf> partial_bufs;
@@ -980,6 +984,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       KScope ks(ctx, K_NUMERATORS, (4.0 * gcols.size() + 16.0 * batches.size()) * (double)(1u << ls));
       launch_accumulate_numerators(ls, d_ents, d_starts, d_bs, (uint32_t)batches.size(), d_outs, aligned, ctx.stream);
     }
+    HMARK("x_qgroup");
     i = j;
   }
   HMARK("q_numer");
@@ -1266,6 +1271,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     }
   }
   auto qpos = sorted_unique(unsorted);
+  HMARK("x_queries");
   Gather g;
   std::vector<FriPlan> fplans;
   fplans.push_back(plan_fri_layer(first, qpos, 1, g));
@@ -1274,6 +1280,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     fplans.push_back(plan_fri_layer(l, lq, l.fold_step, g));
     lq = fold_queries(lq, l.fold_step);
   }
+  HMARK("x_friplan");
   std::vector<MerklePlan> tplans;
   for (auto& t : trees) {
     auto q = tree_local_positions(qpos, Lc, t.max_log());

@@ -159,10 +159,38 @@ def load_library(path=LIB_PATH):
     return L
 
 
+class ColumnList(tuple):
+    """An immutable sequence of device columns that keeps the array of their addresses, so that a
+    library call taking the columns (a whole trace, every proof) does not rebuild it element by
+    element in Python."""
+
+    def __new__(cls, cols):
+        self = super().__new__(cls, cols)
+        self._arr = (ctypes.c_void_p * max(1, len(self)))(
+            *[col.ptr if isinstance(col, DeviceColumn) else col for col in self])
+        return self
+
+
 def _ptrs(cols):
+    if isinstance(cols, ColumnList):
+        return cols._arr
     arr = (ctypes.c_void_p * max(1, len(cols)))()
     for i, col in enumerate(cols):
         arr[i] = col.ptr if isinstance(col, DeviceColumn) else col
+    return arr
+
+
+def _concat_ptrs(lists):
+    """The address array of several column sequences, copied array by array when they are
+    ColumnLists."""
+    n = sum(len(t) for t in lists)
+    arr = (ctypes.c_void_p * max(1, n))()
+    off = 0
+    for t in lists:
+        src = _ptrs(t)
+        ctypes.memmove(ctypes.addressof(arr) + off * ctypes.sizeof(ctypes.c_void_p), src,
+                       len(t) * ctypes.sizeof(ctypes.c_void_p))
+        off += len(t)
     return arr
 
 
@@ -248,14 +276,18 @@ class HipBackend:
         """n_cols columns of n_words each, allocated in one library call."""
         arr = (ctypes.c_void_p * max(1, n_cols))()
         self._check(self.lib.stwo_hip_alloc_columns(self.ctx, n_cols, n_words, int(zeroed), arr))
-        return [DeviceColumn(self, n_words, ptr=arr[k]) for k in range(n_cols)]
+        return ColumnList(DeviceColumn(self, n_words, ptr=arr[k]) for k in range(n_cols))
 
     def free_columns(self, cols):
         """Frees columns in one library call."""
-        live = [col for col in cols if col.ptr]
+        if isinstance(cols, ColumnList) and all(col.ptr for col in cols):
+            live, arr = cols, cols._arr  # the cached address array
+        else:
+            live = [col for col in cols if col.ptr]
+            arr = _ptrs(live)
         if not live:
             return
-        st = self.lib.stwo_hip_free_columns(self.ctx, _ptrs(live), len(live))
+        st = self.lib.stwo_hip_free_columns(self.ctx, arr, len(live))
         for col in live:
             col.ptr = None
         self._check(st)
@@ -440,11 +472,10 @@ class HipBackend:
 
     def prove_wide_fibonacci_multi(self, traces, log_n_rows, config):
         """traces[i]: the columns of component i (2^log_n_rows[i] rows each)."""
-        flat = [col for t in traces for col in t]
         logs = np.array(log_n_rows, np.uint32)
         ncols = np.array([len(t) for t in traces], np.uint32)
         h = ctypes.c_void_p()
-        self._check(self.lib.stwo_hip_prove_wide_fibonacci_multi(self.ctx, _ptrs(flat), len(traces),
+        self._check(self.lib.stwo_hip_prove_wide_fibonacci_multi(self.ctx, _concat_ptrs(traces), len(traces),
                                                                  logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
                                                                  ctypes.byref(config), ctypes.byref(h)))
         return Proof(self, h.value)
