@@ -199,8 +199,9 @@ int stwo_hip_batch_eval_at_point(stwo_hip_ctx* h, const uint32_t* const* coeffs,
     for (uint32_t i = 0; i < n_cols; ++i) tasks.push_back({coeffs[i], log_size, i, {0, 0}});
     uint32_t max_chunks = oods_max_chunks(log_size);
     DevBuf partials(&c, (size_t)n_cols * max_chunks * 4), vals(&c, (size_t)n_cols * 4), w(&c, (size_t)4 << OODS_TABLE_LOG);
-    launch_oods(c.upload_vec(tasks), c.upload_vec(tg), n_cols, log_size, c.upload_vec(f), c.upload_vec(gl), 1,
-                (qm31*)w.p, (qm31*)partials.p, (qm31*)vals.p, c.stream);
+    const std::vector<uint32_t> bs = oods_batch_starts(tg);
+    launch_oods(c.upload_vec(tasks), c.upload_vec(tg), n_cols, c.upload_vec(bs), (uint32_t)bs.size() - 1, log_size,
+                c.upload_vec(f), c.upload_vec(gl), 1, (qm31*)w.p, (qm31*)partials.p, (qm31*)vals.p, c.stream);
     c.d2h(out_host, vals.p, (size_t)n_cols * 16);
     c.reset_arena();
   });

@@ -94,11 +94,24 @@ void launch_accumulate_lifted(uint32_t log_dst, Cols4 dst, uint32_t log_src, CCo
 // group_factors[g*32 + b] is the folding factor of bit b (y, x, pi(x), ...). Scratch: weights holds
 // 2^OODS_TABLE_LOG QM31 per group, partials oods_max_chunks(max_log) QM31 per task.
 constexpr uint32_t OODS_TABLE_LOG = 16;
-inline uint32_t oods_max_chunks(uint32_t max_log) { return max_log > 13 ? 1u << (max_log - 13) : 1u; }
+constexpr uint32_t OODS_BLOCK_LOG = 11;  // coefficients per block of the chunk kernel
+inline uint32_t oods_max_chunks(uint32_t max_log) {
+  return max_log > OODS_BLOCK_LOG ? 1u << (max_log - OODS_BLOCK_LOG) : 1u;
+}
 struct OodsTask { const uint32_t* coeffs; uint32_t log_size; uint32_t out_index; uint32_t pad[2]; };
-void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, uint32_t max_log,
-                 const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,
-                 qm31* partials, qm31* out, hipStream_t stream);
+// Tasks are sorted by group; batch b is tasks [batch_start[b], batch_start[b+1]), at most
+// OODS_BATCH of one group, which share each weight read (oods_batch_starts builds the list).
+constexpr uint32_t OODS_BATCH = 8;
+inline std::vector<uint32_t> oods_batch_starts(const std::vector<uint32_t>& sorted_task_group) {
+  std::vector<uint32_t> st;
+  for (uint32_t i = 0; i < sorted_task_group.size(); ++i)
+    if (st.empty() || i - st.back() == OODS_BATCH || sorted_task_group[i] != sorted_task_group[i - 1]) st.push_back(i);
+  st.push_back((uint32_t)sorted_task_group.size());
+  return st;
+}
+void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, const uint32_t* batch_start,
+                 uint32_t n_batches, uint32_t max_log, const qm31* group_factors, const uint32_t* group_log,
+                 uint32_t n_groups, qm31* weights, qm31* partials, qm31* out, hipStream_t stream);
 
 // Barycentric weights of CanonicCoset(log_size) for p (4 coordinate columns) and the batched
 // sums sum_i evals_c[i] w_i; shifts[j-1] (j = 1 .. log_size-1) as in barycentric_weights_kernel.

@@ -353,16 +353,19 @@ __global__ void __launch_bounds__(256) oods_weights_kernel(const qm31* __restric
   W[((size_t)g << OODS_TABLE_LOG) + i] = w;
 }
 
-constexpr uint32_t OODS_BLOCK_LOG = 13;  // coefficients per block
+// A block takes 2^OODS_BLOCK_LOG coefficients of up to OODS_BATCH columns of one group (tasks
+// batch_start[b] .. batch_start[b+1]), so that each weight it reads from L2 serves every column of
+// the batch; small blocks keep many of them resident to hide the loads.
 __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restrict__ tasks,
                                                          const uint32_t* __restrict__ task_group,
+                                                         const uint32_t* __restrict__ batch_start,
                                                          const qm31* __restrict__ factors,
                                                          const qm31* __restrict__ Wall, qm31* __restrict__ partials,
                                                          uint32_t max_chunks) {
-  __shared__ qm31 red[4];
-  const OodsTask task = tasks[blockIdx.y];
-  const uint32_t grp = task_group[blockIdx.y];
-  const uint32_t k = task.log_size;
+  __shared__ qm31 red[4][OODS_BATCH];
+  const uint32_t t0 = batch_start[blockIdx.y], cnt = batch_start[blockIdx.y + 1] - t0;
+  const uint32_t grp = task_group[t0];
+  const uint32_t k = tasks[t0].log_size;
   const uint32_t n = 1u << k;
   const uint32_t blk = blockIdx.x;
   if (((size_t)blk << OODS_BLOCK_LOG) >= n) return;  // uniform per block
@@ -371,40 +374,58 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
   const qm31* W = Wall + ((size_t)grp << OODS_TABLE_LOG);
   const uint32_t t = threadIdx.x;
   const uint32_t base = blk << OODS_BLOCK_LOG;
-  qacc acc = qacc_zero();
+  const GLOBAL_AS uint32_t* col[OODS_BATCH];
+#pragma unroll
+  for (uint32_t c = 0; c < OODS_BATCH; ++c) col[c] = gmem(tasks[t0 + (c < cnt ? c : 0)].coeffs);
+  qacc acc[OODS_BATCH];
+#pragma unroll
+  for (uint32_t c = 0; c < OODS_BATCH; ++c) acc[c] = qacc_zero();
   if (n >= 4) {
-#pragma unroll 2
-    for (uint32_t r = 0; r < 8; ++r) {
+    for (uint32_t r = 0; r < (1u << OODS_BLOCK_LOG) / 1024; ++r) {
       const uint32_t i = base + 4 * (t + 256 * r);
       if (i < n) {
-        const u32x4 v = *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(task.coeffs) + i);
         const uint32_t wi = i & tmask;
-        qacc_mad(acc, W[wi], v.x);
-        qacc_mad(acc, W[wi + 1], v.y);
-        qacc_fold(acc);
-        qacc_mad(acc, W[wi + 2], v.z);
-        qacc_mad(acc, W[wi + 3], v.w);
-        qacc_fold(acc);
+        const qm31 w0 = W[wi], w1 = W[wi + 1], w2 = W[wi + 2], w3 = W[wi + 3];
+        // every column slot loads and accumulates (slots past cnt repeat column 0 and are never
+        // written): loads behind per-column branches would each wait on their own
+        u32x4 v[OODS_BATCH];
+#pragma unroll
+        for (uint32_t c = 0; c < OODS_BATCH; ++c) v[c] = *reinterpret_cast<const GLOBAL_AS u32x4*>(col[c] + i);
+#pragma unroll
+        for (uint32_t c = 0; c < OODS_BATCH; ++c) {
+          qacc_mad(acc[c], w0, v[c].x);
+          qacc_mad(acc[c], w1, v[c].y);
+          qacc_fold(acc[c]);
+          qacc_mad(acc[c], w2, v[c].z);
+          qacc_mad(acc[c], w3, v[c].w);
+          qacc_fold(acc[c]);
+        }
       }
     }
   } else if (t < n) {
-    qacc_mad(acc, W[t], gmem(task.coeffs)[t]);
+#pragma unroll
+    for (uint32_t c = 0; c < OODS_BATCH; ++c)
+      if (c < cnt) qacc_mad(acc[c], W[t], col[c][t]);
   }
-  qm31 out = qacc_reduce(acc);
-  // wave reduction by shuffles, then across the 4 waves
-  for (uint32_t off = 32; off > 0; off >>= 1) {
-    qm31 o = {(uint32_t)__shfl_down((int)out.a, off), (uint32_t)__shfl_down((int)out.b, off),
-              (uint32_t)__shfl_down((int)out.c, off), (uint32_t)__shfl_down((int)out.d, off)};
-    out = qm_add(out, o);
+  const qm31* f = factors + (size_t)grp * 32;
+#pragma unroll
+  for (uint32_t c = 0; c < OODS_BATCH; ++c) {
+    if (c >= cnt) break;
+    qm31 out = qacc_reduce(acc[c]);
+    // wave reduction by shuffles, then across the 4 waves
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+      qm31 o = {(uint32_t)__shfl_down((int)out.a, off), (uint32_t)__shfl_down((int)out.b, off),
+                (uint32_t)__shfl_down((int)out.c, off), (uint32_t)__shfl_down((int)out.d, off)};
+      out = qm_add(out, o);
+    }
+    if ((t & 63) == 0) red[t >> 6][c] = out;
   }
-  if ((t & 63) == 0) red[t >> 6] = out;
   __syncthreads();
-  if (t == 0) {
-    qm31 s = qm_add(qm_add(red[0], red[1]), qm_add(red[2], red[3]));
-    const qm31* f = factors + (size_t)grp * 32;
+  if (t < cnt) {
+    qm31 s = qm_add(qm_add(red[0][t], red[1][t]), qm_add(red[2][t], red[3][t]));
     for (uint32_t b = OODS_TABLE_LOG; b < k; ++b)
       if ((base >> b) & 1) s = qm_mul(s, f[b]);
-    partials[(size_t)blockIdx.y * max_chunks + blk] = s;
+    partials[(size_t)(t0 + t) * max_chunks + blk] = s;
   }
 }
 
@@ -419,16 +440,16 @@ __global__ void oods_reduce_kernel(const OodsTask* __restrict__ tasks, uint32_t 
   out[tasks[tsk].out_index] = s;
 }
 
-void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, uint32_t max_log,
-                 const qm31* group_factors, const uint32_t* group_log, uint32_t n_groups, qm31* weights,
-                 qm31* partials, qm31* out, hipStream_t stream) {
+void launch_oods(const OodsTask* tasks, const uint32_t* task_group, uint32_t n_tasks, const uint32_t* batch_start,
+                 uint32_t n_batches, uint32_t max_log, const qm31* group_factors, const uint32_t* group_log,
+                 uint32_t n_groups, qm31* weights, qm31* partials, qm31* out, hipStream_t stream) {
   if (n_tasks == 0) return;
   const uint32_t max_chunks = oods_max_chunks(max_log);
   const uint32_t tab = max_log < OODS_TABLE_LOG ? max_log : OODS_TABLE_LOG;
   hipLaunchKernelGGL(oods_weights_kernel, dim3(((1u << tab) + 255) / 256, n_groups), dim3(256), 0, stream,
                      group_factors, group_log, weights);
-  hipLaunchKernelGGL(oods_chunk_kernel, dim3(max_chunks, n_tasks), dim3(256), 0, stream, tasks, task_group,
-                     group_factors, weights, partials, max_chunks);
+  hipLaunchKernelGGL(oods_chunk_kernel, dim3(max_chunks, n_batches), dim3(256), 0, stream, tasks, task_group,
+                     batch_start, group_factors, weights, partials, max_chunks);
   hipLaunchKernelGGL(oods_reduce_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, stream, tasks, n_tasks, partials,
                      max_chunks, out);
 }

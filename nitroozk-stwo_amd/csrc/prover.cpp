@@ -840,6 +840,18 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         max_coeff_log = std::max(max_coeff_log, k);
       }
   const size_t n_vals = tasks.size();
+  {
+    // tasks of one group side by side, in batches that share each weight read (out_index keeps
+    // every value's place)
+    std::vector<uint32_t> perm(n_vals);
+    for (uint32_t i = 0; i < n_vals; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return task_group[a] < task_group[b]; });
+    std::vector<OodsTask> st(n_vals);
+    std::vector<uint32_t> sg(n_vals);
+    for (uint32_t i = 0; i < n_vals; ++i) { st[i] = tasks[perm[i]]; sg[i] = task_group[perm[i]]; }
+    tasks.swap(st);
+    task_group.swap(sg);
+  }
   HMARK("oods_prep");
   S.val.resize(n_vals);
   if (n_vals) {
@@ -851,11 +863,13 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     const OodsTask* dt = ctx.table_vec(tasks);
     const uint32_t* dtg = ctx.table_vec(task_group);
     const uint32_t* dgl = ctx.table_vec(group_log);
+    const std::vector<uint32_t> bstart = oods_batch_starts(task_group);
+    const uint32_t* dbs = ctx.table_vec(bstart);
     const qm31* df = ctx.upload_vec(group_factors);
     {
       KScope ks(ctx, K_OODS, obytes);
-      launch_oods(dt, dtg, (uint32_t)n_vals, max_coeff_log, df, dgl, (uint32_t)group_key.size(), (qm31*)weights.p,
-                  (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
+      launch_oods(dt, dtg, (uint32_t)n_vals, dbs, (uint32_t)bstart.size() - 1, max_coeff_log, df, dgl,
+                  (uint32_t)group_key.size(), (qm31*)weights.p, (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
     }
     ctx.d2h(S.val.data(), dvals.p, n_vals * 16);
     ctx.reset_arena();
