@@ -417,34 +417,67 @@ __device__ __forceinline__ uint32_t coset_storage(uint32_t c, uint32_t log_n) {
 }
 constexpr uint32_t SCAN_BLOCK = 1024;
 
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_local_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols,
-                                                               qm31 shift, uint32_t* __restrict__ block_sums) {
-  __shared__ uint32_t sm[SCAN_BLOCK];
-  uint32_t c = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-  uint32_t k = blockIdx.y;
-  uint32_t sh = k == 0 ? shift.a : k == 1 ? shift.b : k == 2 ? shift.c : shift.d;
-  uint32_t idx = coset_storage(c, log_n);
-  uint32_t v = m31_sub(cols[k][idx], sh);
-  sm[threadIdx.x] = v;
-  __syncthreads();
-  for (uint32_t off = 1; off < SCAN_BLOCK; off <<= 1) {
-    uint32_t add = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
-    __syncthreads();
-    sm[threadIdx.x] = m31_add(sm[threadIdx.x], add);
-    __syncthreads();
-  }
-  cols[k][idx] = sm[threadIdx.x];
-  if (threadIdx.x == SCAN_BLOCK - 1) block_sums[k * gridDim.x + blockIdx.x] = sm[threadIdx.x];
+// Coordinate k of the shift: given, or (sums non-null) the LogUp column sums scaled by 1/n, as
+// Component::cumsum_shift computes it, so that the scan needs no host round trip.
+__device__ __forceinline__ uint32_t scan_shift(qm31 shift, const unsigned long long* __restrict__ sums,
+                                               uint32_t inv_n, uint32_t k) {
+  if (sums) return m31_mul(m31_reduce64(sums[k]), inv_n);
+  return k == 0 ? shift.a : k == 1 ? shift.b : k == 2 ? shift.c : shift.d;
 }
 
-__global__ void scan_blocks_kernel(uint32_t n_blocks, uint32_t* __restrict__ block_sums) {
-  uint32_t k = blockIdx.x;
-  if (threadIdx.x != 0) return;
-  uint32_t run = 0;
-  for (uint32_t b = 0; b < n_blocks; ++b) {
-    uint32_t v = block_sums[k * n_blocks + b];
-    block_sums[k * n_blocks + b] = run;  // exclusive
-    run = m31_add(run, v);
+// Inclusive M31 prefix sum over a workgroup of SCAN_BLOCK lanes: within each wave by shuffles,
+// then over the wave totals.
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* wave_tot) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t u = (uint32_t)__shfl_up((int)v, off);
+    if (lane >= off) v = m31_add(v, u);
+  }
+  if (lane == 63) wave_tot[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t x = lane < SCAN_BLOCK / 64 ? wave_tot[lane] : 0;
+#pragma unroll
+    for (uint32_t off = 1; off < SCAN_BLOCK / 64; off <<= 1) {
+      const uint32_t u = (uint32_t)__shfl_up((int)x, off);
+      if (lane >= off) x = m31_add(x, u);
+    }
+    if (lane < SCAN_BLOCK / 64) wave_tot[lane] = x;
+  }
+  __syncthreads();
+  return w ? m31_add(v, wave_tot[w - 1]) : v;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_local_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols,
+                                                               qm31 shift, const unsigned long long* __restrict__ sums,
+                                                               uint32_t inv_n, uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t wave_tot[SCAN_BLOCK / 64];
+  const uint32_t c = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  const uint32_t idx = coset_storage(c, log_n);
+  GLOBAL_AS uint32_t* col = gmem(cols[k]);
+  const uint32_t v = block_scan(m31_sub(col[idx], scan_shift(shift, sums, inv_n, k)), wave_tot);
+  col[idx] = v;
+  if (threadIdx.x == SCAN_BLOCK - 1) block_sums[k * gridDim.x + blockIdx.x] = v;
+}
+
+// Exclusive scan of the per-block totals, one workgroup per coordinate (n_blocks <= SCAN_BLOCK per
+// pass; longer lists carry the running total from pass to pass).
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_blocks_kernel(uint32_t n_blocks, uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t wave_tot[SCAN_BLOCK / 64];
+  __shared__ uint32_t carry_s;
+  const uint32_t k = blockIdx.x;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < n_blocks; b0 += SCAN_BLOCK) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint32_t v = b < n_blocks ? block_sums[k * n_blocks + b] : 0;
+    const uint32_t incl = block_scan(v, wave_tot);
+    if (b < n_blocks) block_sums[k * n_blocks + b] = m31_add(carry, m31_sub(incl, v));  // exclusive
+    if (threadIdx.x == SCAN_BLOCK - 1) carry_s = m31_add(carry, incl);
+    __syncthreads();
+    carry = carry_s;
+    __syncthreads();
   }
 }
 
@@ -453,15 +486,17 @@ __global__ void scan_add_kernel(uint32_t log_n, uint32_t* const* __restrict__ co
   uint32_t c = blockIdx.x * SCAN_BLOCK + threadIdx.x;
   uint32_t k = blockIdx.y;
   uint32_t idx = coset_storage(c, log_n);
-  cols[k][idx] = m31_add(cols[k][idx], block_sums[k * gridDim.x + blockIdx.x]);
+  GLOBAL_AS uint32_t* col = gmem(cols[k]);
+  col[idx] = m31_add(col[idx], block_sums[k * gridDim.x + blockIdx.x]);
 }
 
 // Small columns (n < SCAN_BLOCK) are scanned by one workgroup with n lanes.
-__global__ void scan_small_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols, qm31 shift) {
+__global__ void scan_small_kernel(uint32_t log_n, uint32_t* const* __restrict__ cols, qm31 shift,
+                                  const unsigned long long* __restrict__ sums, uint32_t inv_n) {
   __shared__ uint32_t sm[SCAN_BLOCK];
   uint32_t n = 1u << log_n;
   uint32_t k = blockIdx.y;
-  uint32_t sh = k == 0 ? shift.a : k == 1 ? shift.b : k == 2 ? shift.c : shift.d;
+  uint32_t sh = scan_shift(shift, sums, inv_n, k);
   uint32_t c = threadIdx.x;
   uint32_t idx = c < n ? coset_storage(c, log_n) : 0;
   sm[c] = c < n ? m31_sub(cols[k][idx], sh) : 0;
@@ -475,16 +510,18 @@ __global__ void scan_small_kernel(uint32_t log_n, uint32_t* const* __restrict__ 
   if (c < n) cols[k][idx] = sm[c];
 }
 
-void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, uint32_t* scratch,
-                             hipStream_t stream) {
-  uint32_t n = 1u << log_n;
+void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, const unsigned long long* shift_sums,
+                             uint32_t* scratch, hipStream_t stream) {
+  const uint32_t n = 1u << log_n, inv_n = m31_inv(n);
   if (n < SCAN_BLOCK) {
-    hipLaunchKernelGGL(scan_small_kernel, dim3(1, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, shift);
+    hipLaunchKernelGGL(scan_small_kernel, dim3(1, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, shift, shift_sums,
+                       inv_n);
     return;
   }
   uint32_t nb = n / SCAN_BLOCK;
-  hipLaunchKernelGGL(scan_local_kernel, dim3(nb, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, shift, scratch);
-  hipLaunchKernelGGL(scan_blocks_kernel, dim3(4), dim3(64), 0, stream, nb, scratch);
+  hipLaunchKernelGGL(scan_local_kernel, dim3(nb, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, shift, shift_sums,
+                     inv_n, scratch);
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(4), dim3(SCAN_BLOCK), 0, stream, nb, scratch);
   hipLaunchKernelGGL(scan_add_kernel, dim3(nb, 4), dim3(SCAN_BLOCK), 0, stream, log_n, cols4, scratch);
 }
 

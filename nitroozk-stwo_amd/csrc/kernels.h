@@ -151,8 +151,10 @@ void launch_gen_poseidon(uint32_t log_n, uint32_t* const* cols, uint32_t ipr, ui
 void launch_poseidon_logup_fracs(uint32_t log_n, uint32_t ipr, const uint32_t* const* trace_cols,
                                  uint32_t* const* inter_cols, const PoseidonParams* params, unsigned long long* sums,
                                  hipStream_t stream);
-void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, uint32_t* scratch,
-                             hipStream_t stream);
+// shift_sums (4 u64 sums of M31 values, may be null) replaces shift by sums / 2^log_n, read on the
+// device.
+void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift, const unsigned long long* shift_sums,
+                             uint32_t* scratch, hipStream_t stream);
 
 void launch_fill_zero(uint32_t* p, size_t n, hipStream_t stream);
 void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t stream);

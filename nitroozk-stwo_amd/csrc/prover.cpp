@@ -1433,7 +1433,9 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     pp.ipr = ipr;
     std::vector<uint32_t*> ip;
     for (auto& b : icols) ip.push_back(b.p);
-    DevBuf sums(&ctx, 8, true);
+    // read back once, after the interaction commit: the 4 column sums (words 0..8, u64) and the
+    // interaction root (words 8..16); the prefix sum takes its shift from the sums on the device
+    DevBuf sums(&ctx, 16, true);
     const uint32_t* const* d_tr = (const uint32_t* const*)ctx.table_vec(trace);
     uint32_t* const* d_ip = (uint32_t* const*)ctx.table_vec(ip);
     const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
@@ -1441,17 +1443,14 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
       KScope ks(ctx, K_LOGUP, (double)(1u << log_n) * (128.0 * ipr + 16.0 * ipr));
       launch_poseidon_logup_fracs(log_n, ipr, d_tr, d_ip, d_pp, (unsigned long long*)sums.p, ctx.stream);
     }
-    unsigned long long s[4];
-    ctx.d2h(s, sums.p, 32);
     HMARK("logup");
-    c.claimed_sum = {m31_reduce64(s[0] % P), m31_reduce64(s[1] % P), m31_reduce64(s[2] % P), m31_reduce64(s[3] % P)};
-    qm31 shift = c.cumsum_shift();
     std::vector<uint32_t*> last4(ip.end() - 4, ip.end());
     uint32_t nb = (1u << log_n) >= 1024 ? (1u << log_n) / 1024 : 1;
     DevBuf scratch(&ctx, 4 * nb);
     {
       KScope ks2(ctx, K_LOGUP, 32.0 * (double)(1u << log_n));
-      launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.table_vec(last4), shift, scratch.p, ctx.stream);
+      launch_coset_prefix_sum(log_n, (uint32_t* const*)ctx.table_vec(last4), qm_zero(),
+                              (const unsigned long long*)sums.p, scratch.p, ctx.stream);
     }
     std::vector<const uint32_t*> ev;
     for (auto& b : icols) ev.push_back(b.p);
@@ -1462,7 +1461,15 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     const uint32_t E = log_n + 2;
     evaluate_into(ctx, trees[2].coeffs, trees[2].coeff_logs, E, trees[2].ext, trees[2].owned, true);
     trees[2].ext_log = E;
-    merkle_commit(ctx, trees[2]);
+    merkle_commit(ctx, trees[2], false);
+    launch_copy_words(sums.p + 8, trees[2].layers[0].p, 8, ctx.stream);
+    uint32_t w[16];
+    ctx.d2h(w, sums.p, sizeof(w));
+    unsigned long long s[4];
+    std::memcpy(s, w, sizeof(s));
+    c.claimed_sum = {m31_reduce64(s[0] % P), m31_reduce64(s[1] % P), m31_reduce64(s[2] % P), m31_reduce64(s[3] % P)};
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 4; ++j) trees[2].root[4 * i + j] = (uint8_t)(w[8 + i] >> (8 * j));
     ch.mix_root(trees[2].root);
     ctx.reset_arena();
   }
