@@ -854,10 +854,14 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   }
   HMARK("oods_prep");
   S.val.resize(n_vals);
+  DevBuf oods_partials, oods_vals, oods_weights;
+  const uint32_t* dvals_p = nullptr;
   if (n_vals) {
     uint32_t max_chunks = oods_max_chunks(max_coeff_log);
-    DevBuf partials(&ctx, (size_t)n_vals * max_chunks * 4), dvals(&ctx, n_vals * 4);
-    DevBuf weights(&ctx, group_key.size() * ((size_t)4 << OODS_TABLE_LOG));
+    oods_partials = DevBuf(&ctx, (size_t)n_vals * max_chunks * 4);
+    oods_vals = DevBuf(&ctx, n_vals * 4);
+    oods_weights = DevBuf(&ctx, group_key.size() * ((size_t)4 << OODS_TABLE_LOG));
+    dvals_p = oods_vals.p;
     double obytes = 0;
     for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
     const OodsTask* dt = ctx.table_vec(tasks);
@@ -866,36 +870,25 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     const std::vector<uint32_t> bstart = oods_batch_starts(task_group);
     const uint32_t* dbs = ctx.table_vec(bstart);
     const qm31* df = ctx.upload_vec(group_factors);
-    {
-      KScope ks(ctx, K_OODS, obytes);
-      launch_oods(dt, dtg, (uint32_t)n_vals, dbs, (uint32_t)bstart.size() - 1, max_coeff_log, df, dgl,
-                  (uint32_t)group_key.size(), (qm31*)weights.p, (qm31*)partials.p, (qm31*)dvals.p, ctx.stream);
-    }
-    ctx.d2h(S.val.data(), dvals.p, n_vals * 16);
-    ctx.reset_arena();
+    KScope ks(ctx, K_OODS, obytes);
+    launch_oods(dt, dtg, (uint32_t)n_vals, dbs, (uint32_t)bstart.size() - 1, max_coeff_log, df, dgl,
+                (uint32_t)group_key.size(), (qm31*)oods_weights.p, (qm31*)oods_partials.p, (qm31*)oods_vals.p,
+                ctx.stream);
   }
-  HMARK("oods_gpu");
-  ch.mix_felts(S.val);
-  HMARK("oods_post");
-  stage_ms[3] = tm.lap();
-
-  // ---------------- FRI quotients ----------------
+  // ---- quotient plan: everything that does not depend on the sampled values, built on the host
+  // while the OODS kernels run ----
   // Quotient samples per column (core/pcs/quotients.rs:80-105,306-366): a column sampled at two
   // points first gets a periodicity sample at its second point shifted by the generator of its
   // evaluation domain, then its samples; consecutive powers of the random coefficient in order.
-  qm31 qrc = ch.draw_secure_felt();
   std::vector<cpqm31> qsample_pts = S.pts;
   std::vector<std::array<uint32_t, 3>> periodic;  // (point id, eval log, new point id)
-  std::vector<uint32_t> qstart(NC + 1, 0), qpid;
-  std::vector<qm31> qval, qrand;
+  std::vector<uint32_t> qstart(NC + 1, 0), qpid, qvi;  // per sample: point, index of its value in S.val
   {
-    qm31 pw = qm_one();
     const cpm31 gen = index_to_point(subgroup_gen(Lc));
     for (size_t t = 0; t < trees.size(); ++t)
       for (size_t c = 0; c < trees[t].evals.size(); ++c) {
-        const uint32_t g = S.col_base[t] + (uint32_t)c, n = S.count(t, c);
-        const qm31* v = S.vals(t, c);
-        const uint32_t* pids = S.pid.data() + S.first(t, c);
+        const uint32_t g = S.col_base[t] + (uint32_t)c, n = S.count(t, c), f0 = S.first(t, c);
+        const uint32_t* pids = S.pid.data() + f0;
         qstart[g] = (uint32_t)qpid.size();
         if (n == 2) {
           const uint32_t el = trees[t].eval_logs[c];
@@ -910,96 +903,136 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
             periodic.push_back({pids[1], el, np});
           }
           qpid.push_back(np);
-          qval.push_back(v[1]);
-          qrand.push_back(pw);
-          pw = qm_mul(pw, qrc);
+          qvi.push_back(f0 + 1);
         }
         for (uint32_t j = 0; j < n; ++j) {
           qpid.push_back(pids[j]);
-          qval.push_back(v[j]);
-          qrand.push_back(pw);
-          pw = qm_mul(pw, qrc);
+          qvi.push_back(f0 + j);
         }
       }
     qstart[NC] = (uint32_t)qpid.size();
   }
-  HMARK("q_samples");
-  struct Item { size_t t, c; uint32_t ls; };
-  std::vector<Item> items;
-  items.reserve(NC);
-  for (size_t t = 0; t < trees.size(); ++t)
-    for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
-  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
-  HMARK("x_qitems");
+  // ColumnSampleBatch per (size group, point); line coefficients (core/constraints.rs
+  // complex_conjugate_line_coeffs)
+  //   a = r (conj(v) - v), c = r (conj(p.y) - p.y), b = r (v (conj(p.y) - p.y) - (conj(v) - v) p.y)
+  // so that over a batch A = sum r (conj(v) - v) and B = cy sum r v - p.y A, with cy = conj(p.y) - p.y.
+  struct QBatch { uint32_t pid; qm31 cy; uint32_t first, count; size_t acc; };
+  struct QGroup {
+    uint32_t ls;
+    size_t n_cols;
+    bool aligned;
+    std::vector<QBatch> batches;
+    std::vector<uint32_t> ent_q;               // per entry (batch-major): the sample index
+    std::vector<const uint32_t*> ent_col;      // per entry: the column
+    const uint32_t* d_starts;
+    uint32_t* const* d_outs;
+  };
   struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; qm31 first_linear; };
   std::vector<Acc> acc_list;
   std::vector<DevBuf> partial_bufs;
-  std::vector<int> batch_of(qsample_pts.size(), -1);
-  for (size_t i = 0; i < items.size();) {
-    size_t j = i;
-    while (j < items.size() && items[j].ls == items[i].ls) ++j;
-    uint32_t ls = items[i].ls;
-    // ColumnSampleBatch per point; line coefficients (core/constraints.rs complex_conjugate_line_coeffs)
-    //   a = r (conj(v) - v), c = r (conj(p.y) - p.y), b = r (v (conj(p.y) - p.y) - (conj(v) - v) p.y)
-    // so that over a batch A = sum r (conj(v) - v) and B = cy sum r v - p.y A, with cy = conj(p.y) - p.y.
-    struct Batch { uint32_t pid; qm31 cy, sum_rv, sum_ra; std::vector<NumEntry> entries; };
-    std::vector<Batch> batches;
-    std::vector<const uint32_t*> gcols;
-    gcols.reserve(j - i);
-    for (size_t k = i; k < j; ++k) {
-      const uint32_t gi = (uint32_t)gcols.size();
-      gcols.push_back(trees[items[k].t].evals[items[k].c]);
-      const uint32_t g = S.col_base[items[k].t] + (uint32_t)items[k].c;
-      for (uint32_t q = qstart[g]; q < qstart[g + 1]; ++q) {
-        const uint32_t pid = qpid[q];
-        if (batch_of[pid] < 0) {
-          batch_of[pid] = (int)batches.size();
-          const qm31 py = qsample_pts[pid].y;
-          batches.push_back({pid, qm_sub(qm_conj(py), py), qm_zero(), qm_zero(), {}});
+  std::vector<QGroup> qgroups;
+  {
+    struct Item { size_t t, c; uint32_t ls; };
+    std::vector<Item> items;
+    items.reserve(NC);
+    for (size_t t = 0; t < trees.size(); ++t)
+      for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
+    std::vector<int> batch_of(qsample_pts.size(), -1);
+    for (size_t i = 0; i < items.size();) {
+      size_t j = i;
+      while (j < items.size() && items[j].ls == items[i].ls) ++j;
+      QGroup G;
+      G.ls = items[i].ls;
+      G.n_cols = j - i;
+      G.aligned = true;
+      // entries per batch in column order: count first, then place
+      std::vector<std::vector<std::pair<uint32_t, const uint32_t*>>> per;
+      for (size_t k = i; k < j; ++k) {
+        const uint32_t* col = trees[items[k].t].evals[items[k].c];
+        G.aligned &= ((uintptr_t)col & 15) == 0;
+        const uint32_t g = S.col_base[items[k].t] + (uint32_t)items[k].c;
+        for (uint32_t q = qstart[g]; q < qstart[g + 1]; ++q) {
+          const uint32_t pid = qpid[q];
+          if (batch_of[pid] < 0) {
+            batch_of[pid] = (int)G.batches.size();
+            const qm31 py = qsample_pts[pid].y;
+            G.batches.push_back({pid, qm_sub(qm_conj(py), py), 0, 0, 0});
+            per.emplace_back();
+          }
+          per[batch_of[pid]].push_back({q, col});
         }
-        Batch& b = batches[batch_of[pid]];
-        const qm31 v = qval[q], r = qrand[q];
-        NumEntry ne{};
-        ne.col = gcols[gi];
-        ne.c = qm_mul(r, b.cy);
-        b.entries.push_back(ne);
-        b.sum_rv = qm_add(b.sum_rv, qm_mul(r, v));
-        b.sum_ra = qm_add(b.sum_ra, qm_mul(r, qm_sub(qm_conj(v), v)));
       }
-    }
-    if (!batches.empty()) {
-      std::vector<NumEntry> ents;
       std::vector<uint32_t> starts;
-      std::vector<qm31> Bs;
       std::vector<uint32_t*> outs;
-      for (auto& b : batches) {
+      for (size_t bi = 0; bi < G.batches.size(); ++bi) {
+        QBatch& b = G.batches[bi];
         batch_of[b.pid] = -1;
-        starts.push_back((uint32_t)ents.size());
-        ents.insert(ents.end(), b.entries.begin(), b.entries.end());
-        Bs.push_back(qm_sub(qm_mul(b.cy, b.sum_rv), qm_mul(qsample_pts[b.pid].y, b.sum_ra)));
+        b.first = (uint32_t)G.ent_q.size();
+        b.count = (uint32_t)per[bi].size();
+        starts.push_back(b.first);
+        for (auto& [q, col] : per[bi]) { G.ent_q.push_back(q); G.ent_col.push_back(col); }
         Acc a;
         a.point = qsample_pts[b.pid];
-        a.log = ls;
-        a.first_linear = b.sum_ra;
+        a.log = G.ls;
+        a.first_linear = qm_zero();
         for (int k = 0; k < 4; ++k) {
-          partial_bufs.emplace_back(&ctx, (size_t)1 << ls);
+          partial_bufs.emplace_back(&ctx, (size_t)1 << G.ls);
           outs.push_back(partial_bufs.back().p);
           a.c[k] = partial_bufs.back().p;
         }
+        b.acc = acc_list.size();
         acc_list.push_back(a);
       }
-      starts.push_back((uint32_t)ents.size());
-      bool aligned = true;
-      for (auto* p : gcols) aligned &= ((uintptr_t)p & 15) == 0;
-      const NumEntry* d_ents = ctx.stage_vec(ents);
-      const uint32_t* d_starts = ctx.table_vec(starts);
-      uint32_t* const* d_outs = (uint32_t* const*)ctx.table_vec(outs);
-      const qm31* d_bs = ctx.upload_vec(Bs);
-      KScope ks(ctx, K_NUMERATORS, (4.0 * gcols.size() + 16.0 * batches.size()) * (double)(1u << ls));
-      launch_accumulate_numerators(ls, d_ents, d_starts, d_bs, (uint32_t)batches.size(), d_outs, aligned, ctx.stream);
+      starts.push_back((uint32_t)G.ent_q.size());
+      G.d_starts = ctx.table_vec(starts);
+      G.d_outs = (uint32_t* const*)ctx.table_vec(outs);
+      qgroups.push_back(std::move(G));
+      i = j;
     }
+  }
+  HMARK("x_qplan");
+  if (n_vals) {
+    ctx.d2h(S.val.data(), dvals_p, n_vals * 16);
+    ctx.reset_arena();
+  }
+  HMARK("oods_gpu");
+  ch.mix_felts(S.val);
+  HMARK("oods_post");
+  stage_ms[3] = tm.lap();
+
+  // ---------------- FRI quotients ----------------
+  const qm31 qrc = ch.draw_secure_felt();
+  std::vector<qm31> qrand(qpid.size());
+  {
+    qm31 pw = qm_one();
+    for (auto& r : qrand) { r = pw; pw = qm_mul(pw, qrc); }
+  }
+  HMARK("q_samples");
+  for (auto& G : qgroups) {
+    if (G.batches.empty()) continue;
+    std::vector<NumEntry> ents(G.ent_q.size());
+    std::vector<qm31> Bs;
+    for (auto& b : G.batches) {
+      qm31 sum_rv = qm_zero(), sum_ra = qm_zero();
+      for (uint32_t e = b.first; e < b.first + b.count; ++e) {
+        const uint32_t q = G.ent_q[e];
+        const qm31 v = S.val[qvi[q]], r = qrand[q];
+        NumEntry& ne = ents[e];
+        ne.col = G.ent_col[e];
+        ne.c = qm_mul(r, b.cy);
+        sum_rv = qm_add(sum_rv, qm_mul(r, v));
+        sum_ra = qm_add(sum_ra, qm_mul(r, qm_sub(qm_conj(v), v)));
+      }
+      Bs.push_back(qm_sub(qm_mul(b.cy, sum_rv), qm_mul(qsample_pts[b.pid].y, sum_ra)));
+      acc_list[b.acc].first_linear = sum_ra;
+    }
+    const NumEntry* d_ents = ctx.stage_vec(ents);
+    const qm31* d_bs = ctx.upload_vec(Bs);
+    KScope ks(ctx, K_NUMERATORS, (4.0 * G.n_cols + 16.0 * G.batches.size()) * (double)(1u << G.ls));
+    launch_accumulate_numerators(G.ls, d_ents, G.d_starts, d_bs, (uint32_t)G.batches.size(), G.d_outs, G.aligned,
+                                 ctx.stream);
     HMARK("x_qgroup");
-    i = j;
   }
   HMARK("q_numer");
   std::stable_sort(acc_list.begin(), acc_list.end(), [](const Acc& a, const Acc& b) { return point_less(a.point, b.point); });
