@@ -32,7 +32,9 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
                            hipStream_t stream);
 // Scans nonce candidates [start, start + count) in search order (hi * 2^20 + lo, lo < 2^20) and
 // atomically keeps the smallest solution in *result.
-void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
+// prefix: the 8 words of the channel's proof-of-work prefix digest.
+struct GrindPrefix { uint32_t w[8]; };
+void launch_grind(const GrindPrefix& prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream);
 
 // ops.hip
@@ -123,6 +125,32 @@ void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, ui
 
 void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
                          uint32_t* out, hipStream_t stream);
+
+// Decommitment reads for a set of query positions, in one launch. A Merkle tree (or FRI layer)
+// decommitment only touches, at each level, the block of 2^w nodes that holds the query's path
+// node: k = pos >> (top - level + w) gives block [k 2^w, (k+1) 2^w). An item copies, for every
+// query, either the values of a chunk of the columns of one level at the nodes of that block
+// (DEC_COLS, node-major rows of row_width words) or the child hashes under the block (DEC_HASH:
+// 2^(w+1) nodes of the level below, 8 words each). pos is the query's position in the item's top
+// level: mode 0 ((p >> 1) << a) + (p & 1), mode 1 ((p >> a) << 1) + (p & 1), mode 2 p >> a.
+// Nodes past the end of their level read as zero.
+enum DecKind : uint32_t { DEC_COLS = 0, DEC_HASH = 1 };
+struct DecItem {
+  const uint32_t* const* cols;  // DEC_COLS: this chunk's column addresses
+  const uint32_t* layer;        // DEC_HASH: the child level's nodes
+  uint32_t kind;
+  uint32_t width;       // words per row read: chunk columns, or 8
+  uint32_t row_width;   // words per row in the output block
+  uint32_t rows_log;    // rows per query: 2^w (DEC_COLS) or 2^(w+1) (DEC_HASH)
+  uint32_t level_log;   // nodes in the level read: 2^level_log
+  uint32_t shift;       // k = pos >> shift; first row node = k << rows_log
+  uint32_t pos_mode, pos_arg;
+  uint32_t out, stride;  // query i's block at out + i * stride
+};
+constexpr uint32_t DEC_MAX_QUERIES = 64;  // positions per launch (kernel argument)
+// Splits into launches of at most DEC_MAX_QUERIES positions; positions[i] is query i.
+void launch_decommit_gather(const DecItem* items, uint32_t n_items, const uint32_t* positions, uint32_t n_queries,
+                            uint32_t* out, hipStream_t stream);
 
 // Composition (constraint quotients on the evaluation domain), added into acc (accumulate) or
 // written to it.

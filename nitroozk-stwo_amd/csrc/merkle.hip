@@ -207,9 +207,8 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
 
 // Candidates are numbered g = hi * 2^20 + lo (lo < 2^20), i.e. in the order of the smallest-nonce
 // search; one launch covers [start, start + count).
-__global__ void __launch_bounds__(256) grind_kernel(const uint32_t* __restrict__ prefix, uint32_t pow_bits,
-                                                    uint64_t start, uint32_t count,
-                                                    unsigned long long* __restrict__ result) {
+__global__ void __launch_bounds__(256) grind_kernel(GrindPrefix prefix, uint32_t pow_bits, uint64_t start,
+                                                    uint32_t count, unsigned long long* __restrict__ result) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   uint64_t g = start + i;
@@ -217,7 +216,7 @@ __global__ void __launch_bounds__(256) grind_kernel(const uint32_t* __restrict__
   uint32_t hi = (uint32_t)(g >> 20);
   uint32_t m[16];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) m[k] = prefix[k];
+  for (int k = 0; k < 8; ++k) m[k] = prefix.w[k];
   m[8] = lo;
   m[9] = hi;
 #pragma unroll
@@ -229,7 +228,7 @@ __global__ void __launch_bounds__(256) grind_kernel(const uint32_t* __restrict__
   if (tz >= pow_bits) atomicMin(result, ((unsigned long long)hi << 32) | lo);
 }
 
-void launch_grind(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
+void launch_grind(const GrindPrefix& prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream) {
   hipLaunchKernelGGL(grind_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, prefix, pow_bits, start, count,
                      result);

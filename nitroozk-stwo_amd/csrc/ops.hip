@@ -467,6 +467,57 @@ void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx,
                      out);
 }
 
+struct DecQueries { uint32_t first, n, p[DEC_MAX_QUERIES]; };
+
+__device__ __forceinline__ uint32_t dec_pos(uint32_t mode, uint32_t a, uint32_t p) {
+  return mode == 0 ? ((p >> 1) << a) + (p & 1) : mode == 1 ? ((p >> a) << 1) + (p & 1) : p >> a;
+}
+
+// One workgroup per (item, query). Each thread reads up to 4 words per step with the loads issued
+// together (indices clamped into range, out-of-level nodes masked afterwards).
+__global__ void __launch_bounds__(256) decommit_gather_kernel(const DecItem* __restrict__ items, DecQueries q,
+                                                              uint32_t* __restrict__ out) {
+  const DecItem it = items[blockIdx.x];
+  const uint32_t i = blockIdx.y;
+  const uint32_t pos = dec_pos(it.pos_mode, it.pos_arg, q.p[i]);
+  const uint32_t node0 = (pos >> it.shift) << it.rows_log;
+  const uint32_t level_n = 1u << it.level_log;
+  const uint32_t total = it.width << it.rows_log;
+  uint32_t* o = out + it.out + (q.first + i) * it.stride;
+  for (uint32_t e0 = 0; e0 < total; e0 += 4 * 256) {
+    uint32_t v[4], dst[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t e = e0 + u * 256 + threadIdx.x;
+      const uint32_t ec = e < total ? e : total - 1;
+      const uint32_t r = ec / it.width, c = ec - r * it.width;
+      const uint32_t node = node0 + r;
+      ok[u] = e < total;
+      dst[u] = r * it.row_width + c;
+      const uint32_t nd = node < level_n ? node : 0;
+      const uint32_t* src = it.kind == DEC_COLS ? it.cols[c] + nd : it.layer + (size_t)nd * 8 + c;
+      v[u] = *src;
+      if (node >= level_n) v[u] = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (ok[u]) o[dst[u]] = v[u];
+  }
+}
+
+void launch_decommit_gather(const DecItem* items, uint32_t n_items, const uint32_t* positions, uint32_t n_queries,
+                            uint32_t* out, hipStream_t stream) {
+  if (!n_items) return;
+  for (uint32_t f = 0; f < n_queries; f += DEC_MAX_QUERIES) {
+    DecQueries q{};
+    q.first = f;
+    q.n = n_queries - f < DEC_MAX_QUERIES ? n_queries - f : DEC_MAX_QUERIES;
+    for (uint32_t i = 0; i < q.n; ++i) q.p[i] = positions[f + i];
+    hipLaunchKernelGGL(decommit_gather_kernel, dim3(n_items, q.n), dim3(256), 0, stream, items, q, out);
+  }
+}
+
 __global__ void fill_zero_kernel(uint32_t* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

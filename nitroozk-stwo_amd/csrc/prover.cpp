@@ -64,38 +64,6 @@ Hash read_hash(Ctx& ctx, const uint32_t* dev) {
   return h;
 }
 
-// Batched gather of device words: requests are collected first, then served by one kernel.
-struct Gather {
-  std::vector<const uint32_t*> bases;
-  std::vector<uint32_t> bidx, offs;
-  uint32_t base(const uint32_t* p) {  // a base may repeat: the table is only read by the gather
-    bases.push_back(p);
-    return (uint32_t)bases.size() - 1;
-  }
-  uint32_t add_at(uint32_t b, uint32_t off) {
-    bidx.push_back(b);
-    offs.push_back(off);
-    return (uint32_t)offs.size() - 1;
-  }
-  uint32_t add(const uint32_t* p, uint32_t off) { return add_at(base(p), off); }
-  uint32_t add_hash(const uint32_t* layer, uint32_t node) {
-    uint32_t b = base(layer);
-    uint32_t first = add_at(b, node * 8);
-    for (uint32_t w = 1; w < 8; ++w) add_at(b, node * 8 + w);
-    return first;
-  }
-  std::vector<uint32_t> run(Ctx& ctx) {
-    std::vector<uint32_t> out(offs.size());
-    if (offs.empty()) return out;
-    const uint32_t* const* b = (const uint32_t* const*)ctx.upload_table(bases.data(), bases.size() * sizeof(void*));
-    const uint32_t* bi = ctx.stage_vec(bidx);
-    const uint32_t* of = ctx.upload_vec(offs);
-    DevBuf o(&ctx, offs.size());
-    launch_gather_words(b, bi, of, (uint32_t)offs.size(), o.p, ctx.stream);
-    ctx.d2h(out.data(), o.p, out.size() * 4);
-    return out;
-  }
-};
 Hash words_to_hash(const std::vector<uint32_t>& w, uint32_t at) {
   Hash h;
   for (int i = 0; i < 8; ++i)
@@ -174,56 +142,146 @@ std::vector<size_t> merkle_order(const std::vector<uint32_t>& logs) {
   return o;
 }
 
-// MerkleProver::decommit (prover/vcs/prover.rs:84-217), in two phases so that every tree and FRI
-// layer of a proof is served by one gather: plan_merkle adds the needed words to the gather,
-// finish_merkle assembles the witness in the reference order.
-struct MerklePlan {
-  std::vector<uint32_t> hw_idx, q_idx, w_idx;
+// MerkleProver::decommit (prover/vcs/prover.rs:84-217) and FriProver::decommit
+// (prover/fri.rs:215-412) served by one device read per proof. At every level the reference walk
+// only touches nodes inside the block of 2^w nodes that holds a query's path node (w = 1 for
+// trees, the fold step for FRI layers), so launch_decommit_gather copies, per query, each level's
+// column values at that block and the child hashes under it; the walk below then picks the words
+// the reference emits, in its order, from that superset.
+struct DecObject {
+  const DevTree* t = nullptr;
+  bool empty = true;  // no query reaches the tree (a tree of one node)
+  uint32_t top = 0, w = 1, pos_mode = 2, pos_arg = 0;
+  std::vector<uint32_t> cols_out, cols_n, hash_out;  // per level: superset offsets (query 0), column counts
 };
-MerklePlan plan_merkle(const DevTree& t, const std::map<uint32_t, std::vector<size_t>>& qpl, Gather& g) {
-  // MerkleProver::decommit (vcs/prover.rs:84-217); query lists in qpl are sorted and unique
-  MerklePlan mp;
-  auto order = merkle_order(t.eval_logs);
+struct DecLayout {
+  std::vector<DecObject> objs;
+  std::vector<DecItem> items;
+  size_t words = 0;
+  uint32_t n_q = 0;  // query blocks reserved per item
+};
+// Adds tree t, its query positions given by (mode, arg) of the global query positions.
+void dec_add_object(Ctx& ctx, DecLayout& L, const DevTree& t, uint32_t w, uint32_t mode, uint32_t arg) {
+  DecObject ob;
+  ob.t = &t;
+  ob.top = t.max_log();
+  ob.w = w;
+  ob.pos_mode = mode;
+  ob.pos_arg = arg;
+  ob.empty = ob.top == 0;  // tree_local_positions: a tree of one node has no queries
+  if (ob.empty) { L.objs.push_back(ob); return; }
+  ob.cols_out.assign(ob.top + 1, UINT32_MAX);
+  ob.cols_n.assign(ob.top + 1, 0);
+  ob.hash_out.assign(ob.top + 1, UINT32_MAX);
+  const auto order = merkle_order(t.eval_logs);
   size_t ci = 0;
-  std::vector<size_t> last, merged, qn, wn;
-  std::vector<uint32_t> cb;
+  std::vector<const uint32_t*> cols;
+  for (int ls = (int)ob.top; ls >= 0; --ls) {
+    const uint32_t shift = ob.top - (uint32_t)ls + w;
+    cols.clear();
+    while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) cols.push_back(t.evals[order[ci++]]);
+    if (!cols.empty()) {
+      const uint32_t nc = (uint32_t)cols.size(), stride = nc << w, chunk = std::max(1u, 1024u >> w);
+      const uint32_t* const* tab = ctx.table_vec(cols);
+      ob.cols_out[ls] = (uint32_t)L.words;
+      ob.cols_n[ls] = nc;
+      for (uint32_t c0 = 0; c0 < nc; c0 += chunk) {
+        DecItem it{};
+        it.cols = tab + c0;
+        it.kind = DEC_COLS;
+        it.width = std::min(chunk, nc - c0);
+        it.row_width = nc;
+        it.rows_log = w;
+        it.level_log = (uint32_t)ls;
+        it.shift = shift;
+        it.pos_mode = mode;
+        it.pos_arg = arg;
+        it.out = (uint32_t)L.words + c0;
+        it.stride = stride;
+        L.items.push_back(it);
+      }
+      L.words += (size_t)stride * L.n_q;
+    }
+    if ((uint32_t)ls < ob.top) {
+      DecItem it{};
+      it.layer = t.layers[ls + 1].p;
+      it.kind = DEC_HASH;
+      it.width = 8;
+      it.row_width = 8;
+      it.rows_log = w + 1;
+      it.level_log = (uint32_t)ls + 1;
+      it.shift = shift;
+      it.pos_mode = mode;
+      it.pos_arg = arg;
+      it.out = (uint32_t)L.words;
+      it.stride = 8u << (w + 1);
+      ob.hash_out[ls] = it.out;
+      L.items.push_back(it);
+      L.words += (size_t)it.stride * L.n_q;
+    }
+  }
+  if (L.words >= ((size_t)1 << 32)) throw HipError(3, "decommitment read too large");
+  L.objs.push_back(std::move(ob));
+}
+uint32_t dec_pos_host(uint32_t mode, uint32_t a, uint32_t p) {
+  return mode == 0 ? ((p >> 1) << a) + (p & 1) : mode == 1 ? ((p >> a) << 1) + (p & 1) : p >> a;
+}
+// Superset offsets of what the reference decommitment emits, in its order: the first word of each
+// witness hash, and runs (offset, length) of queried and witness column values.
+struct DecPlan {
+  std::vector<uint32_t> hash_at;
+  std::vector<std::pair<uint32_t, uint32_t>> qry, wit;
+};
+// The walk of MerkleProver::decommit for object ob. P[i]: query i's position in the top level;
+// qpl: (level, sorted unique queried nodes) for the levels with queried columns.
+void dec_plan_object(const DecObject& ob, const std::vector<uint32_t>& P,
+                     const std::vector<std::pair<uint32_t, std::vector<size_t>>>& qpl, DecPlan& dp) {
+  if (ob.empty) return;
   static const std::vector<size_t> none;
-  for (int ls = (int)t.layers.size() - 1; ls >= 0; --ls) {
-    bool has_prev = (size_t)ls + 1 < t.layers.size();
-    auto it = qpl.find((uint32_t)ls);
-    const std::vector<size_t>& colq = it != qpl.end() ? it->second : none;
+  std::vector<size_t> last, merged;
+  const uint32_t w = ob.w, nodes_in_block = 1u << w;
+  for (int ls = (int)ob.top; ls >= 0; --ls) {
+    const std::vector<size_t>* colq = &none;
+    for (auto& e : qpl)
+      if (e.first == (uint32_t)ls) colq = &e.second;
     merged.clear();
     for (size_t q : last) merged.push_back(q / 2);
-    merged.insert(merged.end(), colq.begin(), colq.end());
+    merged.insert(merged.end(), colq->begin(), colq->end());
     std::sort(merged.begin(), merged.end());
     merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
-    cb.clear();
-    while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) cb.push_back(g.base(t.evals[order[ci++]]));
-    qn.clear();
-    wn.clear();
-    uint32_t pb = has_prev ? g.base(t.layers[ls + 1].p) : 0;
-    for (size_t node : merged) {
-      if (has_prev) {
+    const uint32_t shift = ob.top - (uint32_t)ls + w;
+    auto query_of = [&](size_t block) -> uint32_t {
+      for (uint32_t i = 0; i < P.size(); ++i)
+        if ((P[i] >> shift) == block) return i;
+      throw HipError(4, "decommitment node outside the queried blocks");
+    };
+    if ((uint32_t)ls < ob.top) {
+      const uint32_t hs = 8u << (w + 1);
+      for (size_t node : merged)
         for (size_t child = 2 * node; child < 2 * node + 2; ++child)
-          if (!std::binary_search(last.begin(), last.end(), child)) {
-            mp.hw_idx.push_back(g.add_at(pb, (uint32_t)(child * 8)));
-            for (uint32_t w = 1; w < 8; ++w) g.add_at(pb, (uint32_t)(child * 8 + w));
-          }
-      }
-      if (std::binary_search(colq.begin(), colq.end(), node)) qn.push_back(node); else wn.push_back(node);
+          if (!std::binary_search(last.begin(), last.end(), child))
+            dp.hash_at.push_back(ob.hash_out[ls] + query_of(child >> (w + 1)) * hs +
+                                 (uint32_t)(child & (2 * nodes_in_block - 1)) * 8);
     }
-    for (size_t node : qn) for (uint32_t b : cb) mp.q_idx.push_back(g.add_at(b, (uint32_t)node));
-    for (size_t node : wn) for (uint32_t b : cb) mp.w_idx.push_back(g.add_at(b, (uint32_t)node));
+    const uint32_t nc = ob.cols_n[ls];
+    if (nc) {
+      auto at = [&](size_t node) {
+        return ob.cols_out[ls] + query_of(node >> w) * (nc << w) + (uint32_t)(node & (nodes_in_block - 1)) * nc;
+      };
+      for (size_t node : merged)
+        if (std::binary_search(colq->begin(), colq->end(), node)) dp.qry.push_back({at(node), nc});
+      for (size_t node : merged)
+        if (!std::binary_search(colq->begin(), colq->end(), node)) dp.wit.push_back({at(node), nc});
+    }
     last.swap(merged);
   }
-  return mp;
 }
-std::vector<uint32_t> finish_merkle(const MerklePlan& mp, const std::vector<uint32_t>& vals, MerkleDec& dec) {
-  for (auto i : mp.hw_idx) dec.hash_witness.push_back(words_to_hash(vals, i));
-  for (auto i : mp.w_idx) dec.column_witness.push_back(vals[i]);
-  std::vector<uint32_t> queried;
-  for (auto i : mp.q_idx) queried.push_back(vals[i]);
-  return queried;
+void dec_finish(const DecPlan& dp, const uint32_t* sup, MerkleDec& dec, std::vector<uint32_t>* queried) {
+  dec.hash_witness.resize(dp.hash_at.size());
+  for (size_t k = 0; k < dp.hash_at.size(); ++k) std::memcpy(dec.hash_witness[k].data(), sup + dp.hash_at[k], 32);
+  for (auto& r : dp.wit) dec.column_witness.insert(dec.column_witness.end(), sup + r.first, sup + r.first + r.second);
+  if (queried)
+    for (auto& r : dp.qry) queried->insert(queried->end(), sup + r.first, sup + r.first + r.second);
 }
 
 qm31 qm_pow_u(qm31 b, uint64_t e) { qm31 r = qm_one(); while (e) { if (e & 1) r = qm_mul(r, b); b = qm_mul(b, b); e >>= 1; } return r; }
@@ -402,28 +460,31 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
 
 }  // namespace
 
-uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits) {
+uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits, unsigned long long* slot) {
   if (pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
-  uint32_t pw[8];
+  GrindPrefix pre;
   for (int i = 0; i < 8; ++i)
-    pw[i] = (uint32_t)prefix[4 * i] | ((uint32_t)prefix[4 * i + 1] << 8) | ((uint32_t)prefix[4 * i + 2] << 16) |
-            ((uint32_t)prefix[4 * i + 3] << 24);
-  const uint32_t* dpre = (const uint32_t*)ctx.upload(pw, 32);
-  DevBuf res(&ctx, 2);
+    pre.w[i] = (uint32_t)prefix[4 * i] | ((uint32_t)prefix[4 * i + 1] << 8) | ((uint32_t)prefix[4 * i + 2] << 16) |
+               ((uint32_t)prefix[4 * i + 3] << 24);
+  DevBuf own;
   unsigned long long found = ~0ull;
-  HIP_CHECK(hipMemcpyAsync(res.p, &found, 8, hipMemcpyHostToDevice, ctx.stream));
-  // the expected nonce is about 2^pow_bits: start there and double the range per launch
-  uint64_t count = (uint64_t)1 << (pow_bits + 3 < 12 ? 12 : (pow_bits + 3 > 24 ? 24 : pow_bits + 3));
+  if (!slot) {
+    own = DevBuf(&ctx, 2);
+    slot = (unsigned long long*)own.p;
+    HIP_CHECK(hipMemcpyAsync(slot, &found, 8, hipMemcpyHostToDevice, ctx.stream));
+  }
+  // the expected nonce is about 2^pow_bits: the first launch covers 32 times that (a miss has
+  // probability about e^-32), later ones double the range
+  uint64_t count = (uint64_t)1 << (pow_bits + 5 < 12 ? 12 : (pow_bits + 5 > 24 ? 24 : pow_bits + 5));
   const uint64_t limit = (uint64_t)0x7fffffff << 20;  // hi < P (simd/grind.rs:47-56)
   for (uint64_t start = 0; found == ~0ull; start += count, count = count < (1u << 24) ? count * 2 : count) {
     if (start >= limit) throw HipError(4, "Grind failed to find a solution.");
     {
       KScope ks(ctx, K_GRIND, 0.0);
-      launch_grind(dpre, pow_bits, start, (uint32_t)count, (unsigned long long*)res.p, ctx.stream);
+      launch_grind(pre, pow_bits, start, (uint32_t)count, slot, ctx.stream);
     }
-    ctx.d2h(&found, res.p, 8);
+    ctx.d2h(&found, slot, 8);
   }
-  ctx.reset_arena();
   return found;
 }
 
@@ -596,40 +657,47 @@ struct FriCol {
   CCols4 cc() const { return {{c[0].p, c[1].p, c[2].p, c[3].p}}; }
   Cols4 mc() { return {{c[0].p, c[1].p, c[2].p, c[3].p}}; }
 };
-struct FriPlan {
-  std::vector<uint32_t> at;
-  std::vector<bool> is_witness;
-  MerklePlan mp;
-  Hash commitment;
+// A FRI layer's decommit (prover/fri.rs:291-378) with
+// compute_decommitment_positions_and_witness_evals (prover/fri.rs:383-412): the values of every
+// query's fold coset that are not queried, then the Merkle decommitment of the coset positions. q: the layer's sorted
+// unique query positions; P[i]: query i's position in the layer.
+struct FriDecPlan {
+  std::vector<uint32_t> witness_at;  // first word of each witness value (4 coordinates)
+  DecPlan mp;
 };
-FriPlan plan_fri_layer(const FriCol& f, const std::vector<size_t>& q, uint32_t fold_step, Gather& g) {
-  FriPlan fp;
+FriDecPlan plan_fri_layer(const DecObject& ob, const std::vector<size_t>& q, const std::vector<uint32_t>& P) {
+  FriDecPlan fp;
+  const uint32_t w = ob.w;
   std::vector<size_t> pos;
   size_t i = 0;
   while (i < q.size()) {
     size_t j = i;
-    while (j < q.size() && (q[j] >> fold_step) == (q[i] >> fold_step)) ++j;
-    size_t start = (q[i] >> fold_step) << fold_step;
+    while (j < q.size() && (q[j] >> w) == (q[i] >> w)) ++j;
+    const size_t start = (q[i] >> w) << w;
     size_t k = i;
-    for (size_t p = start; p < start + ((size_t)1 << fold_step); ++p) {
+    for (size_t p = start; p < start + ((size_t)1 << w); ++p) {
       pos.push_back(p);
-      if (k < j && q[k] == p) { ++k; fp.is_witness.push_back(false); }
-      else fp.is_witness.push_back(true);
+      if (k < j && q[k] == p) {
+        ++k;
+        continue;
+      }
+      uint32_t qi = UINT32_MAX;
+      for (uint32_t t = 0; t < P.size() && qi == UINT32_MAX; ++t)
+        if ((P[t] >> w) == (p >> w)) qi = t;
+      if (qi == UINT32_MAX) throw HipError(4, "FRI witness outside the queried blocks");
+      fp.witness_at.push_back(ob.cols_out[ob.top] + qi * (4u << w) + (uint32_t)(p & ((1u << w) - 1)) * 4);
     }
     i = j;
   }
-  for (size_t p : pos) for (int k = 0; k < 4; ++k) fp.at.push_back(g.add(f.c[k].p, (uint32_t)p));
-  std::map<uint32_t, std::vector<size_t>> qpl{{f.log, pos}};
-  fp.mp = plan_merkle(f.tree, qpl, g);
-  fp.commitment = f.tree.root;
+  std::vector<std::pair<uint32_t, std::vector<size_t>>> qpl{{ob.top, pos}};
+  dec_plan_object(ob, P, qpl, fp.mp);
   return fp;
 }
-FriLayer finish_fri_layer(const FriPlan& fp, const std::vector<uint32_t>& v) {
+FriLayer finish_fri_layer(const FriDecPlan& fp, const uint32_t* sup, const Hash& commitment) {
   FriLayer L;
-  for (size_t n = 0; n < fp.is_witness.size(); ++n)
-    if (fp.is_witness[n]) L.witness.push_back({v[fp.at[4 * n]], v[fp.at[4 * n + 1]], v[fp.at[4 * n + 2]], v[fp.at[4 * n + 3]]});
-  finish_merkle(fp.mp, v, L.dec);
-  L.commitment = fp.commitment;
+  for (uint32_t at : fp.witness_at) L.witness.push_back({sup[at], sup[at + 1], sup[at + 2], sup[at + 3]});
+  dec_finish(fp.mp, sup, L.dec, nullptr);
+  L.commitment = commitment;
   return L;
 }
 
@@ -1102,6 +1170,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       st[i] = (uint32_t)ch.digest[4 * i] | (uint32_t)ch.digest[4 * i + 1] << 8 |
               (uint32_t)ch.digest[4 * i + 2] << 16 | (uint32_t)ch.digest[4 * i + 3] << 24;
     st[8] = ch.n_draws;
+    st[14] = st[15] = ~0u;  // the proof-of-work result slot (u64), empty
     launch_copy_words(dstate, ctx.upload_vec(st), st.size(), ctx.stream);
   }
   auto mix_and_draw = [&](FriCol& f, uint32_t layer) {
@@ -1240,6 +1309,26 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   }
   if (layer != n_layers) throw HipError(4, "FRI layer count mismatch");
   HMARK("fri_launch");
+  // Decommitment read layout: it depends only on the proof's shape, so the item table is a cached
+  // table from the second proof of a shape on.
+  DecLayout dl;
+  dl.n_q = cfg.n_queries;
+  for (auto& t : trees) {
+    const uint32_t mx = t.max_log();
+    if (Lc < mx) dec_add_object(ctx, dl, t, 1, 0, mx - Lc + 1);
+    else dec_add_object(ctx, dl, t, 1, 1, Lc - mx + 1);
+  }
+  dec_add_object(ctx, dl, first.tree, 1, 2, 0);
+  {
+    uint32_t a = 1;
+    for (auto& l : inner) {
+      dec_add_object(ctx, dl, l.tree, l.fold_step, 2, a);
+      a += l.fold_step;
+    }
+  }
+  const DecItem* ditems = ctx.table_vec(dl.items);
+  DevBuf dsup(&ctx, std::max<size_t>(dl.words, 1));
+  HMARK("x_declayout");
   // Sanity check (prover/mod.rs:127-144): the composition value at the OODS point against the
   // constraints evaluated from the sampled values. It only sets the status, so it runs on the
   // host while the FRI kernels are in flight.
@@ -1303,7 +1392,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   stage_ms[5] = tm.lap();
 
   // ---------------- proof of work ----------------
-  out.pow = device_grind(ctx, ch.pow_prefix(cfg.pow_bits), cfg.pow_bits);
+  out.pow = device_grind(ctx, ch.pow_prefix(cfg.pow_bits), cfg.pow_bits, (unsigned long long*)(dstate + 14));
   ch.mix_u64(out.pow);
   HMARK("pow");
   stage_ms[6] = tm.lap();
@@ -1317,39 +1406,54 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (auto x : w) { if (unsorted.size() < cfg.n_queries) unsorted.push_back(x & mask); }
     }
   }
-  auto qpos = sorted_unique(unsorted);
+  const auto qpos = sorted_unique(unsorted);
+  const std::vector<uint32_t> qp32(qpos.begin(), qpos.end());
+  const uint32_t nq = (uint32_t)qp32.size();
+  launch_decommit_gather(ditems, (uint32_t)dl.items.size(), qp32.data(), nq, dsup.p, ctx.stream);
+  const uint32_t* sup = (const uint32_t*)ctx.readback_buf(std::max<size_t>(dl.words, 1) * 4);
+  if (dl.words) HIP_CHECK(hipMemcpyAsync((void*)sup, dsup.p, dl.words * 4, hipMemcpyDeviceToHost, ctx.stream));
   HMARK("x_queries");
-  Gather g;
-  std::vector<FriPlan> fplans;
-  fplans.push_back(plan_fri_layer(first, qpos, 1, g));
-  auto lq = fold_queries(qpos, 1);
-  for (auto& l : inner) {
-    fplans.push_back(plan_fri_layer(l, lq, l.fold_step, g));
-    lq = fold_queries(lq, l.fold_step);
-  }
-  HMARK("x_friplan");
-  std::vector<MerklePlan> tplans;
-  for (auto& t : trees) {
-    auto q = tree_local_positions(qpos, Lc, t.max_log());
-    std::map<uint32_t, std::vector<size_t>> qpl;
-    uint32_t mx = t.max_log();
-    for (auto ls : std::set<uint32_t>(t.eval_logs.begin(), t.eval_logs.end())) {
-      uint32_t shift = mx - ls;
+  // the reference walks, on the host while the read is in flight
+  auto positions = [&](const DecObject& ob) {
+    std::vector<uint32_t> P(nq);
+    for (uint32_t i = 0; i < nq; ++i) P[i] = dec_pos_host(ob.pos_mode, ob.pos_arg, qp32[i]);
+    return P;
+  };
+  std::vector<DecPlan> tplans(trees.size());
+  for (size_t t = 0; t < trees.size(); ++t) {
+    const DecObject& ob = dl.objs[t];
+    if (ob.empty) continue;
+    const auto P = positions(ob);
+    // prepare_preprocessed_query_positions, then each column size's queried nodes
+    std::vector<std::pair<uint32_t, std::vector<size_t>>> qpl;
+    for (auto ls : std::set<uint32_t>(trees[t].eval_logs.begin(), trees[t].eval_logs.end())) {
+      const uint32_t shift = ob.top - ls;
       std::vector<size_t> v;
-      for (size_t p : q) v.push_back(((p >> (shift + 1)) << 1) + (p & 1));
-      qpl[ls] = sorted_unique(v);
+      for (uint32_t p : P) v.push_back(((size_t)(p >> (shift + 1)) << 1) + (p & 1));
+      qpl.push_back({ls, sorted_unique(v)});
     }
-    tplans.push_back(plan_merkle(t, qpl, g));
+    dec_plan_object(ob, P, qpl, tplans[t]);
+  }
+  std::vector<FriDecPlan> fplans;
+  {
+    std::vector<size_t> lq = qpos;
+    for (size_t j = 0; j <= inner.size(); ++j) {
+      const DecObject& ob = dl.objs[trees.size() + j];
+      fplans.push_back(plan_fri_layer(ob, lq, positions(ob)));
+      lq = fold_queries(lq, ob.w);
+    }
   }
   HMARK("dec_plan");
-  auto vals = g.run(ctx);
+  ctx.sync_main();
   HMARK("dec_gather");
-  out.first = finish_fri_layer(fplans[0], vals);
-  for (size_t i = 1; i < fplans.size(); ++i) out.inner.push_back(finish_fri_layer(fplans[i], vals));
+  out.first = finish_fri_layer(fplans[0], sup, first.tree.root);
+  for (size_t j = 0; j < inner.size(); ++j) out.inner.push_back(finish_fri_layer(fplans[j + 1], sup, inner[j].tree.root));
   for (size_t t = 0; t < trees.size(); ++t) {
     MerkleDec d;
-    out.queried.push_back(finish_merkle(tplans[t], vals, d));
-    out.decs.push_back(d);
+    std::vector<uint32_t> queried;
+    dec_finish(tplans[t], sup, d, &queried);
+    out.queried.push_back(std::move(queried));
+    out.decs.push_back(std::move(d));
     out.commitments.push_back(trees[t].root);
   }
   ctx.reset_arena();

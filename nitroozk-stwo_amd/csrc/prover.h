@@ -57,7 +57,8 @@ struct ProveResult {
 };
 
 // GrindOps::grind on the device: smallest nonce for the prefixed digest (simd/grind.rs order).
-uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits);
+// slot: a device u64 that holds ~0 (allocated and set here when null); receives the nonce.
+uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits, unsigned long long* slot = nullptr);
 // Builds the Merkle layers of a tree from its evaluation columns (MerkleProver::commit). With
 // read_root false the root stays on the device (layers[0]) and t.root is not set.
 void merkle_commit(Ctx& ctx, DevTree& t, bool read_root = true);

@@ -87,6 +87,7 @@ Ctx::~Ctx() {
   for (auto& ch : bump_chunks) (void)hipFree(ch.first);
   if (dev_arena) (void)hipFree(dev_arena);
   if (pinned) (void)hipHostFree(pinned);
+  if (rb_pinned) (void)hipHostFree(rb_pinned);
   if (table_dev) (void)hipFree(table_dev);
   if (table_pinned) (void)hipHostFree(table_pinned);
   (void)hipEventDestroy(ev_fork);
@@ -251,6 +252,16 @@ void Ctx::reset_arena() {
   pinned_off = 0;
   arena_off = 0;
   staged_from = 0;
+}
+void* Ctx::readback_buf(size_t bytes) {
+  if (bytes > rb_cap) {
+    sync_main();  // no read-back into the old buffer is still queued
+    if (rb_pinned) HIP_CHECK(hipHostFree(rb_pinned));
+    rb_pinned = nullptr;
+    rb_cap = std::max(bytes, (size_t)1 << 20);
+    HIP_CHECK(hipHostMalloc((void**)&rb_pinned, rb_cap, hipHostMallocDefault));
+  }
+  return rb_pinned;
 }
 void Ctx::d2h(void* dst, const void* src, size_t bytes) {
   HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));

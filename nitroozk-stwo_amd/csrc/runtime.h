@@ -144,6 +144,11 @@ struct Ctx {
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
   void d2h(void* dst, const void* src, size_t bytes);  // synchronous
+  // Pinned host memory for asynchronous read-backs on the main stream (kept and grown as needed;
+  // the contents are valid after the next sync).
+  uint8_t* rb_pinned = nullptr;
+  size_t rb_cap = 0;
+  void* readback_buf(size_t bytes);
 };
 
 // While alive, DevBufs of the context come from the proof arena; the arena is rewound (after a
