@@ -236,20 +236,31 @@ __global__ void __launch_bounds__(256) fri_layer_kernel(FriLayerDesc d, uint32_t
   const uint32_t* top = fl_levels(bufA, bufB, C, l, w * C, d.nodes);
   uint32_t root[8];
   if (gridDim.x > 1) {
-    // Wave 0 stored this chunk's root (the last level runs on quad 0); its fence publishes the
-    // root before the ticket is taken.
-    if (t == 0) {
-      __threadfence();
-      last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
-      __threadfence();
+    // The chunk root is published with device-scope atomic exchanges, which are performed where
+    // every XCD reads them; wave 0 waits for their results before it takes the ticket. This orders
+    // root and ticket without a device-scope fence, which would write back the whole L2 of the
+    // XCD once per workgroup.
+    if (t < 8) {
+      uint32_t* slot = d.nodes + 8 * (((size_t)1 << (l - c)) - 1 + w) + t;
+      const uint32_t old = __hip_atomic_exchange(slot, top[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(old) : "memory");
     }
+    if (t == 0)
+      last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1 ? 1u : 0u;
     __syncthreads();
     if (!last) return;
+    // the chunk roots, read with read-modify-write atomics of the same scope (performed where the
+    // exchanges were); the operand is opaque so that they are not turned into plain loads
     const uint32_t R = gridDim.x, lev = l - c;
-    const uint32_t* g = d.nodes + 8 * (((size_t)1 << lev) - 1);
-    for (uint32_t i = t; i < R; i += FL_THREADS)
+    uint32_t* g = d.nodes + 8 * (((size_t)1 << lev) - 1);
+    const uint32_t zero = pin(0u);
+    for (uint32_t i = t; i < R; i += FL_THREADS) {
+      uint32_t r[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) bufA[k * R + i] = __hip_atomic_load(g + 8 * i + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int k = 0; k < 8; ++k) r[k] = __hip_atomic_fetch_or(g + 8 * i + k, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) bufA[k * R + i] = r[k];
+    }
     __syncthreads();
     top = fl_levels(bufA, bufB, R, lev, 0, d.nodes);
   }
