@@ -160,7 +160,7 @@ int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, 
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
       KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
-      launch_fft(true, ds, dd, n_cols, log_size, log_size, c.itw.p, c.tree_len(), c.stream);
+      launch_fft(true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), c.stream);
     }
     c.sync();
     c.reset_arena();
@@ -177,7 +177,7 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
       KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_size) + 4.0 * (double)(1u << (log_size + log_blowup))));
-      launch_fft(false, ds, dd, n_cols, log_size, log_size + log_blowup, c.tw.p, c.tree_len(), c.stream);
+      launch_fft(false, ds, dd, n_cols, log_size, log_size + log_blowup, c.tw2.p, c.tree_len(), c.stream);
     }
     c.sync();
     c.reset_arena();

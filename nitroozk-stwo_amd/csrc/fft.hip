@@ -16,31 +16,46 @@
 
 namespace sh {
 
-template <bool INV>
-__device__ __forceinline__ void bfly(uint32_t& v0, uint32_t& v1, uint32_t t) {
-  if (INV) {  // ibutterfly(v0, v1, t) = (v0 + v1, (v0 - v1) t)
-    uint32_t a = v0, b = v1;
+// Twiddles are read from doubled trees (every entry 2t, below 2^32): a t (mod P) for a <= P is
+// then hi + lo / 2 with a (2t) = hi 2^32 + lo, since a t = hi 2^31 + lo / 2 and 2^31 = 1 (mod P).
+// The sum is below 2P, so one conditional subtraction (a minimum) makes it canonical: 5 operations
+// against 6 for the product with t itself.
+__device__ __forceinline__ uint32_t mul_tw(uint32_t a, uint32_t t2) {
+  const uint64_t p = (uint64_t)a * t2;
+  const uint32_t s = (uint32_t)(p >> 32) + ((uint32_t)p >> 1);
+  const uint32_t r = s - P;
+  return r < s ? r : s;
+}
+// The negation of a doubled twiddle, again doubled: 2P - t2 (0 stays 0).
+__device__ __forceinline__ uint32_t neg_tw(uint32_t t2) { return t2 ? 2 * P - t2 : 0u; }
+
+// NEG: the twiddle is the negation of t2, folded into the butterfly's additions.
+template <bool INV, bool NEG = false>
+__device__ __forceinline__ void bfly(uint32_t& v0, uint32_t& v1, uint32_t t2) {
+  const uint32_t a = v0, b = v1;
+  if (INV) {  // ibutterfly(v0, v1, t) = (v0 + v1, (v0 - v1) t); with -t: (v1 - v0) t
     v0 = m31_add(a, b);
-    v1 = m31_mul(m31_sub(a, b), t);
-  } else {  // butterfly(v0, v1, t) = (v0 + v1 t, v0 - v1 t)
-    uint32_t m = m31_mul(v1, t);
-    uint32_t a = v0;
-    v0 = m31_add(a, m);
-    v1 = m31_sub(a, m);
+    v1 = mul_tw(NEG ? m31_sub(b, a) : m31_sub(a, b), t2);
+  } else {  // butterfly(v0, v1, t) = (v0 + v1 t, v0 - v1 t); with -t the two swap
+    const uint32_t m = mul_tw(b, t2);
+    v0 = NEG ? m31_sub(a, m) : m31_add(a, m);
+    v1 = NEG ? m31_add(a, m) : m31_sub(a, m);
   }
 }
 
+// Doubled circle twiddle of pair h (line layer 0 pairs [x, y] -> [y, -y, -x, x]).
 __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* __restrict__ lt0, uint32_t h) {
-  uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
+  const uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
   switch (h & 3) {
     case 0: return y;
-    case 1: return m31_neg(y);
-    case 2: return m31_neg(x);
+    case 1: return neg_tw(y);
+    case 2: return neg_tw(x);
     default: return x;
   }
 }
 
-// Twiddle of the pair whose lower element has global index idx0 at stage s (see top of file).
+// Doubled twiddle of the pair whose lower element has global index idx0 at stage s (see top of
+// file); tree is a doubled tree.
 __device__ __forceinline__ uint32_t stage_twiddle(const uint32_t* __restrict__ tree, uint32_t tree_len,
                                                   uint32_t log_n, uint32_t s, uint32_t idx0) {
   if (s == 0) return circle_twiddle(tree + tree_len - (1u << (log_n - 1)), idx0 >> 1);
@@ -119,7 +134,7 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
   for (uint32_t l = threadIdx.x; l < tile_n; l += blockDim.x) {
     uint32_t t = l >> c, lo = l & (C - 1);
     uint32_t v = lds[l];
-    out[base | (t << e) | lo] = scale ? m31_mul(v, scale) : v;
+    out[base | (t << e) | lo] = scale ? mul_tw(v, scale) : v;
   }
 }
 
@@ -197,7 +212,7 @@ __device__ __forceinline__ void round_to_global(uint32_t* v, GLOBAL_AS uint32_t*
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
   if (scale) {
 #pragma unroll
-    for (uint32_t i = 0; i < LowShape<S>::V; ++i) v[i] = m31_mul(v[i], scale);
+    for (uint32_t i = 0; i < LowShape<S>::V; ++i) v[i] = mul_tw(v[i], scale);
   }
   if (A == 0 && R == 5 && vec) {
 #pragma unroll
@@ -231,6 +246,21 @@ __device__ __forceinline__ void round_butterflies(uint32_t* v, uint32_t gbase, c
     for (int kk = 0; kk < R; ++kk) {
       const int k = INV ? kk : R - 1 - kk;
       const uint32_t s = A + k;
+      if (A == 0 && k == 0) {
+        // stage 0: pair h = (idx_base >> 1) | u, whose quarter u & 3 is known here, so the negated
+        // circle twiddles become swapped additions
+        const uint32_t* lt0 = tree + tree_len - (1u << (log_n - 1));
+#pragma unroll
+        for (uint32_t u = 0; u < (Q >> 1); u += 4) {
+          const uint32_t h = (idx_base >> 1) | u;
+          const uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
+          bfly<INV, false>(v[g * Q + 2 * u], v[g * Q + 2 * u + 1], y);
+          bfly<INV, true>(v[g * Q + 2 * u + 2], v[g * Q + 2 * u + 3], y);
+          bfly<INV, true>(v[g * Q + 2 * u + 4], v[g * Q + 2 * u + 5], x);
+          bfly<INV, false>(v[g * Q + 2 * u + 6], v[g * Q + 2 * u + 7], x);
+        }
+        continue;
+      }
 #pragma unroll
       for (uint32_t u = 0; u < (Q >> (k + 1)); ++u) {
         const uint32_t tw = stage_twiddle(tree, tree_len, log_n, s, idx_base | ((u << (k + 1)) << A));
@@ -316,7 +346,7 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
   if (n_cols == 0) return;
   const uint32_t prio = background ? 0u : 1u;
   const uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
-  const uint32_t inv_n = inverse ? m31_inv(1u << log_n) : 0;
+  const uint32_t inv_n = inverse ? 2 * m31_inv(1u << log_n) : 0;  // doubled, as the twiddles
   auto run_pass = [&](const FftPass& ps, const uint32_t* const* from, uint32_t mask, uint32_t scale) {
     const uint32_t tiles = 1u << (log_n - ps.S - ps.c), tile_n = 1u << (ps.S + ps.c);
     const uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);

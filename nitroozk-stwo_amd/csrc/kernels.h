@@ -12,8 +12,9 @@ struct Cols4 { uint32_t* c[4]; };
 struct CCols4 { const uint32_t* c[4]; };
 
 // fft.hip
-// background: the transform runs beside critical-path kernels on another stream and leaves them
-// the issue priority.
+// tree: a doubled twiddle tree (every entry 2t; Ctx::tw2 for evaluation, Ctx::itw2 for
+// interpolation). background: the transform runs beside critical-path kernels on another stream
+// and leaves them the issue priority.
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
                 uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background = false);
 

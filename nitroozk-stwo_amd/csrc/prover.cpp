@@ -454,7 +454,7 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
       st = ctx.side;
     }
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)), st);
-    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw.p, ctx.tree_len(), st, on_side);
+    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len(), st, on_side);
   }
 }
 
@@ -570,7 +570,7 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
     KScope ks(ctx, K_INTERPOLATE, (double)idx.size() * 8.0 * (1u << lg));
     launch_fft(true, d_src, d_dst,
-               (uint32_t)idx.size(), lg, lg, ctx.itw.p, ctx.tree_len(), ctx.stream);
+               (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len(), ctx.stream);
   }
 }
 
@@ -595,7 +595,7 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << K)));
     launch_fft(false, d_src, d_dst,
-               (uint32_t)idx.size(), lg, K, ctx.tw.p, ctx.tree_len(), ctx.stream);
+               (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len(), ctx.stream);
   }
 }
 

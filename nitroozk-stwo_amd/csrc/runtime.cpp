@@ -78,6 +78,8 @@ Ctx::~Ctx() {
   for (auto e : event_pool) (void)hipEventDestroy(e);
   tw.reset();
   itw.reset();
+  tw2.reset();
+  itw2.reset();
   domain_points.clear();
   for (auto& [bytes, v] : col_cache)
     for (void* p : v) (void)hipFreeAsync(p, stream);
@@ -304,6 +306,12 @@ void Ctx::ensure_twiddles(uint32_t log_domain) {
   itw = DevBuf(this, len);
   HIP_CHECK(hipMemcpyAsync(tw.p, t.data(), len * 4, hipMemcpyHostToDevice, stream));
   HIP_CHECK(hipMemcpyAsync(itw.p, it.data(), len * 4, hipMemcpyHostToDevice, stream));
+  sync();
+  for (size_t i = 0; i < len; ++i) { t[i] *= 2; it[i] *= 2; }
+  tw2 = DevBuf(this, len);
+  itw2 = DevBuf(this, len);
+  HIP_CHECK(hipMemcpyAsync(tw2.p, t.data(), len * 4, hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipMemcpyAsync(itw2.p, it.data(), len * 4, hipMemcpyHostToDevice, stream));
   sync();
   tw_log = log_domain;
 }

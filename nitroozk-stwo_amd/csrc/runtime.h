@@ -101,6 +101,7 @@ struct Ctx {
   // twiddle tree for the half coset of CanonicCoset(tw_log): 2^(tw_log-1) words each
   uint32_t tw_log = 0;
   DevBuf tw, itw;
+  DevBuf tw2, itw2;  // the same trees with every entry doubled (2t < 2^32), read by the FFT kernels
   std::map<uint32_t, DevBuf> domain_points;  // bit-reversed (x, y) of CanonicCoset(log)
   // small-upload arena (device) and pinned staging (host)
   uint8_t* pinned = nullptr;
