@@ -86,8 +86,22 @@ void launch_accumulate_numerators(uint32_t log_size, const NumEntry* entries, co
                                   const qm31* batch_b, uint32_t n_batches, uint32_t* const* out, bool aligned16,
                                   hipStream_t stream);
 
+// A point's linear term coefficient is first_linear plus the a of each of its parts.
 struct QuotPoint { cm31 prx, pix, pry, piy; qm31 first_linear; uint32_t part_begin, part_end, pad[2]; };
-struct QuotPart { const uint32_t* c[4]; uint32_t log_size; uint32_t pad[3]; };
+struct QuotPart { const uint32_t* c[4]; uint32_t log_size; uint32_t pad[3]; qm31 a; };
+
+// Line coefficients of the quotient batches from the sampled values on the device
+// (core/constraints.rs complex_conjugate_line_coeffs with the batch sums of
+// core/pcs/quotients.rs:200-298). Sample q of the proof has random coefficient r = qrc^q and value
+// vals[vi]; for batch b (entries [first, first + count), sample point y-coordinate py,
+// cy = conj(py) - py): entry e gets c = r cy (out_entries[e] = {col, c}), batch_b[b] =
+// cy sum r v - py sum r (conj(v) - v), and parts[part].a = sum r (conj(v) - v).
+// scratch: 8 words per entry.
+struct QCoefEntry { const uint32_t* col; uint32_t q, vi, batch, pad; };
+struct QCoefBatch { uint32_t first, count, part, pad; qm31 cy, py; };
+void launch_quotient_coeffs(qm31 qrc, const qm31* vals, const QCoefEntry* entries, uint32_t n_entries,
+                            const QCoefBatch* batches, uint32_t n_batches, NumEntry* out_entries, qm31* batch_b,
+                            QuotPart* parts, qm31* scratch, hipStream_t stream);
 void launch_quotients(uint32_t lifting_log, const uint32_t* domain_xy, const QuotPoint* points, uint32_t n_points,
                       const QuotPart* parts, Cols4 out, hipStream_t stream);
 

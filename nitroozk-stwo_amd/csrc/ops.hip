@@ -222,17 +222,85 @@ __global__ void __launch_bounds__(256) quotients_kernel(uint32_t lifting_log, co
     cm31 dy = {m31_sub(pt.pry.a, y), pt.pry.b};
     cm31 den = cm_sub(cm_mul(dx, pt.piy), cm_mul(dy, pt.pix));
     cm31 dinv = cm_inv(den);
-    qm31 num = qm_zero();
+    qm31 num = qm_zero(), lin = pt.first_linear;
     for (uint32_t p = pt.part_begin; p < pt.part_end; ++p) {
-      QuotPart pa = parts[p];
+      const QuotPart& pa = parts[p];
       uint32_t r = lifting_log - pa.log_size;
       uint32_t li = ((row >> (r + 1)) << 1) + (row & 1);
       num = qm_add(num, load_q(pa.c, li));
+      lin = qm_add(lin, pa.a);
     }
-    num = qm_sub(num, qm_mul_m31(pt.first_linear, y));
+    num = qm_sub(num, qm_mul_m31(lin, y));
     q = qm_add(q, qm_mul_cm31(num, dinv));
   }
   store_q(out.c, row, q);
+}
+
+// Two launches: one lane per entry computes r = qrc^q, the entry's coefficient and its two terms
+// r v and r (conj(v) - v) (into scratch); then one workgroup per batch sums its entries' terms.
+__device__ __forceinline__ qm31 qm_pow_small(qm31 b, uint32_t e) {
+  qm31 r = qm_one();
+  while (e) {
+    if (e & 1) r = qm_mul(r, b);
+    b = qm_sqr(b);
+    e >>= 1;
+  }
+  return r;
+}
+__global__ void __launch_bounds__(256) quotient_coeff_entries_kernel(qm31 qrc, const qm31* __restrict__ vals,
+                                                                     const QCoefEntry* __restrict__ entries,
+                                                                     uint32_t n_entries,
+                                                                     const QCoefBatch* __restrict__ batches,
+                                                                     NumEntry* __restrict__ out_entries,
+                                                                     qm31* __restrict__ terms) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_entries) return;
+  const QCoefEntry en = entries[e];
+  const qm31 r = qm_pow_small(qrc, en.q), v = vals[en.vi];
+  NumEntry ne{};
+  ne.col = en.col;
+  ne.c = qm_mul(r, batches[en.batch].cy);
+  out_entries[e] = ne;
+  terms[2 * e] = qm_mul(r, v);
+  terms[2 * e + 1] = qm_mul(r, qm_sub(qm_conj(v), v));
+}
+__global__ void __launch_bounds__(256) quotient_coeff_batches_kernel(const QCoefBatch* __restrict__ batches,
+                                                                     const qm31* __restrict__ terms,
+                                                                     qm31* __restrict__ batch_b,
+                                                                     QuotPart* __restrict__ parts) {
+  __shared__ qm31 red[2][256];
+  const QCoefBatch bt = batches[blockIdx.x];
+  const uint32_t t = threadIdx.x;
+  qm31 srv = qm_zero(), sra = qm_zero();
+  for (uint32_t k = t; k < bt.count; k += 256) {
+    srv = qm_add(srv, terms[2 * (bt.first + k)]);
+    sra = qm_add(sra, terms[2 * (bt.first + k) + 1]);
+  }
+  red[0][t] = srv;
+  red[1][t] = sra;
+  __syncthreads();
+  for (uint32_t h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      red[0][t] = qm_add(red[0][t], red[0][t + h]);
+      red[1][t] = qm_add(red[1][t], red[1][t + h]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    batch_b[blockIdx.x] = qm_sub(qm_mul(bt.cy, red[0][0]), qm_mul(bt.py, red[1][0]));
+    parts[bt.part].a = red[1][0];
+  }
+}
+
+void launch_quotient_coeffs(qm31 qrc, const qm31* vals, const QCoefEntry* entries, uint32_t n_entries,
+                            const QCoefBatch* batches, uint32_t n_batches, NumEntry* out_entries, qm31* batch_b,
+                            QuotPart* parts, qm31* scratch, hipStream_t stream) {
+  if (!n_batches) return;
+  if (n_entries)
+    hipLaunchKernelGGL(quotient_coeff_entries_kernel, dim3((n_entries + 255) / 256), dim3(256), 0, stream, qrc, vals,
+                       entries, n_entries, batches, out_entries, scratch);
+  hipLaunchKernelGGL(quotient_coeff_batches_kernel, dim3(n_batches), dim3(256), 0, stream, batches,
+                     (const qm31*)scratch, batch_b, parts);
 }
 
 void launch_quotients(uint32_t lifting_log, const uint32_t* domain_xy, const QuotPoint* points, uint32_t n_points,

@@ -984,21 +984,22 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   // complex_conjugate_line_coeffs)
   //   a = r (conj(v) - v), c = r (conj(p.y) - p.y), b = r (v (conj(p.y) - p.y) - (conj(v) - v) p.y)
   // so that over a batch A = sum r (conj(v) - v) and B = cy sum r v - p.y A, with cy = conj(p.y) - p.y.
-  struct QBatch { uint32_t pid; qm31 cy; uint32_t first, count; size_t acc; };
+  // The coefficients themselves need the sampled values and the random coefficient, so they are
+  // computed on the device (launch_quotient_coeffs); the host lays out entries and batches here.
   struct QGroup {
     uint32_t ls;
     size_t n_cols;
     bool aligned;
-    std::vector<QBatch> batches;
-    std::vector<uint32_t> ent_q;               // per entry (batch-major): the sample index
-    std::vector<const uint32_t*> ent_col;      // per entry: the column
+    uint32_t ent_off, batch_off, n_batches;  // into the proof's entry and batch arrays
     const uint32_t* d_starts;
     uint32_t* const* d_outs;
   };
-  struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; qm31 first_linear; };
+  struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; uint32_t batch; };
   std::vector<Acc> acc_list;
   std::vector<DevBuf> partial_bufs;
   std::vector<QGroup> qgroups;
+  std::vector<QCoefEntry> qents;
+  std::vector<QCoefBatch> qbatches;
   {
     struct Item { size_t t, c; uint32_t ls; };
     std::vector<Item> items;
@@ -1007,6 +1008,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
     std::vector<int> batch_of(qsample_pts.size(), -1);
+    std::vector<uint32_t> bpid;
+    std::vector<std::vector<std::pair<uint32_t, const uint32_t*>>> per;
     for (size_t i = 0; i < items.size();) {
       size_t j = i;
       while (j < items.size() && items[j].ls == items[i].ls) ++j;
@@ -1014,8 +1017,11 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       G.ls = items[i].ls;
       G.n_cols = j - i;
       G.aligned = true;
-      // entries per batch in column order: count first, then place
-      std::vector<std::vector<std::pair<uint32_t, const uint32_t*>>> per;
+      G.ent_off = (uint32_t)qents.size();
+      G.batch_off = (uint32_t)qbatches.size();
+      // a batch per point, in order of first use; its entries in column order
+      bpid.clear();
+      per.clear();
       for (size_t k = i; k < j; ++k) {
         const uint32_t* col = trees[items[k].t].evals[items[k].c];
         G.aligned &= ((uintptr_t)col & 15) == 0;
@@ -1023,9 +1029,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         for (uint32_t q = qstart[g]; q < qstart[g + 1]; ++q) {
           const uint32_t pid = qpid[q];
           if (batch_of[pid] < 0) {
-            batch_of[pid] = (int)G.batches.size();
-            const qm31 py = qsample_pts[pid].y;
-            G.batches.push_back({pid, qm_sub(qm_conj(py), py), 0, 0, 0});
+            batch_of[pid] = (int)bpid.size();
+            bpid.push_back(pid);
             per.emplace_back();
           }
           per[batch_of[pid]].push_back({q, col});
@@ -1033,76 +1038,38 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       }
       std::vector<uint32_t> starts;
       std::vector<uint32_t*> outs;
-      for (size_t bi = 0; bi < G.batches.size(); ++bi) {
-        QBatch& b = G.batches[bi];
-        batch_of[b.pid] = -1;
-        b.first = (uint32_t)G.ent_q.size();
-        b.count = (uint32_t)per[bi].size();
-        starts.push_back(b.first);
-        for (auto& [q, col] : per[bi]) { G.ent_q.push_back(q); G.ent_col.push_back(col); }
+      for (size_t bi = 0; bi < bpid.size(); ++bi) {
+        const uint32_t pid = bpid[bi];
+        batch_of[pid] = -1;
+        QCoefBatch qb{};
+        qb.first = (uint32_t)qents.size();
+        qb.count = (uint32_t)per[bi].size();
+        const qm31 py = qsample_pts[pid].y;
+        qb.py = py;
+        qb.cy = qm_sub(qm_conj(py), py);
+        starts.push_back(qb.first - G.ent_off);
+        for (auto& [q, col] : per[bi]) qents.push_back({col, q, qvi[q], (uint32_t)qbatches.size(), 0});
         Acc a;
-        a.point = qsample_pts[b.pid];
+        a.point = qsample_pts[pid];
         a.log = G.ls;
-        a.first_linear = qm_zero();
+        a.batch = (uint32_t)qbatches.size();
         for (int k = 0; k < 4; ++k) {
           partial_bufs.emplace_back(&ctx, (size_t)1 << G.ls);
           outs.push_back(partial_bufs.back().p);
           a.c[k] = partial_bufs.back().p;
         }
-        b.acc = acc_list.size();
         acc_list.push_back(a);
+        qbatches.push_back(qb);
       }
-      starts.push_back((uint32_t)G.ent_q.size());
+      G.n_batches = (uint32_t)bpid.size();
+      starts.push_back((uint32_t)qents.size() - G.ent_off);
       G.d_starts = ctx.table_vec(starts);
       G.d_outs = (uint32_t* const*)ctx.table_vec(outs);
-      qgroups.push_back(std::move(G));
+      qgroups.push_back(G);
       i = j;
     }
   }
-  HMARK("x_qplan");
-  if (n_vals) {
-    ctx.d2h(S.val.data(), dvals_p, n_vals * 16);
-    ctx.reset_arena();
-  }
-  HMARK("oods_gpu");
-  ch.mix_felts(S.val);
-  HMARK("oods_post");
-  stage_ms[3] = tm.lap();
-
-  // ---------------- FRI quotients ----------------
-  const qm31 qrc = ch.draw_secure_felt();
-  std::vector<qm31> qrand(qpid.size());
-  {
-    qm31 pw = qm_one();
-    for (auto& r : qrand) { r = pw; pw = qm_mul(pw, qrc); }
-  }
-  HMARK("q_samples");
-  for (auto& G : qgroups) {
-    if (G.batches.empty()) continue;
-    std::vector<NumEntry> ents(G.ent_q.size());
-    std::vector<qm31> Bs;
-    for (auto& b : G.batches) {
-      qm31 sum_rv = qm_zero(), sum_ra = qm_zero();
-      for (uint32_t e = b.first; e < b.first + b.count; ++e) {
-        const uint32_t q = G.ent_q[e];
-        const qm31 v = S.val[qvi[q]], r = qrand[q];
-        NumEntry& ne = ents[e];
-        ne.col = G.ent_col[e];
-        ne.c = qm_mul(r, b.cy);
-        sum_rv = qm_add(sum_rv, qm_mul(r, v));
-        sum_ra = qm_add(sum_ra, qm_mul(r, qm_sub(qm_conj(v), v)));
-      }
-      Bs.push_back(qm_sub(qm_mul(b.cy, sum_rv), qm_mul(qsample_pts[b.pid].y, sum_ra)));
-      acc_list[b.acc].first_linear = sum_ra;
-    }
-    const NumEntry* d_ents = ctx.stage_vec(ents);
-    const qm31* d_bs = ctx.upload_vec(Bs);
-    KScope ks(ctx, K_NUMERATORS, (4.0 * G.n_cols + 16.0 * G.batches.size()) * (double)(1u << G.ls));
-    launch_accumulate_numerators(G.ls, d_ents, G.d_starts, d_bs, (uint32_t)G.batches.size(), G.d_outs, G.aligned,
-                                 ctx.stream);
-    HMARK("x_qgroup");
-  }
-  HMARK("q_numer");
+  // quotient points in the order of their coordinates, each with its batches as parts
   std::stable_sort(acc_list.begin(), acc_list.end(), [](const Acc& a, const Acc& b) { return point_less(a.point, b.point); });
   std::vector<QuotPoint> qpts;
   std::vector<QuotPart> qparts;
@@ -1112,30 +1079,60 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     QuotPoint qp{};
     const cpqm31& p = acc_list[i].point;
     qp.prx = qm_lo(p.x); qp.pix = qm_hi(p.x); qp.pry = qm_lo(p.y); qp.piy = qm_hi(p.y);
-    qp.first_linear = qm_zero();
+    qp.first_linear = qm_zero();  // the parts' a, written on the device, make up the coefficient
     qp.part_begin = (uint32_t)qparts.size();
     for (size_t k = i; k < j; ++k) {
-      qp.first_linear = qm_add(qp.first_linear, acc_list[k].first_linear);
       QuotPart pa{};
       for (int c = 0; c < 4; ++c) pa.c[c] = acc_list[k].c[c];
       pa.log_size = acc_list[k].log;
+      qbatches[acc_list[k].batch].part = (uint32_t)qparts.size();
       qparts.push_back(pa);
     }
     qp.part_end = (uint32_t)qparts.size();
     qpts.push_back(qp);
     i = j;
   }
+  const QCoefEntry* d_qents = ctx.table_vec(qents);
+  const QCoefBatch* d_qbatches = ctx.upload_vec(qbatches);
+  QuotPart* d_qparts = (QuotPart*)ctx.upload_vec(qparts);
+  const QuotPoint* d_qpts = ctx.upload_vec(qpts);
+  DevBuf num_entries(&ctx, std::max<size_t>(qents.size(), 1) * (sizeof(NumEntry) / 4));
+  DevBuf batch_b(&ctx, std::max<size_t>(qbatches.size(), 1) * 4);
+  DevBuf qterms(&ctx, std::max<size_t>(qents.size(), 1) * 8);
+  HMARK("x_qplan");
+  // the uploads above stay in the staging arena until the quotient kernels have run
+  if (n_vals) ctx.d2h(S.val.data(), dvals_p, n_vals * 16);
+  HMARK("oods_gpu");
+  ch.mix_felts(S.val);
+  HMARK("oods_post");
+  stage_ms[3] = tm.lap();
+
+  // ---------------- FRI quotients ----------------
+  const qm31 qrc = ch.draw_secure_felt();
+  NumEntry* d_num = (NumEntry*)num_entries.p;
+  qm31* d_bb = (qm31*)batch_b.p;
+  {
+    KScope ks(ctx, K_NUMERATORS, 32.0 * (double)qents.size());
+    launch_quotient_coeffs(qrc, (const qm31*)dvals_p, d_qents, (uint32_t)qents.size(), d_qbatches,
+                           (uint32_t)qbatches.size(), d_num, d_bb, d_qparts, (qm31*)qterms.p, ctx.stream);
+  }
+  HMARK("q_samples");
+  for (auto& G : qgroups) {
+    if (!G.n_batches) continue;
+    KScope ks(ctx, K_NUMERATORS, (4.0 * G.n_cols + 16.0 * G.n_batches) * (double)(1u << G.ls));
+    launch_accumulate_numerators(G.ls, d_num + G.ent_off, G.d_starts, d_bb + G.batch_off, G.n_batches, G.d_outs,
+                                 G.aligned, ctx.stream);
+  }
+  HMARK("q_numer");
   FriCol first;
   first.log = Lc;
   for (int k = 0; k < 4; ++k) first.c.emplace_back(&ctx, (size_t)1 << Lc);
   const uint32_t* dxy = ctx.domain_xy(Lc);
   double qbytes = 24.0 * (double)(1u << Lc);
   for (auto& pa : qparts) qbytes += 16.0 * (double)(1u << pa.log_size);
-  const QuotPart* dp = ctx.table_vec(qparts);
-  const QuotPoint* dq = ctx.upload_vec(qpts);
   {
     KScope ksq(ctx, K_QUOTIENTS, qbytes);
-    launch_quotients(Lc, dxy, dq, (uint32_t)qpts.size(), dp, first.mc(), ctx.stream);
+    launch_quotients(Lc, dxy, d_qpts, (uint32_t)qpts.size(), d_qparts, first.mc(), ctx.stream);
   }
   HMARK("q_launch");
   // no wait here: the FRI kernels queue behind the quotients while the host prepares them
