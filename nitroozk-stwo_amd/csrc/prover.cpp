@@ -824,19 +824,15 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   }
   accs.clear();
   HMARK("comp_launch");
-  commit_tree(ctx, comp, cfg, ch);
-  HMARK("comp_commit");
+  // the composition commitment is queued; its root is read after the host has laid out the OODS
+  // and quotient work below, which depends on the proof's shape but not on the OODS point
+  evaluate_polynomials(ctx, comp, cfg.log_blowup);
+  merkle_commit(ctx, comp, false);
   trees.push_back(std::move(comp));
-  stage_ms[2] = tm.lap();
+  HMARK("comp_queue");
 
   // ---------------- OODS ----------------
   cpqm31 oods;
-  {
-    qm31 t = ch.draw_secure_felt();
-    qm31 t2 = qm_sqr(t);
-    qm31 d = qm_inv(qm_add(t2, qm_one()));
-    oods = {qm_mul(qm_sub(qm_one(), t2), d), qm_mul(qm_add(t, t), d)};
-  }
   uint32_t split_log = trees.back().max_log();
   uint32_t L = cfg.lifting_log_size ? cfg.lifting_log_size : split_log;
   if (L < split_log) throw HipError(3, "lifting log size below the composition size");
@@ -847,12 +843,11 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   for (size_t t = 0; t < trees.size(); ++t) S.col_base.push_back(S.col_base.back() + (uint32_t)trees[t].evals.size());
   const uint32_t NC = S.col_base.back();
   const cpm31 step = index_to_point(subgroup_gen(max_deg));
-  std::vector<long> pt_off;
+  std::vector<long> pt_off;  // sample point p is the OODS point plus pt_off[p] steps
   auto pid_of = [&](long o) -> uint32_t {
     for (size_t i = 0; i < pt_off.size(); ++i)
       if (pt_off[i] == o) return (uint32_t)i;
     pt_off.push_back(o);
-    S.pts.push_back(cpq_add(oods, cpq_mul_signed(step, o)));
     return (uint32_t)pt_off.size() - 1;
   };
   std::vector<uint8_t> ccount(NC, 0);
@@ -881,7 +876,6 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   tasks.reserve(S.pid.size());
   std::vector<uint32_t> task_group, group_log;
   task_group.reserve(S.pid.size());
-  std::vector<qm31> group_factors;
   struct GKey { uint32_t k, pid, shift; };
   std::vector<GKey> group_key;
   uint32_t max_coeff_log = 0;
@@ -894,14 +888,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         uint32_t g = 0;
         while (g < group_key.size() && !(group_key[g].k == k && group_key[g].pid == pid && group_key[g].shift == shift)) ++g;
         if (g == group_key.size()) {
-          cpqm31 q = cpq_repeated_double(S.pts[pid], shift);
           group_key.push_back({k, pid, shift});
           group_log.push_back(k);
-          std::vector<qm31> f(32, qm_zero());
-          f[0] = q.y;
-          qm31 x = q.x;
-          for (uint32_t b = 1; b < k; ++b) { f[b] = x; x = qm_double_x(x); }
-          group_factors.insert(group_factors.end(), f.begin(), f.end());
         }
         tasks.push_back(OodsTask{trees[t].coeffs[c], k, (uint32_t)tasks.size(), {0, 0}});
         task_group.push_back(g);
@@ -920,39 +908,38 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     tasks.swap(st);
     task_group.swap(sg);
   }
-  HMARK("oods_prep");
   S.val.resize(n_vals);
   DevBuf oods_partials, oods_vals, oods_weights;
   const uint32_t* dvals_p = nullptr;
+  const OodsTask* d_otasks = nullptr;
+  const uint32_t *d_otg = nullptr, *d_ogl = nullptr, *d_obs = nullptr;
+  uint32_t n_obatches = 0;
+  double obytes = 0;
   if (n_vals) {
     uint32_t max_chunks = oods_max_chunks(max_coeff_log);
     oods_partials = DevBuf(&ctx, (size_t)n_vals * max_chunks * 4);
     oods_vals = DevBuf(&ctx, n_vals * 4);
     oods_weights = DevBuf(&ctx, group_key.size() * ((size_t)4 << OODS_TABLE_LOG));
     dvals_p = oods_vals.p;
-    double obytes = 0;
     for (auto& tk : tasks) obytes += 4.0 * (double)(1u << tk.log_size);
-    const OodsTask* dt = ctx.table_vec(tasks);
-    const uint32_t* dtg = ctx.table_vec(task_group);
-    const uint32_t* dgl = ctx.table_vec(group_log);
+    d_otasks = ctx.table_vec(tasks);
+    d_otg = ctx.table_vec(task_group);
+    d_ogl = ctx.table_vec(group_log);
     const std::vector<uint32_t> bstart = oods_batch_starts(task_group);
-    const uint32_t* dbs = ctx.table_vec(bstart);
-    const qm31* df = ctx.upload_vec(group_factors);
-    KScope ks(ctx, K_OODS, obytes);
-    launch_oods(dt, dtg, (uint32_t)n_vals, dbs, (uint32_t)bstart.size() - 1, max_coeff_log, df, dgl,
-                (uint32_t)group_key.size(), (qm31*)oods_weights.p, (qm31*)oods_partials.p, (qm31*)oods_vals.p,
-                ctx.stream);
+    d_obs = ctx.table_vec(bstart);
+    n_obatches = (uint32_t)bstart.size() - 1;
   }
+  HMARK("oods_prep");
   // ---- quotient plan: everything that does not depend on the sampled values, built on the host
   // while the OODS kernels run ----
   // Quotient samples per column (core/pcs/quotients.rs:80-105,306-366): a column sampled at two
   // points first gets a periodicity sample at its second point shifted by the generator of its
   // evaluation domain, then its samples; consecutive powers of the random coefficient in order.
-  std::vector<cpqm31> qsample_pts = S.pts;
+  // Point ids past the sample points' are the periodicity points, listed in `periodic`.
   std::vector<std::array<uint32_t, 3>> periodic;  // (point id, eval log, new point id)
   std::vector<uint32_t> qstart(NC + 1, 0), qpid, qvi;  // per sample: point, index of its value in S.val
+  uint32_t n_qpoints = (uint32_t)pt_off.size();
   {
-    const cpm31 gen = index_to_point(subgroup_gen(Lc));
     for (size_t t = 0; t < trees.size(); ++t)
       for (size_t c = 0; c < trees[t].evals.size(); ++c) {
         const uint32_t g = S.col_base[t] + (uint32_t)c, n = S.count(t, c), f0 = S.first(t, c);
@@ -964,10 +951,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
           for (auto& e : periodic)
             if (e[0] == pids[1] && e[1] == el) np = e[2];
           if (np == UINT32_MAX) {
-            cpm31 pg = gen;
-            for (uint32_t i = 0; i < el; ++i) pg = cp_double(pg);
-            np = (uint32_t)qsample_pts.size();
-            qsample_pts.push_back(cpq_add(qsample_pts[pids[1]], cpq_from(pg)));
+            np = n_qpoints++;
             periodic.push_back({pids[1], el, np});
           }
           qpid.push_back(np);
@@ -994,7 +978,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     const uint32_t* d_starts;
     uint32_t* const* d_outs;
   };
-  struct Acc { cpqm31 point; uint32_t log; const uint32_t* c[4]; uint32_t batch; };
+  struct Acc { cpqm31 point; uint32_t pid, log; const uint32_t* c[4]; uint32_t batch; };
   std::vector<Acc> acc_list;
   std::vector<DevBuf> partial_bufs;
   std::vector<QGroup> qgroups;
@@ -1007,7 +991,7 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     for (size_t t = 0; t < trees.size(); ++t)
       for (size_t c = 0; c < trees[t].evals.size(); ++c) items.push_back({t, c, trees[t].eval_logs[c]});
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
-    std::vector<int> batch_of(qsample_pts.size(), -1);
+    std::vector<int> batch_of(n_qpoints, -1);
     std::vector<uint32_t> bpid;
     std::vector<std::vector<std::pair<uint32_t, const uint32_t*>>> per;
     for (size_t i = 0; i < items.size();) {
@@ -1044,13 +1028,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         QCoefBatch qb{};
         qb.first = (uint32_t)qents.size();
         qb.count = (uint32_t)per[bi].size();
-        const qm31 py = qsample_pts[pid].y;
-        qb.py = py;
-        qb.cy = qm_sub(qm_conj(py), py);
         starts.push_back(qb.first - G.ent_off);
         for (auto& [q, col] : per[bi]) qents.push_back({col, q, qvi[q], (uint32_t)qbatches.size(), 0});
         Acc a;
-        a.point = qsample_pts[pid];
+        a.pid = pid;
         a.log = G.ls;
         a.batch = (uint32_t)qbatches.size();
         for (int k = 0; k < 4; ++k) {
@@ -1068,6 +1049,58 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       qgroups.push_back(G);
       i = j;
     }
+  }
+  const QCoefEntry* d_qents = ctx.table_vec(qents);
+  DevBuf num_entries(&ctx, std::max<size_t>(qents.size(), 1) * (sizeof(NumEntry) / 4));
+  DevBuf batch_b(&ctx, std::max<size_t>(qbatches.size(), 1) * 4);
+  DevBuf qterms(&ctx, std::max<size_t>(qents.size(), 1) * 8);
+  HMARK("x_plan");
+
+  // ---- the composition root, the OODS point (prover/mod.rs:85-95) and the OODS launch ----
+  trees.back().root = read_hash(ctx, trees.back().layers[0].p);
+  ch.mix_root(trees.back().root);
+  ctx.reset_arena();
+  HMARK("comp_commit");
+  stage_ms[2] = tm.lap();
+  {
+    qm31 t = ch.draw_secure_felt();
+    qm31 t2 = qm_sqr(t);
+    qm31 d = qm_inv(qm_add(t2, qm_one()));
+    oods = {qm_mul(qm_sub(qm_one(), t2), d), qm_mul(qm_add(t, t), d)};
+  }
+  for (long o : pt_off) S.pts.push_back(cpq_add(oods, cpq_mul_signed(step, o)));
+  if (n_vals) {
+    std::vector<qm31> group_factors(32 * group_key.size(), qm_zero());
+    for (size_t g = 0; g < group_key.size(); ++g) {
+      const cpqm31 q = cpq_repeated_double(S.pts[group_key[g].pid], group_key[g].shift);
+      qm31* f = group_factors.data() + 32 * g;
+      f[0] = q.y;
+      qm31 x = q.x;
+      for (uint32_t b = 1; b < group_key[g].k; ++b) { f[b] = x; x = qm_double_x(x); }
+    }
+    const qm31* df = ctx.upload_vec(group_factors);
+    KScope ks(ctx, K_OODS, obytes);
+    launch_oods(d_otasks, d_otg, (uint32_t)n_vals, d_obs, n_obatches, max_coeff_log, df, d_ogl,
+                (uint32_t)group_key.size(), (qm31*)oods_weights.p, (qm31*)oods_partials.p, (qm31*)oods_vals.p,
+                ctx.stream);
+  }
+  HMARK("oods_launch");
+  // sample point values: the mask points, then the periodicity points
+  std::vector<cpqm31> qsample_pts = S.pts;
+  qsample_pts.resize(n_qpoints);
+  {
+    const cpm31 gen = index_to_point(subgroup_gen(Lc));
+    for (auto& e : periodic) {
+      cpm31 pg = gen;
+      for (uint32_t i = 0; i < e[1]; ++i) pg = cp_double(pg);
+      qsample_pts[e[2]] = cpq_add(qsample_pts[e[0]], cpq_from(pg));
+    }
+  }
+  for (auto& a : acc_list) {
+    a.point = qsample_pts[a.pid];
+    QCoefBatch& qb = qbatches[a.batch];
+    qb.py = a.point.y;
+    qb.cy = qm_sub(qm_conj(qb.py), qb.py);
   }
   // quotient points in the order of their coordinates, each with its batches as parts
   std::stable_sort(acc_list.begin(), acc_list.end(), [](const Acc& a, const Acc& b) { return point_less(a.point, b.point); });
@@ -1092,13 +1125,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     qpts.push_back(qp);
     i = j;
   }
-  const QCoefEntry* d_qents = ctx.table_vec(qents);
   const QCoefBatch* d_qbatches = ctx.upload_vec(qbatches);
   QuotPart* d_qparts = (QuotPart*)ctx.upload_vec(qparts);
   const QuotPoint* d_qpts = ctx.upload_vec(qpts);
-  DevBuf num_entries(&ctx, std::max<size_t>(qents.size(), 1) * (sizeof(NumEntry) / 4));
-  DevBuf batch_b(&ctx, std::max<size_t>(qbatches.size(), 1) * 4);
-  DevBuf qterms(&ctx, std::max<size_t>(qents.size(), 1) * 8);
   HMARK("x_qplan");
   // the uploads above stay in the staging arena until the quotient kernels have run
   if (n_vals) ctx.d2h(S.val.data(), dvals_p, n_vals * 16);
