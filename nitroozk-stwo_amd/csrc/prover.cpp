@@ -15,6 +15,7 @@
 #include "prover.h"
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -1125,10 +1126,158 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     qpts.push_back(qp);
     i = j;
   }
-  const QCoefBatch* d_qbatches = ctx.upload_vec(qbatches);
-  QuotPart* d_qparts = (QuotPart*)ctx.upload_vec(qparts);
-  const QuotPoint* d_qpts = ctx.upload_vec(qpts);
+  // one transfer for the three point-dependent tables
+  const QCoefBatch* d_qbatches = ctx.stage_vec(qbatches);
+  QuotPart* d_qparts = (QuotPart*)ctx.stage_vec(qparts);
+  const QuotPoint* d_qpts = ctx.stage_vec(qpts);
+  ctx.commit_staged();
   HMARK("x_qplan");
+
+  // ---- FRI layout, also while the OODS kernels run: the layers' buffers, Merkle heaps and kernel
+  // descriptors. The launches are recorded and queued after the quotients (the channel state they
+  // start from is known only then). ----
+  // The channel runs on the device during the commit phase (one record of root + folding
+  // coefficient per layer), so that no layer waits for the host; the host replays the
+  // transcript from the recorded roots afterwards and checks every coefficient.
+  FriCol first;
+  first.log = Lc;
+  for (int k = 0; k < 4; ++k) first.c.emplace_back(&ctx, (size_t)1 << Lc);
+  const uint32_t* dxy = ctx.domain_xy(Lc);
+  ctx.ensure_twiddles(Lc);
+  uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
+  if (Lc - 1 < last_log) throw HipError(3, "FRI: last layer larger than the first folded layer");
+  uint32_t n_layers = 1;
+  for (uint32_t l = Lc - 1; l > last_log;) {
+    uint32_t fs = l > last_log + cfg.fold_step ? cfg.fold_step : l - last_log;
+    ++n_layers;
+    l -= fs;
+  }
+  const size_t last_n = (size_t)1 << last_log;
+  // device block: channel state (16 words), one ticket counter per layer (zero), then the
+  // transcript records (12 words per layer) and the last layer's values
+  const size_t n_tickets = ((size_t)n_layers + 3) & ~(size_t)3;
+  DevBuf chan(&ctx, 16 + n_tickets + 12 * (size_t)n_layers + 4 * last_n);
+  uint32_t* dstate = chan.p;
+  uint32_t* dtickets = chan.p + 16;
+  uint32_t* drec = dtickets + n_tickets;
+  auto mix_and_draw = [&ctx, dstate, drec](FriCol& f, uint32_t layer) {
+    f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
+    f.tree.eval_logs.assign(4, f.log);
+    merkle_commit(ctx, f.tree, false);
+    KScope ks(ctx, K_MISC, 0.0);
+    launch_channel_mix_root_draw(dstate, f.tree.layers[0].p, drec + 12 * layer, ctx.stream);
+    return (const qm31*)(drec + 12 * layer + 8);
+  };
+  std::vector<FriCol> inner;
+  inner.reserve(n_layers);  // FriCol addresses stay valid while layers are appended
+  FriCol cur;
+  uint32_t layer = 1;
+  std::vector<std::function<void()>> fri_launches;  // in stream order
+  std::vector<FriLayerDesc> small;
+  DevBuf fri_buf;
+  if (cfg.fold_step == 1) {
+    // Fused layers (fri.hip): a layer of 2^9 .. 2^18 values is one launch of launch_fri_layer;
+    // layers of at most 2^8 values go to one launch_fri_small at the end; larger layers (and the
+    // fold into a last layer above 2^8) use the separate fold, Merkle and channel kernels.
+    constexpr uint32_t FAST_MAX = 18, SMALL_MAX = 8;
+    // Line layers of at most 2^FAST_MAX values share one block: the values of layer l (l_start
+    // down to last_log) at 4 (2^(l_start+1) - 2^(l+1)) words, coordinate k at + k 2^l, then the
+    // Merkle heaps of the committed ones, then the first layer's heap when it is fused too.
+    const uint32_t l_start = std::min(FAST_MAX, Lc - 1);
+    const size_t tv_words = 4 * (((size_t)2 << l_start) - ((size_t)1 << last_log));
+    std::vector<size_t> heap_off(l_start + 1, 0);
+    size_t tn_words = 0;
+    for (uint32_t l = l_start; l > last_log; --l) {
+      heap_off[l] = tv_words + tn_words;
+      tn_words += 8 * (((size_t)2 << l) - 1);
+    }
+    const bool first_fused = Lc <= FAST_MAX;
+    const size_t fh_words = first_fused ? 8 * (((size_t)2 << Lc) - 1) : 0;
+    fri_buf = DevBuf(&ctx, tv_words + tn_words + fh_words);
+    auto heap_views = [&](FriCol& f, uint32_t* heap) {
+      f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
+      f.tree.eval_logs.assign(4, f.log);
+      f.tree.layers.clear();
+      for (uint32_t j = 0; j <= f.log; ++j)
+        f.tree.layers.push_back(DevBuf::view(&ctx, heap + 8 * (((size_t)1 << j) - 1), (size_t)8 << j));
+    };
+    auto line_col = [&](uint32_t l) {
+      FriCol f;
+      f.log = l;
+      f.fold_step = 1;
+      if (l <= l_start) {
+        uint32_t* v = fri_buf.p + 4 * (((size_t)2 << l_start) - ((size_t)2 << l));
+        for (int k = 0; k < 4; ++k) f.c.push_back(DevBuf::view(&ctx, v + ((size_t)k << l), (size_t)1 << l));
+      } else {
+        for (int k = 0; k < 4; ++k) f.c.emplace_back(&ctx, (size_t)1 << l);
+      }
+      return f;
+    };
+    // one step per layer: li = 0 is the circle layer (the quotients), li = n_layers the last layer
+    FriCol* prev = &first;
+    for (uint32_t li = 0; li <= n_layers; ++li) {
+      const uint32_t l = Lc - li;
+      const bool commit = li < n_layers;
+      // the layer's FriCol at its final place (inner or cur), so that recorded launches can refer to it
+      FriCol* cf = &first;
+      if (li > 0) {
+        if (commit) {
+          inner.push_back(line_col(l));
+          cf = &inner.back();
+        } else {
+          cur = line_col(l);
+          cf = &cur;
+        }
+      }
+      FriLayerDesc d{};
+      d.log = l;
+      d.mode = li == 0 ? FRI_LEAVES : li == 1 ? FRI_FOLD_CIRCLE : FRI_FOLD_LINE;
+      d.commit = commit ? 1u : 0u;
+      d.src = li == 0 ? first.cc() : prev->cc();
+      d.dst = cf->mc();
+      d.tw = li == 1 ? ctx.itw.p + ctx.tree_len() - ((size_t)1 << l) : ctx.itw.p + ctx.tree_len() - ((size_t)2 << l);
+      d.alpha = li > 0 ? drec + 12 * (li - 1) + 8 : nullptr;
+      d.rec = commit ? drec + 12 * li : nullptr;
+      if (commit && l <= FAST_MAX) {
+        d.nodes = li == 0 ? fri_buf.p + tv_words + tn_words : fri_buf.p + heap_off[l];
+        heap_views(*cf, d.nodes);
+      }
+      if (l <= SMALL_MAX) {
+        small.push_back(d);
+      } else if (commit && l <= FAST_MAX) {
+        fri_launches.push_back([&ctx, d, l, li, dstate, dtickets] {
+          KScope ks(ctx, K_FRI_FOLD, (li == 0 ? 16.0 : 48.0) * (double)(1u << l) + 64.0 * (double)(1u << l));
+          launch_fri_layer(d, dstate, dtickets + li, ctx.stream);
+        });
+      } else {
+        if (li > 0) {
+          const CCols4 src = li == 1 ? first.cc() : prev->cc();
+          const Cols4 dst = cf->mc();
+          fri_launches.push_back([&ctx, d, l, li, src, dst, Lc] {
+            KScope ks(ctx, K_FRI_FOLD, 48.0 * (double)(1u << l));
+            const qm31* a = (const qm31*)d.alpha;
+            if (li == 1) launch_fold_circle_into_line_dev(Lc, src, dst, a, true, ctx.itw.p, ctx.tree_len(), ctx.stream);
+            else launch_fold_line_dev(l + 1, src, dst, a, 0, ctx.itw.p, ctx.tree_len(), ctx.stream);
+          });
+        }
+        if (commit) fri_launches.push_back([cf, li, mix_and_draw] { mix_and_draw(*cf, li); });
+      }
+      if (li > 0 && commit) prev = cf;
+    }
+    if (!small.empty()) {
+      double tb = 0;
+      for (auto& d : small) tb += 64.0 * (double)(1u << d.log);
+      const FriLayerDesc* d_small = ctx.table_vec(small);
+      const uint32_t n_small = (uint32_t)small.size();
+      fri_launches.push_back([&ctx, tb, d_small, n_small, dstate] {
+        KScope ks(ctx, K_FRI_TAIL, tb);
+        launch_fri_small(d_small, n_small, dstate, ctx.stream);
+      });
+    }
+    layer = n_layers;
+  }
+  HMARK("x_friprep");
+
   // the uploads above stay in the staging arena until the quotient kernels have run
   if (n_vals) ctx.d2h(S.val.data(), dvals_p, n_vals * 16);
   HMARK("oods_gpu");
@@ -1153,10 +1302,6 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
                                  G.aligned, ctx.stream);
   }
   HMARK("q_numer");
-  FriCol first;
-  first.log = Lc;
-  for (int k = 0; k < 4; ++k) first.c.emplace_back(&ctx, (size_t)1 << Lc);
-  const uint32_t* dxy = ctx.domain_xy(Lc);
   double qbytes = 24.0 * (double)(1u << Lc);
   for (auto& pa : qparts) qbytes += 16.0 * (double)(1u << pa.log_size);
   {
@@ -1164,32 +1309,10 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     launch_quotients(Lc, dxy, d_qpts, (uint32_t)qpts.size(), d_qparts, first.mc(), ctx.stream);
   }
   HMARK("q_launch");
-  // no wait here: the FRI kernels queue behind the quotients while the host prepares them
   partial_bufs.clear();
-  HMARK("q_gpu");
   stage_ms[4] = tm.lap();
 
   // ---------------- FRI commit ----------------
-  // The channel runs on the device during the commit phase (one record of root + folding
-  // coefficient per layer), so that no layer waits for the host; the host replays the
-  // transcript from the recorded roots afterwards and checks every coefficient.
-  ctx.ensure_twiddles(Lc);
-  uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
-  if (Lc - 1 < last_log) throw HipError(3, "FRI: last layer larger than the first folded layer");
-  uint32_t n_layers = 1;
-  for (uint32_t l = Lc - 1; l > last_log;) {
-    uint32_t fs = l > last_log + cfg.fold_step ? cfg.fold_step : l - last_log;
-    ++n_layers;
-    l -= fs;
-  }
-  const size_t last_n = (size_t)1 << last_log;
-  // device block: channel state (16 words), one ticket counter per layer (zero), then the
-  // transcript records (12 words per layer) and the last layer's values
-  const size_t n_tickets = ((size_t)n_layers + 3) & ~(size_t)3;
-  DevBuf chan(&ctx, 16 + n_tickets + 12 * (size_t)n_layers + 4 * last_n);
-  uint32_t* dstate = chan.p;
-  uint32_t* dtickets = chan.p + 16;
-  uint32_t* drec = dtickets + n_tickets;
   {
     std::vector<uint32_t> st(16 + n_tickets, 0);
     for (int i = 0; i < 8; ++i)
@@ -1199,109 +1322,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     st[14] = st[15] = ~0u;  // the proof-of-work result slot (u64), empty
     launch_copy_words(dstate, ctx.upload_vec(st), st.size(), ctx.stream);
   }
-  auto mix_and_draw = [&](FriCol& f, uint32_t layer) {
-    f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
-    f.tree.eval_logs.assign(4, f.log);
-    merkle_commit(ctx, f.tree, false);
-    KScope ks(ctx, K_MISC, 0.0);
-    launch_channel_mix_root_draw(dstate, f.tree.layers[0].p, drec + 12 * layer, ctx.stream);
-    return (const qm31*)(drec + 12 * layer + 8);
-  };
-  std::vector<FriCol> inner;
-  inner.reserve(n_layers);  // FriCol addresses stay valid while layers are appended
-  FriCol cur;
-  uint32_t layer = 1;
-  if (cfg.fold_step == 1) {
-    // Fused layers (fri.hip): a layer of 2^9 .. 2^18 values is one launch of launch_fri_layer;
-    // layers of at most 2^8 values go to one launch_fri_small at the end; larger layers (and the
-    // fold into a last layer above 2^8) use the separate fold, Merkle and channel kernels.
-    constexpr uint32_t FAST_MAX = 18, SMALL_MAX = 8;
-    // Line layers of at most 2^FAST_MAX values share one block: the values of layer l (l_start
-    // down to last_log) at 4 (2^(l_start+1) - 2^(l+1)) words, coordinate k at + k 2^l, then the
-    // Merkle heaps of the committed ones, then the first layer's heap when it is fused too.
-    const uint32_t l_start = std::min(FAST_MAX, Lc - 1);
-    const size_t tv_words = 4 * (((size_t)2 << l_start) - ((size_t)1 << last_log));
-    std::vector<size_t> heap_off(l_start + 1, 0);
-    size_t tn_words = 0;
-    for (uint32_t l = l_start; l > last_log; --l) {
-      heap_off[l] = tv_words + tn_words;
-      tn_words += 8 * (((size_t)2 << l) - 1);
-    }
-    const bool first_fused = Lc <= FAST_MAX;
-    const size_t fh_words = first_fused ? 8 * (((size_t)2 << Lc) - 1) : 0;
-    DevBuf fri_buf(&ctx, tv_words + tn_words + fh_words);
-    auto heap_views = [&](FriCol& f, uint32_t* heap) {
-      f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
-      f.tree.eval_logs.assign(4, f.log);
-      f.tree.layers.clear();
-      for (uint32_t j = 0; j <= f.log; ++j)
-        f.tree.layers.push_back(DevBuf::view(&ctx, heap + 8 * (((size_t)1 << j) - 1), (size_t)8 << j));
-    };
-    auto line_col = [&](uint32_t l) {
-      FriCol f;
-      f.log = l;
-      f.fold_step = 1;
-      if (l <= l_start) {
-        uint32_t* v = fri_buf.p + 4 * (((size_t)2 << l_start) - ((size_t)2 << l));
-        for (int k = 0; k < 4; ++k) f.c.push_back(DevBuf::view(&ctx, v + ((size_t)k << l), (size_t)1 << l));
-      } else {
-        for (int k = 0; k < 4; ++k) f.c.emplace_back(&ctx, (size_t)1 << l);
-      }
-      return f;
-    };
-    std::vector<FriLayerDesc> small;
-    // one step per layer: li = 0 is the circle layer (the quotients), li = n_layers the last layer
-    FriCol* prev = &first;
-    for (uint32_t li = 0; li <= n_layers; ++li) {
-      const uint32_t l = Lc - li;
-      const bool commit = li < n_layers;
-      FriCol f;
-      if (li > 0) f = line_col(l);
-      FriCol& cf = li == 0 ? first : f;
-      FriLayerDesc d{};
-      d.log = l;
-      d.mode = li == 0 ? FRI_LEAVES : li == 1 ? FRI_FOLD_CIRCLE : FRI_FOLD_LINE;
-      d.commit = commit ? 1u : 0u;
-      d.src = li == 0 ? first.cc() : prev->cc();
-      d.dst = cf.mc();
-      d.tw = li == 1 ? ctx.itw.p + ctx.tree_len() - ((size_t)1 << l) : ctx.itw.p + ctx.tree_len() - ((size_t)2 << l);
-      d.alpha = li > 0 ? drec + 12 * (li - 1) + 8 : nullptr;
-      d.rec = commit ? drec + 12 * li : nullptr;
-      if (commit && l <= FAST_MAX) {
-        d.nodes = li == 0 ? fri_buf.p + tv_words + tn_words : fri_buf.p + heap_off[l];
-        heap_views(cf, d.nodes);
-      }
-      if (l <= SMALL_MAX) {
-        small.push_back(d);
-      } else if (commit && l <= FAST_MAX) {
-        KScope ks(ctx, K_FRI_FOLD, (li == 0 ? 16.0 : 48.0) * (double)(1u << l) + 64.0 * (double)(1u << l));
-        launch_fri_layer(d, dstate, dtickets + li, ctx.stream);
-      } else {
-        if (li > 0) {
-          KScope ks(ctx, K_FRI_FOLD, 48.0 * (double)(1u << l));
-          const qm31* a = (const qm31*)d.alpha;
-          if (li == 1) launch_fold_circle_into_line_dev(Lc, first.cc(), cf.mc(), a, true, ctx.itw.p, ctx.tree_len(), ctx.stream);
-          else launch_fold_line_dev(l + 1, prev->cc(), cf.mc(), a, 0, ctx.itw.p, ctx.tree_len(), ctx.stream);
-        }
-        if (commit) mix_and_draw(cf, li);
-      }
-      if (li > 0) {
-        if (commit) {
-          inner.push_back(std::move(f));
-          prev = &inner.back();
-        } else {
-          cur = std::move(f);
-        }
-      }
-    }
-    if (!small.empty()) {
-      double tb = 0;
-      for (auto& d : small) tb += 64.0 * (double)(1u << d.log);
-      KScope ks(ctx, K_FRI_TAIL, tb);
-      launch_fri_small(ctx.upload_vec(small), (uint32_t)small.size(), dstate, ctx.stream);
-    }
-    layer = n_layers;
-  } else {
+  for (auto& f : fri_launches) f();
+  if (cfg.fold_step != 1) {
     const qm31* alpha0 = mix_and_draw(first, 0);
     cur.log = Lc - 1;
     for (int k = 0; k < 4; ++k) cur.c.emplace_back(&ctx, (size_t)1 << cur.log);
