@@ -145,6 +145,8 @@ struct Ctx {
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
   void d2h(void* dst, const void* src, size_t bytes);  // synchronous
+  uint8_t* d2h_pinned = nullptr;  // read-backs of up to d2h_cap bytes land here first
+  static constexpr size_t d2h_cap = 4u << 20;
   // Pinned host memory for asynchronous read-backs on the main stream (kept and grown as needed;
   // the contents are valid after the next sync).
   uint8_t* rb_pinned = nullptr;
