@@ -388,6 +388,24 @@ static void words_from_bytes(const uint8_t* b, uint32_t m[16]) {
 }
 void Blake2sHost::update(const void* data, size_t len) {
   const uint8_t* p = (const uint8_t*)data;
+  // whole blocks straight from the input while more than one block remains (the last block is
+  // held back: it may be the final one)
+  if (buflen == 64 && len) {
+    t += 64;
+    uint32_t m[16];
+    words_from_bytes(buf, m);
+    blake2s_compress(h, m, (uint32_t)t, false);
+    buflen = 0;
+  }
+  if (buflen == 0)
+    while (len > 64) {
+      t += 64;
+      uint32_t m[16];
+      words_from_bytes(p, m);
+      blake2s_compress(h, m, (uint32_t)t, false);
+      p += 64;
+      len -= 64;
+    }
   while (len) {
     if (buflen == 64) {
       t += 64;
@@ -425,8 +443,11 @@ void Channel::mix_u32s(const uint32_t* data, size_t n) {
   update_digest(b.finalize());
 }
 void Channel::mix_felts(const std::vector<qm31>& f) {
-  Blake2sHost b; b.update(digest.data(), 32);
-  for (auto& q : f) { b.update_u32(q.a); b.update_u32(q.b); b.update_u32(q.c); b.update_u32(q.d); }
+  // the felts' words in order, little endian (the host's order): the vector's bytes as they are
+  static_assert(sizeof(qm31) == 16, "qm31 is 4 packed words");
+  Blake2sHost b;
+  b.update(digest.data(), 32);
+  b.update(f.data(), f.size() * sizeof(qm31));
   update_digest(b.finalize());
 }
 std::array<uint32_t, 8> Channel::draw_u32s() {
