@@ -430,7 +430,8 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
                                                          const qm31* __restrict__ factors,
                                                          const qm31* __restrict__ Wall, qm31* __restrict__ partials,
                                                          uint32_t max_chunks) {
-  __shared__ qm31 red[4][OODS_BATCH];
+  __shared__ uint32_t red[4 * OODS_BATCH][256 + 8];  // padded rows: the 8 words a wave sums at once sit in distinct banks
+  __shared__ uint32_t fin[4 * OODS_BATCH];
   const uint32_t t0 = batch_start[blockIdx.y], cnt = batch_start[blockIdx.y + 1] - t0;
   const uint32_t grp = task_group[t0];
   const uint32_t k = tasks[t0].log_size;
@@ -475,22 +476,35 @@ __global__ void __launch_bounds__(256) oods_chunk_kernel(const OodsTask* __restr
     for (uint32_t c = 0; c < OODS_BATCH; ++c)
       if (c < cnt) qacc_mad(acc[c], W[t], col[c][t]);
   }
-  const qm31* f = factors + (size_t)grp * 32;
+  // Block sum of each of the 4 OODS_BATCH accumulator words: every lane folds its words below
+  // 2^31 + 8 into LDS (word-major), then 8 lanes per word add 32 values each in 64 bits and meet
+  // by shuffles (a shuffle reduction of every word over the whole block cost more than the
+  // accumulation itself).
 #pragma unroll
   for (uint32_t c = 0; c < OODS_BATCH; ++c) {
-    if (c >= cnt) break;
-    qm31 out = qacc_reduce(acc[c]);
-    // wave reduction by shuffles, then across the 4 waves
-    for (uint32_t off = 32; off > 0; off >>= 1) {
-      qm31 o = {(uint32_t)__shfl_down((int)out.a, off), (uint32_t)__shfl_down((int)out.b, off),
-                (uint32_t)__shfl_down((int)out.c, off), (uint32_t)__shfl_down((int)out.d, off)};
-      out = qm_add(out, o);
+    red[4 * c + 0][t] = (uint32_t)fold31(fold31(acc[c].a));
+    red[4 * c + 1][t] = (uint32_t)fold31(fold31(acc[c].b));
+    red[4 * c + 2][t] = (uint32_t)fold31(fold31(acc[c].c));
+    red[4 * c + 3][t] = (uint32_t)fold31(fold31(acc[c].d));
+  }
+  __syncthreads();
+  {
+    const uint32_t w = t >> 3, j = t & 7;  // accumulator word w = 4 c + coordinate
+    uint64_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 32; ++i) sum += red[w][(i << 3) | j];
+#pragma unroll
+    for (uint32_t off = 4; off > 0; off >>= 1) {
+      const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)sum, off, 8);
+      const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(sum >> 32), off, 8);
+      sum += ((uint64_t)hi << 32) | lo;
     }
-    if ((t & 63) == 0) red[t >> 6][c] = out;
+    if (j == 0) fin[w] = m31_reduce64(sum);
   }
   __syncthreads();
   if (t < cnt) {
-    qm31 s = qm_add(qm_add(red[0][t], red[1][t]), qm_add(red[2][t], red[3][t]));
+    const qm31* f = factors + (size_t)grp * 32;
+    qm31 s = {fin[4 * t], fin[4 * t + 1], fin[4 * t + 2], fin[4 * t + 3]};
     for (uint32_t b = OODS_TABLE_LOG; b < k; ++b)
       if ((base >> b) & 1) s = qm_mul(s, f[b]);
     partials[(size_t)(t0 + t) * max_chunks + blk] = s;
