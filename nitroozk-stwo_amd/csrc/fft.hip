@@ -185,10 +185,29 @@ __device__ __forceinline__ void round_to_lds(const uint32_t* v, uint32_t* lds) {
 }
 // Global elements gbase | e (index masked with in_mask) of the round's slots. At A = 0 a thread's
 // 32 elements are consecutive: 16-byte loads when the column allows them.
+// Rounds whose lanes own the 2^A low bits of the tile index (A = 10 with 1024 lanes): element q
+// of lane t is the tile's word t + q 2^A, so a buffer access takes the lane offset in a vector
+// register and the element offset in a scalar one, with no vector address arithmetic.
+template <int S, int A> constexpr bool LANE_LOW_BITS = LowShape<S>::X == 0 && LowShape<S>::T == (1u << A);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t column_rsrc(const void* p, uint32_t words) {
+  // raw buffer over the column (stride 0, 32-bit data format); accesses past `words` read 0
+  const uint32_t bytes = words >= (1u << 30) ? 0xffffffffu : words * 4u;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
 template <int S, int A, int R>
 __device__ __forceinline__ void round_from_global(uint32_t* v, const GLOBAL_AS uint32_t* __restrict__ in, uint32_t gbase,
                                                   uint32_t in_mask, bool vec) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+  if constexpr (LANE_LOW_BITS<S, A>) {
+    // (gbase | t | q 2^A) & in_mask with in_mask >= 2^S - 1: the tile's base is gbase & in_mask
+    const __amdgpu_buffer_rsrc_t rs = column_rsrc((const void*)in, in_mask + 1);
+    const uint32_t base = __builtin_amdgcn_readfirstlane(gbase & in_mask);
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q)
+      v[q] = __builtin_amdgcn_raw_buffer_load_b32(rs, threadIdx.x * 4u, (base + (q << A)) * 4u, 0);
+    return;
+  }
   if (A == 0 && R == 5 && vec) {
 #pragma unroll
     for (uint32_t g = 0; g < G; ++g) {
@@ -208,11 +227,19 @@ __device__ __forceinline__ void round_from_global(uint32_t* v, const GLOBAL_AS u
 }
 template <int S, int A, int R>
 __device__ __forceinline__ void round_to_global(uint32_t* v, GLOBAL_AS uint32_t* __restrict__ out, uint32_t gbase,
-                                                uint32_t scale, bool vec) {
+                                                uint32_t scale, bool vec, uint32_t out_words) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
   if (scale) {
 #pragma unroll
     for (uint32_t i = 0; i < LowShape<S>::V; ++i) v[i] = mul_tw(v[i], scale);
+  }
+  if constexpr (LANE_LOW_BITS<S, A>) {
+    const __amdgpu_buffer_rsrc_t rs = column_rsrc((const void*)out, out_words);
+    const uint32_t base = __builtin_amdgcn_readfirstlane(gbase);
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q)
+      __builtin_amdgcn_raw_buffer_store_b32(v[q], rs, threadIdx.x * 4u, (base + (q << A)) * 4u, 0);
+    return;
   }
   if (A == 0 && R == 5 && vec) {
 #pragma unroll
@@ -302,7 +329,7 @@ __global__ void __launch_bounds__(LowShape<S>::T) fft_low_kernel(const uint32_t*
   __syncthreads();
   round_from_lds<S, A3, R3>(v, lds);
   round_butterflies<INV, S, A3, R3>(v, gbase, tree, tree_len, log_n);
-  round_to_global<S, A3, R3>(v, out, gbase, scale, (reinterpret_cast<uintptr_t>(out) & 15) == 0);
+  round_to_global<S, A3, R3>(v, out, gbase, scale, (reinterpret_cast<uintptr_t>(out) & 15) == 0, 1u << log_n);
 }
 
 template <bool INV>
