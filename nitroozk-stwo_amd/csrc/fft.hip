@@ -167,9 +167,21 @@ __device__ __forceinline__ uint32_t round_elem(uint32_t t, uint32_t g, uint32_t 
   return ((rho >> A) << (A + R)) | (rho & ((1u << A) - 1)) | (q << A);
 }
 
+// With one group per lane and A >= 5, slot q's element is the lane's first element plus q 2^A
+// (no carries into the lane's bits), and its padded LDS word is the first one's plus
+// q (2^A + 2^(A-5)): one base address per lane and immediate offsets, instead of three
+// instructions of index arithmetic per slot.
+template <int S, int A, int R>
+constexpr bool LDS_AFFINE = A >= 5 && ((1u << (5 - R)) << LowShape<S>::X) == 1;
 template <int S, int A, int R>
 __device__ __forceinline__ void round_from_lds(uint32_t* v, const uint32_t* lds) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+  if constexpr (LDS_AFFINE<S, A, R>) {
+    const uint32_t* b = lds + lds_pad(round_elem<S, A, R>(threadIdx.x, 0, 0));
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) v[q] = b[q * ((1u << A) + (1u << (A - 5)))];
+    return;
+  }
 #pragma unroll
   for (uint32_t g = 0; g < G; ++g)
 #pragma unroll
@@ -178,6 +190,12 @@ __device__ __forceinline__ void round_from_lds(uint32_t* v, const uint32_t* lds)
 template <int S, int A, int R>
 __device__ __forceinline__ void round_to_lds(const uint32_t* v, uint32_t* lds) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
+  if constexpr (LDS_AFFINE<S, A, R>) {
+    uint32_t* b = lds + lds_pad(round_elem<S, A, R>(threadIdx.x, 0, 0));
+#pragma unroll
+    for (uint32_t q = 0; q < Q; ++q) b[q * ((1u << A) + (1u << (A - 5)))] = v[q];
+    return;
+  }
 #pragma unroll
   for (uint32_t g = 0; g < G; ++g)
 #pragma unroll
