@@ -294,11 +294,12 @@ __device__ __forceinline__ void round_butterflies(uint32_t* v, uint32_t gbase, c
       if (A == 0 && k == 0) {
         // stage 0: pair h = (idx_base >> 1) | u, whose quarter u & 3 is known here, so the negated
         // circle twiddles become swapped additions
-        const uint32_t* lt0 = tree + tree_len - (1u << (log_n - 1));
+        // pair h = (idx_base >> 1) + u (idx_base's low 5 bits are zero): its [x, y] twiddle pair is
+        // at 2 (h >> 2) = 2 (idx_base >> 3) + 2 (u >> 2)
+        const uint32_t* lb = tree + tree_len - (1u << (log_n - 1)) + 2 * (idx_base >> 3);
 #pragma unroll
         for (uint32_t u = 0; u < (Q >> 1); u += 4) {
-          const uint32_t h = (idx_base >> 1) | u;
-          const uint32_t x = lt0[(h >> 2) * 2], y = lt0[(h >> 2) * 2 + 1];
+          const uint32_t x = lb[2 * (u >> 2)], y = lb[2 * (u >> 2) + 1];
           bfly<INV, false>(v[g * Q + 2 * u], v[g * Q + 2 * u + 1], y);
           bfly<INV, true>(v[g * Q + 2 * u + 2], v[g * Q + 2 * u + 3], y);
           bfly<INV, true>(v[g * Q + 2 * u + 4], v[g * Q + 2 * u + 5], x);
@@ -306,9 +307,16 @@ __device__ __forceinline__ void round_butterflies(uint32_t* v, uint32_t gbase, c
         }
         continue;
       }
+      // stage s >= 1: the twiddle of block u is tree[len - 2^(log_n - s) + (idx_base >> (s + 1)) + u]
+      // (idx_base is zero on the round's bits, so u adds without carries): one base per stage
+      // (when the lanes' bits all lie below bit A, the base is the same for the whole workgroup and
+      // the twiddles come through the scalar cache)
+      constexpr bool UNIFORM_TW = T * G <= (1u << A);
+      const uint32_t toff = UNIFORM_TW ? __builtin_amdgcn_readfirstlane(idx_base >> (s + 1)) : idx_base >> (s + 1);
+      const uint32_t* tb = tree + tree_len - (1u << (log_n - s)) + toff;
 #pragma unroll
       for (uint32_t u = 0; u < (Q >> (k + 1)); ++u) {
-        const uint32_t tw = stage_twiddle(tree, tree_len, log_n, s, idx_base | ((u << (k + 1)) << A));
+        const uint32_t tw = tb[u];
 #pragma unroll
         for (uint32_t l = 0; l < (1u << k); ++l) {
           const uint32_t q = (u << (k + 1)) | l;
