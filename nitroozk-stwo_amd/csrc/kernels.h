@@ -27,7 +27,7 @@ void launch_merkle_layer_strided(uint32_t log_size, const uint32_t* prev, const 
 // The same for at most 16 columns, given as a host array of device addresses.
 void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const uint32_t* const* host_cols,
                                uint32_t n_cols, uint32_t* out, hipStream_t stream);
-// `levels` (<= 8, <= log_child) column-free levels above a layer of 2^log_child nodes; outs[j]
+// `levels` (<= 9, <= log_child) column-free levels above a layer of 2^log_child nodes; outs[j]
 // receives level log_child - 1 - j.
 void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t levels, uint32_t* const* outs,
                            hipStream_t stream);

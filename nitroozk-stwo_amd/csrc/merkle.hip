@@ -10,6 +10,7 @@
 // Grind (prover/backend/simd/grind.rs:20-91): the nonce is the smallest (hi << 32) | lo with
 // lo < 2^20 such that the first digest word of H(prefix || lo || hi) has >= pow_bits trailing
 // zeros, prefix = H(0x12345678 || 0^12 || digest || pow_bits).
+#include <stdexcept>
 #include "blake2s.h"
 #include "kernels.h"
 
@@ -152,16 +153,17 @@ void launch_merkle_layer_small(uint32_t log_size, const uint32_t* prev, const ui
                      cp, n_cols, out);
 }
 
-// Up to 8 levels of a Merkle tree with no injected columns in one launch. Workgroup w reads the
+// Up to 9 levels of a Merkle tree with no injected columns in one launch. Workgroup w reads the
 // 2^L children [w 2^L, (w+1) 2^L) of level log_child and writes its 2^(L-j) nodes of level
 // log_child - j (j = 1..L); the levels in between stay in LDS. The levels are latency-bound (each
 // waits for the one below), so every node is hashed by a quad of lanes (blake2s_compress_quad):
 // 4 * 2^(L-1) threads, lane q of a quad holding chaining words q and 4 + q.
-struct LayerPtrs { uint32_t* p[8]; };
-__global__ void __launch_bounds__(512) merkle_subtree_kernel(const uint32_t* __restrict__ child, uint32_t L,
-                                                             LayerPtrs outs) {
+constexpr uint32_t SUBTREE_MAX_LEVELS = 9;
+struct LayerPtrs { uint32_t* p[SUBTREE_MAX_LEVELS]; };
+__global__ void __launch_bounds__(1024) merkle_subtree_kernel(const uint32_t* __restrict__ child, uint32_t L,
+                                                              LayerPtrs outs) {
   CRITICAL_PATH_WAVE();
-  __shared__ uint32_t lds[128 * 8];
+  __shared__ uint32_t lds[(1u << (SUBTREE_MAX_LEVELS - 1)) * 8];
   const uint32_t w = blockIdx.x, nd = threadIdx.x >> 2, q = threadIdx.x & 3;
   uint32_t n = 1u << (L - 1);  // nodes of the first level computed by this workgroup
   for (uint32_t j = 0; j < L; ++j, n >>= 1) {
@@ -196,7 +198,8 @@ __global__ void __launch_bounds__(512) merkle_subtree_kernel(const uint32_t* __r
 
 void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t levels, uint32_t* const* outs,
                            hipStream_t stream) {
-  // levels <= 8 and <= log_child; outs[j] receives level log_child - 1 - j
+  // levels <= SUBTREE_MAX_LEVELS and <= log_child; outs[j] receives level log_child - 1 - j
+  if (levels == 0 || levels > SUBTREE_MAX_LEVELS || levels > log_child) throw std::invalid_argument("merkle subtree: levels");
   LayerPtrs lp{};
   for (uint32_t j = 0; j < levels; ++j) lp.p[j] = outs[j];
   uint32_t wgs = 1u << (log_child - levels);

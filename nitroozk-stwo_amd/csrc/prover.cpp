@@ -511,9 +511,9 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
   int ls = (int)max_log;
   while (ls >= 0) {
     if (!rev.empty() && !has_cols[ls]) {
-      // run of levels without injected columns: fused, up to 8 levels per launch
+      // run of levels without injected columns: fused, up to 9 levels per launch
       uint32_t levels = 0;
-      while ((int)levels < 8 && ls - (int)levels >= 0 && !has_cols[ls - levels]) ++levels;
+      while ((int)levels < 9 && ls - (int)levels >= 0 && !has_cols[ls - levels]) ++levels;
       uint32_t log_child = (uint32_t)ls + 1;
       std::vector<uint32_t*> outs;
       for (uint32_t j = 0; j < levels; ++j) {
