@@ -73,10 +73,12 @@ SHD void blake2s_compress(uint32_t h[8], const uint32_t m[16], uint32_t t, bool 
 // (v[j], v[4+j], v[8+j], v[12+j]) and runs one G of each step; the diagonal step reads rows
 // b, c, d rotated by 1, 2, 3 lanes (DPP quad permutations), then rotates them back. On entry and
 // exit lane j holds chaining words hA = h[j] and hB = h[4+j]; every lane holds the full message.
-// All 4 lanes of the quad must be active.
+// All 4 lanes of the quad must be active. The permutations set bound_ctrl: they never read a lane
+// outside the quad, so it changes no value, and with it the compiler folds most of the moves into
+// the add or xor that consumes them (DPP operand), about 38 fewer instructions per compression.
 template <int CTRL>
 __device__ __forceinline__ uint32_t quad_rot(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 #define B2Q_ROT1 0x39  // lane i reads lane (i + 1) % 4
 #define B2Q_ROT2 0x4E  // lane i reads lane (i + 2) % 4

@@ -418,6 +418,9 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
     for (uint32_t e = S; e < k;) {
       uint32_t st = k - e < 8 ? k - e : 8, c = 12 - st;
       if (c > e) c = e;
+      // few columns (the composition's 4 or 8): narrower tiles, so that the pass still spreads
+      // over at least 2048 workgroups; 2^6 contiguous words keep every wave's access one segment
+      while (c > 6 && ((size_t)n_cols << (log_n - st - c)) < 2048) --c;
       high.push_back({e, st, c});
       e += st;
     }

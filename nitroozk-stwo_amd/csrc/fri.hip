@@ -30,11 +30,11 @@ constexpr uint32_t FL_THREADS = 256;
 // Lane k of the quad, broadcast to the 4 lanes (DPP quad_perm [k, k, k, k]).
 template <int K>
 __device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, K * 0x55, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, K * 0x55, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ uint32_t quad_perm(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 
 // An empty asm that keeps a value in a VGPR: a lane-dependent choice among such values stays a
@@ -249,6 +249,8 @@ __global__ void __launch_bounds__(256) fri_layer_kernel(FriLayerDesc d, uint32_t
       last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1 ? 1u : 0u;
     __syncthreads();
     if (!last) return;
+    // every other workgroup has taken its ticket: the counter goes back to zero for the next launch
+    if (t == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // the chunk roots, read with read-modify-write atomics of the same scope (performed where the
     // exchanges were); the operand is opaque so that they are not turned into plain loads
     const uint32_t R = gridDim.x, lev = l - c;
@@ -269,9 +271,10 @@ __global__ void __launch_bounds__(256) fri_layer_kernel(FriLayerDesc d, uint32_t
     for (int k = 0; k < 8; ++k) root[k] = top[k];
     uint32_t st[9], a[4];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) st[k] = state[k];
+    for (int k = 0; k < 9; ++k) st[k] = d.init ? d.st0[k] : state[k];
     fl_channel(st, root, t, a);
     fl_publish(t, st, root, a, d.rec, state, nullptr);
+    if (d.init && t < 2) state[14 + t] = ~0u;
   }
 }
 

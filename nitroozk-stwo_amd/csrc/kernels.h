@@ -62,7 +62,9 @@ struct FriLayerDesc {
   uint32_t log;     // the layer holds 2^log values
   uint32_t mode;    // FRI_LEAVES: the layer is src itself; folds: src is the layer above (2^(log+1) values)
   uint32_t commit;  // 0 for the last layer: folded only
-  uint32_t pad;
+  uint32_t init;    // nonzero on the launch that opens the commit phase (fri_layer_kernel only): the
+                    // channel state comes from st0, and the launch sets the proof-of-work result
+                    // slot (state words 14 and 15) to all ones
   CCols4 src;
   Cols4 dst;                // folds: the layer's 4 coordinate columns
   const uint32_t* tw;       // FRI_FOLD_CIRCLE: line layer 0 of the inverse twiddle tree of the layer above
@@ -70,9 +72,10 @@ struct FriLayerDesc {
   const uint32_t* alpha;    // folds: the 4 words of the folding coefficient drawn for the layer above
   uint32_t* nodes;          // Merkle heap: level j (2^j nodes of 8 words) at 8 (2^j - 1) words, 16-byte aligned
   uint32_t* rec;            // 12 words: root, then the coefficient drawn after it
+  uint32_t st0[9];          // init: channel digest (8 words) and n_draws at the start of the phase
 };
 // One layer, 9 <= log <= 18, over 2^(log - c) workgroups (c = max(8, log - 8)); *ticket is zero
-// before the launch. state: channel digest (8 words) and n_draws.
+// before the launch, and the launch leaves it at zero. state: channel digest (8 words) and n_draws.
 void launch_fri_layer(const FriLayerDesc& d, uint32_t* state, uint32_t* ticket, hipStream_t stream);
 // Layers of at most 2^8 values in one workgroup, in order (descs in device memory); every entry
 // after the first is a fold of the one before it. Only the last entry may have commit == 0.

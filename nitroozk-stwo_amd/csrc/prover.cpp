@@ -15,6 +15,7 @@
 #include "prover.h"
 
 #include <algorithm>
+#include <array>
 #include <functional>
 #include <chrono>
 #include <cstdio>
@@ -1153,13 +1154,18 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     l -= fs;
   }
   const size_t last_n = (size_t)1 << last_log;
-  // device block: channel state (16 words), one ticket counter per layer (zero), then the
-  // transcript records (12 words per layer) and the last layer's values
-  const size_t n_tickets = ((size_t)n_layers + 3) & ~(size_t)3;
-  DevBuf chan(&ctx, 16 + n_tickets + 12 * (size_t)n_layers + 4 * last_n);
+  // device block: channel state (9 words; words 14 and 15 hold the proof-of-work result), the
+  // transcript records (12 words per layer) and the last layer's values, which the last fold
+  // writes in place, so that one read-back takes both
+  DevBuf chan(&ctx, 16 + 12 * (size_t)n_layers + 4 * last_n);
   uint32_t* dstate = chan.p;
-  uint32_t* dtickets = chan.p + 16;
-  uint32_t* drec = dtickets + n_tickets;
+  uint32_t* dtickets = ctx.fri_ticket_words(n_layers);
+  uint32_t* drec = chan.p + 16;
+  uint32_t* dlast = drec + 12 * (size_t)n_layers;
+  // the channel state at the start of the phase goes to the first layer launch as an argument when
+  // that launch is a fused layer (fri_st0_by_value), else it is uploaded to dstate
+  std::array<uint32_t, 9> fri_st0{};
+  bool fri_st0_by_value = false;
   auto mix_and_draw = [&ctx, dstate, drec](FriCol& f, uint32_t layer) {
     f.tree.evals = {f.c[0].p, f.c[1].p, f.c[2].p, f.c[3].p};
     f.tree.eval_logs.assign(4, f.log);
@@ -1225,7 +1231,9 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
           inner.push_back(line_col(l));
           cf = &inner.back();
         } else {
-          cur = line_col(l);
+          cur.log = l;
+          cur.fold_step = 1;
+          for (int k = 0; k < 4; ++k) cur.c.push_back(DevBuf::view(&ctx, dlast + ((size_t)k << l), (size_t)1 << l));
           cf = &cur;
         }
       }
@@ -1245,9 +1253,15 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
       if (l <= SMALL_MAX) {
         small.push_back(d);
       } else if (commit && l <= FAST_MAX) {
-        fri_launches.push_back([&ctx, d, l, li, dstate, dtickets] {
+        if (li == 0) fri_st0_by_value = true;
+        fri_launches.push_back([&ctx, d, l, li, dstate, dtickets, &fri_st0] {
           KScope ks(ctx, K_FRI_FOLD, (li == 0 ? 16.0 : 48.0) * (double)(1u << l) + 64.0 * (double)(1u << l));
-          launch_fri_layer(d, dstate, dtickets + li, ctx.stream);
+          FriLayerDesc dd = d;
+          if (li == 0) {
+            dd.init = 1;
+            std::copy(fri_st0.begin(), fri_st0.end(), dd.st0);
+          }
+          launch_fri_layer(dd, dstate, dtickets + li, ctx.stream);
         });
       } else {
         if (li > 0) {
@@ -1314,13 +1328,16 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
 
   // ---------------- FRI commit ----------------
   {
-    std::vector<uint32_t> st(16 + n_tickets, 0);
     for (int i = 0; i < 8; ++i)
-      st[i] = (uint32_t)ch.digest[4 * i] | (uint32_t)ch.digest[4 * i + 1] << 8 |
-              (uint32_t)ch.digest[4 * i + 2] << 16 | (uint32_t)ch.digest[4 * i + 3] << 24;
-    st[8] = ch.n_draws;
-    st[14] = st[15] = ~0u;  // the proof-of-work result slot (u64), empty
-    launch_copy_words(dstate, ctx.upload_vec(st), st.size(), ctx.stream);
+      fri_st0[i] = (uint32_t)ch.digest[4 * i] | (uint32_t)ch.digest[4 * i + 1] << 8 |
+                   (uint32_t)ch.digest[4 * i + 2] << 16 | (uint32_t)ch.digest[4 * i + 3] << 24;
+    fri_st0[8] = ch.n_draws;
+    if (!fri_st0_by_value) {
+      std::vector<uint32_t> st(16, 0);
+      std::copy(fri_st0.begin(), fri_st0.end(), st.begin());
+      st[14] = st[15] = ~0u;  // the proof-of-work result slot (u64), empty
+      launch_copy_words(dstate, ctx.upload_vec(st), st.size(), ctx.stream);
+    }
   }
   for (auto& f : fri_launches) f();
   if (cfg.fold_step != 1) {
@@ -1401,8 +1418,8 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   HMARK("sanity");
   {
     // one read-back: the transcript records and the last layer's evaluations
-    uint32_t* dlast = drec + 12 * n_layers;
-    for (int k = 0; k < 4; ++k) launch_copy_words(dlast + k * last_n, cur.c[k].p, last_n, ctx.stream);
+    for (int k = 0; k < 4; ++k)
+      if (cur.c[k].p != dlast + k * last_n) launch_copy_words(dlast + k * last_n, cur.c[k].p, last_n, ctx.stream);
     std::vector<uint32_t> hb(12 * (size_t)n_layers + 4 * last_n);
     ctx.d2h(hb.data(), drec, hb.size() * 4);
     ctx.reset_arena();

@@ -82,6 +82,7 @@ Ctx::~Ctx() {
   tw2.reset();
   itw2.reset();
   domain_points.clear();
+  fri_tickets.reset();
   for (auto& [bytes, v] : col_cache)
     for (void* p : v) (void)hipFreeAsync(p, stream);
   for (auto& [p, bytes] : col_live) (void)hipFreeAsync(p, stream);
@@ -331,6 +332,14 @@ cpm31 canonic_domain_at(uint32_t log, size_t i) {
   size_t h = (size_t)1 << (log - 1);
   if (i < h) return hc.at(i);
   return index_to_point((0x80000000u - hc.index_at(i - h)) & 0x7fffffffu);
+}
+
+uint32_t* Ctx::fri_ticket_words(size_t n) {
+  if (fri_tickets.n < n) {
+    PersistentAllocScope keep(*this);
+    fri_tickets = DevBuf(this, n < 64 ? 64 : n, true);
+  }
+  return fri_tickets.p;
 }
 
 const uint32_t* Ctx::domain_xy(uint32_t log) {

@@ -103,6 +103,10 @@ struct Ctx {
   DevBuf tw, itw;
   DevBuf tw2, itw2;  // the same trees with every entry doubled (2t < 2^32), read by the FFT kernels
   std::map<uint32_t, DevBuf> domain_points;  // bit-reversed (x, y) of CanonicCoset(log)
+  // Ticket counters of the fused FRI layer launches, one per layer: zeroed once here, and every
+  // launch leaves its counter at zero again.
+  DevBuf fri_tickets;
+  uint32_t* fri_ticket_words(size_t n);
   // small-upload arena (device) and pinned staging (host)
   uint8_t* pinned = nullptr;
   size_t pinned_cap = 0, pinned_off = 0;
