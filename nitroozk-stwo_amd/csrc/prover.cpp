@@ -561,11 +561,14 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
     ctx.ensure_twiddles(lg);
     std::vector<const uint32_t*> src;
     std::vector<uint32_t*> dst;
-    for (size_t i : idx) {
-      into.owned.emplace_back(&ctx, (size_t)1 << lg);
+    // the group's columns are consecutive in one allocation
+    into.owned.emplace_back(&ctx, idx.size() << lg);
+    uint32_t* blk = into.owned.back().p;
+    for (size_t j = 0; j < idx.size(); ++j) {
+      const size_t i = idx[j];
       src.push_back(evals[i]);
-      dst.push_back(into.owned.back().p);
-      into.coeffs[base + i] = into.owned.back().p;
+      dst.push_back(blk + (j << lg));
+      into.coeffs[base + i] = dst.back();
       into.coeff_logs[base + i] = lg;
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
@@ -586,11 +589,14 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
     ctx.ensure_twiddles(K);
     std::vector<const uint32_t*> src;
     std::vector<uint32_t*> dst;
-    for (size_t i : idx) {
-      t.owned.emplace_back(&ctx, (size_t)1 << K);
+    // the group's columns are consecutive in one allocation
+    t.owned.emplace_back(&ctx, idx.size() << K);
+    uint32_t* blk = t.owned.back().p;
+    for (size_t j = 0; j < idx.size(); ++j) {
+      const size_t i = idx[j];
       src.push_back(t.coeffs[i]);
-      dst.push_back(t.owned.back().p);
-      t.evals[i] = t.owned.back().p;
+      dst.push_back(blk + (j << K));
+      t.evals[i] = dst.back();
       t.eval_logs[i] = K;
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
