@@ -16,6 +16,7 @@
 // Wide levels hash one node per lane (Blake2s is issue-bound: one wave per SIMD already reaches
 // peak throughput); levels of at most 64 parents hash one node per quad of lanes, which has about
 // half the latency of one lane.
+#include <stdexcept>
 #include <type_traits>
 #include "blake2s.h"
 #include "field.h"
@@ -349,6 +350,64 @@ __global__ void __launch_bounds__(256) fri_small_kernel(const FriLayerDesc* __re
     __syncthreads();
   }
   if (t < 9) state[t] = st_s[t];
+}
+
+// The last layer's interpolation, the channel mix of the last polynomial and the proof-of-work
+// prefix (see launch_fri_last_pow). The butterflies follow line_interpolate on the host: the values
+// in natural order, then per level (a, b) -> (a + b, (a - b) x^-1) on pairs half a level apart.
+__global__ void __launch_bounds__(64) fri_last_pow_kernel(const uint32_t* __restrict__ vals, uint32_t L, uint32_t BL,
+                                                          const uint32_t* __restrict__ xi, uint32_t* __restrict__ state,
+                                                          uint32_t pow_bits, uint32_t* __restrict__ prefix) {
+  __shared__ uint32_t W[4][64];
+  __shared__ uint32_t M[8 + 4 * 16];  // the digest, then the last polynomial's coefficients
+  const uint32_t t = threadIdx.x, n = 1u << L, bound = 1u << BL;
+  if (t < n)
+    for (int k = 0; k < 4; ++k) W[k][t] = gmem(vals)[k * n + bitrev(t, L)];
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t ds = n; ds > 1; ds >>= 1) {
+    const uint32_t half = ds >> 1;
+    if (t < n / 2) {
+      const uint32_t ch = (t / half) * ds, i = t % half, x = gmem(xi)[off + i];
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t a = W[k][ch + i], b = W[k][ch + half + i];
+        W[k][ch + i] = m31_add(a, b);
+        W[k][ch + half + i] = m31_mul(m31_sub(a, b), x);
+      }
+    }
+    off += half;
+    __syncthreads();
+  }
+  // coefficient j of the last polynomial is the interpolant's coefficient bitrev_L(bitrev_BL(j))
+  if (t < bound) {
+    const uint32_t src = bitrev(bitrev(t, BL), L), inv_n = gmem(xi)[n - 1];
+    for (int k = 0; k < 4; ++k) M[8 + 4 * t + k] = m31_mul(W[k][src], inv_n);
+  }
+  if (t < 8) M[t] = state[t];
+  __syncthreads();
+  if (t != 0) return;
+  const uint32_t nbytes = 32 + 16 * bound;
+  uint32_t h[8];
+  blake2s_init(h);
+  for (uint32_t o = 0; o < nbytes; o += 64) {
+    uint32_t m[16];
+    for (int k = 0; k < 16; ++k) m[k] = o + 4 * k < nbytes ? M[o / 4 + k] : 0u;
+    const bool last = o + 64 >= nbytes;
+    blake2s_compress(h, m, last ? nbytes : o + 64, last);
+  }
+  uint32_t m[16] = {0x12345678u, 0, 0, 0, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], pow_bits, 0, 0, 0};
+  for (int k = 0; k < 8; ++k) state[k] = h[k];
+  state[8] = 0;
+  uint32_t pre[8];
+  blake2s_init(pre);
+  blake2s_compress(pre, m, 52, true);
+  for (int k = 0; k < 8; ++k) prefix[k] = pre[k];
+}
+
+void launch_fri_last_pow(const uint32_t* vals, uint32_t L, uint32_t BL, const uint32_t* xi, uint32_t* state,
+                         uint32_t pow_bits, uint32_t* prefix, hipStream_t stream) {
+  if (L > 6 || BL > L || BL > 4) throw std::invalid_argument("launch_fri_last_pow: last layer too large");
+  hipLaunchKernelGGL(fri_last_pow_kernel, dim3(1), dim3(64), 0, stream, vals, L, BL, xi, state, pow_bits, prefix);
 }
 
 void launch_fri_layer(const FriLayerDesc& d, uint32_t* state, uint32_t* ticket, hipStream_t stream) {

@@ -37,6 +37,10 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
 struct GrindPrefix { uint32_t w[8]; };
 void launch_grind(const GrindPrefix& prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream);
+// The same search with the prefix read from device memory (8 words), for a prefix that an earlier
+// kernel on the stream computed.
+void launch_grind_dev(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
+                      unsigned long long* result, hipStream_t stream);
 
 // ops.hip
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
@@ -79,6 +83,14 @@ struct FriLayerDesc {
 void launch_fri_layer(const FriLayerDesc& d, uint32_t* state, uint32_t* ticket, hipStream_t stream);
 // Layers of at most 2^8 values in one workgroup, in order (descs in device memory); every entry
 // after the first is a fold of the one before it. Only the last entry may have commit == 0.
+// The FRI commit phase's end on the device (prover/fri.rs:215-238, core/proof_of_work.rs): the
+// last layer's 2^L values (4 coordinate columns at vals) are interpolated into line coefficients
+// (xi: the inverse x-coordinates of the line domain's points, level by level, 2^L - 1 words, then
+// the inverse of 2^L), the 2^BL coefficients of the last polynomial are mixed into the channel
+// state, and the proof-of-work prefix for pow_bits is written to prefix (8 words). state gets the
+// mixed digest and n_draws = 0. L <= 6, BL <= min(L, 4); one workgroup of 64 threads.
+void launch_fri_last_pow(const uint32_t* vals, uint32_t L, uint32_t BL, const uint32_t* xi, uint32_t* state,
+                         uint32_t pow_bits, uint32_t* prefix, hipStream_t stream);
 void launch_fri_small(const FriLayerDesc* descs_dev, uint32_t n, uint32_t* state, hipStream_t stream);
 
 // Fused accumulate_numerators for one size group. Entries of batch b are

@@ -210,8 +210,8 @@ void launch_merkle_subtree(uint32_t log_child, const uint32_t* child, uint32_t l
 
 // Candidates are numbered g = hi * 2^20 + lo (lo < 2^20), i.e. in the order of the smallest-nonce
 // search; one launch covers [start, start + count).
-__global__ void __launch_bounds__(256) grind_kernel(GrindPrefix prefix, uint32_t pow_bits, uint64_t start,
-                                                    uint32_t count, unsigned long long* __restrict__ result) {
+__device__ __forceinline__ void grind_candidate(const uint32_t* pw, uint32_t pow_bits, uint64_t start, uint32_t count,
+                                                unsigned long long* __restrict__ result) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   uint64_t g = start + i;
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256) grind_kernel(GrindPrefix prefix, uint32_t
   uint32_t hi = (uint32_t)(g >> 20);
   uint32_t m[16];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) m[k] = prefix.w[k];
+  for (int k = 0; k < 8; ++k) m[k] = pw[k];
   m[8] = lo;
   m[9] = hi;
 #pragma unroll
@@ -230,11 +230,28 @@ __global__ void __launch_bounds__(256) grind_kernel(GrindPrefix prefix, uint32_t
   uint32_t tz = h[0] ? __builtin_ctz(h[0]) : 32u;
   if (tz >= pow_bits) atomicMin(result, ((unsigned long long)hi << 32) | lo);
 }
+__global__ void __launch_bounds__(256) grind_kernel(GrindPrefix prefix, uint32_t pow_bits, uint64_t start,
+                                                    uint32_t count, unsigned long long* __restrict__ result) {
+  grind_candidate(prefix.w, pow_bits, start, count, result);
+}
+__global__ void __launch_bounds__(256) grind_dev_kernel(const uint32_t* __restrict__ prefix, uint32_t pow_bits,
+                                                        uint64_t start, uint32_t count,
+                                                        unsigned long long* __restrict__ result) {
+  uint32_t pw[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) pw[k] = prefix[k];
+  grind_candidate(pw, pow_bits, start, count, result);
+}
 
 void launch_grind(const GrindPrefix& prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
                   unsigned long long* result, hipStream_t stream) {
   hipLaunchKernelGGL(grind_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, prefix, pow_bits, start, count,
                      result);
+}
+void launch_grind_dev(const uint32_t* prefix, uint32_t pow_bits, uint64_t start, uint32_t count,
+                      unsigned long long* result, hipStream_t stream) {
+  hipLaunchKernelGGL(grind_dev_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, prefix, pow_bits, start,
+                     count, result);
 }
 
 // ---- device-side Fiat-Shamir channel (crates/stwo/src/core/channel/blake2s.rs:19-140) --------

@@ -462,6 +462,12 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
 
 }  // namespace
 
+// Candidates of a grind's first launch: 32 times the expected nonce (about 2^pow_bits), so that a
+// miss has probability about e^-32, within [2^12, 2^24].
+static uint32_t grind_first_count(uint32_t pow_bits) {
+  return 1u << (pow_bits + 5 < 12 ? 12 : (pow_bits + 5 > 24 ? 24 : pow_bits + 5));
+}
+
 uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits, unsigned long long* slot) {
   if (pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
   GrindPrefix pre;
@@ -475,9 +481,8 @@ uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits, unsigned 
     slot = (unsigned long long*)own.p;
     HIP_CHECK(hipMemcpyAsync(slot, &found, 8, hipMemcpyHostToDevice, ctx.stream));
   }
-  // the expected nonce is about 2^pow_bits: the first launch covers 32 times that (a miss has
-  // probability about e^-32), later ones double the range
-  uint64_t count = (uint64_t)1 << (pow_bits + 5 < 12 ? 12 : (pow_bits + 5 > 24 ? 24 : pow_bits + 5));
+  // later launches double the range
+  uint64_t count = grind_first_count(pow_bits);
   const uint64_t limit = (uint64_t)0x7fffffff << 20;  // hi < P (simd/grind.rs:47-56)
   for (uint64_t start = 0; found == ~0ull; start += count, count = count < (1u << 24) ? count * 2 : count) {
     if (start >= limit) throw HipError(4, "Grind failed to find a solution.");
@@ -1163,11 +1168,28 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   // device block: channel state (9 words; words 14 and 15 hold the proof-of-work result), the
   // transcript records (12 words per layer) and the last layer's values, which the last fold
   // writes in place, so that one read-back takes both
-  DevBuf chan(&ctx, 16 + 12 * (size_t)n_layers + 4 * last_n);
+  DevBuf chan(&ctx, 16 + 12 * (size_t)n_layers + 4 * last_n + 8);
   uint32_t* dstate = chan.p;
   uint32_t* dtickets = ctx.fri_ticket_words(n_layers);
   uint32_t* drec = chan.p + 16;
   uint32_t* dlast = drec + 12 * (size_t)n_layers;
+  // The end of the phase on the device as well (launch_fri_last_pow, then the grind from its
+  // prefix), so that the proof of work needs no round trip of its own: for a last layer folded in
+  // place of at most 2^6 values. The prefix goes after the last layer; the host checks the mixed
+  // digest and the prefix against its own.
+  uint32_t* dprefix = dlast + 4 * last_n;
+  const bool dev_pow = cfg.fold_step == 1 && last_log <= 6 && cfg.log_last_layer <= 4 && cfg.pow_bits <= 32;
+  const uint32_t* d_last_xi = nullptr;
+  if (dev_pow) {
+    // the inverse x-coordinates of the last layer's line domain, level by level, then 1 / 2^L
+    HCoset dom = HCoset::half_odds(Lc - 1);
+    for (uint32_t i = Lc - 1; i > last_log; --i) dom = dom.doubled();
+    std::vector<uint32_t> xi;
+    for (HCoset d = dom; d.log_size > 0; d = d.doubled())
+      for (size_t i = 0; i < ((size_t)1 << d.log_size) / 2; ++i) xi.push_back(m31_inv(d.at(i).x));
+    xi.push_back(m31_inv((uint32_t)last_n));
+    d_last_xi = ctx.table_vec(xi);
+  }
   // the channel state at the start of the phase goes to the first layer launch as an argument when
   // that launch is a fused layer (fri_st0_by_value), else it is uploaded to dstate
   std::array<uint32_t, 9> fri_st0{};
@@ -1346,6 +1368,12 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     }
   }
   for (auto& f : fri_launches) f();
+  if (dev_pow) {
+    KScope ks(ctx, K_GRIND, 0.0);
+    launch_fri_last_pow(dlast, last_log, cfg.log_last_layer, d_last_xi, dstate, cfg.pow_bits, dprefix, ctx.stream);
+    launch_grind_dev(dprefix, cfg.pow_bits, 0, grind_first_count(cfg.pow_bits), (unsigned long long*)(dstate + 14),
+                     ctx.stream);
+  }
   if (cfg.fold_step != 1) {
     const qm31* alpha0 = mix_and_draw(first, 0);
     cur.log = Lc - 1;
@@ -1422,12 +1450,15 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
     status = qm_eq(committed, acc) ? 0 : 2;
   }
   HMARK("sanity");
+  std::vector<uint32_t> fri_hb;  // the FRI block as read back
   {
     // one read-back: the transcript records and the last layer's evaluations
     for (int k = 0; k < 4; ++k)
       if (cur.c[k].p != dlast + k * last_n) launch_copy_words(dlast + k * last_n, cur.c[k].p, last_n, ctx.stream);
-    std::vector<uint32_t> hb(12 * (size_t)n_layers + 4 * last_n);
-    ctx.d2h(hb.data(), drec, hb.size() * 4);
+    // the whole block: channel state and proof-of-work slot, records, last layer, prefix
+    fri_hb.assign(16 + 12 * (size_t)n_layers + 4 * last_n + 8, 0);
+    ctx.d2h(fri_hb.data(), dstate, fri_hb.size() * 4);
+    const uint32_t* hb = fri_hb.data() + 16;
     ctx.reset_arena();
     HMARK("fri_wait");
     auto replay = [&](DevTree& t, uint32_t l) {
@@ -1435,13 +1466,13 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
         for (int j = 0; j < 4; ++j) t.root[4 * i + j] = (uint8_t)(hb[12 * l + i] >> (8 * j));
       ch.mix_root(t.root);
       qm31 a = ch.draw_secure_felt();
-      const uint32_t* r = hb.data() + 12 * l + 8;
+      const uint32_t* r = hb + 12 * l + 8;
       if (!qm_eq(a, qm31{r[0], r[1], r[2], r[3]})) throw HipError(4, "device channel diverged from the transcript");
     };
     replay(first.tree, 0);
     for (uint32_t l = 1; l < n_layers; ++l) replay(inner[l - 1].tree, l);
     const size_t n = last_n;
-    const uint32_t* w = hb.data() + 12 * (size_t)n_layers;
+    const uint32_t* w = hb + 12 * (size_t)n_layers;
     std::vector<qm31> v(n);
     for (size_t i = 0; i < n; ++i) v[i] = {w[i], w[n + i], w[2 * n + i], w[3 * n + i]};
     HCoset dom = HCoset::half_odds(Lc - 1);
@@ -1463,7 +1494,23 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   stage_ms[5] = tm.lap();
 
   // ---------------- proof of work ----------------
-  out.pow = device_grind(ctx, ch.pow_prefix(cfg.pow_bits), cfg.pow_bits, (unsigned long long*)(dstate + 14));
+  if (dev_pow) {
+    const uint32_t* dw = fri_hb.data();
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t hw = (uint32_t)ch.digest[4 * i] | (uint32_t)ch.digest[4 * i + 1] << 8 |
+                          (uint32_t)ch.digest[4 * i + 2] << 16 | (uint32_t)ch.digest[4 * i + 3] << 24;
+      if (dw[i] != hw) throw HipError(4, "device channel diverged from the transcript");
+    }
+    const Hash prefix = ch.pow_prefix(cfg.pow_bits);
+    const uint32_t* dp = dw + 16 + 12 * (size_t)n_layers + 4 * last_n;
+    for (int i = 0; i < 32; ++i)
+      if (prefix[i] != (uint8_t)(dp[i / 4] >> (8 * (i % 4)))) throw HipError(4, "device proof-of-work prefix differs");
+    out.pow = (uint64_t)dw[14] | (uint64_t)dw[15] << 32;
+    // a nonce beyond the first launch's range (probability about e^-32): the search continues
+    if (out.pow == ~0ull) out.pow = device_grind(ctx, prefix, cfg.pow_bits);
+  } else {
+    out.pow = device_grind(ctx, ch.pow_prefix(cfg.pow_bits), cfg.pow_bits, (unsigned long long*)(dstate + 14));
+  }
   ch.mix_u64(out.pow);
   HMARK("pow");
   stage_ms[6] = tm.lap();
