@@ -385,23 +385,28 @@ __global__ void __launch_bounds__(64) fri_last_pow_kernel(const uint32_t* __rest
   }
   if (t < 8) M[t] = state[t];
   __syncthreads();
-  if (t != 0) return;
-  const uint32_t nbytes = 32 + 16 * bound;
-  uint32_t h[8];
-  blake2s_init(h);
+  // the two hashes on one quad of lanes (lane q holds chaining words q and q + 4)
+  if (t >= 4) return;
+  const uint32_t q = t, nbytes = 32 + 16 * bound;
+  uint32_t hA, hB;
+  blake2s_init_quad(q, hA, hB);
   for (uint32_t o = 0; o < nbytes; o += 64) {
     uint32_t m[16];
     for (int k = 0; k < 16; ++k) m[k] = o + 4 * k < nbytes ? M[o / 4 + k] : 0u;
     const bool last = o + 64 >= nbytes;
-    blake2s_compress(h, m, last ? nbytes : o + 64, last);
+    blake2s_compress_quad(hA, hB, m, last ? nbytes : o + 64, last, q);
   }
-  uint32_t m[16] = {0x12345678u, 0, 0, 0, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], pow_bits, 0, 0, 0};
-  for (int k = 0; k < 8; ++k) state[k] = h[k];
-  state[8] = 0;
-  uint32_t pre[8];
-  blake2s_init(pre);
-  blake2s_compress(pre, m, 52, true);
-  for (int k = 0; k < 8; ++k) prefix[k] = pre[k];
+  const uint32_t m[16] = {0x12345678u,     0, 0, 0, quad_bcast<0>(hA), quad_bcast<1>(hA), quad_bcast<2>(hA),
+                          quad_bcast<3>(hA), quad_bcast<0>(hB), quad_bcast<1>(hB), quad_bcast<2>(hB),
+                          quad_bcast<3>(hB), pow_bits, 0, 0, 0};
+  state[q] = hA;
+  state[4 + q] = hB;
+  if (q == 0) state[8] = 0;
+  uint32_t pA, pB;
+  blake2s_init_quad(q, pA, pB);
+  blake2s_compress_quad(pA, pB, m, 52, true, q);
+  prefix[q] = pA;
+  prefix[4 + q] = pB;
 }
 
 void launch_fri_last_pow(const uint32_t* vals, uint32_t L, uint32_t BL, const uint32_t* xi, uint32_t* state,

@@ -615,11 +615,85 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
 namespace {
 
 // CommitmentSchemeProver::commit + TreeBuilder::commit for trees given as coefficients.
-void commit_tree(Ctx& ctx, DevTree& t, const PcsCfg& cfg, Channel& ch) {
+// `before_root` runs on the host after the tree is queued and before its root is read.
+void commit_tree(Ctx& ctx, DevTree& t, const PcsCfg& cfg, Channel& ch, const std::function<void()>& before_root = {}) {
   evaluate_polynomials(ctx, t, cfg.log_blowup);
-  merkle_commit(ctx, t);
+  merkle_commit(ctx, t, false);
+  if (before_root) before_root();
+  if (!t.evals.empty()) t.root = read_hash(ctx, t.layers[0].p);
   ch.mix_root(t.root);
-  ctx.reset_arena();  // merkle_commit synchronised the stream
+  ctx.reset_arena();  // the root read synchronised the stream
+}
+
+// The composition's launches up to its random coefficient: per component the accumulator, the
+// columns on the evaluation domain (extensions queued here when needed), the vanishing
+// denominators and the column tables. Laid out before the last commitment's root is read, so the
+// host does it while the device still works on that commitment.
+struct CompLaunch {
+  uint32_t E = 0;
+  bool accumulate = false, join_side = false;
+  Cols4 a4{};
+  std::vector<const uint32_t*> tcols, icols;
+  const uint32_t* const* d_t = nullptr;
+  const uint32_t* const* d_i = nullptr;
+  const uint32_t* d_dinv = nullptr;
+  double cbytes = 0;
+  std::vector<DevBuf> ext_t, ext_i, ext_keep;  // alive until the composition is queued
+};
+struct CompPrep {
+  bool ready = false;
+  uint32_t max_log = 0;
+  size_t T = 0;
+  std::map<uint32_t, std::vector<DevBuf>> accs;
+  std::vector<CompLaunch> items;
+};
+
+void prepare_composition(Ctx& ctx, std::vector<DevTree>& trees, const std::vector<Component>& comps, CompPrep& P) {
+  for (auto& c : comps) { P.max_log = std::max(P.max_log, c.eval_log()); P.T += c.n_constraints(); }
+  P.items.resize(comps.size());
+  for (size_t ci = 0; ci < comps.size(); ++ci) {
+    const Component& c = comps[ci];
+    CompLaunch& L = P.items[ci];
+    const uint32_t E = c.eval_log();
+    L.E = E;
+    ctx.ensure_twiddles(E);
+    auto& acc = P.accs[E];
+    // the first component of an evaluation size writes the accumulator, later ones add to it
+    L.accumulate = !acc.empty();
+    if (acc.empty()) for (int k = 0; k < 4; ++k) acc.emplace_back(&ctx, (size_t)1 << E);
+    L.a4 = {{acc[0].p, acc[1].p, acc[2].p, acc[3].p}};
+    // columns on the evaluation domain: committed evaluations when sizes match, else extended
+    auto cols_on_domain = [&](const DevTree& t, size_t start, size_t n, std::vector<DevBuf>& ext) {
+      std::vector<const uint32_t*> r(n);
+      bool need = false;
+      for (size_t i = 0; i < n; ++i) need |= t.eval_logs[start + i] != E;
+      if (!need) { for (size_t i = 0; i < n; ++i) r[i] = t.evals[start + i]; return r; }
+      if (t.ext_log == E && t.ext.size() >= start + n) {
+        for (size_t i = 0; i < n; ++i) r[i] = t.ext[start + i].p;
+        L.join_side = true;
+        return r;
+      }
+      std::vector<const uint32_t*> cf(t.coeffs.begin() + start, t.coeffs.begin() + start + n);
+      std::vector<uint32_t> lg(t.coeff_logs.begin() + start, t.coeff_logs.begin() + start + n);
+      evaluate_into(ctx, cf, lg, E, ext, L.ext_keep);
+      for (size_t i = 0; i < n; ++i) r[i] = ext[i].p;
+      return r;
+    };
+    L.tcols = cols_on_domain(trees[1], c.trace_start, c.n_trace_cols(), L.ext_t);
+    const uint32_t le = E - c.log_n;
+    std::vector<uint32_t> dinv(1u << le);
+    for (uint32_t i = 0; i < (1u << le); ++i) dinv[i] = m31_inv(coset_vanishing_m31(c.log_n, canonic_domain_at(E, i)));
+    std::vector<uint32_t> dinv_br(dinv.size());
+    for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
+    L.d_dinv = ctx.table_vec(dinv_br);
+    L.d_t = (const uint32_t* const*)ctx.table_vec(L.tcols);
+    L.cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
+    if (c.kind != AIR_WIDE_FIB) {
+      L.icols = cols_on_domain(trees[2], c.inter_start, c.n_inter_cols(), L.ext_i);
+      L.d_i = (const uint32_t* const*)ctx.table_vec(L.icols);
+    }
+  }
+  P.ready = true;
 }
 
 struct ProofParts {
@@ -740,81 +814,61 @@ std::vector<qm31> line_interpolate(std::vector<qm31> v, HCoset dom) {
 
 // prove_ex + prove_values. `trees` holds the committed preprocessed, trace (and interaction) trees.
 int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vector<Component>& comps,
-             const PcsCfg& cfg, ProofParts& out, double* stage_ms) {
+             const PcsCfg& cfg, ProofParts& out, double* stage_ms, CompPrep* prep = nullptr) {
   Timer tm;
   HMARK("prove_ex");
   // ---------------- composition ----------------
+  CompPrep own_prep;
+  if (!prep || !prep->ready) {
+    prepare_composition(ctx, trees, comps, own_prep);
+    prep = &own_prep;
+  }
   qm31 random_coeff = ch.draw_secure_felt();
-  uint32_t max_log = 0;
-  size_t T = 0;
-  for (auto& c : comps) { max_log = std::max(max_log, c.eval_log()); T += c.n_constraints(); }
+  const uint32_t max_log = prep->max_log;
+  const size_t T = prep->T;
+  // random_coeff^i, in four interleaved chains (exact, so the same values as one chain)
   std::vector<qm31> powers(T);
-  { qm31 p = qm_one(); for (size_t i = 0; i < T; ++i) { powers[i] = p; p = qm_mul(p, random_coeff); } }
+  {
+    const qm31 a2 = qm_sqr(random_coeff), a4 = qm_sqr(a2);
+    qm31 p[4] = {qm_one(), random_coeff, a2, qm_mul(a2, random_coeff)};
+    for (size_t i = 0; i < T; i += 4)
+      for (size_t j = 0; j < 4 && i + j < T; ++j) {
+        powers[i + j] = p[j];
+        p[j] = qm_mul(p[j], a4);
+      }
+  }
   HMARK("x_pows");
-  std::map<uint32_t, std::vector<DevBuf>> accs;
-  for (auto& c : comps) {
+  auto& accs = prep->accs;
+  for (size_t ci = 0; ci < comps.size(); ++ci) {
+    const Component& c = comps[ci];
+    CompLaunch& L = prep->items[ci];
     size_t nc = c.n_constraints();
     std::vector<qm31> cp(powers.end() - nc, powers.end());
     powers.resize(powers.size() - nc);
     std::reverse(cp.begin(), cp.end());
-    uint32_t E = c.eval_log();
-    ctx.ensure_twiddles(E);
-    auto& acc = accs[E];
-    // the first component of an evaluation size writes the accumulator, later ones add to it
-    const bool accumulate = !acc.empty();
-    if (acc.empty()) for (int k = 0; k < 4; ++k) acc.emplace_back(&ctx, (size_t)1 << E);
-    Cols4 a4 = {{acc[0].p, acc[1].p, acc[2].p, acc[3].p}};
-    // columns on the evaluation domain: committed evaluations when sizes match, else extended
-    std::vector<DevBuf> ext_t, ext_i, ext_keep;
-    bool join_side = false;
-    auto cols_on_domain = [&](const DevTree& t, size_t start, size_t n, std::vector<DevBuf>& ext) {
-      std::vector<const uint32_t*> r(n);
-      bool need = false;
-      for (size_t i = 0; i < n; ++i) need |= t.eval_logs[start + i] != E;
-      if (!need) { for (size_t i = 0; i < n; ++i) r[i] = t.evals[start + i]; return r; }
-      if (t.ext_log == E && t.ext.size() >= start + n) {
-        for (size_t i = 0; i < n; ++i) r[i] = t.ext[start + i].p;
-        join_side = true;
-        return r;
-      }
-      std::vector<const uint32_t*> cf(t.coeffs.begin() + start, t.coeffs.begin() + start + n);
-      std::vector<uint32_t> lg(t.coeff_logs.begin() + start, t.coeff_logs.begin() + start + n);
-      evaluate_into(ctx, cf, lg, E, ext, ext_keep);
-      for (size_t i = 0; i < n; ++i) r[i] = ext[i].p;
-      return r;
-    };
-    auto tcols = cols_on_domain(trees[1], c.trace_start, c.n_trace_cols(), ext_t);
-    uint32_t le = E - c.log_n;
-    std::vector<uint32_t> dinv(1u << le);
-    for (uint32_t i = 0; i < (1u << le); ++i) dinv[i] = m31_inv(coset_vanishing_m31(c.log_n, canonic_domain_at(E, i)));
-    std::vector<uint32_t> dinv_br(dinv.size());
-    for (uint32_t i = 0; i < dinv.size(); ++i) dinv_br[bitrev(i, le)] = dinv[i];
-    const uint32_t* d_dinv = ctx.table_vec(dinv_br);
+    const uint32_t E = L.E;
     const qm31* d_pows = ctx.stage_vec(cp);
-    const uint32_t* const* d_t = (const uint32_t* const*)ctx.table_vec(tcols);
     HMARK("x_tables");
-    double cbytes = (4.0 * (c.n_trace_cols() + c.n_inter_cols()) + 32.0) * (double)(1u << E);
-    if (join_side) ctx.join();
+    if (L.join_side) ctx.join();
     if (c.kind == AIR_WIDE_FIB) {
       ctx.commit_staged();
-      KScope ks(ctx, K_COMPOSITION, cbytes);
-      launch_wide_fib_composition(E, c.log_n, d_t, c.n_cols, d_pows, d_dinv, a4, accumulate, ctx.stream);
+      KScope ks(ctx, K_COMPOSITION, L.cbytes);
+      launch_wide_fib_composition(E, c.log_n, L.d_t, c.n_cols, d_pows, L.d_dinv, L.a4, L.accumulate, ctx.stream);
     } else {
-      auto icols = cols_on_domain(trees[2], c.inter_start, c.n_inter_cols(), ext_i);
       PoseidonParams pp{};
       pp.z = c.z;
       for (int i = 0; i < 16; ++i) pp.alpha_pow[i] = c.alpha_pow[i];
       pp.shift = c.cumsum_shift();
       pp.ipr = c.ipr;
-      const uint32_t* const* d_i = (const uint32_t* const*)ctx.table_vec(icols);
       const PoseidonParams* d_pp = (const PoseidonParams*)ctx.upload(&pp, sizeof(pp));
-      KScope ks(ctx, K_COMPOSITION, cbytes);
+      KScope ks(ctx, K_COMPOSITION, L.cbytes);
       const size_t stride = (size_t)1 << E;
-      if (is_strided(tcols, stride) && is_strided(icols, stride))
-        launch_poseidon_composition_strided(E, c.log_n, tcols[0], icols[0], stride, d_pp, c.ipr, d_pows, d_dinv, a4,
-                                            accumulate, ctx.stream);
+      if (is_strided(L.tcols, stride) && is_strided(L.icols, stride))
+        launch_poseidon_composition_strided(E, c.log_n, L.tcols[0], L.icols[0], stride, d_pp, c.ipr, d_pows, L.d_dinv,
+                                            L.a4, L.accumulate, ctx.stream);
       else
-        launch_poseidon_composition(E, c.log_n, d_t, d_i, d_pp, c.ipr, d_pows, d_dinv, a4, accumulate, ctx.stream);
+        launch_poseidon_composition(E, c.log_n, L.d_t, L.d_i, d_pp, c.ipr, d_pows, L.d_dinv, L.a4, L.accumulate,
+                                    ctx.stream);
     }
   }
   // finalize: lift smaller accumulations into the largest, then interpolate 4 coordinate columns
@@ -1593,14 +1647,15 @@ ProveResult prove_wide_fib(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   ctx.ensure_twiddles(log_n + 1 + cfg.log_blowup);
   commit_tree(ctx, trees[0], cfg, ch);  // empty preprocessed tree
   interpolate_columns(ctx, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
-  commit_tree(ctx, trees[1], cfg, ch);
+  std::vector<Component> comps(1);
+  comps[0].kind = AIR_WIDE_FIB;
+  comps[0].log_n = log_n;
+  comps[0].n_cols = n_cols;
+  CompPrep prep;
+  commit_tree(ctx, trees[1], cfg, ch, [&] { prepare_composition(ctx, trees, comps, prep); });
   r.stage_ms[0] = tm.lap();
-  Component c;
-  c.kind = AIR_WIDE_FIB;
-  c.log_n = log_n;
-  c.n_cols = n_cols;
   ProofParts parts;
-  r.status = prove_ex(ctx, ch, trees, {c}, cfg, parts, r.stage_ms);
+  r.status = prove_ex(ctx, ch, trees, comps, cfg, parts, r.stage_ms, &prep);
   r.proof = encode(cfg, parts);
   r.digest = ch.digest;
   return r;
@@ -1632,10 +1687,11 @@ ProveResult prove_wide_fib_multi(Ctx& ctx, const std::vector<const uint32_t*>& t
   ctx.ensure_twiddles(max_n + 1 + cfg.log_blowup);
   commit_tree(ctx, trees[0], cfg, ch);  // empty preprocessed tree
   interpolate_columns(ctx, trace, col_logs, trees[1]);
-  commit_tree(ctx, trees[1], cfg, ch);
+  CompPrep prep;
+  commit_tree(ctx, trees[1], cfg, ch, [&] { prepare_composition(ctx, trees, comps, prep); });
   r.stage_ms[0] = tm.lap();
   ProofParts parts;
-  r.status = prove_ex(ctx, ch, trees, comps, cfg, parts, r.stage_ms);
+  r.status = prove_ex(ctx, ch, trees, comps, cfg, parts, r.stage_ms, &prep);
   r.proof = encode(cfg, parts);
   r.digest = ch.digest;
   return r;
@@ -1673,6 +1729,8 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   c.kind = AIR_POSEIDON;
   c.log_n = log_n;
   c.ipr = ipr;
+  std::vector<Component> comps;  // {c}, once the LogUp draws are in it; the composition refers to it
+  CompPrep prep;
   auto le = ch.draw_secure_felts(2);
   c.z = le[0];
   c.alpha = le[1];
@@ -1718,11 +1776,14 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
     trees[2].ext_log = E;
     merkle_commit(ctx, trees[2], false);
     launch_copy_words(sums.p + 8, trees[2].layers[0].p, 8, ctx.stream);
+    comps.assign(1, c);
+    prepare_composition(ctx, trees, comps, prep);
     uint32_t w[16];
     ctx.d2h(w, sums.p, sizeof(w));
     unsigned long long s[4];
     std::memcpy(s, w, sizeof(s));
     c.claimed_sum = {m31_reduce64(s[0] % P), m31_reduce64(s[1] % P), m31_reduce64(s[2] % P), m31_reduce64(s[3] % P)};
+    comps[0].claimed_sum = c.claimed_sum;
     for (int i = 0; i < 8; ++i)
       for (int j = 0; j < 4; ++j) trees[2].root[4 * i + j] = (uint8_t)(w[8 + i] >> (8 * j));
     ch.mix_root(trees[2].root);
@@ -1733,7 +1794,7 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
   r.stage_ms[1] = tm.lap();
   r.claimed_sum = c.claimed_sum;
   ProofParts parts;
-  r.status = prove_ex(ctx, ch, trees, {c}, cfg, parts, r.stage_ms);
+  r.status = prove_ex(ctx, ch, trees, comps, cfg, parts, r.stage_ms, &prep);
   r.proof = encode(cfg, parts);
   r.digest = ch.digest;
   HMARK("encode");
