@@ -214,7 +214,7 @@ __device__ __forceinline__ void fl_publish(uint32_t q, const uint32_t st[9], con
 }  // namespace
 
 // One FRI layer of 2^d.log values (9 <= log <= 18) over 2^(log - c) workgroups of 256 threads,
-// 2^c leaves each (8 <= c <= 10, log - c <= 8). `ticket` is zero before the launch.
+// 2^c leaves each (7 <= c <= 10, log - c <= 8). `ticket` is zero before the launch.
 __global__ void __launch_bounds__(256) fri_layer_kernel(FriLayerDesc d, uint32_t c, uint32_t* __restrict__ state,
                                                         uint32_t* __restrict__ ticket) {
   __shared__ __align__(16) uint32_t bufA[8 << 10];
@@ -416,7 +416,10 @@ void launch_fri_last_pow(const uint32_t* vals, uint32_t L, uint32_t BL, const ui
 }
 
 void launch_fri_layer(const FriLayerDesc& d, uint32_t* state, uint32_t* ticket, hipStream_t stream) {
-  const uint32_t c = d.log > 16 ? d.log - 8 : 8;  // chunk: 2^c leaves, at most 2^8 chunks
+  // chunk: 2^c leaves, at most 2^8 chunks. Up to 2^14 values, chunks of 2^7 leaves, whose levels
+  // above the leaves are all quad levels (at most 64 parents), at the price of one more quad level
+  // above the chunk roots.
+  const uint32_t c = d.log > 16 ? d.log - 8 : (d.log > 14 ? 8 : 7);
   hipLaunchKernelGGL(fri_layer_kernel, dim3(1u << (d.log - c)), dim3(FL_THREADS), 0, stream, d, c, state, ticket);
 }
 

@@ -78,8 +78,9 @@ struct FriLayerDesc {
   uint32_t* rec;            // 12 words: root, then the coefficient drawn after it
   uint32_t st0[9];          // init: channel digest (8 words) and n_draws at the start of the phase
 };
-// One layer, 9 <= log <= 18, over 2^(log - c) workgroups (c = max(8, log - 8)); *ticket is zero
-// before the launch, and the launch leaves it at zero. state: channel digest (8 words) and n_draws.
+// One layer, 9 <= log <= 18, over 2^(log - c) workgroups (c = 7 up to log 14, then max(8, log - 8));
+// *ticket is zero before the launch, and the launch leaves it at zero. state: channel digest (8
+// words) and n_draws.
 void launch_fri_layer(const FriLayerDesc& d, uint32_t* state, uint32_t* ticket, hipStream_t stream);
 // Layers of at most 2^8 values in one workgroup, in order (descs in device memory); every entry
 // after the first is a fold of the one before it. Only the last entry may have commit == 0.
