@@ -247,60 +247,31 @@ SecureCol lift_and_accumulate(std::vector<SecureCol> cols) {
   return curr;
 }
 
-SecureCol compute_fri_quotients(const std::vector<std::vector<const std::vector<M31>*>>& columns,
-                                const std::vector<std::vector<std::vector<PointSample>>>& samples, QM31 random_coeff,
-                                uint32_t lifting_log_size) {
-  std::vector<std::vector<uint32_t>> log_sizes(columns.size());
-  for (size_t t = 0; t < columns.size(); ++t)
-    for (auto* c : columns[t]) log_sizes[t].push_back(log2_size(c->size()));
-  auto swr = samples_with_randomness(samples, log_sizes, lifting_log_size, random_coeff);
-  struct Item { const std::vector<M31>* col; const std::vector<std::pair<PointSample, QM31>>* s; uint32_t ls; };
-  std::vector<Item> items;
-  for (size_t t = 0; t < columns.size(); ++t)
-    for (size_t c = 0; c < columns[t].size(); ++c) items.push_back({columns[t][c], &swr[t][c], log_sizes[t][c]});
-  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
-  std::vector<AccNum> accs;
-  for (size_t i = 0; i < items.size();) {
-    size_t j = i;
-    while (j < items.size() && items[j].ls == items[i].ls) ++j;
-    std::vector<const std::vector<std::pair<PointSample, QM31>>*> sv;
-    std::vector<const std::vector<M31>*> cv;
-    for (size_t k = i; k < j; ++k) { sv.push_back(items[k].s); cv.push_back(items[k].col); }
-    auto batches = sample_batches(sv);
-    size_t n = cv[0]->size();
-    for (auto& b : batches) {
-      std::vector<LineCoeffs> lc;
-      for (auto& nd : b.cols) lc.push_back(line_coeffs(b.point, nd.value, nd.rand));
-      AccNum an;
-      an.point = b.point;
-      an.partial.resize(n);
-      an.first_linear = QM31::zero();
-      for (auto& l : lc) an.first_linear = an.first_linear + l.a;
+// QuotientOps::accumulate_numerators for one batch (prover/pcs/quotient_ops.rs:19-43; CPU form
+// prover/backend/cpu/quotients.rs): partial[row] = sum_k (c_k * col_k[row] - b_k),
+// first_linear = sum_k a_k, with (a, b, c) the batch's line coefficients times the random power.
+AccNum accumulate_batch(const SampleBatch& b, const std::vector<const std::vector<M31>*>& cv) {
+  std::vector<LineCoeffs> lc;
+  for (auto& nd : b.cols) lc.push_back(line_coeffs(b.point, nd.value, nd.rand));
+  size_t n = cv[0]->size();
+  AccNum an;
+  an.point = b.point;
+  an.partial.resize(n);
+  an.first_linear = QM31::zero();
+  for (auto& l : lc) an.first_linear = an.first_linear + l.a;
 #pragma omp parallel for schedule(static) if (n >= 4096)
-      for (long row = 0; row < (long)n; ++row) {
-        QM31 acc = QM31::zero();
-        for (size_t k = 0; k < b.cols.size(); ++k) acc = acc + (lc[k].c * (*cv[b.cols[k].column_index])[row] - lc[k].b);
-        an.partial.set(row, acc);
-      }
-      accs.push_back(std::move(an));
-    }
-    i = j;
+  for (long row = 0; row < (long)n; ++row) {
+    QM31 acc = QM31::zero();
+    for (size_t k = 0; k < b.cols.size(); ++k) acc = acc + (lc[k].c * (*cv[b.cols[k].column_index])[row] - lc[k].b);
+    an.partial.set(row, acc);
   }
-  std::stable_sort(accs.begin(), accs.end(), [](const AccNum& a, const AccNum& b) { return point_less(a.point, b.point); });
-  std::vector<AccNum> per_point;
-  for (size_t i = 0; i < accs.size();) {
-    size_t j = i;
-    while (j < accs.size() && accs[j].point == accs[i].point) ++j;
-    AccNum a;
-    a.point = accs[i].point;
-    a.first_linear = QM31::zero();
-    std::vector<SecureCol> parts;
-    for (size_t k = i; k < j; ++k) { a.first_linear = a.first_linear + accs[k].first_linear; parts.push_back(std::move(accs[k].partial)); }
-    a.partial = lift_and_accumulate(std::move(parts));
-    per_point.push_back(std::move(a));
-    i = j;
-  }
-  // compute_quotients_and_combine (cpu/quotients.rs)
+  return an;
+}
+
+// QuotientOps::compute_quotients_and_combine (cpu/quotients.rs, simd/quotients.rs:73-121): on every
+// row of CanonicCoset(lifting_log_size), bit-reversed, sum over the accumulations of
+// (lift(partial)[row] - first_linear * y) / den, den = (Re p.x - x) Im p.y - (Re p.y - y) Im p.x.
+SecureCol quotients_and_combine(const std::vector<AccNum>& per_point, uint32_t lifting_log_size) {
   CircleDomain dom = canonic_domain(lifting_log_size);
   size_t N = (size_t)1 << lifting_log_size;
   SecureCol q; q.resize(N);
@@ -320,6 +291,45 @@ SecureCol compute_fri_quotients(const std::vector<std::vector<const std::vector<
     q.set(row, acc);
   }
   return q;
+}
+
+SecureCol compute_fri_quotients(const std::vector<std::vector<const std::vector<M31>*>>& columns,
+                                const std::vector<std::vector<std::vector<PointSample>>>& samples, QM31 random_coeff,
+                                uint32_t lifting_log_size) {
+  std::vector<std::vector<uint32_t>> log_sizes(columns.size());
+  for (size_t t = 0; t < columns.size(); ++t)
+    for (auto* c : columns[t]) log_sizes[t].push_back(log2_size(c->size()));
+  auto swr = samples_with_randomness(samples, log_sizes, lifting_log_size, random_coeff);
+  struct Item { const std::vector<M31>* col; const std::vector<std::pair<PointSample, QM31>>* s; uint32_t ls; };
+  std::vector<Item> items;
+  for (size_t t = 0; t < columns.size(); ++t)
+    for (size_t c = 0; c < columns[t].size(); ++c) items.push_back({columns[t][c], &swr[t][c], log_sizes[t][c]});
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.ls < b.ls; });
+  std::vector<AccNum> accs;
+  for (size_t i = 0; i < items.size();) {
+    size_t j = i;
+    while (j < items.size() && items[j].ls == items[i].ls) ++j;
+    std::vector<const std::vector<std::pair<PointSample, QM31>>*> sv;
+    std::vector<const std::vector<M31>*> cv;
+    for (size_t k = i; k < j; ++k) { sv.push_back(items[k].s); cv.push_back(items[k].col); }
+    for (auto& b : sample_batches(sv)) accs.push_back(accumulate_batch(b, cv));
+    i = j;
+  }
+  std::stable_sort(accs.begin(), accs.end(), [](const AccNum& a, const AccNum& b) { return point_less(a.point, b.point); });
+  std::vector<AccNum> per_point;
+  for (size_t i = 0; i < accs.size();) {
+    size_t j = i;
+    while (j < accs.size() && accs[j].point == accs[i].point) ++j;
+    AccNum a;
+    a.point = accs[i].point;
+    a.first_linear = QM31::zero();
+    std::vector<SecureCol> parts;
+    for (size_t k = i; k < j; ++k) { a.first_linear = a.first_linear + accs[k].first_linear; parts.push_back(std::move(accs[k].partial)); }
+    a.partial = lift_and_accumulate(std::move(parts));
+    per_point.push_back(std::move(a));
+    i = j;
+  }
+  return quotients_and_combine(per_point, lifting_log_size);
 }
 
 std::vector<size_t> prepare_pp_positions(const std::vector<size_t>& q, uint32_t max_log, uint32_t pp_max_log) {
@@ -1051,6 +1061,105 @@ void oracle_commit_on_layer(uint32_t log_size, const uint8_t* prev, uint32_t n_c
   }
   auto r = commit_on_layer(log_size, prev ? &pv : nullptr, refs);
   memcpy(out, r.data(), r.size() * 32);
+}
+// QuotientOps::accumulate_numerators over columns of one size (prover/pcs/quotient_ops.rs:19-43).
+// Batch b: point at points[8b..8b+8), samples [starts[b], starts[b+1]) of (col_idx, value, rand);
+// out receives 4 coordinate columns of 2^log_size per batch, first_linear 4 words per batch.
+void oracle_accumulate_numerators(uint32_t log_size, uint32_t n_cols, const uint32_t* const* cols, uint32_t n_batches,
+                                  const uint32_t* points, const uint32_t* starts, const uint32_t* col_idx,
+                                  const uint32_t* values, const uint32_t* rands, uint32_t* const* out,
+                                  uint32_t* first_linear) {
+  const size_t n = (size_t)1 << log_size;
+  std::vector<std::vector<M31>> cv(n_cols, std::vector<M31>(n));
+  std::vector<const std::vector<M31>*> refs;
+  for (uint32_t c = 0; c < n_cols; ++c) {
+    for (size_t i = 0; i < n; ++i) cv[c][i] = M31(cols[c][i]);
+    refs.push_back(&cv[c]);
+  }
+  for (uint32_t b = 0; b < n_batches; ++b) {
+    SampleBatch sb;
+    const uint32_t* pt = points + 8 * b;
+    sb.point = {QM31::from_u32(pt[0], pt[1], pt[2], pt[3]), QM31::from_u32(pt[4], pt[5], pt[6], pt[7])};
+    for (uint32_t j = starts[b]; j < starts[b + 1]; ++j)
+      sb.cols.push_back({col_idx[j], QM31::from_u32(values[4 * j], values[4 * j + 1], values[4 * j + 2], values[4 * j + 3]),
+                         QM31::from_u32(rands[4 * j], rands[4 * j + 1], rands[4 * j + 2], rands[4 * j + 3])});
+    AccNum a = accumulate_batch(sb, refs);
+    for (int k = 0; k < 4; ++k) {
+      for (size_t i = 0; i < n; ++i) out[4 * b + k][i] = a.partial.c[k][i].v;
+      first_linear[4 * b + k] = a.first_linear.coord(k).v;
+    }
+  }
+}
+// QuotientOps::compute_quotients_and_combine (quotient_ops.rs:34-43): accumulation a has its sample
+// point at points[8a..8a+8), 4 partial columns partials[4a..4a+4) of 2^log_sizes[a] words and its
+// first linear term at first_linear[4a..4a+4); out: 4 columns of 2^lifting_log_size.
+void oracle_compute_quotients_and_combine(uint32_t n_accs, const uint32_t* points, const uint32_t* const* partials,
+                                          const uint32_t* log_sizes, const uint32_t* first_linear,
+                                          uint32_t lifting_log_size, uint32_t* const* out) {
+  std::vector<AccNum> accs(n_accs);
+  for (uint32_t a = 0; a < n_accs; ++a) {
+    const uint32_t* pt = points + 8 * a;
+    accs[a].point = {QM31::from_u32(pt[0], pt[1], pt[2], pt[3]), QM31::from_u32(pt[4], pt[5], pt[6], pt[7])};
+    accs[a].first_linear = QM31::from_u32(first_linear[4 * a], first_linear[4 * a + 1], first_linear[4 * a + 2],
+                                          first_linear[4 * a + 3]);
+    const size_t n = (size_t)1 << log_sizes[a];
+    accs[a].partial.resize(n);
+    for (int k = 0; k < 4; ++k)
+      for (size_t i = 0; i < n; ++i) accs[a].partial.c[k][i] = M31(partials[4 * a + k][i]);
+  }
+  SecureCol q = quotients_and_combine(accs, lifting_log_size);
+  for (int k = 0; k < 4; ++k)
+    for (size_t i = 0; i < q.size(); ++i) out[k][i] = q.c[k][i].v;
+}
+// ComponentProver::evaluate_constraint_quotients_on_domain for one component
+// (crates/constraint-framework/src/prover/component_prover.rs:32-169): for every row of the
+// evaluation domain, acc[row] += (sum_k pows[k] C_k(row)) * denom_inv[row >> trace_log]. air 0:
+// wide Fibonacci with n_trace columns; air 1: Poseidon2 with ipr instances per row, lookup
+// elements (z, alpha) and the LogUp cumsum shift. Columns are evaluations on CanonicCoset(eval_log).
+void oracle_eval_constraints_on_domain(uint32_t air, uint32_t ipr, uint32_t trace_log, uint32_t eval_log,
+                                       const uint32_t* const* trace, uint32_t n_trace, const uint32_t* const* inter,
+                                       uint32_t n_inter, const uint32_t* pows, uint32_t n_constraints,
+                                       const uint32_t* denom_inv, const uint32_t z[4], const uint32_t alpha[4],
+                                       const uint32_t shift[4], uint32_t* const* acc) {
+  const size_t n = (size_t)1 << eval_log;
+  Component comp;
+  comp.kind = air == 0 ? WIDE_FIB : POSEIDON;
+  comp.log_n_rows = trace_log;
+  comp.n_cols = n_trace;
+  comp.ipr = ipr;
+  if (air == 1) {
+    comp.le.z = QM31::from_u32(z[0], z[1], z[2], z[3]);
+    comp.le.alpha = QM31::from_u32(alpha[0], alpha[1], alpha[2], alpha[3]);
+    QM31 cur = QM31::one();
+    for (uint32_t i = 0; i < POSEIDON_N_STATE; ++i) { comp.le.alpha_powers[i] = cur; cur = cur * comp.le.alpha; }
+  }
+  if (comp.n_constraints() != n_constraints) throw std::runtime_error("constraint count");
+  std::vector<std::vector<std::vector<M31>>> store(3);
+  std::vector<std::vector<const std::vector<M31>*>> refs(3);
+  auto load = [&](int t, const uint32_t* const* cols, uint32_t nc) {
+    store[t].assign(nc, std::vector<M31>(n));
+    for (uint32_t c = 0; c < nc; ++c)
+      for (size_t i = 0; i < n; ++i) store[t][c][i] = M31(cols[c][i]);
+    for (auto& v : store[t]) refs[t].push_back(&v);
+  };
+  load(1, trace, n_trace);
+  load(2, inter, n_inter);
+  std::vector<QM31> pw(n_constraints);
+  for (uint32_t k = 0; k < n_constraints; ++k)
+    pw[k] = QM31::from_u32(pows[4 * k], pows[4 * k + 1], pows[4 * k + 2], pows[4 * k + 3]);
+  const QM31 sh = QM31::from_u32(shift[0], shift[1], shift[2], shift[3]);
+  for (size_t row = 0; row < n; ++row) {
+    DomainEval e;
+    e.cols = &refs;
+    e.row = row;
+    e.trace_log = trace_log;
+    e.eval_log = eval_log;
+    e.pows = pw.data();
+    e.shift = sh;
+    eval_component(e, comp);
+    QM31 v = QM31::from_u32(acc[0][row], acc[1][row], acc[2][row], acc[3][row]) + e.row_res * M31(denom_inv[row >> trace_log]);
+    for (int k = 0; k < 4; ++k) acc[k][row] = v.coord(k).v;
+  }
 }
 // Root of a mixed-degree Merkle tree over columns of lengths 2^log_sizes[i].
 void oracle_merkle_root(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* log_sizes, uint8_t root[32]) {

@@ -265,3 +265,62 @@ def gen_poseidon(log_n, ipr, mode, seed):
     out = np.zeros((158 * ipr) << log_n, np.uint32)
     lib().oracle_gen_poseidon(log_n, ipr, mode, seed, u32ptr(out))
     return out.reshape(158 * ipr, 1 << log_n)
+
+
+def _q4(v):
+    return np.asarray(v, np.uint32).reshape(4)
+
+
+def accumulate_numerators(cols, batches):
+    """QuotientOps::accumulate_numerators over columns of one size. batches: list of
+    (point8, [(col_idx, value4, rand4), ...]). Returns ([4 partial columns per batch], [first_linear4])."""
+    cols = [np.ascontiguousarray(c, np.uint32) for c in cols]
+    n = len(cols[0])
+    log_size = n.bit_length() - 1
+    pts = np.array([list(pt) for pt, _ in batches], np.uint32).reshape(-1)
+    starts, idx, vals, rnds = [0], [], [], []
+    for _, ents in batches:
+        for ci, v, r in ents:
+            idx.append(ci)
+            vals += list(v)
+            rnds += list(r)
+        starts.append(len(idx))
+    starts = np.array(starts, np.uint32)
+    idx = np.array(idx, np.uint32)
+    vals = np.array(vals, np.uint32)
+    rnds = np.array(rnds, np.uint32)
+    outs = [np.zeros(n, np.uint32) for _ in range(4 * len(batches))]
+    fl = np.zeros(4 * len(batches), np.uint32)
+    lib().oracle_accumulate_numerators(ctypes.c_uint32(log_size), ctypes.c_uint32(len(cols)), ptr_array(cols),
+                                       ctypes.c_uint32(len(batches)), u32ptr(pts), u32ptr(starts), u32ptr(idx),
+                                       u32ptr(vals), u32ptr(rnds), ptr_array(outs), u32ptr(fl))
+    return [outs[4 * b:4 * b + 4] for b in range(len(batches))], [tuple(int(x) for x in fl[4 * b:4 * b + 4])
+                                                                   for b in range(len(batches))]
+
+
+def compute_quotients_and_combine(accs, lifting_log_size):
+    """accs: list of (point8, [4 partial columns], first_linear4). Returns 4 columns of 2^lifting_log_size."""
+    pts = np.array([list(a[0]) for a in accs], np.uint32).reshape(-1)
+    parts = [np.ascontiguousarray(c, np.uint32) for a in accs for c in a[1]]
+    logs = np.array([len(a[1][0]).bit_length() - 1 for a in accs], np.uint32)
+    fl = np.array([list(a[2]) for a in accs], np.uint32).reshape(-1)
+    out = [np.zeros(1 << lifting_log_size, np.uint32) for _ in range(4)]
+    lib().oracle_compute_quotients_and_combine(ctypes.c_uint32(len(accs)), u32ptr(pts), ptr_array(parts), u32ptr(logs),
+                                               u32ptr(fl), ctypes.c_uint32(lifting_log_size), ptr_array(out))
+    return out
+
+
+def eval_constraints_on_domain(air, ipr, trace_log, eval_log, trace, inter, pows, denom_inv, acc4, z=(0, 0, 0, 0),
+                               alpha=(0, 0, 0, 0), shift=(0, 0, 0, 0)):
+    """ComponentProver::evaluate_constraint_quotients_on_domain for one component; returns the new acc4."""
+    trace = [np.ascontiguousarray(c, np.uint32) for c in trace]
+    inter = [np.ascontiguousarray(c, np.uint32) for c in inter]
+    acc = [np.ascontiguousarray(c, np.uint32).copy() for c in acc4]
+    pw = np.array([list(p) for p in pows], np.uint32).reshape(-1)
+    dinv = np.ascontiguousarray(denom_inv, np.uint32)
+    lib().oracle_eval_constraints_on_domain(
+        ctypes.c_uint32(0 if air == "wide_fib" else 1), ctypes.c_uint32(ipr), ctypes.c_uint32(trace_log),
+        ctypes.c_uint32(eval_log), ptr_array(trace), ctypes.c_uint32(len(trace)),
+        ptr_array(inter) if inter else None, ctypes.c_uint32(len(inter)), u32ptr(pw), ctypes.c_uint32(len(pows)),
+        u32ptr(dinv), u32ptr(_q4(z).copy()), u32ptr(_q4(alpha).copy()), u32ptr(_q4(shift).copy()), ptr_array(acc))
+    return acc
