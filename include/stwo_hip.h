@@ -6,7 +6,9 @@
  * Conventions
  *  - Every function returns an int status: 0 ok, 1 invalid argument, 2 HIP runtime error,
  *    3 unsupported size or configuration, 4 protocol failure (e.g. FRI degree check),
- *    5 unknown component. The message of the last failure is in stwo_hip_last_error().
+ *    5 unknown component, 6 constraints not satisfied (prove_ex's sanity check; the proof
+ *    handle is still set, see the whole-proof entry points). The message of the last failure
+ *    is in stwo_hip_last_error().
  *    (The reference exits the process on any device error, CU/utils.cuh:37-40.)
  *  - Column arguments are device pointers to u32 words; "const uint32_t* const* cols" is a
  *    HOST array of device pointers. Every M31 word produced is canonical, in [0, 2^31 - 1).
@@ -68,13 +70,58 @@ int stwo_hip_copy_d2h(stwo_hip_ctx* ctx, uint32_t* dst, const uint32_t* src, siz
 int stwo_hip_batch_at_multi(stwo_hip_ctx* ctx, const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows,
                             uint32_t n_rows, uint32_t* out_host);
 
+/* ColumnOps::bit_reverse_column (prover/backend/mod.rs:39-42; bit_reverse_base_field, used by
+ * prover/backend/cuda/column.rs:14-21): in place, 2^log_size words. */
+int stwo_hip_bit_reverse_column(stwo_hip_ctx* ctx, uint32_t* col, uint32_t log_size);
+/* Column::at / Column::set (prover/backend/mod.rs:59-62; cuda_get_uint32_t / cuda_set_uint32_t,
+ * bindings.rs:84-131). at is synchronous; set is stream-ordered and complete on return. */
+int stwo_hip_column_at(stwo_hip_ctx* ctx, const uint32_t* col, size_t index, uint32_t* out);
+int stwo_hip_column_set(stwo_hip_ctx* ctx, uint32_t* col, size_t index, uint32_t value);
+/* Column::split_at_mid (prover/backend/mod.rs:90-91) and PolyOps::split_at_mid
+ * (prover/poly/circle/ops.rs:127-145, whose coefficient halves are p_left and p_right of
+ * p = p_left + pi^(L-2)(x) p_right): the halves of a column of n_words (even, at least 2) as
+ * borrowed views, *left = col and *right = col + n_words / 2. No device work; the views live as
+ * long as col and are not freed on their own. */
+int stwo_hip_split_at_mid(stwo_hip_ctx* ctx, uint32_t* col, size_t n_words, uint32_t** left, uint32_t** right);
+
 /* ---------------------------------------------------------------- PolyOps -------------- */
+/* PolyOps::extend (prover/poly/circle/ops.rs:86-90; cpu/circle.rs:182-188, cuda/poly.rs:409-420):
+ * out (2^new_log_size words) = coeffs (2^log_size words) followed by zeros; out may equal coeffs
+ * when that buffer already holds 2^new_log_size words. */
+int stwo_hip_extend(stwo_hip_ctx* ctx, const uint32_t* coeffs, uint32_t log_size, uint32_t new_log_size,
+                    uint32_t* out);
+/* PolyOps::precompute_twiddles(coset) for any Coset::new(coset_initial_index, coset_log_size)
+ * (prover/poly/circle/ops.rs:122; precompute_twiddles(initial, step, size), bindings.rs:165-169):
+ * the tree and its inverse, built on the device and owned by the handle (freed by
+ * stwo_hip_twiddles_free or with the context). Status 1 when a twiddle is zero (no inverse).
+ * stwo_hip_twiddles_read copies 2^coset_log_size words of each (host pointers may be NULL). */
+typedef struct stwo_hip_twiddles stwo_hip_twiddles;
+int stwo_hip_twiddles_create(stwo_hip_ctx* ctx, uint32_t coset_initial_index, uint32_t coset_log_size,
+                             stwo_hip_twiddles** out);
+int stwo_hip_twiddles_read(stwo_hip_ctx* ctx, const stwo_hip_twiddles* tw, uint32_t* tw_host, uint32_t* itw_host);
+int stwo_hip_twiddles_free(stwo_hip_ctx* ctx, stwo_hip_twiddles* tw);
+/* PolyOps::interpolate / evaluate on any CircleDomain (ops.rs:23-26,92-96; cpu/circle.rs:28-80,
+ * 190-228): the domain is CircleDomain::new(Coset::new(half_coset_initial_index, log_size - 1)),
+ * whose half coset must be a doubling of the tree's coset (status 1 otherwise). Evaluations are
+ * in bit-reversed domain order. evaluate zero-extends 2^log_coeffs coefficients to the domain. */
+int stwo_hip_interpolate_on_domain(stwo_hip_ctx* ctx, const stwo_hip_twiddles* tw, uint32_t half_coset_initial_index,
+                                   uint32_t log_size, const uint32_t* const* evals, uint32_t* const* coeffs,
+                                   uint32_t n_cols);
+int stwo_hip_evaluate_on_domain(stwo_hip_ctx* ctx, const stwo_hip_twiddles* tw, uint32_t half_coset_initial_index,
+                                uint32_t log_size, const uint32_t* const* coeffs, uint32_t log_coeffs,
+                                uint32_t* const* evals, uint32_t n_cols);
+/* PolyOps::eval_at_point_by_folding (ops.rs:77-84; cpu/circle.rs:143-180): the value at point of
+ * the polynomial whose bit-reversed evaluations on CanonicCoset(log_size) are evals, by one circle
+ * fold and log_size - 1 line folds with the point's coordinates; synchronous. */
+int stwo_hip_eval_at_point_by_folding(stwo_hip_ctx* ctx, const uint32_t* evals, uint32_t log_size,
+                                      stwo_circle_point_qm31 point, stwo_qm31* out);
 /* precompute_twiddles (bindings.rs:165-169; prover/poly/circle/ops.rs:122): caches the twiddle
  * tree of the half coset of CanonicCoset(log_domain) in the context. A tree serves every canonic
  * domain up to its size. tw/itw (host, may be NULL) receive 2^(log_domain-1) words each. */
 int stwo_hip_precompute_twiddles(stwo_hip_ctx* ctx, uint32_t log_domain, uint32_t* tw_host, uint32_t* itw_host);
 /* interpolate_columns (ops.rs:23-39; ntt_b2n_column, bindings.rs:503-510): evals[i] (bit-reversed
- * evaluations on CanonicCoset(log_size)) -> coeffs[i]; in place when the pointers are equal. */
+ * evaluations on CanonicCoset(log_size), log_size >= 1) -> coeffs[i]; in place when the pointers
+ * are equal. */
 int stwo_hip_interpolate_columns(stwo_hip_ctx* ctx, const uint32_t* const* evals, uint32_t* const* coeffs,
                                  uint32_t n_cols, uint32_t log_size);
 /* evaluate_polynomials (ops.rs:98-120; ntt_n2b_columns, bindings.rs:512-519): coeffs (2^log_size)
@@ -108,10 +155,12 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_
  * 2^(log_src-1)) = dst * alpha^2 + fold(src), src on CanonicCoset(log_src). Uses the cached twiddles. */
 int stwo_hip_fold_circle_into_line(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4],
                                    uint32_t* const dst[4], stwo_qm31 alpha);
-/* fold_line (prover/fri.rs:31-44): src on the line domain of log size log_src
- * (Coset::half_odds(log_src)), dst has 2^(log_src-1) words per coordinate. */
+/* fold_line (prover/fri.rs:31-44; fold_line, bindings.rs:196-212): src on the line domain of log
+ * size log_src (Coset::half_odds(log_src)); fold_step folds, the coefficient squared before each
+ * fold after the first (prover/backend/cpu/fri.rs:15-30), so dst has 2^(log_src - fold_step)
+ * words per coordinate. */
 int stwo_hip_fold_line(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
-                       stwo_qm31 alpha);
+                       stwo_qm31 alpha, uint32_t fold_step);
 
 /* ---------------------------------------------------------------- QuotientOps ---------- */
 /* One ColumnSampleBatch (core/pcs/quotients.rs:56-100): a point and, per sampled column, the
@@ -176,8 +225,10 @@ int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* ctx, uint32_t
 /* prove() over the wide-Fibonacci AIR (crates/examples/src/wide_fibonacci/mod.rs:186-261) and the
  * Poseidon2 AIR (crates/examples/src/poseidon/mod.rs:337-419), with the trace already resident:
  * trace[i] are device columns of 2^log_n_rows words (left untouched). The proof is written in the
- * library's canonical byte encoding. status_out: 0 ok, 2 ConstraintsNotSatisfied (prove_ex's
- * sanity check, prover/mod.rs:127-144). Synchronous. */
+ * library's canonical byte encoding. Synchronous. When prove_ex's sanity check fails
+ * (prover/mod.rs:127-144: the reference returns Err(ConstraintsNotSatisfied) and no proof) the
+ * call returns 6, *out is still set, and stwo_proof_status reports 2: the caller inspects the
+ * bytes or frees the handle. */
 typedef struct stwo_proof stwo_proof;
 int stwo_hip_prove_wide_fibonacci(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t n_cols,
                                   uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
@@ -214,6 +265,10 @@ int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* ctx, uint32_t* const* cols, uint32
  * 2 merkle, 3 composition, 4 accumulate_numerators, 5 quotients, 6 fri_fold, 7 oods, 8 logup,
  * 9 grind, 10 misc, 11 fri_tail (FRI layers of at most 2^12 values: Merkle, channel and fold). */
 int stwo_hip_set_kernel_timing(stwo_hip_ctx* ctx, int enable);
+/* No reference counterpart. The device's 32-bit VALU issue rate in lane-instructions per second,
+ * measured with independent chains of add / xor / rotate (the operations of a Blake2s round): the
+ * peak against which VALU-bound kernels (Blake2s hashing, M31 butterflies) are reported. */
+int stwo_hip_measure_int32_issue(stwo_hip_ctx* ctx, double* lane_instr_per_s);
 int stwo_hip_kernel_stats(stwo_hip_ctx* ctx, uint32_t kernel_class, double* ms, double* bytes, uint64_t* launches);
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* ctx);
 

@@ -2,14 +2,21 @@
 #include "../../include/stwo_hip.h"
 
 #include <cstring>
+#include <memory>
+#include <set>
 
 #include "prover.h"
 
 using namespace sh;
 
+struct stwo_hip_twiddles {
+  uint32_t init_index = 0, log_size = 0;  // the root coset, Coset::new(init_index, log_size)
+  DevBuf tw, itw, tw2, itw2;
+};
 struct stwo_hip_ctx {
   Ctx* ctx = nullptr;
   std::string err;
+  std::set<stwo_hip_twiddles*> trees;  // freed with the context if the caller has not
 };
 struct stwo_proof {
   ProveResult r;
@@ -52,6 +59,32 @@ PcsCfg cfg_of(const stwo_pcs_config* c) {
     throw HipError(1, "invalid PcsConfig");
   return r;
 }
+// prove_ex's sanity check (crates/stwo/src/prover/mod.rs:127-144): the reference returns
+// Err(ConstraintsNotSatisfied) and no proof. Here the call returns status 6 and *out still holds
+// the proof, which the caller inspects or frees.
+void reject_unsatisfied(const stwo_proof& p) {
+  if (p.r.status != 0)
+    throw HipError(6, "ConstraintsNotSatisfied: the composition's OODS value differs from the constraints' "
+                      "(prove_ex's sanity check); the proof handle is set and must be freed");
+}
+// A circle transform of same-size columns on the domain whose half coset starts at half_init:
+// domains of log size 1 and 2 by the explicit formulas, larger ones by the FFT kernels over a
+// doubled tree whose last 2^(log_n - 1) entries are the domain's line twiddles.
+void run_fft(Ctx& c, bool inverse, const uint32_t* const* ds, uint32_t* const* dd, uint32_t n_cols, uint32_t log_in,
+             uint32_t log_n, const uint32_t* tree2, uint32_t tree_len, uint32_t half_init) {
+  if (log_n <= 2) launch_fft_tiny(inverse, ds, dd, n_cols, log_in, log_n, index_to_point(half_init), c.stream);
+  else launch_fft(inverse, ds, dd, n_cols, log_in, log_n, tree2, tree_len, c.stream);
+}
+uint32_t canonic_half_init(uint32_t log_n) { return HCoset::half_odds(log_n - 1).initial_index; }
+void check_tree(const stwo_hip_twiddles* t, uint32_t half_init, uint32_t log_n) {
+  // domain.half_coset.is_doubling_of(twiddles.root_coset) (prover/backend/cpu/circle.rs:195)
+  if (!t) throw HipError(1, "null twiddle tree");
+  if (log_n < 1 || log_n - 1 > t->log_size) throw HipError(1, "domain larger than the twiddle tree's coset");
+  const uint32_t d = t->log_size - (log_n - 1);
+  const uint32_t want = d >= 31 ? 0u : (uint32_t)(((uint64_t)t->init_index << d) & 0x7fffffffu);
+  if ((half_init & 0x7fffffffu) != want) throw HipError(1, "the domain's half coset is not a doubling of the tree's coset");
+}
+
 uint32_t fnv1a(const char* s) {
   uint32_t h = 0x811C9DC5u;
   for (; *s; ++s) { h ^= (uint8_t)*s; h *= 0x01000193u; }
@@ -77,6 +110,8 @@ int stwo_hip_create(int device, stwo_hip_ctx** out) {
 }
 void stwo_hip_destroy(stwo_hip_ctx* h) {
   if (!h) return;
+  for (auto* t : h->trees) delete t;
+  h->trees.clear();
   delete h->ctx;
   delete h;
 }
@@ -152,15 +187,15 @@ int stwo_hip_precompute_twiddles(stwo_hip_ctx* h, uint32_t log_domain, uint32_t*
 int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, uint32_t* const* coeffs,
                                  uint32_t n_cols, uint32_t log_size) {
   return guarded(h, [&](Ctx& c) {
-    if (log_size < 3 || log_size > 30) throw HipError(3, "log_size must be in [3, 30]");
-    c.ensure_twiddles(log_size);
+    if (log_size < 1 || log_size > 30) throw HipError(3, "log_size must be in [1, 30]");
+    c.ensure_twiddles(log_size < 2 ? 2 : log_size);
     std::vector<const uint32_t*> s(evals, evals + n_cols);
     std::vector<uint32_t*> d(coeffs, coeffs + n_cols);
     const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
       KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
-      launch_fft(true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), c.stream);
+      run_fft(c, true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), canonic_half_init(log_size));
     }
     c.sync();
     c.reset_arena();
@@ -169,15 +204,16 @@ int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, 
 int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs, uint32_t* const* evals,
                                   uint32_t n_cols, uint32_t log_size, uint32_t log_blowup) {
   return guarded(h, [&](Ctx& c) {
-    if (log_size < 3 || log_size + log_blowup > 30) throw HipError(3, "sizes out of range");
-    c.ensure_twiddles(log_size + log_blowup);
+    if (log_size < 1 || log_size + log_blowup > 30) throw HipError(3, "sizes out of range");
+    const uint32_t log_n = log_size + log_blowup;
+    c.ensure_twiddles(log_n < 2 ? 2 : log_n);
     std::vector<const uint32_t*> s(coeffs, coeffs + n_cols);
     std::vector<uint32_t*> d(evals, evals + n_cols);
     const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
-      KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_size) + 4.0 * (double)(1u << (log_size + log_blowup))));
-      launch_fft(false, ds, dd, n_cols, log_size, log_size + log_blowup, c.tw2.p, c.tree_len(), c.stream);
+      KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_size) + 4.0 * (double)(1u << log_n)));
+      run_fft(c, false, ds, dd, n_cols, log_size, log_n, c.tw2.p, c.tree_len(), canonic_half_init(log_n));
     }
     c.sync();
     c.reset_arena();
@@ -279,15 +315,35 @@ int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint
   });
 }
 int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
-                       stwo_qm31 alpha) {
+                       stwo_qm31 alpha, uint32_t fold_step) {
   return guarded(h, [&](Ctx& c) {
     if (log_src < 1 || log_src > 29) throw HipError(3, "log_src must be in [1, 29]");
+    if (fold_step < 1 || fold_step > log_src) throw HipError(1, "fold_step must be in [1, log_src]");
     check_canonical(alpha);
     c.ensure_twiddles(log_src + 1);
-    {
-      KScope ks(c, K_FRI_FOLD, 24.0 * (double)(1u << log_src));
-      launch_fold_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}}, q(alpha), c.itw.p,
-                       c.tree_len(), c.stream);
+    // fold_step folds, the coefficient squared before each after the first (prover/backend/cpu/fri.rs:15-30);
+    // the intermediate layers live in one temporary buffer: layers after even steps at its start
+    // (at most 4 x 2^(log_src-1) words), after odd steps from word 2^(log_src+1) on (at most
+    // 4 x 2^(log_src-2) words)
+    const size_t tmp_words = fold_step > 1 ? (size_t)3 << log_src : 0;
+    DevBuf tmp(&c, tmp_words ? tmp_words : 1);
+    CCols4 from = {{src[0], src[1], src[2], src[3]}};
+    qm31 a = q(alpha);
+    for (uint32_t st = 0; st < fold_step; ++st) {
+      const uint32_t ls = log_src - st;
+      const size_t half = (size_t)1 << (ls - 1);
+      Cols4 to = {{dst[0], dst[1], dst[2], dst[3]}};
+      if (st + 1 < fold_step) {
+        // alternate between the two halves of the temporary buffer
+        uint32_t* base = tmp.p + ((st & 1) ? ((size_t)2 << log_src) : 0);
+        for (int k = 0; k < 4; ++k) to.c[k] = base + (size_t)k * half;
+      }
+      {
+        KScope ks(c, K_FRI_FOLD, 24.0 * (double)(1u << ls));
+        launch_fold_line(ls, from, to, a, c.itw.p, c.tree_len(), c.stream);
+      }
+      from = {{to.c[0], to.c[1], to.c[2], to.c[3]}};
+      a = qm_mul(a, a);
     }
     c.sync();
   });
@@ -461,6 +517,7 @@ int stwo_hip_prove_wide_fibonacci(stwo_hip_ctx* h, const uint32_t* const* trace,
       throw;
     }
     *out = p;
+    reject_unsatisfied(*p);
   });
 }
 int stwo_hip_prove_wide_fibonacci_multi(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
@@ -486,6 +543,7 @@ int stwo_hip_prove_wide_fibonacci_multi(stwo_hip_ctx* h, const uint32_t* const* 
       throw;
     }
     *out = p;
+    reject_unsatisfied(*p);
   });
 }
 int stwo_hip_prove_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr, uint32_t log_n_rows,
@@ -504,6 +562,7 @@ int stwo_hip_prove_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint3
       throw;
     }
     *out = p;
+    reject_unsatisfied(*p);
   });
 }
 int stwo_proof_status(const stwo_proof* p) { return p ? p->r.status : -1; }
@@ -532,6 +591,161 @@ int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* h, uint32_t* const* cols, uint32_t
     launch_gen_poseidon(log_n_rows, (uint32_t* const*)c.upload_vec(v), ipr, mode, seed, c.stream);
     c.sync();
     c.reset_arena();
+  });
+}
+
+// ---- ColumnOps / Column ------------------------------------------------------------------------
+int stwo_hip_bit_reverse_column(stwo_hip_ctx* h, uint32_t* col, uint32_t log_size) {
+  return guarded(h, [&](Ctx& c) {
+    if (!col) throw HipError(1, "null column");
+    if (log_size > 31) throw HipError(3, "log_size out of range");
+    launch_bit_reverse(col, log_size, c.stream);
+    c.sync();
+  });
+}
+int stwo_hip_column_at(stwo_hip_ctx* h, const uint32_t* col, size_t index, uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!col || !out) throw HipError(1, "null argument");
+    c.d2h(out, col + index, 4);
+  });
+}
+int stwo_hip_column_set(stwo_hip_ctx* h, uint32_t* col, size_t index, uint32_t value) {
+  return guarded(h, [&](Ctx& c) {
+    if (!col) throw HipError(1, "null column");
+    const uint32_t* v = (const uint32_t*)c.upload(&value, 4);
+    launch_copy_words(col + index, v, 1, c.stream);
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_split_at_mid(stwo_hip_ctx* h, uint32_t* col, size_t n_words, uint32_t** left, uint32_t** right) {
+  return guarded(h, [&](Ctx&) {
+    if (!col || !left || !right) throw HipError(1, "null argument");
+    if (n_words < 2 || (n_words & 1)) throw HipError(1, "split_at_mid needs an even length of at least 2");
+    *left = col;
+    *right = col + n_words / 2;
+  });
+}
+
+// ---- PolyOps: extend, twiddles of any coset, transforms on any domain, folding evaluation ------
+int stwo_hip_extend(stwo_hip_ctx* h, const uint32_t* coeffs, uint32_t log_size, uint32_t new_log_size, uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!coeffs || !out) throw HipError(1, "null argument");
+    if (new_log_size < log_size || new_log_size > 31) throw HipError(1, "new log size must be at least the old one");
+    const size_t n = (size_t)1 << log_size, m = (size_t)1 << new_log_size;
+    if (out != coeffs) launch_copy_words(out, coeffs, n, c.stream);
+    if (m > n) launch_fill_zero(out + n, m - n, c.stream);
+    c.sync();
+  });
+}
+int stwo_hip_twiddles_create(stwo_hip_ctx* h, uint32_t coset_initial_index, uint32_t coset_log_size,
+                             stwo_hip_twiddles** out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!out) throw HipError(1, "null out");
+    if (coset_log_size > 29) throw HipError(3, "coset log size must be at most 29");
+    auto t = std::make_unique<stwo_hip_twiddles>();
+    t->init_index = coset_initial_index & 0x7fffffffu;
+    t->log_size = coset_log_size;
+    build_twiddle_tree(c, t->init_index, coset_log_size, t->tw, t->itw, t->tw2, t->itw2);
+    h->trees.insert(t.get());
+    *out = t.release();
+  });
+}
+int stwo_hip_twiddles_read(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uint32_t* tw_host, uint32_t* itw_host) {
+  return guarded(h, [&](Ctx& c) {
+    if (!t || !h->trees.count(const_cast<stwo_hip_twiddles*>(t))) throw HipError(1, "unknown twiddle tree");
+    const size_t n = (size_t)1 << t->log_size;
+    if (tw_host) c.d2h(tw_host, t->tw.p, n * 4);
+    if (itw_host) c.d2h(itw_host, t->itw.p, n * 4);
+  });
+}
+int stwo_hip_twiddles_free(stwo_hip_ctx* h, stwo_hip_twiddles* t) {
+  return guarded(h, [&](Ctx& c) {
+    if (!t || !h->trees.erase(t)) throw HipError(1, "unknown twiddle tree");
+    c.sync();
+    delete t;
+  });
+}
+int stwo_hip_interpolate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uint32_t half_coset_initial_index,
+                                   uint32_t log_size, const uint32_t* const* evals, uint32_t* const* coeffs,
+                                   uint32_t n_cols) {
+  return guarded(h, [&](Ctx& c) {
+    if (!t || !h->trees.count(const_cast<stwo_hip_twiddles*>(t))) throw HipError(1, "unknown twiddle tree");
+    check_tree(t, half_coset_initial_index, log_size);
+    if (n_cols == 0) return;
+    std::vector<const uint32_t*> s(evals, evals + n_cols);
+    std::vector<uint32_t*> d(coeffs, coeffs + n_cols);
+    {
+      KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
+      run_fft(c, true, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
+              log_size, t->itw2.p, 1u << t->log_size, half_coset_initial_index);
+    }
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_evaluate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uint32_t half_coset_initial_index,
+                                uint32_t log_size, const uint32_t* const* coeffs, uint32_t log_coeffs,
+                                uint32_t* const* evals, uint32_t n_cols) {
+  return guarded(h, [&](Ctx& c) {
+    if (!t || !h->trees.count(const_cast<stwo_hip_twiddles*>(t))) throw HipError(1, "unknown twiddle tree");
+    check_tree(t, half_coset_initial_index, log_size);
+    if (log_coeffs < 1 || log_coeffs > log_size) throw HipError(1, "coefficients larger than the domain");
+    if (n_cols == 0) return;
+    std::vector<const uint32_t*> s(coeffs, coeffs + n_cols);
+    std::vector<uint32_t*> d(evals, evals + n_cols);
+    {
+      KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_coeffs) + 4.0 * (double)(1u << log_size)));
+      run_fft(c, false, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_coeffs,
+              log_size, t->tw2.p, 1u << t->log_size, half_coset_initial_index);
+    }
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_eval_at_point_by_folding(stwo_hip_ctx* h, const uint32_t* evals, uint32_t log_size,
+                                      stwo_circle_point_qm31 point, stwo_qm31* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (!evals || !out) throw HipError(1, "null argument");
+    if (log_size < 3 || log_size > 30) throw HipError(3, "log_size must be in [3, 30]");
+    check_canonical(point.x);
+    check_canonical(point.y);
+    c.ensure_twiddles(log_size);
+    // folding coefficients (core/poly/utils.rs:45-70): p.y for the circle fold, then p.x and its
+    // successive doublings for the line folds
+    const cpqm31 p = cp(point);
+    const size_t half = (size_t)1 << (log_size - 1);
+    // zero coordinates 1..3 of the M31 column, then two ping-pong line layers of 4 coordinates
+    DevBuf zero(&c, half * 2, true), la(&c, 4 * half), lb(&c, 4 * (half / 2 ? half / 2 : 1));
+    CCols4 src = {{evals, zero.p, zero.p, zero.p}};
+    Cols4 a4 = {{la.p, la.p + half, la.p + 2 * half, la.p + 3 * half}};
+    launch_fold_circle_into_line_zero(log_size, src, a4, p.y, c.itw.p, c.tree_len(), c.stream);
+    qm31 alpha = p.x;
+    Cols4 cur = a4;
+    DevBuf* bufs[2] = {&la, &lb};
+    for (uint32_t ls = log_size - 1, k = 1; ls >= 1; --ls, ++k) {
+      const size_t nh = (size_t)1 << (ls - 1);
+      DevBuf& nb = *bufs[k & 1];
+      Cols4 nxt = {{nb.p, nb.p + nh, nb.p + 2 * nh, nb.p + 3 * nh}};
+      launch_fold_line(ls, {{cur.c[0], cur.c[1], cur.c[2], cur.c[3]}}, nxt, alpha, c.itw.p, c.tree_len(), c.stream);
+      cur = nxt;
+      alpha = qm_double_x(alpha);
+    }
+    uint32_t v[4];
+    for (int k = 0; k < 4; ++k) c.d2h(&v[k], cur.c[k], 4);
+    const qm31 r = qm_mul_m31({v[0], v[1], v[2], v[3]}, m31_inv(1u << log_size));
+    *out = q(r);
+  });
+}
+
+int stwo_hip_measure_int32_issue(stwo_hip_ctx* h, double* lane_instr_per_s) {
+  return guarded(h, [&](Ctx& c) {
+    if (!lane_instr_per_s) throw HipError(1, "null out");
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    DevBuf scratch(&c, (size_t)cus * 16);
+    c.sync();
+    *lane_instr_per_s = measure_int32_issue(scratch.p, c.stream);
   });
 }
 

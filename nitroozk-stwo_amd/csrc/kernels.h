@@ -45,6 +45,9 @@ void launch_grind_dev(const uint32_t* prefix, uint32_t pow_bits, uint64_t start,
 // ops.hip
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                   uint32_t tree_len, hipStream_t stream);
+// The same into a destination whose previous contents are ignored (dst = fold(src)).
+void launch_fold_circle_into_line_zero(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
+                                       uint32_t tree_len, hipStream_t stream);
 void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
                       hipStream_t stream);
 
@@ -217,5 +220,22 @@ void launch_coset_prefix_sum(uint32_t log_n, uint32_t* const* cols4, qm31 shift,
 
 void launch_fill_zero(uint32_t* p, size_t n, hipStream_t stream);
 void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t stream);
+
+// columns.hip
+// Powers g^(2^b), b < 31, of the circle generator, passed by value to the twiddle-tree kernel.
+struct GenPowers { cpm31 p[31]; };
+// The twiddle tree of Coset::new(init_index, log_size) and its inverse, plain and doubled
+// (2^log_size words each); zero_flag (one word, cleared by the caller) is raised when an
+// x-coordinate is zero.
+void launch_twiddle_tree(uint32_t init_index, uint32_t log_size, uint32_t* tw, uint32_t* itw, uint32_t* tw2,
+                         uint32_t* itw2, uint32_t* zero_flag, hipStream_t stream);
+void launch_bit_reverse(uint32_t* col, uint32_t log_n, hipStream_t stream);
+// Circle transforms on domains of log size 1 or 2 (the half coset's first point given).
+void launch_fft_tiny(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
+                     uint32_t log_n, cpm31 half_coset_initial, hipStream_t stream);
+
+// measure.hip: 32-bit VALU lane-instructions per second on the current device (synchronous; scratch:
+// one word per workgroup, 16 x CUs words).
+double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
 
 }  // namespace sh

@@ -48,12 +48,87 @@ __global__ void __launch_bounds__(256) fold_circle_kernel(uint32_t n_out, CCols4
   store_q(dst.c, i, fr);
 }
 
+// Vector form: lane i produces outputs 4i..4i+3 from source words 8i..8i+7 of each coordinate (two
+// 16-byte loads per coordinate column). The four outputs share one pair [x, y] of line layer 0:
+// their inverse twiddles are y, -y, -x, x, so the odd output of each pair takes its difference
+// the other way round instead of a negated twiddle.
+__device__ __forceinline__ u32x4 ld4(const uint32_t* c, uint32_t i) {
+  return *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(c) + i);
+}
+__device__ __forceinline__ void st4(uint32_t* c, uint32_t i, u32x4 v) { *reinterpret_cast<GLOBAL_AS u32x4*>(gmem(c) + i) = v; }
+__global__ void __launch_bounds__(256) fold_circle_vec_kernel(uint32_t n_quads, CCols4 src, Cols4 dst, qm31 alpha,
+                                                              qm31 alpha2, const uint32_t* __restrict__ lt0,
+                                                              uint32_t dst_zero) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_quads) return;
+  const uint32_t x = lt0[2 * i], y = lt0[2 * i + 1];
+  u32x4 lo[4], hi[4], d[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lo[k] = ld4(src.c[k], 8 * i);
+    hi[k] = ld4(src.c[k], 8 * i + 4);
+  }
+  if (!dst_zero)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = ld4(dst.c[k], 4 * i);
+  u32x4 o[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // source pair (2q, 2q+1) of the lane's eight words
+    qm31 fp, fn;
+    if (q < 2) {
+      fp = {lo[0][2 * q], lo[1][2 * q], lo[2][2 * q], lo[3][2 * q]};
+      fn = {lo[0][2 * q + 1], lo[1][2 * q + 1], lo[2][2 * q + 1], lo[3][2 * q + 1]};
+    } else {
+      fp = {hi[0][2 * q - 4], hi[1][2 * q - 4], hi[2][2 * q - 4], hi[3][2 * q - 4]};
+      fn = {hi[0][2 * q - 3], hi[1][2 * q - 3], hi[2][2 * q - 3], hi[3][2 * q - 3]};
+    }
+    const uint32_t t = q < 2 ? y : x;
+    const bool swap = q == 1 || q == 2;  // twiddles -y and -x
+    qm31 f0 = qm_add(fp, fn);
+    qm31 f1 = qm_mul_m31(swap ? qm_sub(fn, fp) : qm_sub(fp, fn), t);
+    qm31 fr = qm_add(f0, qm_mul(alpha, f1));
+    if (!dst_zero) fr = qm_add(qm_mul(qm31{d[0][q], d[1][q], d[2][q], d[3][q]}, alpha2), fr);
+    o[0][q] = fr.a;
+    o[1][q] = fr.b;
+    o[2][q] = fr.c;
+    o[3][q] = fr.d;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) st4(dst.c[k], 4 * i, o[k]);
+}
+
+static bool aligned16(const uint32_t* const* c, int n) {
+  for (int k = 0; k < n; ++k)
+    if ((reinterpret_cast<uintptr_t>(c[k]) & 15) != 0) return false;
+  return true;
+}
+
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                   uint32_t tree_len, hipStream_t stream) {
   uint32_t n = 1u << (log_src - 1);
   const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
+  if (log_src >= 3 && aligned16(src.c, 4) && aligned16((const uint32_t* const*)dst.c, 4)) {
+    const uint32_t nq = n / 4;
+    hipLaunchKernelGGL(fold_circle_vec_kernel, dim3((nq + 255) / 256), dim3(256), 0, stream, nq, src, dst, alpha,
+                       qm_mul(alpha, alpha), lt0, 0u);
+    return;
+  }
   hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha,
                      qm_mul(alpha, alpha), lt0, (const qm31*)nullptr, 0u);
+}
+void launch_fold_circle_into_line_zero(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
+                                       uint32_t tree_len, hipStream_t stream) {
+  uint32_t n = 1u << (log_src - 1);
+  const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
+  if (log_src >= 3 && aligned16(src.c, 4) && aligned16((const uint32_t* const*)dst.c, 4)) {
+    const uint32_t nq = n / 4;
+    hipLaunchKernelGGL(fold_circle_vec_kernel, dim3((nq + 255) / 256), dim3(256), 0, stream, nq, src, dst, alpha,
+                       qm_mul(alpha, alpha), lt0, 1u);
+    return;
+  }
+  hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha,
+                     qm_mul(alpha, alpha), lt0, (const qm31*)nullptr, 1u);
 }
 void launch_fold_circle_into_line_dev(uint32_t log_src, CCols4 src, Cols4 dst, const qm31* alpha_dev, bool dst_zero,
                                       const uint32_t* itree, uint32_t tree_len, hipStream_t stream) {
@@ -79,10 +154,52 @@ __global__ void __launch_bounds__(256) fold_line_kernel(uint32_t n_out, CCols4 s
   store_q(dst.c, i, qm_add(f0, qm_mul(alpha, f1)));
 }
 
+// Vector form: lane i produces outputs 4i..4i+3 from source words 8i..8i+7 of each coordinate and
+// the four inverse twiddles lt[4i..4i+3] (one 16-byte load each).
+__global__ void __launch_bounds__(256) fold_line_vec_kernel(uint32_t n_quads, CCols4 src, Cols4 dst, qm31 alpha,
+                                                            const uint32_t* __restrict__ lt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_quads) return;
+  const u32x4 t = ld4(lt, 4 * i);
+  u32x4 lo[4], hi[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lo[k] = ld4(src.c[k], 8 * i);
+    hi[k] = ld4(src.c[k], 8 * i + 4);
+  }
+  u32x4 o[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    qm31 fx, fnx;
+    if (q < 2) {
+      fx = {lo[0][2 * q], lo[1][2 * q], lo[2][2 * q], lo[3][2 * q]};
+      fnx = {lo[0][2 * q + 1], lo[1][2 * q + 1], lo[2][2 * q + 1], lo[3][2 * q + 1]};
+    } else {
+      fx = {hi[0][2 * q - 4], hi[1][2 * q - 4], hi[2][2 * q - 4], hi[3][2 * q - 4]};
+      fnx = {hi[0][2 * q - 3], hi[1][2 * q - 3], hi[2][2 * q - 3], hi[3][2 * q - 3]};
+    }
+    qm31 f0 = qm_add(fx, fnx);
+    qm31 f1 = qm_mul_m31(qm_sub(fx, fnx), t[q]);
+    qm31 fr = qm_add(f0, qm_mul(alpha, f1));
+    o[0][q] = fr.a;
+    o[1][q] = fr.b;
+    o[2][q] = fr.c;
+    o[3][q] = fr.d;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) st4(dst.c[k], 4 * i, o[k]);
+}
+
 void launch_fold_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
                       hipStream_t stream) {
   uint32_t n = 1u << (log_src - 1);
   const uint32_t* lt = itree + tree_len - (1u << log_src);
+  if (log_src >= 3 && aligned16(src.c, 4) && aligned16((const uint32_t* const*)dst.c, 4) &&
+      (reinterpret_cast<uintptr_t>(lt) & 15) == 0) {
+    const uint32_t nq = n / 4;
+    hipLaunchKernelGGL(fold_line_vec_kernel, dim3((nq + 255) / 256), dim3(256), 0, stream, nq, src, dst, alpha, lt);
+    return;
+  }
   hipLaunchKernelGGL(fold_line_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha, lt,
                      (const qm31*)nullptr, 0u);
 }

@@ -297,49 +297,32 @@ void Ctx::d2h(void* dst, const void* src, size_t bytes) {
   sync_main();
 }
 
-// Twiddle tree of the half coset of CanonicCoset(log_domain) (prover/backend/cpu/circle.rs:274-291):
-// layer j = x-coordinates of the first half of the coset doubled j times, bit-reversed; pad 1.
-// A tree built for a larger domain serves every smaller canonic domain: line twiddles are read
-// from the end of the buffer (core/poly/utils.rs:98-114).
+// Twiddle tree of the coset (init_index, log_size) (prover/backend/cpu/circle.rs:274-291): layer j =
+// x-coordinates of the first half of the coset doubled j times, bit-reversed; pad 1. Built on the
+// device (one lane per entry: the point by the generator's powers, the inverse by Fermat), so
+// that a tree of 2^29 entries costs milliseconds instead of a serial host loop.
+void build_twiddle_tree(Ctx& c, uint32_t init_index, uint32_t log_size, DevBuf& tw, DevBuf& itw, DevBuf& tw2,
+                        DevBuf& itw2) {
+  if (log_size > 30) throw HipError(3, "coset log size must be at most 30");
+  PersistentAllocScope keep(c);
+  const size_t len = (size_t)1 << log_size;
+  tw = DevBuf(&c, len);
+  itw = DevBuf(&c, len);
+  tw2 = DevBuf(&c, len);
+  itw2 = DevBuf(&c, len);
+  DevBuf flag(&c, 1, true);
+  launch_twiddle_tree(init_index, log_size, tw.p, itw.p, tw2.p, itw2.p, flag.p, c.stream);
+  uint32_t zero = 0;
+  c.d2h(&zero, flag.p, 4);
+  if (zero) throw HipError(1, "the coset has a point with x = 0, whose twiddle has no inverse");
+}
+
+// The canonic tree (root coset: the half coset of CanonicCoset(log_domain)) serves every smaller
+// canonic domain: line twiddles are read from the end of the buffer (core/poly/utils.rs:98-114).
 void Ctx::ensure_twiddles(uint32_t log_domain) {
   if (log_domain <= tw_log) return;
-  PersistentAllocScope keep(*this);
-  uint32_t R = log_domain - 1;  // root coset log size
-  size_t len = (size_t)1 << R;
-  std::vector<uint32_t> t, it;
-  t.reserve(len);
-  HCoset c = HCoset::half_odds(R);
-  for (uint32_t j = 0; j < R; ++j) {
-    size_t half = (size_t)1 << (c.log_size - 1);
-    size_t i0 = t.size();
-    cpm31 p = index_to_point(c.initial_index), st = index_to_point(c.step_size);
-    for (size_t i = 0; i < half; ++i) { t.push_back(p.x); p = cp_add(p, st); }
-    uint32_t lg = c.log_size - 1;
-    for (size_t i = 0; i < half; ++i) {
-      size_t k = bitrev((uint32_t)i, lg);
-      if (k > i) std::swap(t[i0 + i], t[i0 + k]);
-    }
-    c = c.doubled();
-  }
-  t.push_back(1);
-  // batch inverse
-  it.resize(t.size());
-  std::vector<uint32_t> pre(t.size());
-  uint32_t acc = 1;
-  for (size_t i = 0; i < t.size(); ++i) { pre[i] = acc; acc = m31_mul(acc, t[i]); }
-  uint32_t inv = m31_inv(acc);
-  for (size_t i = t.size(); i-- > 0;) { it[i] = m31_mul(inv, pre[i]); inv = m31_mul(inv, t[i]); }
-  tw = DevBuf(this, len);
-  itw = DevBuf(this, len);
-  HIP_CHECK(hipMemcpyAsync(tw.p, t.data(), len * 4, hipMemcpyHostToDevice, stream));
-  HIP_CHECK(hipMemcpyAsync(itw.p, it.data(), len * 4, hipMemcpyHostToDevice, stream));
-  sync();
-  for (size_t i = 0; i < len; ++i) { t[i] *= 2; it[i] *= 2; }
-  tw2 = DevBuf(this, len);
-  itw2 = DevBuf(this, len);
-  HIP_CHECK(hipMemcpyAsync(tw2.p, t.data(), len * 4, hipMemcpyHostToDevice, stream));
-  HIP_CHECK(hipMemcpyAsync(itw2.p, it.data(), len * 4, hipMemcpyHostToDevice, stream));
-  sync();
+  const uint32_t R = log_domain - 1;  // root coset log size
+  build_twiddle_tree(*this, HCoset::half_odds(R).initial_index, R, tw, itw, tw2, itw2);
   tw_log = log_domain;
 }
 

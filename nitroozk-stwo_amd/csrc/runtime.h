@@ -211,6 +211,10 @@ struct UploadBatch {
   ~UploadBatch() { if (!committed) c.commit_staged(); }
 };
 
+// The twiddle tree of Coset::new(init_index, log_size), plain and doubled (2^log_size words each).
+void build_twiddle_tree(Ctx& c, uint32_t init_index, uint32_t log_size, DevBuf& tw, DevBuf& itw, DevBuf& tw2,
+                        DevBuf& itw2);
+
 // ---- host Blake2s and channel ------------------------------------------------------------
 struct Blake2sHost {
   uint32_t h[8];

@@ -16,11 +16,16 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libstwo_hip.so")
 P = (1 << 31) - 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "unsupported size", 4: "protocol failure",
-          5: "unknown component"}
+          5: "unknown component", 6: "constraints not satisfied"}
+ST_UNSATISFIED = 6
 
 
 class StwoHipError(RuntimeError):
-    pass
+    """A non-zero status from the library; .status holds the C-ABI code."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 class QM31(ctypes.Structure):
@@ -78,6 +83,10 @@ EXPORTS = [
     "stwo_proof_status", "stwo_proof_size", "stwo_proof_bytes", "stwo_proof_digest", "stwo_proof_claimed_sum",
     "stwo_proof_stage_ms", "stwo_proof_free", "stwo_hip_gen_trace_wide_fibonacci", "stwo_hip_gen_trace_poseidon",
     "stwo_hip_set_kernel_timing", "stwo_hip_kernel_stats", "stwo_hip_reset_kernel_stats",
+    "stwo_hip_bit_reverse_column", "stwo_hip_column_at", "stwo_hip_column_set", "stwo_hip_split_at_mid",
+    "stwo_hip_extend", "stwo_hip_twiddles_create", "stwo_hip_twiddles_read", "stwo_hip_twiddles_free",
+    "stwo_hip_interpolate_on_domain", "stwo_hip_evaluate_on_domain", "stwo_hip_eval_at_point_by_folding",
+    "stwo_hip_measure_int32_issue",
 ]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
@@ -117,7 +126,7 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_batch_barycentric_eval.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, _VPP, c.POINTER(QM31)]
     L.stwo_hip_commit_on_layer.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32, c.c_void_p]
     L.stwo_hip_fold_circle_into_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
-    L.stwo_hip_fold_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
+    L.stwo_hip_fold_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31, c.c_uint32]
     L.stwo_hip_accumulate_numerators.argtypes = [c.c_void_p, c.c_uint32, _VPP, c.c_uint32, c.POINTER(SampleBatch),
                                                  c.c_uint32, _VPP, c.POINTER(QM31)]
     L.stwo_hip_compute_quotients_and_combine.argtypes = [c.c_void_p, c.POINTER(AccumulatedNumerators), c.c_uint32,
@@ -155,6 +164,22 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_kernel_stats.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double), c.POINTER(c.c_double),
                                         c.POINTER(c.c_uint64)]
     L.stwo_hip_reset_kernel_stats.argtypes = [c.c_void_p]
+    L.stwo_hip_bit_reverse_column.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32]
+    L.stwo_hip_column_at.argtypes = [c.c_void_p, c.c_void_p, c.c_size_t, _U32P]
+    L.stwo_hip_column_set.argtypes = [c.c_void_p, c.c_void_p, c.c_size_t, c.c_uint32]
+    L.stwo_hip_split_at_mid.argtypes = [c.c_void_p, c.c_void_p, c.c_size_t, c.POINTER(c.c_void_p),
+                                        c.POINTER(c.c_void_p)]
+    L.stwo_hip_extend.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, c.c_uint32, c.c_void_p]
+    L.stwo_hip_twiddles_create.argtypes = [c.c_void_p, c.c_uint32, c.c_uint32, c.POINTER(c.c_void_p)]
+    L.stwo_hip_twiddles_read.argtypes = [c.c_void_p, c.c_void_p, _U32P, _U32P]
+    L.stwo_hip_twiddles_free.argtypes = [c.c_void_p, c.c_void_p]
+    L.stwo_hip_interpolate_on_domain.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, c.c_uint32, _VPP, _VPP,
+                                                 c.c_uint32]
+    L.stwo_hip_evaluate_on_domain.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, c.c_uint32, _VPP, c.c_uint32,
+                                              _VPP, c.c_uint32]
+    L.stwo_hip_eval_at_point_by_folding.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, CirclePointQM31,
+                                                    c.POINTER(QM31)]
+    L.stwo_hip_measure_int32_issue.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
     _lib = L
     return L
 
@@ -197,10 +222,12 @@ def _concat_ptrs(lists):
 class DeviceColumn:
     """A device-resident column of u32 words (M31 values), freed with its backend."""
 
-    def __init__(self, backend, n_words, zeroed=False, ptr=None):
+    def __init__(self, backend, n_words, zeroed=False, ptr=None, parent=None):
         self.backend = backend
         self.n = int(n_words)
-        if ptr is not None:  # an address the backend already allocated (columns())
+        # a view (split_at_mid) borrows its parent's memory and keeps the parent alive
+        self.parent = parent
+        if ptr is not None:  # an address the backend already allocated (columns()), or a view
             self.ptr = ptr
             return
         p = ctypes.c_void_p()
@@ -212,9 +239,26 @@ class DeviceColumn:
         return self.n.bit_length() - 1
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.parent is None:
             self.backend.lib.stwo_hip_free(self.backend.ctx, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    # Column::at / set / split_at_mid (crates/stwo/src/prover/backend/mod.rs:59-62,90-91)
+    def at(self, index):
+        v = ctypes.c_uint32()
+        self.backend._check(self.backend.lib.stwo_hip_column_at(self.backend.ctx, self.ptr, index, ctypes.byref(v)))
+        return v.value
+
+    def set(self, index, value):
+        self.backend._check(self.backend.lib.stwo_hip_column_set(self.backend.ctx, self.ptr, index, value))
+
+    def split_at_mid(self):
+        lft, rgt = ctypes.c_void_p(), ctypes.c_void_p()
+        self.backend._check(self.backend.lib.stwo_hip_split_at_mid(self.backend.ctx, self.ptr, self.n,
+                                                                   ctypes.byref(lft), ctypes.byref(rgt)))
+        root = self.parent if self.parent is not None else self
+        return (DeviceColumn(self.backend, self.n // 2, ptr=lft.value, parent=root),
+                DeviceColumn(self.backend, self.n // 2, ptr=rgt.value, parent=root))
 
     def to_cpu(self):
         out = np.empty(self.n, np.uint32)
@@ -226,6 +270,30 @@ class DeviceColumn:
             self.free()
         except Exception:
             pass
+
+
+class TwiddleTree:
+    """A twiddle tree of any coset, owned by the device (TwiddleTree<B>, prover/poly/twiddles.rs:9-14)."""
+
+    def __init__(self, backend, coset_initial_index, coset_log_size):
+        self.backend = backend
+        self.log_size = coset_log_size
+        h = ctypes.c_void_p()
+        backend._check(backend.lib.stwo_hip_twiddles_create(backend.ctx, coset_initial_index, coset_log_size,
+                                                            ctypes.byref(h)))
+        self.handle = h.value
+
+    def read(self):
+        n = 1 << self.log_size
+        tw, itw = np.empty(n, np.uint32), np.empty(n, np.uint32)
+        self.backend._check(self.backend.lib.stwo_hip_twiddles_read(self.backend.ctx, self.handle,
+                                                                    tw.ctypes.data_as(_U32P), itw.ctypes.data_as(_U32P)))
+        return tw, itw
+
+    def free(self):
+        if self.handle and self.backend.ctx:
+            self.backend._check(self.backend.lib.stwo_hip_twiddles_free(self.backend.ctx, self.handle))
+        self.handle = None
 
 
 class Proof:
@@ -255,7 +323,7 @@ class HipBackend:
         p = ctypes.c_void_p()
         st = self.lib.stwo_hip_create(device, ctypes.byref(p))
         if st != 0:
-            raise StwoHipError("stwo_hip_create failed (status %d): is a GPU visible?" % st)
+            raise StwoHipError("stwo_hip_create failed (status %d): is a GPU visible?" % st, st)
         self.ctx = p.value
 
     def close(self):
@@ -266,7 +334,7 @@ class HipBackend:
     def _check(self, st):
         if st != 0:
             msg = self.lib.stwo_hip_last_error(self.ctx)
-            raise StwoHipError("%s: %s" % (STATUS.get(st, st), msg.decode() if msg else ""))
+            raise StwoHipError("%s: %s" % (STATUS.get(st, st), msg.decode() if msg else ""), st)
 
     # ---- columns ----
     def column(self, n_words, zeroed=False):
@@ -328,6 +396,42 @@ class HipBackend:
                                                            log_blowup))
         return out
 
+    def bit_reverse_column(self, col, log_size):
+        """ColumnOps::bit_reverse_column, in place."""
+        self._check(self.lib.stwo_hip_bit_reverse_column(self.ctx, col.ptr, log_size))
+
+    def extend(self, coeffs, log_size, new_log_size):
+        """PolyOps::extend: the coefficients followed by zeros, in a new column."""
+        out = self.column(1 << new_log_size)
+        self._check(self.lib.stwo_hip_extend(self.ctx, coeffs.ptr, log_size, new_log_size, out.ptr))
+        return out
+
+    def split_at_mid(self, coeffs):
+        """PolyOps::split_at_mid: (p_left, p_right) as views of the coefficient column."""
+        return coeffs.split_at_mid()
+
+    def twiddles(self, coset_initial_index, coset_log_size):
+        """PolyOps::precompute_twiddles for Coset::new(coset_initial_index, coset_log_size)."""
+        return TwiddleTree(self, coset_initial_index, coset_log_size)
+
+    def interpolate_on_domain(self, tree, half_coset_initial_index, log_size, evals):
+        out = [self.column(1 << log_size) for _ in evals]
+        self._check(self.lib.stwo_hip_interpolate_on_domain(self.ctx, tree.handle, half_coset_initial_index, log_size,
+                                                            _ptrs(evals), _ptrs(out), len(evals)))
+        return out
+
+    def evaluate_on_domain(self, tree, half_coset_initial_index, log_size, coeffs, log_coeffs):
+        out = [self.column(1 << log_size) for _ in coeffs]
+        self._check(self.lib.stwo_hip_evaluate_on_domain(self.ctx, tree.handle, half_coset_initial_index, log_size,
+                                                         _ptrs(coeffs), log_coeffs, _ptrs(out), len(coeffs)))
+        return out
+
+    def eval_at_point_by_folding(self, evals, log_size, point):
+        out = QM31()
+        pt = CirclePointQM31(QM31.of(point[:4]), QM31.of(point[4:]))
+        self._check(self.lib.stwo_hip_eval_at_point_by_folding(self.ctx, evals.ptr, log_size, pt, ctypes.byref(out)))
+        return out.tuple()
+
     def batch_eval_at_point(self, coeffs, log_size, point):
         out = (QM31 * len(coeffs))()
         pt = CirclePointQM31(QM31.of(point[:4]), QM31.of(point[4:]))
@@ -360,9 +464,10 @@ class HipBackend:
         self._check(self.lib.stwo_hip_fold_circle_into_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst4),
                                                             QM31.of(alpha)))
 
-    def fold_line(self, src4, alpha, log_src):
-        dst = [self.column(1 << (log_src - 1)) for _ in range(4)]
-        self._check(self.lib.stwo_hip_fold_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst), QM31.of(alpha)))
+    def fold_line(self, src4, alpha, log_src, fold_step=1):
+        dst = [self.column(1 << (log_src - fold_step)) for _ in range(4)]
+        self._check(self.lib.stwo_hip_fold_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst), QM31.of(alpha),
+                                                fold_step))
         return dst
 
     # ---- QuotientOps ----
@@ -437,6 +542,12 @@ class HipBackend:
     def set_kernel_timing(self, enable):
         self._check(self.lib.stwo_hip_set_kernel_timing(self.ctx, int(enable)))
 
+    def measure_int32_issue(self):
+        """32-bit VALU lane-instructions per second of this device (the VALU roofline's peak)."""
+        v = ctypes.c_double()
+        self._check(self.lib.stwo_hip_measure_int32_issue(self.ctx, ctypes.byref(v)))
+        return v.value
+
     def reset_kernel_stats(self):
         self._check(self.lib.stwo_hip_reset_kernel_stats(self.ctx))
 
@@ -464,27 +575,40 @@ class HipBackend:
                                                          seed))
         return cols
 
-    def prove_wide_fibonacci(self, trace, log_n_rows, config):
-        h = ctypes.c_void_p()
-        self._check(self.lib.stwo_hip_prove_wide_fibonacci(self.ctx, _ptrs(trace), len(trace), log_n_rows,
-                                                           ctypes.byref(config), ctypes.byref(h)))
+    def _proof(self, st, h, allow_unsatisfied):
+        """A proof from a whole-proof call. Status 6 is prove_ex's ConstraintsNotSatisfied: the
+        reference returns no proof, so this raises unless the caller asked for the bytes anyway."""
+        if st == ST_UNSATISFIED and h.value:
+            pr = Proof(self, h.value)
+            if allow_unsatisfied:
+                return pr
+        self._check(st)
         return Proof(self, h.value)
 
-    def prove_wide_fibonacci_multi(self, traces, log_n_rows, config):
+    def prove_wide_fibonacci(self, trace, log_n_rows, config, allow_unsatisfied=False):
+        h = ctypes.c_void_p()
+        st = self.lib.stwo_hip_prove_wide_fibonacci(self.ctx, _ptrs(trace), len(trace), log_n_rows,
+                                                    ctypes.byref(config), ctypes.byref(h))
+        return self._proof(st, h, allow_unsatisfied)
+
+    def prove_wide_fibonacci_multi(self, traces, log_n_rows, config, allow_unsatisfied=False):
         """traces[i]: the columns of component i (2^log_n_rows[i] rows each)."""
         logs = np.array(log_n_rows, np.uint32)
         ncols = np.array([len(t) for t in traces], np.uint32)
         h = ctypes.c_void_p()
-        self._check(self.lib.stwo_hip_prove_wide_fibonacci_multi(self.ctx, _concat_ptrs(traces), len(traces),
-                                                                 logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
-                                                                 ctypes.byref(config), ctypes.byref(h)))
-        return Proof(self, h.value)
+        st = self.lib.stwo_hip_prove_wide_fibonacci_multi(self.ctx, _concat_ptrs(traces), len(traces),
+                                                          logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
+                                                          ctypes.byref(config), ctypes.byref(h))
+        return self._proof(st, h, allow_unsatisfied)
 
-    def prove_poseidon(self, trace, instances_per_row, log_n_rows, config):
+    def prove_poseidon(self, trace, instances_per_row, log_n_rows, config, allow_unsatisfied=False):
+        """The Poseidon2 AIR's proofs fail prove_ex's sanity check in this protocol version (its
+        constraint degree exceeds what the lifted protocol supports), so a caller that wants the
+        bytes passes allow_unsatisfied=True and finds status 2 on the proof."""
         h = ctypes.c_void_p()
-        self._check(self.lib.stwo_hip_prove_poseidon(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
-                                                     ctypes.byref(config), ctypes.byref(h)))
-        return Proof(self, h.value)
+        st = self.lib.stwo_hip_prove_poseidon(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
+                                              ctypes.byref(config), ctypes.byref(h))
+        return self._proof(st, h, allow_unsatisfied)
 
 
 def fnv1a(name: str) -> int:

@@ -274,9 +274,20 @@ def test_wide_fib_multi_component_byte_identical(hb, logs, ncols, cfg):
 def test_poseidon_proof_byte_identical(hb, log_n, ipr, mode):
     say("poseidon proof", log_n, ipr, mode)
     trace = hb.gen_trace_poseidon(ipr, log_n, mode, 33)
-    pr = hb.prove_poseidon(trace, ipr, log_n, cfg_of(WF))
+    pr = hb.prove_poseidon(trace, ipr, log_n, cfg_of(WF), allow_unsatisfied=True)
     st, proof, dig, cs, _ = O.prove(air="poseidon", log_n_rows=log_n, ipr=ipr, input_mode=mode, seed=33, **WF)
     assert pr.claimed_sum == tuple(map(int, cs))
     assert pr.status == st
     assert pr.bytes == proof
     assert pr.digest == dig
+
+
+def test_poseidon_unsatisfied_proof_is_an_error_by_default(hb):
+    """prove_ex returns Err(ConstraintsNotSatisfied) for this AIR (prover/mod.rs:127-144): the
+    library reports status 6 unless the caller asks for the bytes."""
+    trace = hb.gen_trace_poseidon(1, 5, 1, 33)
+    with pytest.raises(SH.StwoHipError) as e:
+        hb.prove_poseidon(trace, 1, 5, cfg_of(WF))
+    assert e.value.status == 6
+    pr = hb.prove_poseidon(trace, 1, 5, cfg_of(WF), allow_unsatisfied=True)
+    assert pr.status == 2 and len(pr.bytes) > 0
