@@ -286,6 +286,74 @@ def serve(args, m):
         sys.exit(1)
 
 
+def proof_forms(args, hb, m, cfg, trace, log_rows, fused_times):
+    """The metric's second half: the plain per-call form (every device step a separate C-ABI trait
+    call that synchronises, as the reference's generic prove() drives a backend) against the
+    batched form (the fused, device-resident prover on --contexts contexts proving concurrently
+    from as many host threads). Both prove the same trace; proofs/s and p50/p99 per form."""
+    import threading
+    is_pos = args.workload == "poseidon"
+    n_plain = max(3, min(args.steps, 10))
+    if is_pos:
+        plain_one = lambda: hb.prove_poseidon_per_call(trace, args.ipr, log_rows, cfg, allow_unsatisfied=True)
+    else:
+        plain_one = lambda: hb.prove_wide_fibonacci_per_call([trace], [log_rows], cfg)
+    ref = plain_one()  # warm-up (and the bytes the other forms must equal)
+    device_sync(hb)
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(n_plain):
+        s0 = time.perf_counter()
+        pr = plain_one()
+        times.append((time.perf_counter() - s0) * 1e3)
+    wall = time.perf_counter() - t0
+    pct = lambda v, q: round(sorted(v)[max(0, int(round(q * len(v))) - 1)], 3)
+    plain = {"proofs": n_plain, "ms_per_proof": round(wall * 1e3 / n_plain, 3), "proofs_per_s": round(n_plain / wall, 2),
+             "p50_ms": pct(times, 0.5), "p99_ms": pct(times, 0.99), "bytes_equal_fused": pr.bytes == ref.bytes}
+    # batched: K contexts, each with its own copy of the trace, proving concurrently
+    K = max(1, args.contexts)
+    per = max(2, args.steps // K)
+    backends, traces = [], []
+    for _ in range(K):
+        b = m.HipBackend(0)
+        tr = (b.gen_trace_poseidon(args.ipr, log_rows, 1, args.seed) if is_pos
+              else b.gen_trace_wide_fibonacci(args.n_cols, log_rows, 1, args.seed))
+        backends.append(b)
+        traces.append(tr)
+    prove_on = ((lambda b, tr: b.prove_poseidon(tr, args.ipr, log_rows, cfg, allow_unsatisfied=True)) if is_pos
+                else (lambda b, tr: b.prove_wide_fibonacci(tr, log_rows, cfg)))
+    for b, tr in zip(backends, traces):  # warm-up per context
+        prove_on(b, tr)
+        prove_on(b, tr)
+    lat, same, lock = [], [], threading.Lock()
+
+    def worker(b, tr):
+        for _ in range(per):
+            s0 = time.perf_counter()
+            pr = prove_on(b, tr)
+            d = (time.perf_counter() - s0) * 1e3
+            with lock:
+                lat.append(d)
+                same.append(pr.bytes == ref.bytes)
+
+    th = [threading.Thread(target=worker, args=(b, tr)) for b, tr in zip(backends, traces)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    bwall = time.perf_counter() - t0
+    for b in backends:
+        b.close()
+    batched = {"contexts": K, "proofs": len(lat), "proofs_per_s": round(len(lat) / bwall, 2),
+               "ms_per_proof": round(bwall * 1e3 / len(lat), 3), "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+               "bytes_equal_plain": all(same)}
+    fused = {"contexts": 1, "proofs": len(fused_times), "proofs_per_s": round(1e3 / statistics.mean(fused_times), 2),
+             "p50_ms": pct(fused_times, 0.5), "p99_ms": pct(fused_times, 0.99)}
+    return {"forms": {"plain_per_call": plain, "fused_one_context": fused, "batched": batched,
+                      "batched_over_plain_proofs_per_s": round(batched["proofs_per_s"] / plain["proofs_per_s"], 2)}}
+
+
 def poseidon_extras(args, hb, m, cfg, trace, log_rows):
     """Two more Poseidon2 regions after the metric's: the end-to-end time per proof with the trace
     generated on the device inside the step (as the reference README times it), and the README's
@@ -339,6 +407,8 @@ def main():
                     help="PoW bits (default: 5 for wide-Fibonacci and mixed, as the wide-Fibonacci "
                          "configuration states; 10, the reference's PcsConfig default, for Poseidon2)")
     ap.add_argument("--cpu-runs", type=int, default=5, help="oracle runs for the CPU baseline median")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the plain per-call and batched (multi-context) regions")
     ap.add_argument("--no-extras", action="store_true",
                     help="Poseidon2: skip the end-to-end (trace generation included) and 1-per-row regions")
     ap.add_argument("--seed", type=int, default=2026)
@@ -430,8 +500,10 @@ def main():
     stage = {k: round(v, 2) for k, v in last.stage_ms.items()}
 
     extras = {}
+    if not args.no_forms:
+        extras.update(proof_forms(args, hb, m, cfg, trace, log_rows, times))
     if args.workload == "poseidon" and not args.no_extras:
-        extras = poseidon_extras(args, hb, m, cfg, trace, log_rows)
+        extras.update(poseidon_extras(args, hb, m, cfg, trace, log_rows))
 
     cpu = None
     parity = None

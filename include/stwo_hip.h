@@ -241,6 +241,28 @@ int stwo_hip_prove_poseidon(stwo_hip_ctx* ctx, const uint32_t* const* trace, uin
 int stwo_hip_prove_wide_fibonacci_multi(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t n_components,
                                         const uint32_t* log_n_rows, const uint32_t* n_cols,
                                         const stwo_pcs_config* config, stwo_proof** out);
+/* The same proofs driven through the trait entry points above, one C-ABI call per trait-method
+ * invocation, in the order the reference's generic prove() makes them on a backend
+ * (prover/mod.rs:28-150, pcs/mod.rs, fri.rs, vcs/prover.rs, quotient_ops.rs): the host side a Rust
+ * HipBackend runs above this ABI, in C++. This is the plain per-call form of the prover; the
+ * entry points above are the fused, device-resident form. Both produce the same bytes.
+ * Synchronous; status 6 as above. */
+int stwo_hip_prove_wide_fibonacci_per_call(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t n_components,
+                                            const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                            const stwo_pcs_config* config, stwo_proof** out);
+int stwo_hip_prove_poseidon_per_call(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t instances_per_row,
+                                     uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
+/* verify (core/verifier.rs:19-118 with CommitmentSchemeVerifier::verify_values,
+ * core/pcs/verifier.rs:60-158) on the host, for proofs in the library's encoding: the statement is
+ * the components' shapes (and, for Poseidon2, the claimed LogUp sum) and the expected PcsConfig,
+ * which the proof's own must equal. Returns 0 when the proof is accepted, 4 when it is rejected
+ * (err receives the reference's error name, e.g. "OodsNotMatching", "ProofOfWork",
+ * "Merkle (tree): RootMismatch"), 1 for invalid arguments. Needs no context and no GPU;
+ * thread-safe. */
+int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_components, const uint32_t* log_n_rows,
+                               const uint32_t* n_cols, const stwo_pcs_config* config, char* err, size_t err_len);
+int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t instances_per_row, uint32_t log_n_rows,
+                         stwo_qm31 claimed_sum, const stwo_pcs_config* config, char* err, size_t err_len);
 int stwo_proof_status(const stwo_proof* p);
 size_t stwo_proof_size(const stwo_proof* p);
 const uint8_t* stwo_proof_bytes(const stwo_proof* p);
@@ -257,6 +279,14 @@ int stwo_hip_gen_trace_wide_fibonacci(stwo_hip_ctx* ctx, uint32_t* const* cols, 
                                       uint32_t mode, uint64_t seed);
 int stwo_hip_gen_trace_poseidon(stwo_hip_ctx* ctx, uint32_t* const* cols, uint32_t instances_per_row,
                                 uint32_t log_n_rows, uint32_t mode, uint64_t seed);
+/* The Poseidon2 AIR's LogUp interaction trace (crates/examples/src/poseidon/mod.rs:305-335 with
+ * LogupTraceGenerator, constraint-framework/src/prover/logup.rs:21-200): out receives 4 x
+ * instances_per_row columns of 2^log_n_rows (the fractions of each pair of instances as QM31
+ * coordinates, the last 4 the cumulative sum shifted by claimed_sum / 2^log_n_rows); z and alpha
+ * are the lookup elements drawn from the channel. Synchronous. */
+int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t instances_per_row,
+                                            uint32_t log_n_rows, stwo_qm31 z, stwo_qm31 alpha, uint32_t* const* out,
+                                            stwo_qm31* claimed_sum);
 
 /* ---------------------------------------------------------------- measurement -------- */
 /* No reference counterpart (its CUDA timer is a stub, CU/timer.cu:3-15). When enabled, the kernels

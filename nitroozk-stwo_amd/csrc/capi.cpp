@@ -1,11 +1,14 @@
 // C entry points of libstwo_hip (declared in include/stwo_hip.h).
 #include "../../include/stwo_hip.h"
 
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <set>
 
 #include "prover.h"
+#include "protocol.h"
 
 using namespace sh;
 
@@ -565,6 +568,143 @@ int stwo_hip_prove_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint3
     reject_unsatisfied(*p);
   });
 }
+int stwo_hip_prove_wide_fibonacci_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
+                                            const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                            const stwo_pcs_config* config, stwo_proof** out) {
+  return guarded(h, [&](Ctx&) {
+    if (!out || !log_n_rows || !n_cols) throw HipError(1, "null argument");
+    if (n_components == 0) throw HipError(1, "no components");
+    std::vector<uint32_t> logs(log_n_rows, log_n_rows + n_components), nc(n_cols, n_cols + n_components);
+    size_t total = 0;
+    for (uint32_t i = 0; i < n_components; ++i) {
+      if (nc[i] < 3) throw HipError(1, "wide-fib needs at least 3 columns per component");
+      if (logs[i] < 3 || logs[i] > 26) throw HipError(3, "log_n_rows must be in [3, 26]");
+      total += nc[i];
+    }
+    PcsCfg cfg = cfg_of(config);
+    std::vector<const uint32_t*> t(trace, trace + total);
+    auto* p = new stwo_proof();
+    try {
+      p->r = percall::prove_wide_fib_multi(h, t, logs, nc, cfg);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    reject_unsatisfied(*p);
+  });
+}
+int stwo_hip_prove_poseidon_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr, uint32_t log_n_rows,
+                                     const stwo_pcs_config* config, stwo_proof** out) {
+  return guarded(h, [&](Ctx&) {
+    if (!out) throw HipError(1, "null out");
+    if (ipr < 1 || ipr > 64) throw HipError(1, "instances_per_row must be in [1, 64]");
+    if (log_n_rows < 3 || log_n_rows > 24) throw HipError(3, "log_n_rows must be in [3, 24]");
+    PcsCfg cfg = cfg_of(config);
+    std::vector<const uint32_t*> t(trace, trace + 158 * ipr);
+    auto* p = new stwo_proof();
+    try {
+      p->r = percall::prove_poseidon(h, t, log_n_rows, ipr, cfg);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    reject_unsatisfied(*p);
+  });
+}
+int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr,
+                                            uint32_t log_n_rows, stwo_qm31 z, stwo_qm31 alpha, uint32_t* const* out,
+                                            stwo_qm31* claimed_sum) {
+  return guarded(h, [&](Ctx& c) {
+    if (!trace || !out || !claimed_sum) throw HipError(1, "null argument");
+    if (ipr < 1 || ipr > 64 || log_n_rows < 3 || log_n_rows > 26) throw HipError(1, "bad trace shape");
+    check_canonical(z);
+    check_canonical(alpha);
+    PoseidonParams pp{};
+    pp.z = q(z);
+    qm31 a = q(alpha), pw = qm_one();
+    for (int i = 0; i < 16; ++i) { pp.alpha_pow[i] = pw; pw = qm_mul(pw, a); }
+    pp.ipr = ipr;
+    std::vector<const uint32_t*> tv(trace, trace + 158 * ipr);
+    std::vector<uint32_t*> ov(out, out + 4 * ipr);
+    // fractions per row, the column sums (u64 per coordinate) for the cumulative-sum shift, then
+    // the coset-order prefix sum of the last 4 columns shifted by sum / 2^log_n
+    DevBuf sums(&c, 8, true);
+    {
+      KScope ks(c, K_LOGUP, (double)(1u << log_n_rows) * (128.0 * ipr + 16.0 * ipr));
+      launch_poseidon_logup_fracs(log_n_rows, ipr, (const uint32_t* const*)c.upload_vec(tv),
+                                  (uint32_t* const*)c.upload_vec(ov), (const PoseidonParams*)c.upload(&pp, sizeof(pp)),
+                                  (unsigned long long*)sums.p, c.stream);
+    }
+    std::vector<uint32_t*> last4(ov.end() - 4, ov.end());
+    const uint32_t nb = (1u << log_n_rows) >= 1024 ? (1u << log_n_rows) / 1024 : 1;
+    DevBuf scratch(&c, 4 * (size_t)nb);
+    {
+      KScope ks(c, K_LOGUP, 32.0 * (double)(1u << log_n_rows));
+      launch_coset_prefix_sum(log_n_rows, (uint32_t* const*)c.upload_vec(last4), qm_zero(),
+                              (const unsigned long long*)sums.p, scratch.p, c.stream);
+    }
+    unsigned long long sv[4];
+    c.d2h(sv, sums.p, sizeof(sv));
+    *claimed_sum = {m31_reduce64(sv[0] % P), m31_reduce64(sv[1] % P), m31_reduce64(sv[2] % P), m31_reduce64(sv[3] % P)};
+    c.reset_arena();
+  });
+}
+
+int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_components, const uint32_t* log_n_rows,
+                               const uint32_t* n_cols, const stwo_pcs_config* config, char* err, size_t err_len) {
+  std::string e;
+  int st = 0;
+  try {
+    if (!proof || !log_n_rows || !n_cols || !config || n_components == 0) throw HipError(1, "null argument");
+    std::vector<Component> comps;
+    for (uint32_t i = 0; i < n_components; ++i) {
+      if (n_cols[i] < 3 || log_n_rows[i] < 1 || log_n_rows[i] > 28) throw HipError(1, "bad component shape");
+      Component c;
+      c.kind = AIR_WIDE_FIB;
+      c.log_n = log_n_rows[i];
+      c.n_cols = n_cols[i];
+      comps.push_back(c);
+    }
+    e = verify_proof(proof, len, comps, cfg_of(config));
+    st = e.empty() ? 0 : 4;
+  } catch (const HipError& x) {
+    e = x.what();
+    st = x.code;
+  } catch (const std::exception& x) {
+    e = x.what();
+    st = 4;
+  }
+  if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
+  return st;
+}
+int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t ipr, uint32_t log_n_rows, stwo_qm31 claimed_sum,
+                         const stwo_pcs_config* config, char* err, size_t err_len) {
+  std::string e;
+  int st = 0;
+  try {
+    if (!proof || !config) throw HipError(1, "null argument");
+    if (ipr < 1 || ipr > 64 || log_n_rows < 1 || log_n_rows > 28) throw HipError(1, "bad component shape");
+    check_canonical(claimed_sum);
+    Component c;
+    c.kind = AIR_POSEIDON;
+    c.log_n = log_n_rows;
+    c.ipr = ipr;
+    c.claimed_sum = q(claimed_sum);
+    e = verify_proof(proof, len, {c}, cfg_of(config));
+    st = e.empty() ? 0 : 4;
+  } catch (const HipError& x) {
+    e = x.what();
+    st = x.code;
+  } catch (const std::exception& x) {
+    e = x.what();
+    st = 4;
+  }
+  if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
+  return st;
+}
+
 int stwo_proof_status(const stwo_proof* p) { return p ? p->r.status : -1; }
 size_t stwo_proof_size(const stwo_proof* p) { return p ? p->r.proof.size() : 0; }
 const uint8_t* stwo_proof_bytes(const stwo_proof* p) { return p ? p->r.proof.data() : nullptr; }

@@ -78,3 +78,16 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
                            const PcsCfg& cfg);
 
 }  // namespace sh
+
+typedef struct stwo_hip_ctx stwo_hip_ctx;
+namespace sh {
+namespace percall {
+// prove() with every device step through the C ABI's trait entry points (percall.cpp): the plain
+// per-call form, byte-identical to the device-resident prover.
+ProveResult prove_wide_fib_multi(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace,
+                                 const std::vector<uint32_t>& logs, const std::vector<uint32_t>& ncols,
+                                 const PcsCfg& cfg);
+ProveResult prove_poseidon(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
+                           const PcsCfg& cfg);
+}  // namespace percall
+}  // namespace sh

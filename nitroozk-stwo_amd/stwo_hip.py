@@ -86,7 +86,9 @@ EXPORTS = [
     "stwo_hip_bit_reverse_column", "stwo_hip_column_at", "stwo_hip_column_set", "stwo_hip_split_at_mid",
     "stwo_hip_extend", "stwo_hip_twiddles_create", "stwo_hip_twiddles_read", "stwo_hip_twiddles_free",
     "stwo_hip_interpolate_on_domain", "stwo_hip_evaluate_on_domain", "stwo_hip_eval_at_point_by_folding",
-    "stwo_hip_measure_int32_issue",
+    "stwo_hip_measure_int32_issue", "stwo_hip_prove_wide_fibonacci_per_call", "stwo_hip_prove_poseidon_per_call",
+    "stwo_hip_gen_interaction_trace_poseidon",
+    "stwo_verify_wide_fibonacci", "stwo_verify_poseidon",
 ]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
@@ -180,6 +182,16 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_eval_at_point_by_folding.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, CirclePointQM31,
                                                     c.POINTER(QM31)]
     L.stwo_hip_measure_int32_issue.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
+    L.stwo_verify_wide_fibonacci.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, _U32P, _U32P, c.POINTER(PcsConfig),
+                                             c.c_char_p, c.c_size_t]
+    L.stwo_verify_poseidon.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, c.c_uint32, QM31, c.POINTER(PcsConfig),
+                                       c.c_char_p, c.c_size_t]
+    L.stwo_hip_prove_wide_fibonacci_per_call.argtypes = [c.c_void_p, _VPP, c.c_uint32, _U32P, _U32P,
+                                                         c.POINTER(PcsConfig), c.POINTER(c.c_void_p)]
+    L.stwo_hip_prove_poseidon_per_call.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.POINTER(PcsConfig),
+                                                   c.POINTER(c.c_void_p)]
+    L.stwo_hip_gen_interaction_trace_poseidon.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, QM31, QM31, _VPP,
+                                                          c.POINTER(QM31)]
     _lib = L
     return L
 
@@ -601,6 +613,32 @@ class HipBackend:
                                                           ctypes.byref(config), ctypes.byref(h))
         return self._proof(st, h, allow_unsatisfied)
 
+    def prove_wide_fibonacci_per_call(self, traces, log_n_rows, config, allow_unsatisfied=False):
+        """The same proof through the trait entry points, one C-ABI call per trait-method call (the
+        plain per-call form). traces[i]: the columns of component i."""
+        logs = np.array(log_n_rows, np.uint32)
+        ncols = np.array([len(t) for t in traces], np.uint32)
+        h = ctypes.c_void_p()
+        st = self.lib.stwo_hip_prove_wide_fibonacci_per_call(self.ctx, _concat_ptrs(traces), len(traces),
+                                                             logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
+                                                             ctypes.byref(config), ctypes.byref(h))
+        return self._proof(st, h, allow_unsatisfied)
+
+    def prove_poseidon_per_call(self, trace, instances_per_row, log_n_rows, config, allow_unsatisfied=False):
+        h = ctypes.c_void_p()
+        st = self.lib.stwo_hip_prove_poseidon_per_call(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
+                                                       ctypes.byref(config), ctypes.byref(h))
+        return self._proof(st, h, allow_unsatisfied)
+
+    def gen_interaction_trace_poseidon(self, trace, instances_per_row, log_n_rows, z, alpha):
+        """The LogUp interaction columns (4 per instance of a row) and the claimed sum."""
+        cols = self.columns(4 * instances_per_row, 1 << log_n_rows)
+        cs = QM31()
+        self._check(self.lib.stwo_hip_gen_interaction_trace_poseidon(self.ctx, _ptrs(trace), instances_per_row,
+                                                                     log_n_rows, QM31.of(z), QM31.of(alpha),
+                                                                     _ptrs(cols), ctypes.byref(cs)))
+        return cols, cs.tuple()
+
     def prove_poseidon(self, trace, instances_per_row, log_n_rows, config, allow_unsatisfied=False):
         """The Poseidon2 AIR's proofs fail prove_ex's sanity check in this protocol version (its
         constraint degree exceeds what the lifted protocol supports), so a caller that wants the
@@ -609,6 +647,30 @@ class HipBackend:
         st = self.lib.stwo_hip_prove_poseidon(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
                                               ctypes.byref(config), ctypes.byref(h))
         return self._proof(st, h, allow_unsatisfied)
+
+
+def verify_wide_fibonacci(proof: bytes, log_n_rows, n_cols, config):
+    """verify() of a wide-Fibonacci proof (one or several components) on the host: "" when it is
+    accepted, else the verifier's error. log_n_rows / n_cols: per component."""
+    L = load_library()
+    logs = np.array(log_n_rows, np.uint32).reshape(-1)
+    nc = np.array(n_cols, np.uint32).reshape(-1)
+    err = ctypes.create_string_buffer(256)
+    st = L.stwo_verify_wide_fibonacci(proof, len(proof), len(logs), logs.ctypes.data_as(_U32P),
+                                      nc.ctypes.data_as(_U32P), ctypes.byref(config), err, 256)
+    if st == 1:
+        raise StwoHipError("invalid argument: " + err.value.decode(), st)
+    return "" if st == 0 else err.value.decode()
+
+
+def verify_poseidon(proof: bytes, instances_per_row, log_n_rows, claimed_sum, config):
+    L = load_library()
+    err = ctypes.create_string_buffer(256)
+    st = L.stwo_verify_poseidon(proof, len(proof), instances_per_row, log_n_rows, QM31.of(claimed_sum),
+                                ctypes.byref(config), err, 256)
+    if st == 1:
+        raise StwoHipError("invalid argument: " + err.value.decode(), st)
+    return "" if st == 0 else err.value.decode()
 
 
 def fnv1a(name: str) -> int:
