@@ -175,13 +175,29 @@ def mixed(args, m):
     hb.close()
 
 
+def footprint_bytes(kind, log_rows, width, blowup=1):
+    """HBM footprint estimate of one request, the admission formula: trace 4*2^n*C, its coefficients
+    4*2^n*C (kept for the out-of-domain samples), the evaluation 4*2^(n+b)*C, the Merkle layers
+    32*2^(n+b+1), composition and quotient accumulators 16*2^E*(batches + 2) and FRI 32*2^E; a
+    Poseidon2 request adds its interaction columns and the extension of all columns to 2^(n+2)."""
+    n = log_rows
+    if kind == "wide_fib":
+        C, E, batches, ext = width, n + blowup, 1, 0
+    else:
+        C, E, batches = 162 * width, n + 2, 3
+        ext = 4 * (1 << E) * C
+    return (8 * (1 << n) * C + 4 * (1 << (n + blowup)) * C + 32 * (1 << (n + blowup + 1))
+            + 16 * (1 << E) * (batches + 2) + 32 * (1 << E) + ext)
+
+
 def serve(args, m):
-    """Serving: a seeded stream of requests, 70% wide-Fibonacci (2^n rows, n in 12..20, 32..100
-    columns) and 30% Poseidon2 (2^12..2^18 instances, 8 per row), each generating its trace on the
-    device and proving it. Requests arrive all at once (throughput) or as a Poisson stream of
-    --rate per second. "plain" serves them one after another on one context; "batched" runs
-    --contexts contexts from as many host threads, so that the latency-bound phases of one proof
-    overlap the bandwidth-bound phases of others. Latency = completion - arrival."""
+    """Proving service: a seeded Poisson stream (--rate requests/s; 0 = all at t = 0) of --requests
+    requests, 70% wide-Fibonacci (2^n rows, n in 12..20, 32..100 columns) and 30% Poseidon2
+    (2^12..2^18 instances, 8 per row). Each request generates its trace on the device, is admitted
+    when the HBM footprint of the requests in flight plus its own (footprint_bytes) fits the budget
+    (--hbm-budget-gb), is proved, and its proof is verified on the host by the library's verifier
+    (stwo_verify_*) in a pool of verifier threads. "plain" serves on one context; "batched" on
+    --contexts contexts from as many host threads. Latency = proof completion - arrival."""
     import queue
     import random
     import threading
@@ -190,99 +206,140 @@ def serve(args, m):
     t = 0.0
     for i in range(args.requests):
         if rnd.random() < 0.7:
-            reqs.append(("wide_fib", rnd.randint(12, 20), rnd.randint(32, 100)))
+            kind, log_rows, width = "wide_fib", rnd.randint(12, 20), rnd.randint(32, 100)
         else:
-            li = rnd.randint(12, 18)
-            reqs.append(("poseidon", li - 3, 8))
+            kind, log_rows, width = "poseidon", rnd.randint(12, 18) - 3, 8
         if args.rate > 0:
             t += rnd.expovariate(args.rate)
-        reqs[-1] = reqs[-1] + (t,)
-    cfg = m.PcsConfig.make(args.pow_bits, 1, 0, 3, 1, 0)
+        reqs.append((i, kind, log_rows, width, t))
+    cfgs = {"wide_fib": m.PcsConfig.make(5, 1, 0, 3, 1, 0), "poseidon": m.PcsConfig.make(10, 1, 0, 3, 1, 0)}
+    budget = int(args.hbm_budget_gb * (1 << 30))
 
     def run(n_ctx):
         backends = [m.HipBackend(0) for _ in range(n_ctx)]
-        q = queue.Queue()
+        # warm-up per context, untimed: one proof of every (kind, size) in the stream at its widest
+        widest = {}
+        for _, kind, log_rows, width, _ in reqs:
+            widest[(kind, log_rows)] = max(width, widest.get((kind, log_rows), 0))
+        for hb in backends:
+            for (kind, log_rows), width in sorted(widest.items()):
+                tr = (hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed) if kind == "wide_fib"
+                      else hb.gen_trace_poseidon(width, log_rows, 1, args.seed))
+                (hb.prove_wide_fibonacci(tr, log_rows, cfgs[kind]) if kind == "wide_fib"
+                 else hb.prove_poseidon(tr, width, log_rows, cfgs[kind], allow_unsatisfied=True))
+                hb.free_columns(tr)
+        work = queue.Queue()
         for r in reqs:
-            q.put(r)
-        lat, prove_ms, gen_ms, free_ms, errors = [], [], [], [], []
+            work.put(r)
+        verify_q = queue.Queue()
+        adm = threading.Condition()
+        state = {"inflight": 0, "peak": 0, "waits": 0}
+        lat, errors, results = [], [], {"accepted": 0, "rejected": {}, "status": {}}
         lock = threading.Lock()
         t0 = time.perf_counter()
 
-        def worker(hb):
+        def prover(hb):
             while True:
                 try:
-                    kind, log_rows, width, arrival = q.get_nowait()
+                    rid, kind, log_rows, width, arrival = work.get_nowait()
                 except queue.Empty:
                     return
                 now = time.perf_counter() - t0
                 if arrival > now:
                     time.sleep(arrival - now)
+                fp = footprint_bytes(kind, log_rows, width)
+                with adm:
+                    if state["inflight"] + fp > budget and state["inflight"] > 0:
+                        state["waits"] += 1
+                    while state["inflight"] + fp > budget and state["inflight"] > 0:
+                        adm.wait()
+                    state["inflight"] += fp
+                    state["peak"] = max(state["peak"], state["inflight"])
                 try:
-                    s0 = time.perf_counter()
                     if kind == "wide_fib":
-                        tr = hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed)
-                        s1 = time.perf_counter()
-                        pr = hb.prove_wide_fibonacci(tr, log_rows, cfg)
+                        tr = hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed + rid)
+                        pr = hb.prove_wide_fibonacci(tr, log_rows, cfgs[kind])
                     else:
-                        tr = hb.gen_trace_poseidon(width, log_rows, 1, args.seed)
-                        s1 = time.perf_counter()
-                        pr = hb.prove_poseidon(tr, width, log_rows, cfg, allow_unsatisfied=True)
-                    _ = pr.bytes
+                        tr = hb.gen_trace_poseidon(width, log_rows, 1, args.seed + rid)
+                        pr = hb.prove_poseidon(tr, width, log_rows, cfgs[kind], allow_unsatisfied=True)
                     done = time.perf_counter()
                     hb.free_columns(tr)
-                    freed = time.perf_counter()
                     with lock:
-                        gen_ms.append((s1 - s0) * 1e3)
-                        prove_ms.append((done - s1) * 1e3)
-                        free_ms.append((freed - done) * 1e3)
                         lat.append((done - t0 - arrival) * 1e3)
-                except Exception as e:  # reported in the JSON line
+                    verify_q.put((kind, log_rows, width, pr))
+                except Exception as e:  # reported in the JSON line, and the run exits non-zero
                     with lock:
-                        errors.append(str(e))
+                        errors.append("%s 2^%d x %d: %s" % (kind, log_rows, width, e))
+                finally:
+                    with adm:
+                        state["inflight"] -= fp
+                        adm.notify_all()
 
-        # warm-up per context, untimed: the largest proof of each kind (twiddles, proof arena) and
-        # one trace of every column size at its widest (column cache)
-        widest = {}
-        for kind, log_rows, width, _ in reqs:
-            widest[(kind, log_rows)] = max(width, widest.get((kind, log_rows), 0))
-        big = {k: max((lr for (kd, lr) in widest if kd == k), default=None) for k in ("wide_fib", "poseidon")}
-        for hb in backends:
-            for (kind, log_rows), width in widest.items():
-                tr = (hb.gen_trace_wide_fibonacci(width, log_rows, 1, args.seed) if kind == "wide_fib"
-                      else hb.gen_trace_poseidon(width, log_rows, 1, args.seed))
-                if big[kind] == log_rows:
-                    (hb.prove_wide_fibonacci(tr, log_rows, cfg) if kind == "wide_fib"
-                     else hb.prove_poseidon(tr, width, log_rows, cfg, allow_unsatisfied=True))
-                hb.free_columns(tr)
-        t0 = time.perf_counter()
-        threads = [threading.Thread(target=worker, args=(hb,)) for hb in backends]
-        for th in threads:
+        def verifier():
+            while True:
+                item = verify_q.get()
+                if item is None:
+                    return
+                kind, log_rows, width, pr = item
+                if kind == "wide_fib":
+                    err = m.verify_wide_fibonacci(pr.bytes, [log_rows], [width], cfgs[kind])
+                else:
+                    err = m.verify_poseidon(pr.bytes, width, log_rows, pr.claimed_sum, cfgs[kind])
+                with lock:
+                    results["status"][pr.status] = results["status"].get(pr.status, 0) + 1
+                    if err == "":
+                        results["accepted"] += 1
+                    else:
+                        key = "%s: %s" % (kind, err)
+                        results["rejected"][key] = results["rejected"].get(key, 0) + 1
+
+        vthreads = [threading.Thread(target=verifier) for _ in range(args.verifiers)]
+        for th in vthreads:
             th.start()
-        for th in threads:
+        pthreads = [threading.Thread(target=prover, args=(hb,)) for hb in backends]
+        for th in pthreads:
+            th.start()
+        for th in pthreads:
             th.join()
         wall = time.perf_counter() - t0
+        for _ in vthreads:
+            verify_q.put(None)
+        for th in vthreads:
+            th.join()
+        vwall = time.perf_counter() - t0
         for hb in backends:
             hb.close()
         lat.sort()
-        pct = lambda p: round(lat[min(len(lat) - 1, int(round(p * len(lat))) - 1)], 3) if lat else None
+        pct = lambda p: round(lat[min(len(lat) - 1, max(0, int(round(p * len(lat))) - 1))], 3) if lat else None
         return {"contexts": n_ctx, "proofs": len(lat), "proofs_per_s": round(len(lat) / wall, 2),
-                "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_prove_ms": round(statistics.mean(prove_ms), 3),
-                "mean_trace_gen_ms": round(statistics.mean(gen_ms), 3), "mean_free_ms": round(statistics.mean(free_ms), 3),
-                "errors": errors[:3]}
+                "p50_ms": pct(0.50), "p99_ms": pct(0.99), "wall_s": round(wall, 3),
+                "wall_with_verification_s": round(vwall, 3), "verified": results,
+                "admission": {"budget_gb": args.hbm_budget_gb, "peak_inflight_gb": round(state["peak"] / 2 ** 30, 2),
+                              "requests_that_waited": state["waits"]},
+                "errors": errors[:3], "n_errors": len(errors)}
 
     plain = run(1)
     batched = run(args.contexts)
     out = {
-        "metric": "proofs/s (mixed serving: 70%% wide-Fibonacci, 30%% Poseidon2; %d contexts)" % args.contexts,
+        "metric": "proofs/s (serving: 70%% wide-Fibonacci, 30%% Poseidon2, Poisson %s/s; %d contexts)"
+                  % (args.rate or "inf", args.contexts),
         "value": batched["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": args.requests, "warmup": 1,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (M31 field, exact)",
-        "data": "synthetic: seeded request mix and seeded uniform traces generated on the device (seed %d)" % args.seed,
-        "config": {"workload": "serve", "requests": args.requests, "rate_per_s": args.rate, "pow_bits": args.pow_bits,
-                   "log_blowup": 1, "n_queries": 3},
+        "data": "synthetic: seeded request mix, Poisson arrivals and seeded uniform traces generated on the device "
+                "(seed %d)" % args.seed,
+        "config": {"workload": "serve", "requests": args.requests, "rate_per_s": args.rate,
+                   "pow_bits": {"wide_fib": 5, "poseidon": 10}, "log_blowup": 1, "n_queries": 3},
         "plain": plain, "batched": batched,
+        "batched_over_plain_proofs_per_s": round(batched["proofs_per_s"] / plain["proofs_per_s"], 2)
+        if plain["proofs_per_s"] else None,
+        "note": "Poseidon2 proofs are rejected with OodsNotMatching by this verifier as by the reference's "
+                "(its prove/verify tests are ignored there for this AIR in the lifted protocol); every "
+                "wide-Fibonacci proof must be accepted",
     }
     print(json.dumps(out), flush=True)
-    if plain["errors"] or batched["errors"]:
+    wide_rejected = sum(v for k, v in batched["verified"]["rejected"].items() if k.startswith("wide_fib")) + \
+        sum(v for k, v in plain["verified"]["rejected"].items() if k.startswith("wide_fib"))
+    if plain["n_errors"] or batched["n_errors"] or wide_rejected:
         sys.exit(1)
 
 
@@ -395,9 +452,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["poseidon", "wide_fib", "kernels", "serve", "mixed"], default="poseidon")
     ap.add_argument("--components", type=int, default=48, help="mixed: number of wide-Fibonacci components")
-    ap.add_argument("--requests", type=int, default=200, help="serve: number of requests")
+    ap.add_argument("--requests", type=int, default=10000, help="serve: number of requests")
     ap.add_argument("--contexts", type=int, default=4, help="serve: concurrent contexts (batched mode)")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrivals per second (0 = all at once)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=96.0, help="serve: HBM admitted to requests in flight")
+    ap.add_argument("--verifiers", type=int, default=4, help="serve: host verifier threads")
     ap.add_argument("--kb-cols", type=int, default=256, help="kernel batch: columns")
     ap.add_argument("--kb-log", type=int, default=20, help="kernel batch: log column size")
     ap.add_argument("--log-instances", type=int, default=18)
