@@ -133,6 +133,17 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* ctx, const uint32_t* const* coef
 int stwo_hip_batch_eval_at_point(stwo_hip_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
                                  stwo_circle_point_qm31 point, stwo_qm31* out_host);
 
+/* No reference counterpart: batch_eval_at_point for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The same
+ * values as stwo_hip_batch_eval_at_point; every device buffer comes from a 32 MiB array that the
+ * library holds on each device for the process's lifetime, so the call allocates nothing. Limit:
+ * n_cols x 2^max(0, log_size - 11) x 16 bytes of partial sums plus n_cols x 16 bytes of values
+ * must fit in 31 MiB (status 3 otherwise), e.g. up to 3960 columns at log_size 20. Calls on one
+ * device share the array: concurrent calls from several threads are serialised by a lock, and
+ * each call leaves its contents behind. Synchronous. */
+int stwo_hip_batch_eval_at_point_noalloc(stwo_hip_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols,
+                                         uint32_t log_size, stwo_circle_point_qm31 point, stwo_qm31* out_host);
+
 /* barycentric_weights (ops.rs:61-76; barycentric_* in bindings.rs:141-158; cpu/circle.rs:101-132):
  * weights[0..3] (coordinate columns of 2^log_size) receive w_i = V_n(p) / (S_i V_i(p)) over the
  * bit-reversed circle domain of CanonicCoset(log_size). Synchronous. */

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 
 #include "prover.h"
@@ -24,6 +25,10 @@ struct stwo_hip_ctx {
 struct stwo_proof {
   ProveResult r;
 };
+
+// File-scope device array behind the no-allocation entry point (see resolve_scratch below).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -88,6 +93,35 @@ void check_tree(const stwo_hip_twiddles* t, uint32_t half_init, uint32_t log_n) 
   if ((half_init & 0x7fffffffu) != want) throw HipError(1, "the domain's half coset is not a doubling of the tree's coset");
 }
 
+// Scratch of the no-allocation entry point (stwo_hip_batch_eval_at_point_noalloc): one copy per
+// device for the process's lifetime. Host code never uses the symbol's name as an address: the
+// device address of each device's copy comes from hipGetSymbolAddress, taken once per device
+// when a context is created on it, and calls on one device are serialised by its lock.
+struct DeviceScratch {
+  uint8_t* ptr = nullptr;
+  std::mutex mu;
+};
+std::mutex g_scratch_mu;
+std::map<int, std::unique_ptr<DeviceScratch>> g_scratch;  // by device
+// With the device current: the device address of its scratch copy (status 2 if the runtime cannot
+// resolve it).
+int resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto& e = g_scratch[device];
+  if (e && e->ptr) return 0;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return 2;
+  if (!e) e = std::make_unique<DeviceScratch>();
+  e->ptr = (uint8_t*)p;
+  return 0;
+}
+DeviceScratch& scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  if (it == g_scratch.end() || !it->second || !it->second->ptr) throw HipError(2, "no-allocation scratch not resolved");
+  return *it->second;
+}
+
 uint32_t fnv1a(const char* s) {
   uint32_t h = 0x811C9DC5u;
   for (; *s; ++s) { h ^= (uint8_t)*s; h *= 0x01000193u; }
@@ -102,8 +136,14 @@ int stwo_hip_create(int device, stwo_hip_ctx** out) {
   if (!out) return 1;
   auto* h = new stwo_hip_ctx();
   try {
-    h->ctx = new Ctx(device);
+    h->ctx = new Ctx(device);  // makes the device current
   } catch (const std::exception& e) {
+    delete h;
+    *out = nullptr;
+    return 2;
+  }
+  if (resolve_scratch(device) != 0) {
+    delete h->ctx;
     delete h;
     *out = nullptr;
     return 2;
@@ -242,6 +282,57 @@ int stwo_hip_batch_eval_at_point(stwo_hip_ctx* h, const uint32_t* const* coeffs,
     launch_oods(c.upload_vec(tasks), c.upload_vec(tg), n_cols, c.upload_vec(bs), (uint32_t)bs.size() - 1, log_size,
                 c.upload_vec(f), c.upload_vec(gl), 1, (qm31*)w.p, (qm31*)partials.p, (qm31*)vals.p, c.stream);
     c.d2h(out_host, vals.p, (size_t)n_cols * 16);
+    c.reset_arena();
+  });
+}
+
+int stwo_hip_batch_eval_at_point_noalloc(stwo_hip_ctx* h, const uint32_t* const* coeffs, uint32_t n_cols,
+                                         uint32_t log_size, stwo_circle_point_qm31 point, stwo_qm31* out_host) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size > 30) throw HipError(3, "log_size out of range");
+    if (!coeffs || !out_host) throw HipError(1, "null argument");
+    check_canonical(point.x);
+    check_canonical(point.y);
+    if (n_cols == 0) return;
+    // the same work as stwo_hip_batch_eval_at_point, every device buffer carved from the scratch:
+    // weights table, values, per-chunk partial sums (none needs clearing: each is written before
+    // it is read)
+    const uint32_t max_chunks = oods_max_chunks(log_size);
+    const size_t w_bytes = ((size_t)4 << OODS_TABLE_LOG) * 4, v_bytes = ((size_t)n_cols * 16 + 255) & ~(size_t)255;
+    const size_t p_bytes = (size_t)n_cols * max_chunks * 16;
+    if (w_bytes + v_bytes + p_bytes > NOALLOC_SCRATCH_BYTES)
+      throw HipError(3, "batch_eval_at_point_noalloc: n_cols x 2^max(0, log_size - 11) x 16 bytes of partial sums "
+                        "exceed the 32 MiB scratch (31 MiB for partials and values)");
+    std::vector<OodsTask> tasks;
+    std::vector<qm31> f(32, qm_zero());
+    const cpqm31 p = cp(point);
+    f[0] = p.y;
+    qm31 x = p.x;
+    for (uint32_t b = 1; b < log_size; ++b) { f[b] = x; x = qm_double_x(x); }
+    std::vector<uint32_t> tg(n_cols, 0), gl(1, log_size);
+    for (uint32_t i = 0; i < n_cols; ++i) tasks.push_back({coeffs[i], log_size, i, {0, 0}});
+    const std::vector<uint32_t> bs = oods_batch_starts(tg);
+    // the small tables go through the context's staging buffers, allocated when the context was
+    // created; the call is refused rather than let them overflow into an allocation
+    const size_t tab = ((tasks.size() * sizeof(OodsTask) + 255) & ~(size_t)255) + ((tg.size() * 4 + 255) & ~(size_t)255) +
+                       ((bs.size() * 4 + 255) & ~(size_t)255) + 512 + 256;
+    if (c.pinned_off + tab > c.pinned_cap || c.arena_off + tab > c.arena_cap || (size_t)n_cols * 16 > c.d2h_cap)
+      throw HipError(3, "batch_eval_at_point_noalloc: tables exceed the context's staging buffers");
+    DeviceScratch& sc = scratch_of(c.device);
+    std::lock_guard<std::mutex> lk(sc.mu);
+    uint8_t* base = sc.ptr;
+    qm31* w = (qm31*)base;
+    qm31* vals = (qm31*)(base + w_bytes);
+    qm31* partials = (qm31*)(base + w_bytes + v_bytes);
+    const OodsTask* d_tasks = c.stage_vec(tasks);
+    const uint32_t* d_tg = c.stage_vec(tg);
+    const uint32_t* d_bs = c.stage_vec(bs);
+    const qm31* d_f = c.stage_vec(f);
+    const uint32_t* d_gl = c.stage_vec(gl);
+    c.commit_staged();
+    launch_oods(d_tasks, d_tg, n_cols, d_bs, (uint32_t)bs.size() - 1, log_size, d_f, d_gl, 1, w, partials, vals,
+                c.stream);
+    c.d2h(out_host, vals, (size_t)n_cols * 16);
     c.reset_arena();
   });
 }

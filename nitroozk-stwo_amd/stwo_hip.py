@@ -87,7 +87,7 @@ EXPORTS = [
     "stwo_hip_extend", "stwo_hip_twiddles_create", "stwo_hip_twiddles_read", "stwo_hip_twiddles_free",
     "stwo_hip_interpolate_on_domain", "stwo_hip_evaluate_on_domain", "stwo_hip_eval_at_point_by_folding",
     "stwo_hip_measure_int32_issue", "stwo_hip_prove_wide_fibonacci_per_call", "stwo_hip_prove_poseidon_per_call",
-    "stwo_hip_gen_interaction_trace_poseidon",
+    "stwo_hip_gen_interaction_trace_poseidon", "stwo_hip_batch_eval_at_point_noalloc",
     "stwo_verify_wide_fibonacci", "stwo_verify_poseidon",
 ]
 
@@ -124,6 +124,8 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_evaluate_polynomials.argtypes = [c.c_void_p, _VPP, _VPP, c.c_uint32, c.c_uint32, c.c_uint32]
     L.stwo_hip_batch_eval_at_point.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, CirclePointQM31,
                                                c.POINTER(QM31)]
+    L.stwo_hip_batch_eval_at_point_noalloc.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, CirclePointQM31,
+                                                       c.POINTER(QM31)]
     L.stwo_hip_barycentric_weights.argtypes = [c.c_void_p, c.c_uint32, CirclePointQM31, _VPP]
     L.stwo_hip_batch_barycentric_eval.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, _VPP, c.POINTER(QM31)]
     L.stwo_hip_commit_on_layer.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32, c.c_void_p]
