@@ -78,8 +78,8 @@ def kernel_batch(args, m):
         hb.commit_on_layer(k + 1, None, ext, out=layers[k + 1])
         for j in range(k, -1, -1):
             hb.commit_on_layer(j, layers[j + 1], [], out=layers[j])
-        for j in range(n_cols // 4):
-            hb.fold_circle_into_line(dst[j], ext[4 * j:4 * j + 4], alpha, k + 1)
+        # the 64 secure columns in one grouped launch (the batched form of fold_circle_into_line)
+        hb.fold_circle_into_line_grouped(dst, [ext[4 * j:4 * j + 4] for j in range(n_cols // 4)], alpha, k + 1)
 
     step()  # first step: parity spot check against the oracle on one column / one QM31 column
     parity = None

@@ -166,6 +166,11 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_
  * 2^(log_src-1)) = dst * alpha^2 + fold(src), src on CanonicCoset(log_src). Uses the cached twiddles. */
 int stwo_hip_fold_circle_into_line(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4],
                                    uint32_t* const dst[4], stwo_qm31 alpha);
+/* Grouped form of fold_circle_into_line for the batched form (no reference counterpart:
+ * CudaBackend folds one secure column per call): n_groups secure columns of one size, src and dst
+ * holding 4 coordinate addresses per group, folded with the same alpha in one launch. */
+int stwo_hip_fold_circle_into_line_grouped(stwo_hip_ctx* ctx, uint32_t log_src, uint32_t n_groups,
+                                           const uint32_t* const* src, uint32_t* const* dst, stwo_qm31 alpha);
 /* fold_line (prover/fri.rs:31-44; fold_line, bindings.rs:196-212): src on the line domain of log
  * size log_src (Coset::half_odds(log_src)); fold_step folds, the coefficient squared before each
  * fold after the first (prover/backend/cpu/fri.rs:15-30), so dst has 2^(log_src - fold_step)

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <set>
 
+#include "compiler.h"
 #include "prover.h"
 #include "protocol.h"
 
@@ -33,11 +34,14 @@ static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
 namespace {
 
 template <class F>
-int guarded(stwo_hip_ctx* h, F&& f) {
+int guarded(stwo_hip_ctx* h, F&& f, bool may_recycle = true) {
   if (!h || !h->ctx) return 1;
   try {
     // the calling thread may differ from the one that created the context: select its device
     HIP_CHECK(hipSetDevice(h->ctx->device));
+    // between entry points no host code holds a table address, so a full table store is emptied
+    // here (the no-allocation entry point skips this: it may free a spilled table)
+    if (may_recycle) h->ctx->recycle_tables();
     f(*h->ctx);
     return 0;
   } catch (const HipError& e) {
@@ -334,7 +338,7 @@ int stwo_hip_batch_eval_at_point_noalloc(stwo_hip_ctx* h, const uint32_t* const*
                 c.stream);
     c.d2h(out_host, vals, (size_t)n_cols * 16);
     c.reset_arena();
-  });
+  }, /*may_recycle=*/false);
 }
 
 int stwo_hip_barycentric_weights(stwo_hip_ctx* h, uint32_t log_size, stwo_circle_point_qm31 point,
@@ -406,6 +410,34 @@ int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint
                                    q(alpha), c.itw.p, c.tree_len(), c.stream);
     }
     c.sync();
+  });
+}
+int stwo_hip_fold_circle_into_line_grouped(stwo_hip_ctx* h, uint32_t log_src, uint32_t n_groups,
+                                           const uint32_t* const* src, uint32_t* const* dst, stwo_qm31 alpha) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_src < 3 || log_src > 30) throw HipError(3, "log_src must be in [3, 30]");
+    if (n_groups == 0) return;
+    if (n_groups > 65535) throw HipError(3, "at most 65535 groups per call");
+    check_canonical(alpha);
+    c.ensure_twiddles(log_src);
+    bool aligned = true;
+    for (uint32_t k = 0; k < 4 * n_groups; ++k) aligned &= (((uintptr_t)src[k] | (uintptr_t)dst[k]) & 15) == 0;
+    const double bytes = (double)n_groups * (16.0 * (double)(1u << log_src) + 32.0 * (double)(1u << (log_src - 1)));
+    KScope ks(c, K_FRI_FOLD, bytes);
+    if (aligned) {
+      std::vector<const uint32_t*> sv(src, src + 4 * (size_t)n_groups);
+      std::vector<uint32_t*> dv(dst, dst + 4 * (size_t)n_groups);
+      launch_fold_circle_into_line_grouped(log_src, n_groups, (const uint32_t* const*)c.upload_vec(sv),
+                                           (uint32_t* const*)c.upload_vec(dv), q(alpha), c.itw.p, c.tree_len(),
+                                           c.stream);
+    } else {
+      for (uint32_t g = 0; g < n_groups; ++g)
+        launch_fold_circle_into_line(log_src, {{src[4 * g], src[4 * g + 1], src[4 * g + 2], src[4 * g + 3]}},
+                                     {{dst[4 * g], dst[4 * g + 1], dst[4 * g + 2], dst[4 * g + 3]}}, q(alpha), c.itw.p,
+                                     c.tree_len(), c.stream);
+    }
+    c.sync();
+    c.reset_arena();
   });
 }
 int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
@@ -587,6 +619,34 @@ int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* 
       launch_poseidon_composition(eval_log, trace_log, (const uint32_t* const*)c.upload_vec(tv),
                                   (const uint32_t* const*)c.upload_vec(iv), (const PoseidonParams*)c.upload(&pp, sizeof(pp)),
                                   pp.ipr, c.upload_vec(pows), c.upload_vec(dinv), a4, true, c.stream);
+    } else if (id == fnv1a("expr_component")) {
+      // a constraint program, compiled into a kernel on first use: u32 program words, the
+      // program, u32 parameter count, the parameters (QM31 each); parameter 0 is cumsum_shift
+      if (blob_len < 8) throw HipError(1, "expr_component eval_blob too short");
+      uint32_t nw;
+      memcpy(&nw, blob + 4, 4);
+      if (blob_len < 8 + 4 * (size_t)nw + 4) throw HipError(1, "expr_component eval_blob too short");
+      std::vector<uint32_t> words(nw);
+      memcpy(words.data(), blob + 8, 4 * (size_t)nw);
+      uint32_t np;
+      memcpy(&np, blob + 8 + 4 * (size_t)nw, 4);
+      if (blob_len != 12 + 4 * (size_t)nw + 16 * (size_t)np) throw HipError(1, "expr_component eval_blob length");
+      std::vector<qm31> params(1 + (size_t)np);
+      params[0] = q(cumsum_shift);
+      memcpy(params.data() + 1, blob + 12 + 4 * (size_t)nw, 16 * (size_t)np);
+      for (auto& x : params)
+        if (x.a >= P || x.b >= P || x.c >= P || x.d >= P) throw HipError(1, "non-canonical parameter");
+      if (compiled_component_constraints(words.data(), words.size()) != n_constraints)
+        throw HipError(1, "expr_component: n_constraints differs from the program's");
+      std::vector<const uint32_t*> iv(interaction, interaction + n_interaction);
+      const auto* d_t = (const uint32_t* const*)c.upload_vec(tv);
+      const auto* d_i = n_interaction ? (const uint32_t* const*)c.upload_vec(iv) : nullptr;
+      const qm31* d_p = c.upload_vec(params);
+      const qm31* d_w = c.upload_vec(pows);
+      const uint32_t* d_d = c.upload_vec(dinv);
+      KScope ks(c, K_COMPOSITION, (4.0 * (n_trace + n_interaction) + 32.0) * (double)(1u << eval_log));
+      evaluate_compiled_component(c.device, words.data(), words.size(), eval_log, trace_log, d_t, n_trace, d_i,
+                                  n_interaction, d_p, (uint32_t)params.size(), d_w, n_constraints, d_d, a4, c.stream);
     } else {
       throw HipError(5, "no device kernel for this component");
     }

@@ -45,6 +45,11 @@ void launch_grind_dev(const uint32_t* prefix, uint32_t pow_bits, uint64_t start,
 // ops.hip
 void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                   uint32_t tree_len, hipStream_t stream);
+// n_groups secure columns of one size in one launch (srcs, dsts: device tables of 4 addresses per
+// group, all 16-byte aligned; log_src >= 3), each dst = dst * alpha^2 + fold(src).
+void launch_fold_circle_into_line_grouped(uint32_t log_src, uint32_t n_groups, const uint32_t* const* srcs,
+                                          uint32_t* const* dsts, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
+                                          hipStream_t stream);
 // The same into a destination whose previous contents are ignored (dst = fold(src)).
 void launch_fold_circle_into_line_zero(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                        uint32_t tree_len, hipStream_t stream);

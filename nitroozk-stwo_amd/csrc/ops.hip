@@ -56,11 +56,8 @@ __device__ __forceinline__ u32x4 ld4(const uint32_t* c, uint32_t i) {
   return *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(c) + i);
 }
 __device__ __forceinline__ void st4(uint32_t* c, uint32_t i, u32x4 v) { *reinterpret_cast<GLOBAL_AS u32x4*>(gmem(c) + i) = v; }
-__global__ void __launch_bounds__(256) fold_circle_vec_kernel(uint32_t n_quads, CCols4 src, Cols4 dst, qm31 alpha,
-                                                              qm31 alpha2, const uint32_t* __restrict__ lt0,
-                                                              uint32_t dst_zero) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_quads) return;
+__device__ __forceinline__ void fold_circle_quad(uint32_t i, const CCols4& src, const Cols4& dst, qm31 alpha,
+                                                 qm31 alpha2, const uint32_t* __restrict__ lt0, uint32_t dst_zero) {
   const uint32_t x = lt0[2 * i], y = lt0[2 * i + 1];
   u32x4 lo[4], hi[4], d[4];
 #pragma unroll
@@ -97,6 +94,25 @@ __global__ void __launch_bounds__(256) fold_circle_vec_kernel(uint32_t n_quads, 
 #pragma unroll
   for (int k = 0; k < 4; ++k) st4(dst.c[k], 4 * i, o[k]);
 }
+__global__ void __launch_bounds__(256) fold_circle_vec_kernel(uint32_t n_quads, CCols4 src, Cols4 dst, qm31 alpha,
+                                                              qm31 alpha2, const uint32_t* __restrict__ lt0,
+                                                              uint32_t dst_zero) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_quads) return;
+  fold_circle_quad(i, src, dst, alpha, alpha2, lt0, dst_zero);
+}
+// Grouped form: secure column g of the launch is blockIdx.y, its coordinate columns 4g..4g+3 of the
+// address tables.
+__global__ void __launch_bounds__(256) fold_circle_grouped_kernel(uint32_t n_quads, const uint32_t* const* srcs,
+                                                                  uint32_t* const* dsts, qm31 alpha, qm31 alpha2,
+                                                                  const uint32_t* __restrict__ lt0) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_quads) return;
+  const uint32_t g = blockIdx.y;
+  const CCols4 s = {{srcs[4 * g], srcs[4 * g + 1], srcs[4 * g + 2], srcs[4 * g + 3]}};
+  const Cols4 d = {{dsts[4 * g], dsts[4 * g + 1], dsts[4 * g + 2], dsts[4 * g + 3]}};
+  fold_circle_quad(i, s, d, alpha, alpha2, lt0, 0u);
+}
 
 static bool aligned16(const uint32_t* const* c, int n) {
   for (int k = 0; k < n; ++k)
@@ -116,6 +132,14 @@ void launch_fold_circle_into_line(uint32_t log_src, CCols4 src, Cols4 dst, qm31 
   }
   hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, src, dst, alpha,
                      qm_mul(alpha, alpha), lt0, (const qm31*)nullptr, 0u);
+}
+void launch_fold_circle_into_line_grouped(uint32_t log_src, uint32_t n_groups, const uint32_t* const* srcs,
+                                          uint32_t* const* dsts, qm31 alpha, const uint32_t* itree, uint32_t tree_len,
+                                          hipStream_t stream) {
+  const uint32_t nq = (1u << (log_src - 1)) / 4;
+  const uint32_t* lt0 = itree + tree_len - (1u << (log_src - 1));
+  hipLaunchKernelGGL(fold_circle_grouped_kernel, dim3((nq + 255) / 256, n_groups), dim3(256), 0, stream, nq, srcs, dsts,
+                     alpha, qm_mul(alpha, alpha), lt0);
 }
 void launch_fold_circle_into_line_zero(uint32_t log_src, CCols4 src, Cols4 dst, qm31 alpha, const uint32_t* itree,
                                        uint32_t tree_len, hipStream_t stream) {

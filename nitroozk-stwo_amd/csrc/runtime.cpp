@@ -1,6 +1,7 @@
 // Host runtime of libstwo_hip (see runtime.h).
 #include "runtime.h"
 #include <cstring>
+#include <cstdlib>
 
 #include <algorithm>
 
@@ -199,18 +200,41 @@ const void* Ctx::upload_table(const void* src, size_t bytes) {
       return table_dev + it->second.off;
   const size_t a = (bytes + 255) & ~(size_t)255;
   if (!table_dev) {
+    // STWO_HIP_TABLE_STORE_BYTES (at least 4096) sizes the store; 16 MiB by default
     table_cap = 16u << 20;
+    if (const char* e = std::getenv("STWO_HIP_TABLE_STORE_BYTES")) {
+      const unsigned long long v = std::strtoull(e, nullptr, 10);
+      if (v >= 4096 && v <= (1ull << 30)) table_cap = (size_t)v & ~(size_t)255;
+    }
     HIP_CHECK(hipMalloc((void**)&table_dev, table_cap));
     HIP_CHECK(hipHostMalloc((void**)&table_pinned, table_cap, hipHostMallocDefault));
   }
-  if (a == 0 || table_off + a > table_cap) return upload(src, bytes);
-  // the pinned copy is never overwritten, so the queued transfer may read it at any time
+  if (a == 0) return table_dev;  // an empty table is never read
+  if (table_off + a > table_cap) {
+    // not into the staging arena: reset_arena() rewinds it while the caller may still hold the
+    // address. The source is host memory of the caller's, so the copy completes before return.
+    void* d = col_alloc(a);
+    table_spill.push_back(d);
+    HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream));
+    sync_main();
+    return d;
+  }
+  // the pinned copy is not overwritten before recycle_tables(), which synchronises first, so the
+  // queued transfer may read it at any time
   std::memcpy(table_pinned + table_off, src, bytes);
   HIP_CHECK(hipMemcpyAsync(table_dev + table_off, table_pinned + table_off, bytes, hipMemcpyHostToDevice, stream));
   const void* d = table_dev + table_off;
   table_cache.emplace(key, TableEntry{table_off, bytes});
   table_off += a;
   return d;
+}
+void Ctx::recycle_tables() {
+  if (table_off <= table_cap / 2 && table_spill.empty()) return;
+  sync();  // no queued transfer or kernel still reads a table
+  for (void* p : table_spill) col_free(p);
+  table_spill.clear();
+  table_cache.clear();
+  table_off = 0;
 }
 void* Ctx::upload(const void* src, size_t bytes) {
   void* d = stage(src, bytes);

@@ -133,8 +133,10 @@ struct Ctx {
   void commit_staged();
   template <class T> const T* stage_vec(const std::vector<T>& v) { return (const T*)stage(v.data(), v.size() * sizeof(T)); }
   // Tables that repeat from call to call (column address tables, per-shape constants) are kept in
-  // device memory keyed by their bytes: a table seen before costs no transfer. Falls back to
-  // upload() once the table store is full.
+  // device memory keyed by their bytes: a table seen before costs no transfer. A table's address
+  // stays valid until the next recycle_tables(), across reset_arena() (the prover lays tables out
+  // before a commitment's root is read and launches on them after). Once the store is full, a
+  // table goes to a buffer of its own, with the same lifetime.
   const void* upload_table(const void* src, size_t bytes);
   template <class T> const T* table_vec(const std::vector<T>& v) { return (const T*)upload_table(v.data(), v.size() * sizeof(T)); }
   struct TableEntry { size_t off, bytes; };
@@ -142,6 +144,10 @@ struct Ctx {
   uint8_t* table_dev = nullptr;
   uint8_t* table_pinned = nullptr;
   size_t table_cap = 0, table_off = 0;
+  std::vector<void*> table_spill;  // tables that did not fit the store
+  // Empties the store (after a full sync) once it is half full or has spilled. Called between
+  // top-level calls only, where no host code holds a table address.
+  void recycle_tables();
   void sync();       // both streams
   void sync_main();  // the main stream only
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued

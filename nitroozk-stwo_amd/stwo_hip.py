@@ -88,7 +88,7 @@ EXPORTS = [
     "stwo_hip_interpolate_on_domain", "stwo_hip_evaluate_on_domain", "stwo_hip_eval_at_point_by_folding",
     "stwo_hip_measure_int32_issue", "stwo_hip_prove_wide_fibonacci_per_call", "stwo_hip_prove_poseidon_per_call",
     "stwo_hip_gen_interaction_trace_poseidon", "stwo_hip_batch_eval_at_point_noalloc",
-    "stwo_verify_wide_fibonacci", "stwo_verify_poseidon",
+    "stwo_verify_wide_fibonacci", "stwo_verify_poseidon", "stwo_hip_fold_circle_into_line_grouped",
 ]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
@@ -131,6 +131,7 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_commit_on_layer.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32, c.c_void_p]
     L.stwo_hip_fold_circle_into_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31]
     L.stwo_hip_fold_line.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, QM31, c.c_uint32]
+    L.stwo_hip_fold_circle_into_line_grouped.argtypes = [c.c_void_p, c.c_uint32, c.c_uint32, _VPP, _VPP, QM31]
     L.stwo_hip_accumulate_numerators.argtypes = [c.c_void_p, c.c_uint32, _VPP, c.c_uint32, c.POINTER(SampleBatch),
                                                  c.c_uint32, _VPP, c.POINTER(QM31)]
     L.stwo_hip_compute_quotients_and_combine.argtypes = [c.c_void_p, c.POINTER(AccumulatedNumerators), c.c_uint32,
@@ -477,6 +478,13 @@ class HipBackend:
     def fold_circle_into_line(self, dst4, src4, alpha, log_src):
         self._check(self.lib.stwo_hip_fold_circle_into_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst4),
                                                             QM31.of(alpha)))
+
+    def fold_circle_into_line_grouped(self, dst4s, src4s, alpha, log_src):
+        """Grouped fold: dst4s[g] = dst4s[g] * alpha^2 + fold(src4s[g]) for every group, one launch."""
+        src = [c for g in src4s for c in g]
+        dst = [c for g in dst4s for c in g]
+        self._check(self.lib.stwo_hip_fold_circle_into_line_grouped(self.ctx, log_src, len(src4s), _ptrs(src),
+                                                                    _ptrs(dst), QM31.of(alpha)))
 
     def fold_line(self, src4, alpha, log_src, fold_step=1):
         dst = [self.column(1 << (log_src - fold_step)) for _ in range(4)]

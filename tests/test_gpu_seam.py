@@ -256,3 +256,23 @@ def test_int32_issue_rate_is_plausible(hb):
     v = hb.measure_int32_issue()
     say("int32 issue rate %.3g lane-instr/s" % v)
     assert 1e13 < v < 6e13
+
+
+@pytest.mark.parametrize("log_src,n_groups,misalign", [(3, 2, False), (10, 5, False), (12, 3, True)])
+def test_grouped_fold_matches_per_group(hb, log_src, n_groups, misalign):
+    say("grouped fold", log_src, n_groups, misalign)
+    rng = np.random.default_rng(700 + log_src)
+    srcs = [[rng.integers(0, P, 1 << log_src, dtype=np.uint32) for _ in range(4)] for _ in range(n_groups)]
+    dsts = [[rng.integers(0, P, 1 << (log_src - 1), dtype=np.uint32) for _ in range(4)] for _ in range(n_groups)]
+    alpha = tuple(int(x) for x in rng.integers(0, P, 4))
+    if misalign:
+        big = hb.from_cpu(np.concatenate([np.zeros(1, np.uint32)] + [c for g in srcs for c in g]))
+        dsrc = [[big.ptr + 4 * (1 + ((4 * g + k) << log_src)) for k in range(4)] for g in range(n_groups)]
+    else:
+        dsrc = [[hb.from_cpu(c) for c in g] for g in srcs]
+    ddst = [[hb.from_cpu(c) for c in g] for g in dsts]
+    hb.fold_circle_into_line_grouped(ddst, dsrc, alpha, log_src)
+    for g in range(n_groups):
+        exp = O.fold_circle_into_line(srcs[g], dsts[g], alpha)
+        for k in range(4):
+            assert np.array_equal(ddst[g][k].to_cpu(), exp[k]), (g, k)
