@@ -32,6 +32,36 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {("poseidon2", 18, 8, "evaluate"): 1413.9e6}
 
 
+def roofline_entry(dom, by_per_step, ms_per_step, traffic, units_per_step, unit_peak):
+    """The roofline object of a bench line for the dominant kernel class: its HBM figures, and for
+    the transform classes its butterflies per second against the butterfly ceiling measured live on
+    the same device (stwo_hip_measure_butterfly_rate: the transform kernels' own butterfly code on
+    register-resident values). `bound` names the resource the class is closer to saturating."""
+    hbm_ach = by_per_step / (ms_per_step / 1e3) / 1e9
+    hbm = {"achieved": round(hbm_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 4),
+           "traffic": traffic, "traffic_unit": "HBM bytes per step over the class's launches" if traffic else None,
+           "traffic_over_algorithmic": round(traffic / by_per_step, 3) if traffic else None}
+    valu = None
+    if units_per_step and unit_peak:
+        ach = units_per_step / (ms_per_step / 1e3)
+        valu = {"achieved": round(ach / 1e12, 3), "peak": round(unit_peak / 1e12, 3), "unit": "T butterflies/s",
+                "frac": round(ach / unit_peak, 4), "butterflies_per_step": units_per_step,
+                "peak_source": "M31 butterflies on register-resident data, measured live "
+                               "(stwo_hip_measure_butterfly_rate)"}
+    if valu and valu["frac"] > hbm["frac"]:
+        r = {"bound": "valu", "kernel": dom}
+        r.update({k: valu[k] for k in ("achieved", "peak", "unit", "frac")})
+        r["traffic"] = traffic
+        r["hbm"] = hbm
+        r["valu"] = valu
+        return r
+    r = {"bound": "hbm", "kernel": dom}
+    r.update(hbm)
+    if valu:
+        r["valu"] = valu
+    return r
+
+
 def load_pkg():
     import importlib.util
     spec = importlib.util.spec_from_file_location("stwo_hip", os.path.join(ROOT, "nitroozk-stwo_amd", "stwo_hip.py"))
@@ -93,6 +123,7 @@ def kernel_batch(args, m):
                       and all(np.array_equal(dst[0][i].to_cpu(), f0[i]) for i in range(4)))
     for _ in range(max(0, args.warmup - 1)):
         step()
+    bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
     hb.reset_kernel_stats()
     hb.set_kernel_timing(True)
     device_sync(hb)
@@ -103,6 +134,7 @@ def kernel_batch(args, m):
     wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     hb.set_kernel_timing(False)
     stats = hb.kernel_stats()
+    units = hb.kernel_units()
     names = {"interpolate": "interpolate", "evaluate": "evaluate", "merkle": "merkle", "fri_fold": "fold_circle_into_line"}
     stages = {}
     tot_b = tot_ms = 0.0
@@ -113,8 +145,12 @@ def kernel_batch(args, m):
         tot_b += by
         tot_ms += ms
         stages[label] = {"ms": round(ms, 3), "GB": round(by / 1e9, 3), "GB_per_s": round(by / (ms / 1e3) / 1e9, 1) if ms else None}
-    dom = max(stages, key=lambda s: stages[s]["ms"])
-    ach = stages[dom]["GB_per_s"]
+        if cls in units and ms:
+            stages[label]["T_butterflies_per_s"] = round(units[cls] / args.steps / (ms / 1e3) / 1e12, 3)
+            stages[label]["butterfly_ceiling_frac"] = round(units[cls] / args.steps / (ms / 1e3) / bfly_peak[cls], 4)
+    dom_label = max(stages, key=lambda s: stages[s]["ms"])
+    dom = {v: k for k, v in names.items()}[dom_label]
+    dms, dby, _ = stats[dom]
     out = {
         "metric": "kernel batch GB/s (%d x 2^%d: interpolate, evaluate x2, Merkle, fold)" % (n_cols, k),
         "value": round(tot_b / (tot_ms / 1e3) / 1e9, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -124,8 +160,8 @@ def kernel_batch(args, m):
         "config": {"workload": "kernel_batch", "columns": n_cols, "log_size": k, "log_blowup": 1,
                    "fold_columns": n_cols // 4},
         "stages": stages, "parity_spot_check": parity,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None, "traffic": None},
+        "roofline": roofline_entry(dom, dby / args.steps, dms / args.steps, None, units.get(dom, 0.0) / args.steps,
+                                   bfly_peak.get(dom)),
     }
     print(json.dumps(out), flush=True)
     hb.close()
@@ -454,7 +490,7 @@ def main():
     ap.add_argument("--components", type=int, default=48, help="mixed: number of wide-Fibonacci components")
     ap.add_argument("--requests", type=int, default=10000, help="serve: number of requests")
     ap.add_argument("--contexts", type=int, default=4, help="serve: concurrent contexts (batched mode)")
-    ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrivals per second (0 = all at once)")
+    ap.add_argument("--rate", type=float, default=600.0, help="serve: Poisson arrivals per second (0 = all at once)")
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0, help="serve: HBM admitted to requests in flight")
     ap.add_argument("--verifiers", type=int, default=4, help="serve: host verifier threads")
     ap.add_argument("--kb-cols", type=int, default=256, help="kernel batch: columns")
@@ -518,6 +554,9 @@ def main():
                      "channel": "blake2s", "seed": args.seed})
 
     device_sync(hb)  # brings up torch's device context before anything is timed
+    # the VALU ceilings of this device (untimed): 32-bit add issue, and the transform butterflies
+    valu_peak = hb.measure_int32_issue()
+    bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
     for _ in range(args.warmup):
         prove()
 
@@ -541,6 +580,7 @@ def main():
     ev_total_ms, _, ev_proofs = timed(args.steps)
     hb.set_kernel_timing(False)
     stats = hb.kernel_stats()
+    units = hb.kernel_units()
     proofs += ev_proofs
     ms_per_step = total_ms / args.steps
     last = proofs[-1]
@@ -549,13 +589,17 @@ def main():
     # roofline: kernel class with the largest device time
     dom = max((k for k in stats if k != "grind"), key=lambda k: stats[k][0])
     ms, by, n = stats[dom]
-    achieved = by / (ms / 1e3) / 1e9
     traffic = None
     if args.workload == "poseidon":
         traffic = MEASURED_TRAFFIC.get(("poseidon2", args.log_instances, args.ipr, dom))
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[2] // args.steps,
                    "GB_per_s": round(v[1] / (v[0] / 1e3) / 1e9, 1) if v[0] > 0 and v[1] > 0 else None}
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
+    for k, u in units.items():
+        if k in kernels and stats[k][0] > 0:
+            rate = u / (stats[k][0] / 1e3)
+            kernels[k]["T_butterflies_per_s"] = round(rate / 1e12, 3)
+            kernels[k]["butterfly_ceiling_frac"] = round(rate / bfly_peak[k], 4) if bfly_peak.get(k) else None
     stage = {k: round(v, 2) for k, v in last.stage_ms.items()}
 
     extras = {}
@@ -612,10 +656,11 @@ def main():
         "deterministic": deterministic,
         "byte_identical_to_oracle": parity,
         "stage_ms": stage,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_unit": "HBM bytes per proof over the class's launches" if traffic else None,
-                     "traffic_over_algorithmic": round(traffic / (by / args.steps), 3) if traffic else None},
+        "roofline": roofline_entry(dom, by / args.steps, ms / args.steps, traffic, units.get(dom, 0.0) / args.steps,
+                                   bfly_peak.get(dom)),
+        "valu_ceilings": {"int32_add_issue_T_lane_instr_per_s": round(valu_peak / 1e12, 2),
+                          "forward_butterflies_T_per_s": round(bfly_peak["evaluate"] / 1e12, 3),
+                          "inverse_butterflies_T_per_s": round(bfly_peak["interpolate"] / 1e12, 3)},
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -304,6 +304,20 @@ int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* ctx, const uint32_t* c
                                             uint32_t log_n_rows, stwo_qm31 z, stwo_qm31 alpha, uint32_t* const* out,
                                             stwo_qm31* claimed_sum);
 
+/* ---------------------------------------------------------------- FFT plans ---------- */
+/* No reference counterpart (the reference's transforms take no launch parameters). A circle FFT
+ * of k >= 11 stages runs stages [0, S) in one kernel whose workgroup owns 2^S consecutive values
+ * (11 <= S <= min(k, 15)) and the stages above in strided passes. The context picks S per shape:
+ * the largest tile when its grid fills the device four times over, else the fastest candidate,
+ * timed once per (device, direction, columns, sizes) and kept in the cache directory
+ * (STWO_HIP_KERNEL_CACHE, else ~/.cache/stwo_hip: fft_plans_<arch>_<CUs>.txt). Results never depend
+ * on S. stwo_hip_fft_low_log returns the plan of a shape (tuning it if needed; 0 when the
+ * transform has fewer than 11 stages); stwo_hip_set_fft_low_log fixes S for every later transform
+ * of the context (clamped to each shape's range; 0 restores planning). */
+int stwo_hip_fft_low_log(stwo_hip_ctx* ctx, int inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n,
+                         uint32_t* low_log);
+int stwo_hip_set_fft_low_log(stwo_hip_ctx* ctx, uint32_t low_log);
+
 /* ---------------------------------------------------------------- measurement -------- */
 /* No reference counterpart (its CUDA timer is a stub, CU/timer.cu:3-15). When enabled, the kernels
  * of each class are bracketed with HIP events on the context stream; stats accumulate the event
@@ -315,7 +329,16 @@ int stwo_hip_set_kernel_timing(stwo_hip_ctx* ctx, int enable);
  * measured with independent chains of add / xor / rotate (the operations of a Blake2s round): the
  * peak against which VALU-bound kernels (Blake2s hashing, M31 butterflies) are reported. */
 int stwo_hip_measure_int32_issue(stwo_hip_ctx* ctx, double* lane_instr_per_s);
+/* No reference counterpart. The device's forward (inverse != 0: inverse) circle-FFT butterflies per
+ * second with the transform kernels' own butterfly code on register-resident values and twiddles:
+ * the arithmetic ceiling of the interpolate / evaluate kernels, against which their butterflies per
+ * second (stwo_hip_kernel_units over the class's event time) are reported. */
+int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* ctx, int inverse, double* butterflies_per_s);
 int stwo_hip_kernel_stats(stwo_hip_ctx* ctx, uint32_t kernel_class, double* ms, double* bytes, uint64_t* launches);
+/* No reference counterpart. Algorithmic work units of a kernel class since the last reset: circle-FFT
+ * butterflies for interpolate (n_cols 2^(log-1) log) and evaluate (n_cols 2^(log-1) k, k the
+ * coefficients' log size); 0 for the other classes. */
+int stwo_hip_kernel_units(stwo_hip_ctx* ctx, uint32_t kernel_class, double* units);
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* ctx);
 
 #ifdef __cplusplus

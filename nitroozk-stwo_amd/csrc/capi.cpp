@@ -82,10 +82,15 @@ void reject_unsatisfied(const stwo_proof& p) {
 // A circle transform of same-size columns on the domain whose half coset starts at half_init:
 // domains of log size 1 and 2 by the explicit formulas, larger ones by the FFT kernels over a
 // doubled tree whose last 2^(log_n - 1) entries are the domain's line twiddles.
+// `low` is the plan's low-stage tile (Ctx::fft_low_log), taken before the caller's timing scope.
 void run_fft(Ctx& c, bool inverse, const uint32_t* const* ds, uint32_t* const* dd, uint32_t n_cols, uint32_t log_in,
-             uint32_t log_n, const uint32_t* tree2, uint32_t tree_len, uint32_t half_init) {
+             uint32_t log_n, const uint32_t* tree2, uint32_t tree_len, uint32_t half_init, uint32_t low) {
   if (log_n <= 2) launch_fft_tiny(inverse, ds, dd, n_cols, log_in, log_n, index_to_point(half_init), c.stream);
-  else launch_fft(inverse, ds, dd, n_cols, log_in, log_n, tree2, tree_len, c.stream);
+  else launch_fft(inverse, ds, dd, n_cols, log_in, log_n, tree2, tree_len, c.stream, false, low);
+}
+uint32_t plan_fft(Ctx& c, bool inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n, const uint32_t* tree2,
+                  uint32_t tree_len) {
+  return log_n <= 2 ? 0u : c.fft_low_log(inverse, n_cols, log_in, log_n, tree2, tree_len);
 }
 uint32_t canonic_half_init(uint32_t log_n) { return HCoset::half_odds(log_n - 1).initial_index; }
 void check_tree(const stwo_hip_twiddles* t, uint32_t half_init, uint32_t log_n) {
@@ -241,8 +246,10 @@ int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, 
     const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
+      const uint32_t low = plan_fft(c, true, n_cols, log_size, log_size, c.itw2.p, c.tree_len());
       KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
-      run_fft(c, true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), canonic_half_init(log_size));
+      ks.units(fft_butterflies(n_cols, log_size, log_size));
+      run_fft(c, true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), canonic_half_init(log_size), low);
     }
     c.sync();
     c.reset_arena();
@@ -259,8 +266,10 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs
     const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
+      const uint32_t low = plan_fft(c, false, n_cols, log_size, log_n, c.tw2.p, c.tree_len());
       KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_size) + 4.0 * (double)(1u << log_n)));
-      run_fft(c, false, ds, dd, n_cols, log_size, log_n, c.tw2.p, c.tree_len(), canonic_half_init(log_n));
+      ks.units(fft_butterflies(n_cols, log_n, log_size));
+      run_fft(c, false, ds, dd, n_cols, log_size, log_n, c.tw2.p, c.tree_len(), canonic_half_init(log_n), low);
     }
     c.sync();
     c.reset_arena();
@@ -967,9 +976,11 @@ int stwo_hip_interpolate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, 
     std::vector<const uint32_t*> s(evals, evals + n_cols);
     std::vector<uint32_t*> d(coeffs, coeffs + n_cols);
     {
+      const uint32_t low = plan_fft(c, true, n_cols, log_size, log_size, t->itw2.p, 1u << t->log_size);
       KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
+      ks.units(fft_butterflies(n_cols, log_size, log_size));
       run_fft(c, true, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
-              log_size, t->itw2.p, 1u << t->log_size, half_coset_initial_index);
+              log_size, t->itw2.p, 1u << t->log_size, half_coset_initial_index, low);
     }
     c.sync();
     c.reset_arena();
@@ -986,9 +997,11 @@ int stwo_hip_evaluate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uin
     std::vector<const uint32_t*> s(coeffs, coeffs + n_cols);
     std::vector<uint32_t*> d(evals, evals + n_cols);
     {
+      const uint32_t low = plan_fft(c, false, n_cols, log_coeffs, log_size, t->tw2.p, 1u << t->log_size);
       KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_coeffs) + 4.0 * (double)(1u << log_size)));
+      ks.units(fft_butterflies(n_cols, log_size, log_coeffs));
       run_fft(c, false, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_coeffs,
-              log_size, t->tw2.p, 1u << t->log_size, half_coset_initial_index);
+              log_size, t->tw2.p, 1u << t->log_size, half_coset_initial_index, low);
     }
     c.sync();
     c.reset_arena();
@@ -1040,6 +1053,37 @@ int stwo_hip_measure_int32_issue(stwo_hip_ctx* h, double* lane_instr_per_s) {
   });
 }
 
+int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* h, int inverse, double* butterflies_per_s) {
+  return guarded(h, [&](Ctx& c) {
+    if (!butterflies_per_s) throw HipError(1, "null out");
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    DevBuf scratch(&c, (size_t)cus * 16);
+    c.sync();
+    *butterflies_per_s = measure_butterfly_rate(inverse != 0, scratch.p, c.stream);
+  });
+}
+
+int stwo_hip_set_fft_low_log(stwo_hip_ctx* h, uint32_t low_log) {
+  return guarded(h, [&](Ctx& c) {
+    if (low_log != 0 && (low_log < 11 || low_log > 15)) throw HipError(1, "low_log must be 0 or in 11..15");
+    c.fft_low_override = low_log;
+  });
+}
+int stwo_hip_fft_low_log(stwo_hip_ctx* h, int inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n,
+                         uint32_t* low_log) {
+  return guarded(h, [&](Ctx& c) {
+    if (!low_log) throw HipError(1, "null out");
+    if (log_in > log_n || log_n > 30 || (inverse && log_in != log_n)) throw HipError(3, "sizes out of range");
+    *low_log = 0;
+    if (log_n <= 2 || n_cols == 0) return;
+    c.ensure_twiddles(log_n);
+    *low_log = inverse ? c.fft_low_log(true, n_cols, log_in, log_n, c.itw2.p, c.tree_len())
+                       : c.fft_low_log(false, n_cols, log_in, log_n, c.tw2.p, c.tree_len());
+    c.sync();
+  });
+}
+
 int stwo_hip_set_kernel_timing(stwo_hip_ctx* h, int enable) {
   return guarded(h, [&](Ctx& c) { c.sync(); c.timing = enable != 0; });
 }
@@ -1051,6 +1095,15 @@ int stwo_hip_kernel_stats(stwo_hip_ctx* h, uint32_t cls, double* ms, double* byt
     if (ms) *ms = c.kstats[cls].ms;
     if (bytes) *bytes = c.kstats[cls].bytes;
     if (launches) *launches = c.kstats[cls].launches;
+  });
+}
+int stwo_hip_kernel_units(stwo_hip_ctx* h, uint32_t cls, double* units) {
+  return guarded(h, [&](Ctx& c) {
+    if (cls >= K_NUM_CLASSES) throw HipError(1, "kernel class out of range");
+    if (!units) throw HipError(1, "null out");
+    c.sync();
+    c.collect_timing();
+    *units = c.kstats[cls].units;
   });
 }
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* h) {

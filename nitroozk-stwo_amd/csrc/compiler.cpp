@@ -24,7 +24,6 @@
 #include <mutex>
 #include <sstream>
 
-#include <sys/stat.h>
 
 #include "compiler.h"
 
@@ -352,9 +351,7 @@ const Entry& get_compiled(int device, const uint32_t* w, size_t n) {
   // code objects persist across processes in a cache directory, keyed by source and target
   std::string cache_path;
   {
-    const char* dir = std::getenv("STWO_HIP_KERNEL_CACHE");
-    const char* home = std::getenv("HOME");
-    std::string d = dir ? dir : (home ? std::string(home) + "/.cache/stwo_hip" : std::string());
+    const std::string d = cache_directory();
     if (!d.empty()) {
       const std::string key = src + "\n" + arch;
       char name[40];
@@ -377,8 +374,6 @@ const Entry& get_compiled(int device, const uint32_t* w, size_t n) {
         e->mod = nullptr;
         e->fn = nullptr;
       }
-      for (size_t k = 1; k <= d.size(); ++k)  // the directory and its parents
-        if (k == d.size() || d[k] == '/') (void)mkdir(d.substr(0, k).c_str(), 0755);
     }
   }
   hiprtcProgram rp;

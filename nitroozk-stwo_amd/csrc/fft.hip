@@ -11,37 +11,12 @@
 // MI355X mapping: one workgroup owns a tile of 2^S * 2^c words in LDS and runs S consecutive
 // stages on it. The first (lowest) stages use contiguous tiles (c = 0); higher stages use
 // strided tiles made of 2^c adjacent sub-columns so that global loads stay coalesced.
+#include <stdexcept>
+#include "butterfly.h"
 #include "field.h"
 #include "kernels.h"
 
 namespace sh {
-
-// Twiddles are read from doubled trees (every entry 2t, below 2^32): a t (mod P) for a <= P is
-// then hi + lo / 2 with a (2t) = hi 2^32 + lo, since a t = hi 2^31 + lo / 2 and 2^31 = 1 (mod P).
-// The sum is below 2P, so one conditional subtraction (a minimum) makes it canonical: 5 operations
-// against 6 for the product with t itself.
-__device__ __forceinline__ uint32_t mul_tw(uint32_t a, uint32_t t2) {
-  const uint64_t p = (uint64_t)a * t2;
-  const uint32_t s = (uint32_t)(p >> 32) + ((uint32_t)p >> 1);
-  const uint32_t r = s - P;
-  return r < s ? r : s;
-}
-// The negation of a doubled twiddle, again doubled: 2P - t2 (0 stays 0).
-__device__ __forceinline__ uint32_t neg_tw(uint32_t t2) { return t2 ? 2 * P - t2 : 0u; }
-
-// NEG: the twiddle is the negation of t2, folded into the butterfly's additions.
-template <bool INV, bool NEG = false>
-__device__ __forceinline__ void bfly(uint32_t& v0, uint32_t& v1, uint32_t t2) {
-  const uint32_t a = v0, b = v1;
-  if (INV) {  // ibutterfly(v0, v1, t) = (v0 + v1, (v0 - v1) t); with -t: (v1 - v0) t
-    v0 = m31_add(a, b);
-    v1 = mul_tw(NEG ? m31_sub(b, a) : m31_sub(a, b), t2);
-  } else {  // butterfly(v0, v1, t) = (v0 + v1 t, v0 - v1 t); with -t the two swap
-    const uint32_t m = mul_tw(b, t2);
-    v0 = NEG ? m31_sub(a, m) : m31_add(a, m);
-    v1 = NEG ? m31_add(a, m) : m31_sub(a, m);
-  }
-}
 
 // Doubled circle twiddle of pair h (line layer 0 pairs [x, y] -> [y, -y, -x, x]).
 __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* __restrict__ lt0, uint32_t h) {
@@ -395,7 +370,8 @@ static std::vector<FftPass> plan_passes(uint32_t k, uint32_t K) {
 }
 
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
-                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background) {
+                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background,
+                uint32_t low_log) {
   if (n_cols == 0) return;
   const uint32_t prio = background ? 0u : 1u;
   const uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
@@ -413,7 +389,9 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
   const uint32_t full = (1u << log_n) - 1, in_mask = inverse ? full : (1u << log_in) - 1;
   if (k >= 11) {
     // stages [0, S) in one pass of 2^S-element blocks; stages [S, k) in strided passes
-    const uint32_t S = k < 15 ? k : 15;
+    const uint32_t smax = k < 15 ? k : 15;
+    if (low_log != 0 && (low_log < 11 || low_log > smax)) throw std::invalid_argument("launch_fft: low_log out of range");
+    const uint32_t S = low_log ? low_log : smax;
     std::vector<FftPass> high;
     for (uint32_t e = S; e < k;) {
       uint32_t st = k - e < 8 ? k - e : 8, c = 12 - st;

@@ -14,9 +14,12 @@ struct CCols4 { const uint32_t* c[4]; };
 // fft.hip
 // tree: a doubled twiddle tree (every entry 2t; Ctx::tw2 for evaluation, Ctx::itw2 for
 // interpolation). background: the transform runs beside critical-path kernels on another stream
-// and leaves them the issue priority.
+// and leaves them the issue priority. low_log: the tile of the low-stage kernel, 2^low_log values
+// per workgroup (11 <= low_log <= min(k, 15) for a transform of k >= 11 stages; 0 = the largest),
+// the stages above it run in strided passes.
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
-                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background = false);
+                uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background = false,
+                uint32_t low_log = 0);
 
 // merkle.hip
 void launch_merkle_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
@@ -242,5 +245,7 @@ void launch_fft_tiny(bool inverse, const uint32_t* const* src, uint32_t* const* 
 // measure.hip: 32-bit VALU lane-instructions per second on the current device (synchronous; scratch:
 // one word per workgroup, 16 x CUs words).
 double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
+// Forward (or inverse) FFT butterflies per second on register-resident data (butterfly.h).
+double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t stream);
 
 }  // namespace sh

@@ -245,13 +245,15 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
+    const uint32_t low = ctx.fft_low_log(false, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len());
     hipStream_t st = ctx.stream;
     if (on_side) {
       ctx.fork();  // the side stream starts after the upload and everything queued before it
       st = ctx.side;
     }
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << target_log)), st);
-    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len(), st, on_side);
+    ks.units(fft_butterflies(idx.size(), target_log, lg));
+    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len(), st, on_side, low);
   }
 }
 
@@ -373,9 +375,10 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
+    const uint32_t low = ctx.fft_low_log(true, (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len());
     KScope ks(ctx, K_INTERPOLATE, (double)idx.size() * 8.0 * (1u << lg));
-    launch_fft(true, d_src, d_dst,
-               (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len(), ctx.stream);
+    ks.units(fft_butterflies(idx.size(), lg, lg));
+    launch_fft(true, d_src, d_dst, (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len(), ctx.stream, false, low);
   }
 }
 
@@ -401,9 +404,10 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
+    const uint32_t low = ctx.fft_low_log(false, (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len());
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << K)));
-    launch_fft(false, d_src, d_dst,
-               (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len(), ctx.stream);
+    ks.units(fft_butterflies(idx.size(), K, lg));
+    launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len(), ctx.stream, false, low);
   }
 }
 

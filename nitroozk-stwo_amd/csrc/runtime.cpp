@@ -65,6 +65,7 @@ Ctx::Ctx(int dev) : device(dev) {
   props.location.type = hipMemLocationTypeDevice;
   props.location.id = dev;
   HIP_CHECK(hipMemPoolCreate(&pool, &props));
+  HIP_CHECK(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
   uint64_t thr = UINT64_MAX;  // keep freed blocks cached in the pool
   HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
   pinned_cap = 64u << 20;
@@ -279,6 +280,7 @@ void Ctx::collect_timing() {
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
       kstats[p.cls].ms += ms;
       kstats[p.cls].bytes += p.bytes;
+      kstats[p.cls].units += p.units;
       kstats[p.cls].launches += 1;
     }
     event_pool.push_back(p.a);

@@ -28,6 +28,10 @@ struct HipError : std::runtime_error {
 
 struct Ctx;
 
+// The directory of files kept across processes (compiled constraint kernels, FFT plans):
+// STWO_HIP_KERNEL_CACHE, else ~/.cache/stwo_hip; created on first use. Empty if neither is set.
+std::string cache_directory();
+
 // Owning device buffer of u32 words, freed stream-ordered.
 struct DevBuf {
   uint32_t* p = nullptr;
@@ -67,13 +71,14 @@ enum KernelClass {
 struct KernelStat {
   double ms = 0;
   double bytes = 0;  // algorithmic HBM bytes
+  double units = 0;  // algorithmic work units (butterflies for the transform classes)
   uint64_t launches = 0;
 };
 
 struct Ctx {
   int device = 0;
   bool timing = false;
-  struct Pending { int cls; double bytes; hipEvent_t a, b; };
+  struct Pending { int cls; double bytes, units; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
   KernelStat kstats[K_NUM_CLASSES];
@@ -152,6 +157,12 @@ struct Ctx {
   void sync_main();  // the main stream only
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
   void ensure_twiddles(uint32_t log_domain);
+  // Circle-FFT launch plans (fft_plan.cpp): the low-stage tile (2^S values per workgroup) for a
+  // transform shape, as launch_fft's low_log; 0 for transforms of fewer than 11 stages.
+  uint32_t fft_low_log(bool inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n, const uint32_t* tree2,
+                       uint32_t tree_len);
+  uint32_t fft_low_override = 0;  // a fixed tile for every shape (stwo_hip_set_fft_low_log); 0 = planned
+  int cu_count = 0;
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
   void d2h(void* dst, const void* src, size_t bytes);  // synchronous
@@ -190,20 +201,28 @@ struct PersistentAllocScope {
 struct KScope {
   Ctx& c;
   int cls;
-  double bytes;
+  double bytes, n_units = 0;
   hipStream_t st;
   hipEvent_t a = nullptr;
   KScope(Ctx& ctx, int k, double b, hipStream_t s = nullptr) : c(ctx), cls(k), bytes(b), st(s ? s : ctx.stream) {
     if (c.timing) { a = c.get_event(); (void)hipEventRecord(a, st); }
   }
+  // The launch group's algorithmic work units (KernelStat::units).
+  KScope& units(double u) { n_units = u; return *this; }
   ~KScope() {
     if (c.timing && a) {
       hipEvent_t b = c.get_event();
       (void)hipEventRecord(b, st);
-      c.pending.push_back({cls, bytes, a, b});
+      c.pending.push_back({cls, bytes, n_units, a, b});
     }
   }
 };
+// Butterflies of a circle FFT over n_cols columns of 2^log_n values that runs `stages` stages
+// (interpolation: all log_n; evaluation from 2^k coefficients: k, the top stages only copy).
+inline double fft_butterflies(size_t n_cols, uint32_t log_n, uint32_t stages) {
+  if (log_n == 0 || stages == 0) return 0.0;
+  return (double)n_cols * (double)((size_t)1 << (log_n - 1)) * (double)stages;
+}
 
 // Several host arrays sent to the device in one transfer: add() stages each array and returns its
 // device address; commit() queues the copy and must precede the first launch that reads them.

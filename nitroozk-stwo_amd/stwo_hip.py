@@ -89,6 +89,7 @@ EXPORTS = [
     "stwo_hip_measure_int32_issue", "stwo_hip_prove_wide_fibonacci_per_call", "stwo_hip_prove_poseidon_per_call",
     "stwo_hip_gen_interaction_trace_poseidon", "stwo_hip_batch_eval_at_point_noalloc",
     "stwo_verify_wide_fibonacci", "stwo_verify_poseidon", "stwo_hip_fold_circle_into_line_grouped",
+    "stwo_hip_measure_butterfly_rate", "stwo_hip_kernel_units", "stwo_hip_fft_low_log", "stwo_hip_set_fft_low_log",
 ]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
@@ -185,6 +186,10 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_eval_at_point_by_folding.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, CirclePointQM31,
                                                     c.POINTER(QM31)]
     L.stwo_hip_measure_int32_issue.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
+    L.stwo_hip_measure_butterfly_rate.argtypes = [c.c_void_p, c.c_int, c.POINTER(c.c_double)]
+    L.stwo_hip_kernel_units.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double)]
+    L.stwo_hip_fft_low_log.argtypes = [c.c_void_p, c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, _U32P]
+    L.stwo_hip_set_fft_low_log.argtypes = [c.c_void_p, c.c_uint32]
     L.stwo_verify_wide_fibonacci.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, _U32P, _U32P, c.POINTER(PcsConfig),
                                              c.c_char_p, c.c_size_t]
     L.stwo_verify_poseidon.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, c.c_uint32, QM31, c.POINTER(PcsConfig),
@@ -570,8 +575,38 @@ class HipBackend:
         self._check(self.lib.stwo_hip_measure_int32_issue(self.ctx, ctypes.byref(v)))
         return v.value
 
+    def fft_low_log(self, inverse, n_cols, log_in, log_n):
+        """The low-stage tile log (2^S values per workgroup) this context uses for a transform
+        shape; 0 for transforms of fewer than 11 stages."""
+        v = ctypes.c_uint32()
+        self._check(self.lib.stwo_hip_fft_low_log(self.ctx, int(bool(inverse)), n_cols, log_in, log_n,
+                                                  ctypes.byref(v)))
+        return v.value
+
+    def set_fft_low_log(self, low_log):
+        """Fix the low-stage tile log for every later transform (0 = planned per shape)."""
+        self._check(self.lib.stwo_hip_set_fft_low_log(self.ctx, low_log))
+
+    def measure_butterfly_rate(self, inverse=False):
+        """Forward (or inverse) circle-FFT butterflies per second with register-resident data: the
+        ceiling of the transform kernels' arithmetic on this device."""
+        v = ctypes.c_double()
+        self._check(self.lib.stwo_hip_measure_butterfly_rate(self.ctx, int(bool(inverse)), ctypes.byref(v)))
+        return v.value
+
     def reset_kernel_stats(self):
         self._check(self.lib.stwo_hip_reset_kernel_stats(self.ctx))
+
+    def kernel_units(self):
+        """{class: algorithmic work units} for the classes that count them (butterflies for
+        interpolate and evaluate)."""
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            u = ctypes.c_double()
+            self._check(self.lib.stwo_hip_kernel_units(self.ctx, i, ctypes.byref(u)))
+            if u.value:
+                out[name] = u.value
+        return out
 
     def kernel_stats(self):
         """{class: (total_ms, algorithmic_bytes, launches)} for the classes that ran."""
