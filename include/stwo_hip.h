@@ -313,7 +313,10 @@ int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* ctx, const uint32_t* c
  * (STWO_HIP_KERNEL_CACHE, else ~/.cache/stwo_hip: fft_plans_<arch>_<CUs>.txt). Results never depend
  * on S. stwo_hip_fft_low_log returns the plan of a shape (tuning it if needed; 0 when the
  * transform has fewer than 11 stages); stwo_hip_set_fft_low_log fixes S for every later transform
- * of the context (clamped to each shape's range; 0 restores planning). */
+ * of the context (clamped to each shape's range; 0 restores planning). A plan may carry the flag
+ * 0x100 (S = 15): the low-stage kernel then runs one workgroup per CU, each looping over tiles and
+ * loading its next tile while it computes the current one; a transform whose output columns are not
+ * all 16-byte aligned runs without it. */
 int stwo_hip_fft_low_log(stwo_hip_ctx* ctx, int inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n,
                          uint32_t* low_log);
 int stwo_hip_set_fft_low_log(stwo_hip_ctx* ctx, uint32_t low_log);

@@ -89,8 +89,8 @@ void run_fft(Ctx& c, bool inverse, const uint32_t* const* ds, uint32_t* const* d
   else launch_fft(inverse, ds, dd, n_cols, log_in, log_n, tree2, tree_len, c.stream, false, low);
 }
 uint32_t plan_fft(Ctx& c, bool inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n, const uint32_t* tree2,
-                  uint32_t tree_len) {
-  return log_n <= 2 ? 0u : c.fft_low_log(inverse, n_cols, log_in, log_n, tree2, tree_len);
+                  uint32_t tree_len, uint32_t* const* dst) {
+  return log_n <= 2 ? 0u : Ctx::fft_plan_for(c.fft_low_log(inverse, n_cols, log_in, log_n, tree2, tree_len), dst, n_cols);
 }
 uint32_t canonic_half_init(uint32_t log_n) { return HCoset::half_odds(log_n - 1).initial_index; }
 void check_tree(const stwo_hip_twiddles* t, uint32_t half_init, uint32_t log_n) {
@@ -246,7 +246,7 @@ int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, 
     const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
-      const uint32_t low = plan_fft(c, true, n_cols, log_size, log_size, c.itw2.p, c.tree_len());
+      const uint32_t low = plan_fft(c, true, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), d.data());
       KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
       ks.units(fft_butterflies(n_cols, log_size, log_size));
       run_fft(c, true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), canonic_half_init(log_size), low);
@@ -266,7 +266,7 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs
     const uint32_t* const* ds = (const uint32_t* const*)c.upload_vec(s);
     uint32_t* const* dd = (uint32_t* const*)c.upload_vec(d);
     {
-      const uint32_t low = plan_fft(c, false, n_cols, log_size, log_n, c.tw2.p, c.tree_len());
+      const uint32_t low = plan_fft(c, false, n_cols, log_size, log_n, c.tw2.p, c.tree_len(), d.data());
       KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_size) + 4.0 * (double)(1u << log_n)));
       ks.units(fft_butterflies(n_cols, log_n, log_size));
       run_fft(c, false, ds, dd, n_cols, log_size, log_n, c.tw2.p, c.tree_len(), canonic_half_init(log_n), low);
@@ -976,7 +976,7 @@ int stwo_hip_interpolate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, 
     std::vector<const uint32_t*> s(evals, evals + n_cols);
     std::vector<uint32_t*> d(coeffs, coeffs + n_cols);
     {
-      const uint32_t low = plan_fft(c, true, n_cols, log_size, log_size, t->itw2.p, 1u << t->log_size);
+      const uint32_t low = plan_fft(c, true, n_cols, log_size, log_size, t->itw2.p, 1u << t->log_size, d.data());
       KScope ks(c, K_INTERPOLATE, (double)n_cols * 8.0 * (double)(1u << log_size));
       ks.units(fft_butterflies(n_cols, log_size, log_size));
       run_fft(c, true, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
@@ -997,7 +997,7 @@ int stwo_hip_evaluate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uin
     std::vector<const uint32_t*> s(coeffs, coeffs + n_cols);
     std::vector<uint32_t*> d(evals, evals + n_cols);
     {
-      const uint32_t low = plan_fft(c, false, n_cols, log_coeffs, log_size, t->tw2.p, 1u << t->log_size);
+      const uint32_t low = plan_fft(c, false, n_cols, log_coeffs, log_size, t->tw2.p, 1u << t->log_size, d.data());
       KScope ks(c, K_EVALUATE, (double)n_cols * (4.0 * (double)(1u << log_coeffs) + 4.0 * (double)(1u << log_size)));
       ks.units(fft_butterflies(n_cols, log_size, log_coeffs));
       run_fft(c, false, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_coeffs,
@@ -1066,7 +1066,9 @@ int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* h, int inverse, double* butter
 
 int stwo_hip_set_fft_low_log(stwo_hip_ctx* h, uint32_t low_log) {
   return guarded(h, [&](Ctx& c) {
-    if (low_log != 0 && (low_log < 11 || low_log > 15)) throw HipError(1, "low_log must be 0 or in 11..15");
+    const uint32_t lg = low_log & ~FFT_LOW_LOOP;
+    if (low_log != 0 && (lg < 11 || lg > 15 || ((low_log & FFT_LOW_LOOP) && lg != 15) || (low_log & ~(FFT_LOW_LOOP | 0xFFu))))
+      throw HipError(1, "low_log must be 0, 11..15, or 15 with the loop flag 0x100");
     c.fft_low_override = low_log;
   });
 }

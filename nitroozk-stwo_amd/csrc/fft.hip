@@ -218,7 +218,9 @@ __device__ __forceinline__ void round_from_global(uint32_t* v, const GLOBAL_AS u
 #pragma unroll
     for (uint32_t q = 0; q < Q; ++q) v[g * Q + q] = in[(gbase | round_elem<S, A, R>(threadIdx.x, g, q)) & in_mask];
 }
-template <int S, int A, int R>
+// ALWAYS_VEC: the caller guarantees 16-byte aligned columns, and the round at bit 0 stores with
+// 16-byte writes on one code path.
+template <int S, int A, int R, bool ALWAYS_VEC = false>
 __device__ __forceinline__ void round_to_global(uint32_t* v, GLOBAL_AS uint32_t* __restrict__ out, uint32_t gbase,
                                                 uint32_t scale, bool vec, uint32_t out_words) {
   constexpr uint32_t G = (1u << (5 - R)) << LowShape<S>::X, Q = 1u << R;
@@ -234,7 +236,7 @@ __device__ __forceinline__ void round_to_global(uint32_t* v, GLOBAL_AS uint32_t*
       __builtin_amdgcn_raw_buffer_store_b32(v[q], rs, threadIdx.x * 4u, (base + (q << A)) * 4u, 0);
     return;
   }
-  if (A == 0 && R == 5 && vec) {
+  if (A == 0 && R == 5 && (ALWAYS_VEC || vec)) {
 #pragma unroll
     for (uint32_t g = 0; g < G; ++g) {
       const uint32_t e0 = 32 * (threadIdx.x + g * LowShape<S>::T);
@@ -287,11 +289,26 @@ __device__ __forceinline__ void round_butterflies(uint32_t* v, uint32_t gbase, c
       // (when the lanes' bits all lie below bit A, the base is the same for the whole workgroup and
       // the twiddles come through the scalar cache)
       constexpr bool UNIFORM_TW = T * G <= (1u << A);
-      const uint32_t toff = UNIFORM_TW ? __builtin_amdgcn_readfirstlane(idx_base >> (s + 1)) : idx_base >> (s + 1);
+      // The round at bit 5 with one group per lane: the lanes' bits above the round are t >> 5, two
+      // values in a wave (its halves), so both halves' twiddles load through the scalar cache (the
+      // upper half's 2^(R-1-k) entries further on) and each lane selects its own. No vector load
+      // waits here on older loads and stores still in flight.
+      constexpr bool HALF_TW = !UNIFORM_TW && A == 5 && G == 1;
+      const uint32_t toff = (UNIFORM_TW || HALF_TW) ? __builtin_amdgcn_readfirstlane(idx_base >> (s + 1))
+                                                    : idx_base >> (s + 1);
       const uint32_t* tb = tree + tree_len - (1u << (log_n - s)) + toff;
+      const bool upper = (threadIdx.x & 32) != 0;
 #pragma unroll
       for (uint32_t u = 0; u < (Q >> (k + 1)); ++u) {
-        const uint32_t tw = tb[u];
+        uint32_t tw;
+        if constexpr (HALF_TW) {
+          // both values made wave-uniform, so that the select is not folded into one per-lane load
+          const uint32_t lo = __builtin_amdgcn_readfirstlane(tb[u]);
+          const uint32_t hi = __builtin_amdgcn_readfirstlane(tb[u + (1u << (R - 1 - k))]);
+          tw = upper ? hi : lo;
+        } else {
+          tw = tb[u];
+        }
 #pragma unroll
         for (uint32_t l = 0; l < (1u << k); ++l) {
           const uint32_t q = (u << (k + 1)) | l;
@@ -331,6 +348,161 @@ __global__ void __launch_bounds__(LowShape<S>::T) fft_low_kernel(const uint32_t*
   round_from_lds<S, A3, R3>(v, lds);
   round_butterflies<INV, S, A3, R3>(v, gbase, tree, tree_len, log_n);
   round_to_global<S, A3, R3>(v, out, gbase, scale, (reinterpret_cast<uintptr_t>(out) & 15) == 0, 1u << log_n);
+}
+
+// The per-lane twiddles of the round at bit 0 (R = 5, one group per lane), loaded ahead of the
+// butterflies: stage 0's circle pairs (x, y) for its four quarters, then the line twiddles of
+// stages 1..4 (8 + 4 + 2 + 1, stage k from 16 - 2^(5-k)).
+struct Round0Tw {
+  uint32_t cx[4], cy[4], t[15];
+};
+template <int S>
+__device__ __forceinline__ void load_round0_tw(Round0Tw& tw, uint32_t gbase, const uint32_t* __restrict__ tree,
+                                               uint32_t tree_len, uint32_t log_n) {
+  static_assert(LowShape<S>::X == 0, "one group per lane");
+  const uint32_t idx_base = gbase | (threadIdx.x << 5);
+  const uint32_t* lb = tree + tree_len - (1u << (log_n - 1)) + 2 * (idx_base >> 3);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) {
+    tw.cx[j] = lb[2 * j];
+    tw.cy[j] = lb[2 * j + 1];
+  }
+#pragma unroll
+  for (uint32_t k = 1; k < 5; ++k) {
+    const uint32_t* tb = tree + tree_len - (1u << (log_n - k)) + (idx_base >> (k + 1));
+#pragma unroll
+    for (uint32_t u = 0; u < (16u >> k); ++u) tw.t[16 - (32u >> k) + u] = tb[u];
+  }
+}
+// round_butterflies<INV, S, 0, 5> with the twiddles of load_round0_tw.
+template <bool INV>
+__device__ __forceinline__ void round0_butterflies_pre(uint32_t* v, const Round0Tw& tw) {
+#pragma unroll
+  for (int kk = 0; kk < 5; ++kk) {
+    const int k = INV ? kk : 4 - kk;
+    if (k == 0) {
+#pragma unroll
+      for (uint32_t u = 0; u < 16; u += 4) {
+        const uint32_t x = tw.cx[u >> 2], y = tw.cy[u >> 2];
+        bfly<INV, false>(v[2 * u], v[2 * u + 1], y);
+        bfly<INV, true>(v[2 * u + 2], v[2 * u + 3], y);
+        bfly<INV, true>(v[2 * u + 4], v[2 * u + 5], x);
+        bfly<INV, false>(v[2 * u + 6], v[2 * u + 7], x);
+      }
+      continue;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < (16u >> k); ++u) {
+      const uint32_t t = tw.t[16 - (32u >> k) + u];
+#pragma unroll
+      for (uint32_t l = 0; l < (1u << k); ++l) {
+        const uint32_t q = (u << (k + 1)) | l;
+        bfly<INV>(v[q], v[q | (1u << k)], t);
+      }
+    }
+  }
+}
+
+// The low stages with each workgroup looping over tiles of 2^15 values (output columns 16-byte
+// aligned): one workgroup per CU, whose LDS holds one such tile and no second. As soon as a tile's first round has gone to
+// LDS, the workgroup issues the first round's loads of its next tile, and computes and stores the
+// second and third rounds while they are in flight. Vector memory operations complete in issue
+// order, so every vector load a tile still waits for is issued before the prefetch: the rounds at
+// bits 5 and 10 take their twiddles through the scalar cache, and the per-lane twiddles of the
+// round at bit 0 are loaded ahead (the forward transform's last round: with the tile's own
+// prefetch; the inverse transform's first round: together with the next tile's values). The only
+// wait on a prefetch is then the next tile's first round.
+// The tiles are split into 8 contiguous ranges, one per XCD (workgroup n runs on XCD n mod 8), and
+// the workgroups of an XCD take its tiles in turn: tiles of one column run side by side on one XCD
+// and share its L2. Every branch on `more` is uniform over the workgroup, so all waves meet the
+// same barriers and leave the loop together.
+// The looping kernel's first round (peeled off its loop: inside the loop, the round's waits for
+// the prefetch then have one path, after the previous tile's stores, and leave those in flight).
+template <bool INV, int S>
+__device__ __forceinline__ void loop_first_round(uint32_t* v, const Round0Tw& tw, uint32_t* lds, uint32_t gbase,
+                                                 const uint32_t* __restrict__ tree, uint32_t tree_len, uint32_t log_n) {
+  constexpr int A1 = INV ? 0 : 10, R1 = INV ? 5 : S - 10;
+  if constexpr (INV) round0_butterflies_pre<true>(v, tw);
+  else round_butterflies<false, S, A1, R1>(v, gbase, tree, tree_len, log_n);
+  round_to_lds<S, A1, R1>(v, lds);
+}
+
+// 4 waves per SIMD: one 2^15 tile (1024 threads) per CU.
+template <bool INV, int S>
+__global__ void __launch_bounds__(LowShape<S>::T, 4) fft_low_loop_kernel(const uint32_t* const* __restrict__ src,
+                                                                      uint32_t* const* __restrict__ dst, uint32_t log_n,
+                                                                      uint32_t in_mask, const uint32_t* __restrict__ tree,
+                                                                      uint32_t tree_len, uint32_t scale, uint32_t prio,
+                                                                      uint32_t n_tiles) {
+  if (prio) CRITICAL_PATH_WAVE();
+  constexpr uint32_t N = 1u << S;
+  // rounds in execution order: the inverse runs bits 0-4, 5-9, 10..S-1; the forward the reverse
+  constexpr int A1 = INV ? 0 : 10, R1 = INV ? 5 : S - 10;
+  constexpr int A3 = INV ? 10 : 0, R3 = INV ? S - 10 : 5;
+  __shared__ uint32_t lds[N + (N >> 5)];
+  const uint32_t per_xcd = gridDim.x >> 3, xcd = blockIdx.x & 7;
+  const uint32_t t_end = (uint32_t)(((uint64_t)n_tiles * (xcd + 1)) >> 3);
+  uint32_t tile = (uint32_t)(((uint64_t)n_tiles * xcd) >> 3) + (blockIdx.x >> 3);
+  if (tile >= t_end) return;
+  const uint32_t blk_log = log_n - S, blk_mask = (1u << blk_log) - 1;
+  uint32_t v[LowShape<S>::V], w[LowShape<S>::V];
+  Round0Tw tw;  // the round at bit 0: the inverse's first (next tile's), the forward's last
+  {
+    const GLOBAL_AS uint32_t* __restrict__ in = gmem(src[tile >> blk_log]);
+    round_from_global<S, A1, R1>(v, in, (tile & blk_mask) << S, in_mask, (reinterpret_cast<uintptr_t>(in) & 15) == 0);
+    if constexpr (INV) load_round0_tw<S>(tw, (tile & blk_mask) << S, tree, tree_len, log_n);
+    loop_first_round<INV, S>(v, tw, lds, (tile & blk_mask) << S, tree, tree_len, log_n);
+  }
+  for (;;) {
+    const uint32_t gbase = (tile & blk_mask) << S;
+    if constexpr (!INV) load_round0_tw<S>(tw, gbase, tree, tree_len, log_n);
+    const uint32_t next = tile + per_xcd;
+    const bool more = next < t_end;
+    {
+      // v is free: the next tile's first-round values (the last tile loads its own again, unused:
+      // a branch here would make the compiler's waits below cover the prefetch)
+      const uint32_t pf = more ? next : tile;
+      const GLOBAL_AS uint32_t* __restrict__ nin = gmem(src[pf >> blk_log]);
+      round_from_global<S, A1, R1>(v, nin, (pf & blk_mask) << S, in_mask, (reinterpret_cast<uintptr_t>(nin) & 15) == 0);
+      if constexpr (INV) load_round0_tw<S>(tw, (pf & blk_mask) << S, tree, tree_len, log_n);
+    }
+    // the loads above stay here: the compiler may not sink them past this point
+    asm volatile("" ::: "memory");
+    __syncthreads();
+    round_from_lds<S, 5, 5>(w, lds);
+    round_butterflies<INV, S, 5, 5>(w, gbase, tree, tree_len, log_n);
+    round_to_lds<S, 5, 5>(w, lds);
+    __syncthreads();
+    round_from_lds<S, A3, R3>(w, lds);
+    if constexpr (INV) round_butterflies<true, S, A3, R3>(w, gbase, tree, tree_len, log_n);
+    else round0_butterflies_pre<false>(w, tw);
+    GLOBAL_AS uint32_t* __restrict__ out = gmem(dst[tile >> blk_log]);
+    // one store path (the launch requires 16-byte aligned output columns): with two, the compiler
+    // could not tell how many stores are in flight when the next tile's first round waits
+    round_to_global<S, A3, R3, true>(w, out, gbase, scale, true, 1u << log_n);
+    if (!more) break;
+    __syncthreads();  // this tile's last LDS reads end before the next tile's first writes
+    tile = next;
+    loop_first_round<INV, S>(v, tw, lds, (tile & blk_mask) << S, tree, tree_len, log_n);
+  }
+}
+
+template <bool INV>
+static void launch_low_loop(uint32_t S, uint32_t n_cols, const uint32_t* const* src, uint32_t* const* dst,
+                            uint32_t log_n, uint32_t in_mask, const uint32_t* tree, uint32_t tree_len, uint32_t scale,
+                            uint32_t prio, hipStream_t stream) {
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const uint32_t n_tiles = n_cols << (log_n - S);
+  // one workgroup per CU (its LDS holds one 2^15 tile), a multiple of 8 (the XCD split), no more
+  // than the tiles
+  uint32_t n_wg = (uint32_t)cus / 8 * 8;
+  if (n_wg < 8) n_wg = 8;
+  while (n_wg > 8 && n_wg > n_tiles) n_wg -= 8;
+  const dim3 grid(n_wg), block(1u << (S - 5));
+  hipLaunchKernelGGL((fft_low_loop_kernel<INV, 15>), grid, block, 0, stream, src, dst, log_n, in_mask, tree, tree_len, scale,
+                     prio, n_tiles);
 }
 
 template <bool INV>
@@ -390,8 +562,11 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
   if (k >= 11) {
     // stages [0, S) in one pass of 2^S-element blocks; stages [S, k) in strided passes
     const uint32_t smax = k < 15 ? k : 15;
+    const bool loop = (low_log & FFT_LOW_LOOP) != 0;
+    low_log &= ~FFT_LOW_LOOP;
     if (low_log != 0 && (low_log < 11 || low_log > smax)) throw std::invalid_argument("launch_fft: low_log out of range");
     const uint32_t S = low_log ? low_log : smax;
+    if (loop && S != 15) throw std::invalid_argument("launch_fft: the looping low kernel needs low_log 15");
     std::vector<FftPass> high;
     for (uint32_t e = S; e < k;) {
       uint32_t st = k - e < 8 ? k - e : 8, c = 12 - st;
@@ -402,16 +577,19 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
       high.push_back({e, st, c});
       e += st;
     }
+    auto low = [&](bool inv, const uint32_t* const* from, uint32_t mask, uint32_t scale) {
+      if (inv) (loop ? launch_low_loop<true> : launch_low<true>)(S, n_cols, from, dst, log_n, mask, tree, tree_len, scale, prio, stream);
+      else (loop ? launch_low_loop<false> : launch_low<false>)(S, n_cols, from, dst, log_n, mask, tree, tree_len, scale, prio, stream);
+    };
     if (inverse) {
-      launch_low<true>(S, n_cols, src, dst, log_n, full, tree, tree_len, high.empty() ? inv_n : 0u, prio, stream);
+      low(true, src, full, high.empty() ? inv_n : 0u);
       for (size_t i = 0; i < high.size(); ++i)
         run_pass(high[i], (const uint32_t* const*)dst, full, i + 1 == high.size() ? inv_n : 0u);
     } else {
       for (size_t ii = high.size(); ii-- > 0;)
         run_pass(high[ii], ii + 1 == high.size() ? src : (const uint32_t* const*)dst,
                  ii + 1 == high.size() ? in_mask : full, 0u);
-      launch_low<false>(S, n_cols, high.empty() ? src : (const uint32_t* const*)dst, dst, log_n,
-                        high.empty() ? in_mask : full, tree, tree_len, 0u, prio, stream);
+      low(false, high.empty() ? src : (const uint32_t* const*)dst, high.empty() ? in_mask : full, 0u);
     }
     return;
   }

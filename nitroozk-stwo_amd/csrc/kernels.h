@@ -16,7 +16,10 @@ struct CCols4 { const uint32_t* c[4]; };
 // interpolation). background: the transform runs beside critical-path kernels on another stream
 // and leaves them the issue priority. low_log: the tile of the low-stage kernel, 2^low_log values
 // per workgroup (11 <= low_log <= min(k, 15) for a transform of k >= 11 stages; 0 = the largest),
-// the stages above it run in strided passes.
+// the stages above it run in strided passes. low_log | FFT_LOW_LOOP (low_log 15; every output
+// column 16-byte aligned): the low-stage kernel runs one workgroup per CU, each looping over tiles
+// and loading its next tile while it computes the current one.
+constexpr uint32_t FFT_LOW_LOOP = 0x100;
 void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols, uint32_t log_in,
                 uint32_t log_n, const uint32_t* tree, uint32_t tree_len, hipStream_t stream, bool background = false,
                 uint32_t low_log = 0);

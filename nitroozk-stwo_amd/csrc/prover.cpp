@@ -245,7 +245,11 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
-    const uint32_t low = ctx.fft_low_log(false, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len());
+    // beside critical-path kernels, not the looping kernel: its workgroups would hold every CU
+    const uint32_t low =
+        Ctx::fft_plan_for(ctx.fft_low_log(false, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len()),
+                          dst.data(), dst.size()) &
+        (on_side ? ~FFT_LOW_LOOP : ~0u);
     hipStream_t st = ctx.stream;
     if (on_side) {
       ctx.fork();  // the side stream starts after the upload and everything queued before it
@@ -375,7 +379,8 @@ void interpolate_columns(Ctx& ctx, const std::vector<const uint32_t*>& evals, co
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
-    const uint32_t low = ctx.fft_low_log(true, (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len());
+    const uint32_t low = Ctx::fft_plan_for(ctx.fft_low_log(true, (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len()),
+                                           dst.data(), dst.size());
     KScope ks(ctx, K_INTERPOLATE, (double)idx.size() * 8.0 * (1u << lg));
     ks.units(fft_butterflies(idx.size(), lg, lg));
     launch_fft(true, d_src, d_dst, (uint32_t)idx.size(), lg, lg, ctx.itw2.p, ctx.tree_len(), ctx.stream, false, low);
@@ -404,7 +409,8 @@ void evaluate_polynomials(Ctx& ctx, DevTree& t, uint32_t log_blowup) {
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
-    const uint32_t low = ctx.fft_low_log(false, (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len());
+    const uint32_t low = Ctx::fft_plan_for(ctx.fft_low_log(false, (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len()),
+                                           dst.data(), dst.size());
     KScope ks(ctx, K_EVALUATE, (double)idx.size() * (4.0 * (1u << lg) + 4.0 * (1u << K)));
     ks.units(fft_butterflies(idx.size(), K, lg));
     launch_fft(false, d_src, d_dst, (uint32_t)idx.size(), lg, K, ctx.tw2.p, ctx.tree_len(), ctx.stream, false, low);

@@ -162,6 +162,9 @@ struct Ctx {
   uint32_t fft_low_log(bool inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n, const uint32_t* tree2,
                        uint32_t tree_len);
   uint32_t fft_low_override = 0;  // a fixed tile for every shape (stwo_hip_set_fft_low_log); 0 = planned
+  // The plan for output columns `dst` (host copies of their addresses): without the looping kernel
+  // unless every column is 16-byte aligned.
+  static uint32_t fft_plan_for(uint32_t plan, uint32_t* const* dst, size_t n_cols);
   int cu_count = 0;
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
