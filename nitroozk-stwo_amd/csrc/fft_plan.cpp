@@ -17,10 +17,10 @@
 // (device, direction, columns, sizes) on scratch columns and keeps the fastest; transforms of at
 // most 15 stages whose largest-tile grid fills the device four times over take S = k without
 // timing (looping when forward and S = 15), larger ones are timed on a subset of their columns.
-// A transform beside critical-path work on another stream leaves the looping kernel out (the
-// caller's choice): its workgroups hold every CU until the transform ends. Choices persist across processes in the cache directory as
-// fft_plans_<arch>_<CUs>.txt, one "inverse n_cols log_in log_n plan" line per shape (plan: S, plus
-// 256 for the looping kernel).
+// A transform beside critical-path work on another stream is planned, and timed, without the
+// looping kernel: its workgroups would hold every CU until the transform ends. Choices persist across processes in the cache directory as
+// fft_plans_<arch>_<CUs>.txt, one "direction n_cols log_in log_n plan" line per shape (direction: 1
+// for inverse, plus 2 for a background transform; plan: S, plus 256 for the looping kernel).
 // STWO_HIP_FFT_TUNE=0 turns the timing off (a fixed rule picks S instead).
 #include <fcntl.h>
 #include <sys/stat.h>
@@ -49,7 +49,8 @@ std::string cache_directory() {
 
 namespace {
 
-using PlanKey = std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t>;  // device, inverse, n_cols, log_in, log_n
+// device, inverse (+2 for background transforms), n_cols, log_in, log_n
+using PlanKey = std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t>;
 std::mutex g_mu;
 std::map<PlanKey, uint32_t> g_plans;
 std::set<int> g_loaded;  // devices whose plan file has been read
@@ -71,12 +72,12 @@ void load_plans(int device, int cus) {
   if (path.empty()) return;
   std::FILE* fp = std::fopen(path.c_str(), "r");
   if (!fp) return;
-  unsigned inv, n, li, ln, s;
-  while (std::fscanf(fp, "%u %u %u %u %u", &inv, &n, &li, &ln, &s) == 5) {
-    const uint32_t k = inv ? ln : li, tile = s & ~FFT_LOW_LOOP, loop = s & FFT_LOW_LOOP;
-    if (k >= 11 && tile >= 11 && tile <= std::min<uint32_t>(k, 15) && li <= ln && ln <= 31 &&
-        (s & ~(FFT_LOW_LOOP | 0xFFu)) == 0 && (!loop || tile == 15))
-      g_plans[PlanKey{device, inv ? 1u : 0u, n, li, ln}] = s;
+  unsigned dir, n, li, ln, s;
+  while (std::fscanf(fp, "%u %u %u %u %u", &dir, &n, &li, &ln, &s) == 5) {
+    const uint32_t k = (dir & 1) ? ln : li, tile = s & ~FFT_LOW_LOOP, loop = s & FFT_LOW_LOOP;
+    if (dir < 4 && k >= 11 && tile >= 11 && tile <= std::min<uint32_t>(k, 15) && li <= ln && ln <= 31 &&
+        (s & ~(FFT_LOW_LOOP | 0xFFu)) == 0 && (!loop || (tile == 15 && dir < 2)))
+      g_plans[PlanKey{device, dir, n, li, ln}] = s;
   }
   std::fclose(fp);
 }
@@ -109,13 +110,13 @@ uint32_t Ctx::fft_plan_for(uint32_t plan, uint32_t* const* dst, size_t n_cols) {
 }
 
 uint32_t Ctx::fft_low_log(bool inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n, const uint32_t* tree2,
-                          uint32_t tree_len) {
+                          uint32_t tree_len, bool background) {
   const uint32_t k = inverse ? log_n : log_in;
   if (k < 11 || n_cols == 0) return 0;
   const uint32_t smax = k < 15 ? k : 15;
   if (fft_low_override) {
     const uint32_t s = std::min(std::max(fft_low_override & ~FFT_LOW_LOOP, 11u), smax);
-    const bool loop = (fft_low_override & FFT_LOW_LOOP) && s == 15;
+    const bool loop = (fft_low_override & FFT_LOW_LOOP) && s == 15 && !background;
     return s | (loop ? FFT_LOW_LOOP : 0u);
   }
   const uint64_t cus = (uint64_t)(cu_count > 0 ? cu_count : 1);
@@ -124,8 +125,8 @@ uint32_t Ctx::fft_low_log(bool inverse, uint32_t n_cols, uint32_t log_in, uint32
   // transforms of at most 15 stages that fill the device four times over: one pass of the largest
   // tile, for forward transforms in the looping kernel, whose workgroups load their next tile while
   // they compute (the inverse gains nothing from it)
-  if (fills && k <= 15) return smax | (!inverse && smax == 15 ? FFT_LOW_LOOP : 0u);
-  const PlanKey key{device, inverse ? 1u : 0u, n_cols, log_in, log_n};
+  if (fills && k <= 15) return smax | (!inverse && !background && smax == 15 ? FFT_LOW_LOOP : 0u);
+  const PlanKey key{device, (inverse ? 1u : 0u) + (background ? 2u : 0u), n_cols, log_in, log_n};
   {
     std::lock_guard<std::mutex> lk(g_mu);
     load_plans(device, cu_count);
@@ -160,7 +161,7 @@ uint32_t Ctx::fft_low_log(bool inverse, uint32_t n_cols, uint32_t log_in, uint32
   // candidates: every tile, and the looping kernel with 2^15 values
   std::vector<uint32_t> cand;
   for (uint32_t s = 11; s <= smax; ++s) cand.push_back(s);
-  if (smax == 15) cand.push_back(15 | FFT_LOW_LOOP);
+  if (smax == 15 && !background) cand.push_back(15 | FFT_LOW_LOOP);
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   for (uint32_t c : cand) {
     launch_fft(inverse, ds, dd, n_cols, log_in, log_n, tree2, tree_len, stream, false, c);  // warm-up

@@ -245,11 +245,9 @@ void evaluate_into(Ctx& ctx, const std::vector<const uint32_t*>& coeffs, const s
     }
     const uint32_t* const* d_src = (const uint32_t* const*)ctx.table_vec(src);
     uint32_t* const* d_dst = (uint32_t* const*)ctx.table_vec(dst);
-    // beside critical-path kernels, not the looping kernel: its workgroups would hold every CU
-    const uint32_t low =
-        Ctx::fft_plan_for(ctx.fft_low_log(false, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len()),
-                          dst.data(), dst.size()) &
-        (on_side ? ~FFT_LOW_LOOP : ~0u);
+    const uint32_t low = Ctx::fft_plan_for(
+        ctx.fft_low_log(false, (uint32_t)idx.size(), lg, target_log, ctx.tw2.p, ctx.tree_len(), on_side), dst.data(),
+        dst.size());
     hipStream_t st = ctx.stream;
     if (on_side) {
       ctx.fork();  // the side stream starts after the upload and everything queued before it

@@ -158,9 +158,11 @@ struct Ctx {
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
   void ensure_twiddles(uint32_t log_domain);
   // Circle-FFT launch plans (fft_plan.cpp): the low-stage tile (2^S values per workgroup) for a
-  // transform shape, as launch_fft's low_log; 0 for transforms of fewer than 11 stages.
+  // transform shape, as launch_fft's low_log; 0 for transforms of fewer than 11 stages. background:
+  // the transform runs beside critical-path kernels on another stream, and its plan leaves out the
+  // looping kernel (whose workgroups would hold every CU until the transform ends).
   uint32_t fft_low_log(bool inverse, uint32_t n_cols, uint32_t log_in, uint32_t log_n, const uint32_t* tree2,
-                       uint32_t tree_len);
+                       uint32_t tree_len, bool background = false);
   uint32_t fft_low_override = 0;  // a fixed tile for every shape (stwo_hip_set_fft_low_log); 0 = planned
   // The plan for output columns `dst` (host copies of their addresses): without the looping kernel
   // unless every column is 16-byte aligned.
