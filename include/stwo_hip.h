@@ -161,6 +161,16 @@ int stwo_hip_batch_barycentric_eval(stwo_hip_ctx* ctx, const uint32_t* const* ev
 int stwo_hip_commit_on_layer(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols,
                              uint32_t n_cols, uint32_t* out);
 
+/* MerkleOps<Poseidon252MerkleHasher>::commit_on_layer (the second Merkle channel:
+ * core/vcs/poseidon252_merkle.rs:18-41; poseidon252_commit_on_first_layer / _with_previous,
+ * stwo_cuda/cuda/poseidon252.cuh, via prover/backend/cuda/poseidon252.rs). A hash is a canonical
+ * felt252 as 8 little-endian 32-bit words (starknet FieldElement::to_bytes_be read backwards in
+ * 4-byte words). Node i = poseidon_hash(prev[2i], prev[2i+1]) when n_cols is 0, else
+ * poseidon_hash_many(prev[2i], prev[2i+1] (when prev is not NULL), the columns' values at i packed
+ * 8 per felt252 with the length of a short last group in bits 248..250). out: 2^log_size x 8 words. */
+int stwo_hip_commit_on_layer_poseidon252(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_t* prev,
+                                         const uint32_t* const* cols, uint32_t n_cols, uint32_t* out);
+
 /* ---------------------------------------------------------------- FriOps --------------- */
 /* fold_circle_into_line (prover/fri.rs:46-66; bindings.rs:196-212): dst (4 coordinate columns of
  * 2^(log_src-1)) = dst * alpha^2 + fold(src), src on CanonicCoset(log_src). Uses the cached twiddles. */
@@ -220,6 +230,17 @@ int stwo_hip_generate_secure_powers(stwo_hip_ctx* ctx, stwo_qm31 felt, uint32_t 
 /* GrindOps<Blake2sChannel>::grind (core/proof_of_work.rs:3-7; grind_blake2s, bindings.rs:615):
  * smallest nonce (hi << 32) | lo, lo < 2^20, for the channel digest; synchronous. */
 int stwo_hip_grind(stwo_hip_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce);
+/* GrindOps<Poseidon252Channel>::grind (prover/backend/simd/grind.rs:150-190, which the CUDA
+ * backend calls, cuda/backend.rs:52-56): the first nonce (hi << 32) | lo, lo < 2^14, in (hi, lo)
+ * order, with poseidon_hash(poseidon_hash_many(0x12345678, digest, pow_bits), nonce) ending in
+ * pow_bits zero bits (channel/poseidon252.rs:146-156). digest: a canonical felt252 (8 words LE);
+ * pow_bits <= 64 (status 3 beyond); synchronous. */
+int stwo_hip_grind_poseidon252(stwo_hip_ctx* ctx, const uint32_t digest[8], uint32_t pow_bits, uint64_t* nonce);
+/* No context, no GPU: starknet-crypto's poseidon_hash_many of n canonical felt252s (8 words LE
+ * each), and poseidon_permute_comp of a canonical state of 3, on the host. Status 1 for a
+ * non-canonical input. For bindings and channels that hash on the host. */
+int stwo_poseidon252_hash_many(const uint32_t* values, size_t n, uint32_t out[8]);
+int stwo_poseidon252_permute(uint32_t state[24]);
 
 /* ---------------------------------------------------------------- ComponentProver ------ */
 /* evaluate_constraint_quotients_on_domain (bindings.rs:469-490; constraint-framework

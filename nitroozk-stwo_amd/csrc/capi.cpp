@@ -590,6 +590,45 @@ int stwo_hip_grind(stwo_hip_ctx* h, const uint8_t digest[32], uint32_t pow_bits,
   });
 }
 
+int stwo_hip_commit_on_layer_poseidon252(stwo_hip_ctx* h, uint32_t log_size, const uint32_t* prev,
+                                         const uint32_t* const* cols, uint32_t n_cols, uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_size > 30) throw HipError(3, "log_size out of range");
+    if (!out || (n_cols && !cols)) throw HipError(1, "null argument");
+    std::vector<const uint32_t*> v(cols, cols + n_cols);
+    const uint32_t* const* d = n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
+    {
+      KScope ks(c, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * n_cols + 32.0) * (double)(1u << log_size));
+      launch_p252_layer(log_size, prev, d, n_cols, out, c.stream);
+    }
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_grind_poseidon252(stwo_hip_ctx* h, const uint32_t digest[8], uint32_t pow_bits, uint64_t* nonce) {
+  return guarded(h, [&](Ctx& c) {
+    if (!digest || !nonce) throw HipError(1, "null argument");
+    if (!p252_canonical(digest)) throw HipError(1, "digest is not a canonical felt252");
+    if (pow_bits > 64) throw HipError(3, "pow_bits above 64");
+    DevBuf best(&c, 2);
+    *nonce = p252_grind(digest, pow_bits, reinterpret_cast<unsigned long long*>(best.p), c.stream);
+  });
+}
+int stwo_poseidon252_hash_many(const uint32_t* values, size_t n, uint32_t out[8]) {
+  if ((n && !values) || !out) return 1;
+  for (size_t k = 0; k < n; ++k)
+    if (!p252_canonical(values + 8 * k)) return 1;
+  p252_hash_many_host(values, n, out);
+  return 0;
+}
+int stwo_poseidon252_permute(uint32_t state[24]) {
+  if (!state) return 1;
+  for (int k = 0; k < 3; ++k)
+    if (!p252_canonical(state + 8 * k)) return 1;
+  p252_permute_host(state);
+  return 0;
+}
+
 int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* const acc[4],
                                                      const uint32_t* const* trace, uint32_t n_trace,
                                                      const uint32_t* const* interaction, uint32_t n_interaction,

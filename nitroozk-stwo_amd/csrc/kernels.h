@@ -251,4 +251,18 @@ double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
 // Forward (or inverse) FFT butterflies per second on register-resident data (butterfly.h).
 double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t stream);
 
+// poseidon252.hip: the Poseidon252 Merkle channel. Hashes are canonical felt252s, 8 little-endian
+// words each. prev: 2^(log_size+1) child hashes or null; cols: n_cols M31 columns of 2^log_size.
+void launch_p252_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
+                       uint32_t* out, hipStream_t stream);
+// The first nonce (hi << 32) | lo, lo < 2^14, in (hi, lo) order, whose hash with the proof-of-work
+// prefix of `digest` (canonical) has pow_bits trailing zero bits; d_best: 8 bytes of device
+// scratch. Synchronises the stream.
+uint64_t p252_grind(const uint32_t digest[8], uint32_t pow_bits, unsigned long long* d_best, hipStream_t stream);
+// Host: poseidon_hash_many of n canonical values (8 words each); poseidon_permute_comp of a
+// canonical state of 3; whether 8 words hold a canonical felt252.
+void p252_hash_many_host(const uint32_t* values, size_t n, uint32_t out[8]);
+void p252_permute_host(uint32_t state[24]);
+bool p252_canonical(const uint32_t* v);
+
 }  // namespace sh

@@ -90,6 +90,8 @@ EXPORTS = [
     "stwo_hip_gen_interaction_trace_poseidon", "stwo_hip_batch_eval_at_point_noalloc",
     "stwo_verify_wide_fibonacci", "stwo_verify_poseidon", "stwo_hip_fold_circle_into_line_grouped",
     "stwo_hip_measure_butterfly_rate", "stwo_hip_kernel_units", "stwo_hip_fft_low_log", "stwo_hip_set_fft_low_log",
+    "stwo_hip_commit_on_layer_poseidon252", "stwo_hip_grind_poseidon252", "stwo_poseidon252_hash_many",
+    "stwo_poseidon252_permute",
 ]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
@@ -189,6 +191,11 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_measure_butterfly_rate.argtypes = [c.c_void_p, c.c_int, c.POINTER(c.c_double)]
     L.stwo_hip_kernel_units.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double)]
     L.stwo_hip_fft_low_log.argtypes = [c.c_void_p, c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, _U32P]
+    L.stwo_hip_commit_on_layer_poseidon252.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32,
+                                                       c.c_void_p]
+    L.stwo_hip_grind_poseidon252.argtypes = [c.c_void_p, _U32P, c.c_uint32, c.POINTER(c.c_uint64)]
+    L.stwo_poseidon252_hash_many.argtypes = [_U32P, c.c_size_t, _U32P]
+    L.stwo_poseidon252_permute.argtypes = [_U32P]
     L.stwo_hip_set_fft_low_log.argtypes = [c.c_void_p, c.c_uint32]
     L.stwo_verify_wide_fibonacci.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, _U32P, _U32P, c.POINTER(PcsConfig),
                                              c.c_char_p, c.c_size_t]
@@ -479,6 +486,14 @@ class HipBackend:
                                                       _ptrs(cols), len(cols), out.ptr))
         return out
 
+    def commit_on_layer_poseidon252(self, log_size, prev, cols, out=None):
+        """MerkleOps<Poseidon252MerkleHasher>::commit_on_layer: 2^log_size canonical felt252 hashes,
+        8 little-endian words each (felt252_to_words / felt252_from_words convert)."""
+        out = out if out is not None else self.column(8 << log_size)
+        self._check(self.lib.stwo_hip_commit_on_layer_poseidon252(
+            self.ctx, log_size, prev.ptr if prev is not None else None, _ptrs(cols), len(cols), out.ptr))
+        return out
+
     # ---- FriOps ----
     def fold_circle_into_line(self, dst4, src4, alpha, log_src):
         self._check(self.lib.stwo_hip_fold_circle_into_line(self.ctx, log_src, _ptrs(src4), _ptrs(dst4),
@@ -553,6 +568,14 @@ class HipBackend:
     def grind(self, digest, pow_bits):
         nonce = ctypes.c_uint64()
         self._check(self.lib.stwo_hip_grind(self.ctx, digest, pow_bits, ctypes.byref(nonce)))
+        return nonce.value
+
+    def grind_poseidon252(self, digest, pow_bits):
+        """GrindOps<Poseidon252Channel>::grind for a channel digest (a felt252 as an int)."""
+        nonce = ctypes.c_uint64()
+        d = felt252_to_words(digest)
+        self._check(self.lib.stwo_hip_grind_poseidon252(self.ctx, d.ctypes.data_as(_U32P), pow_bits,
+                                                        ctypes.byref(nonce)))
         return nonce.value
 
     # ---- ComponentProver ----
@@ -692,6 +715,34 @@ class HipBackend:
         st = self.lib.stwo_hip_prove_poseidon(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
                                               ctypes.byref(config), ctypes.byref(h))
         return self._proof(st, h, allow_unsatisfied)
+
+
+def felt252_to_words(v):
+    """A felt252 (int) as the library's 8 little-endian 32-bit words."""
+    return np.array([(int(v) >> (32 * i)) & 0xFFFFFFFF for i in range(8)], np.uint32)
+
+
+def felt252_from_words(w):
+    return sum(int(x) << (32 * i) for i, x in enumerate(np.asarray(w, np.uint32)[:8]))
+
+
+def poseidon252_hash_many(values):
+    """starknet-crypto's poseidon_hash_many on the host (no GPU)."""
+    L = load_library()
+    v = np.concatenate([felt252_to_words(x) for x in values]) if values else np.zeros(8, np.uint32)
+    out = np.zeros(8, np.uint32)
+    if L.stwo_poseidon252_hash_many(v.ctypes.data_as(_U32P), len(values), out.ctypes.data_as(_U32P)) != 0:
+        raise StwoHipError("non-canonical felt252", 1)
+    return felt252_from_words(out)
+
+
+def poseidon252_permute(state):
+    """poseidon_permute_comp of 3 felt252s on the host (no GPU)."""
+    L = load_library()
+    v = np.concatenate([felt252_to_words(x) for x in state])
+    if L.stwo_poseidon252_permute(v.ctypes.data_as(_U32P)) != 0:
+        raise StwoHipError("non-canonical felt252", 1)
+    return [felt252_from_words(v[8 * i:8 * i + 8]) for i in range(3)]
 
 
 def verify_wide_fibonacci(proof: bytes, log_n_rows, n_cols, config):
