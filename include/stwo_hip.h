@@ -300,6 +300,25 @@ int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_comp
                                const uint32_t* n_cols, const stwo_pcs_config* config, char* err, size_t err_len);
 int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t instances_per_row, uint32_t log_n_rows,
                          stwo_qm31 claimed_sum, const stwo_pcs_config* config, char* err, size_t err_len);
+/* prove() and verify over Poseidon252MerkleChannel (core/vcs/poseidon252_merkle.rs:78-85 with
+ * Poseidon252Channel, core/channel/poseidon252.rs:17-156): the same AIRs, configuration and
+ * per-call sequence as stwo_hip_prove_*_per_call, with every tree committed by
+ * stwo_hip_commit_on_layer_poseidon252, the proof of work ground by stwo_hip_grind_poseidon252 and
+ * the channel on felt252 (the reference's test_wide_fib_prove_with_poseidon and
+ * test_poseidon_prove_with_poseidon, crates/examples). Every hash in the proof, and the digest
+ * stwo_proof_digest returns, is a felt252 as 32 little-endian bytes; the verifier rejects one at or
+ * above the Stark prime as "InvalidStructure". Status codes as above. */
+int stwo_hip_prove_wide_fibonacci_poseidon252(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t n_components,
+                                               const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                               const stwo_pcs_config* config, stwo_proof** out);
+int stwo_hip_prove_poseidon_poseidon252(stwo_hip_ctx* ctx, const uint32_t* const* trace, uint32_t instances_per_row,
+                                        uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out);
+int stwo_verify_wide_fibonacci_poseidon252(const uint8_t* proof, size_t len, uint32_t n_components,
+                                           const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                           const stwo_pcs_config* config, char* err, size_t err_len);
+int stwo_verify_poseidon_poseidon252(const uint8_t* proof, size_t len, uint32_t instances_per_row,
+                                     uint32_t log_n_rows, stwo_qm31 claimed_sum, const stwo_pcs_config* config,
+                                     char* err, size_t err_len);
 int stwo_proof_status(const stwo_proof* p);
 size_t stwo_proof_size(const stwo_proof* p);
 const uint8_t* stwo_proof_bytes(const stwo_proof* p);

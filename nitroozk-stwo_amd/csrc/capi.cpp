@@ -767,9 +767,9 @@ int stwo_hip_prove_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint3
     reject_unsatisfied(*p);
   });
 }
-int stwo_hip_prove_wide_fibonacci_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
-                                            const uint32_t* log_n_rows, const uint32_t* n_cols,
-                                            const stwo_pcs_config* config, stwo_proof** out) {
+static int prove_wide_fib_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
+                                   const uint32_t* log_n_rows, const uint32_t* n_cols, const stwo_pcs_config* config,
+                                   stwo_proof** out, MerkleKind mc) {
   return guarded(h, [&](Ctx&) {
     if (!out || !log_n_rows || !n_cols) throw HipError(1, "null argument");
     if (n_components == 0) throw HipError(1, "no components");
@@ -784,7 +784,7 @@ int stwo_hip_prove_wide_fibonacci_per_call(stwo_hip_ctx* h, const uint32_t* cons
     std::vector<const uint32_t*> t(trace, trace + total);
     auto* p = new stwo_proof();
     try {
-      p->r = percall::prove_wide_fib_multi(h, t, logs, nc, cfg);
+      p->r = percall::prove_wide_fib_multi(h, t, logs, nc, cfg, mc);
     } catch (...) {
       delete p;
       throw;
@@ -793,8 +793,8 @@ int stwo_hip_prove_wide_fibonacci_per_call(stwo_hip_ctx* h, const uint32_t* cons
     reject_unsatisfied(*p);
   });
 }
-int stwo_hip_prove_poseidon_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr, uint32_t log_n_rows,
-                                     const stwo_pcs_config* config, stwo_proof** out) {
+static int prove_poseidon_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr, uint32_t log_n_rows,
+                                   const stwo_pcs_config* config, stwo_proof** out, MerkleKind mc) {
   return guarded(h, [&](Ctx&) {
     if (!out) throw HipError(1, "null out");
     if (ipr < 1 || ipr > 64) throw HipError(1, "instances_per_row must be in [1, 64]");
@@ -803,7 +803,7 @@ int stwo_hip_prove_poseidon_per_call(stwo_hip_ctx* h, const uint32_t* const* tra
     std::vector<const uint32_t*> t(trace, trace + 158 * ipr);
     auto* p = new stwo_proof();
     try {
-      p->r = percall::prove_poseidon(h, t, log_n_rows, ipr, cfg);
+      p->r = percall::prove_poseidon(h, t, log_n_rows, ipr, cfg, mc);
     } catch (...) {
       delete p;
       throw;
@@ -811,6 +811,24 @@ int stwo_hip_prove_poseidon_per_call(stwo_hip_ctx* h, const uint32_t* const* tra
     *out = p;
     reject_unsatisfied(*p);
   });
+}
+int stwo_hip_prove_wide_fibonacci_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
+                                            const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                            const stwo_pcs_config* config, stwo_proof** out) {
+  return prove_wide_fib_per_call(h, trace, n_components, log_n_rows, n_cols, config, out, MerkleKind::Blake2s);
+}
+int stwo_hip_prove_poseidon_per_call(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr, uint32_t log_n_rows,
+                                     const stwo_pcs_config* config, stwo_proof** out) {
+  return prove_poseidon_per_call(h, trace, ipr, log_n_rows, config, out, MerkleKind::Blake2s);
+}
+int stwo_hip_prove_wide_fibonacci_poseidon252(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t n_components,
+                                               const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                               const stwo_pcs_config* config, stwo_proof** out) {
+  return prove_wide_fib_per_call(h, trace, n_components, log_n_rows, n_cols, config, out, MerkleKind::Poseidon252);
+}
+int stwo_hip_prove_poseidon_poseidon252(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr,
+                                        uint32_t log_n_rows, const stwo_pcs_config* config, stwo_proof** out) {
+  return prove_poseidon_per_call(h, trace, ipr, log_n_rows, config, out, MerkleKind::Poseidon252);
 }
 int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* h, const uint32_t* const* trace, uint32_t ipr,
                                             uint32_t log_n_rows, stwo_qm31 z, stwo_qm31 alpha, uint32_t* const* out,
@@ -851,8 +869,9 @@ int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* h, const uint32_t* con
   });
 }
 
-int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_components, const uint32_t* log_n_rows,
-                               const uint32_t* n_cols, const stwo_pcs_config* config, char* err, size_t err_len) {
+static int verify_wide_fib(const uint8_t* proof, size_t len, uint32_t n_components, const uint32_t* log_n_rows,
+                           const uint32_t* n_cols, const stwo_pcs_config* config, char* err, size_t err_len,
+                           MerkleKind mc) {
   std::string e;
   int st = 0;
   try {
@@ -866,7 +885,7 @@ int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_comp
       c.n_cols = n_cols[i];
       comps.push_back(c);
     }
-    e = verify_proof(proof, len, comps, cfg_of(config));
+    e = verify_proof(proof, len, comps, cfg_of(config), mc);
     st = e.empty() ? 0 : 4;
   } catch (const HipError& x) {
     e = x.what();
@@ -878,8 +897,8 @@ int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_comp
   if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
   return st;
 }
-int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t ipr, uint32_t log_n_rows, stwo_qm31 claimed_sum,
-                         const stwo_pcs_config* config, char* err, size_t err_len) {
+static int verify_poseidon(const uint8_t* proof, size_t len, uint32_t ipr, uint32_t log_n_rows, stwo_qm31 claimed_sum,
+                           const stwo_pcs_config* config, char* err, size_t err_len, MerkleKind mc) {
   std::string e;
   int st = 0;
   try {
@@ -891,7 +910,7 @@ int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t ipr, uint32_
     c.log_n = log_n_rows;
     c.ipr = ipr;
     c.claimed_sum = q(claimed_sum);
-    e = verify_proof(proof, len, {c}, cfg_of(config));
+    e = verify_proof(proof, len, {c}, cfg_of(config), mc);
     st = e.empty() ? 0 : 4;
   } catch (const HipError& x) {
     e = x.what();
@@ -902,6 +921,24 @@ int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t ipr, uint32_
   }
   if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
   return st;
+}
+int stwo_verify_wide_fibonacci(const uint8_t* proof, size_t len, uint32_t n_components, const uint32_t* log_n_rows,
+                               const uint32_t* n_cols, const stwo_pcs_config* config, char* err, size_t err_len) {
+  return verify_wide_fib(proof, len, n_components, log_n_rows, n_cols, config, err, err_len, MerkleKind::Blake2s);
+}
+int stwo_verify_poseidon(const uint8_t* proof, size_t len, uint32_t ipr, uint32_t log_n_rows, stwo_qm31 claimed_sum,
+                         const stwo_pcs_config* config, char* err, size_t err_len) {
+  return verify_poseidon(proof, len, ipr, log_n_rows, claimed_sum, config, err, err_len, MerkleKind::Blake2s);
+}
+int stwo_verify_wide_fibonacci_poseidon252(const uint8_t* proof, size_t len, uint32_t n_components,
+                                           const uint32_t* log_n_rows, const uint32_t* n_cols,
+                                           const stwo_pcs_config* config, char* err, size_t err_len) {
+  return verify_wide_fib(proof, len, n_components, log_n_rows, n_cols, config, err, err_len, MerkleKind::Poseidon252);
+}
+int stwo_verify_poseidon_poseidon252(const uint8_t* proof, size_t len, uint32_t ipr, uint32_t log_n_rows,
+                                     stwo_qm31 claimed_sum, const stwo_pcs_config* config, char* err,
+                                     size_t err_len) {
+  return verify_poseidon(proof, len, ipr, log_n_rows, claimed_sum, config, err, err_len, MerkleKind::Poseidon252);
 }
 
 int stwo_proof_status(const stwo_proof* p) { return p ? p->r.status : -1; }

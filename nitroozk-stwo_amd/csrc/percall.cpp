@@ -59,6 +59,31 @@ struct Col {
   }
 };
 
+// The MerkleChannel the generic prover is instantiated with (BackendForChannel<MC>,
+// prover/backend/mod.rs:34-37): its channel type and the backend's MerkleOps<MC::H>::commit_on_layer
+// and GrindOps<MC::C>::grind entry points.
+struct Blake2sMC {
+  using Ch = Channel;
+  static int commit_on_layer(stwo_hip_ctx* h, uint32_t ls, const uint32_t* prev, const uint32_t* const* cols,
+                             uint32_t n, uint32_t* out) {
+    return stwo_hip_commit_on_layer(h, ls, prev, cols, n, out);
+  }
+  static int grind(stwo_hip_ctx* h, const Ch& ch, uint32_t bits, uint64_t* nonce) {
+    return stwo_hip_grind(h, ch.digest.data(), bits, nonce);
+  }
+};
+struct Poseidon252MC {
+  using Ch = P252Channel;
+  static int commit_on_layer(stwo_hip_ctx* h, uint32_t ls, const uint32_t* prev, const uint32_t* const* cols,
+                             uint32_t n, uint32_t* out) {
+    return stwo_hip_commit_on_layer_poseidon252(h, ls, prev, cols, n, out);
+  }
+  static int grind(stwo_hip_ctx* h, const Ch& ch, uint32_t bits, uint64_t* nonce) {
+    const auto w = ch.digest_words();
+    return stwo_hip_grind_poseidon252(h, w.data(), bits, nonce);
+  }
+};
+
 stwo_qm31 wq(qm31 v) { return {v.a, v.b, v.c, v.d}; }
 qm31 rq(stwo_qm31 v) { return {v.a, v.b, v.c, v.d}; }
 stwo_circle_point_qm31 wp(const cpqm31& p) { return {wq(p.x), wq(p.y)}; }
@@ -107,11 +132,12 @@ void interpolate_into(const Api& a, const std::vector<const uint32_t*>& evals, c
 }
 
 // MerkleProver::commit: columns by decreasing size, one commit_on_layer per layer.
+template <class MC>
 void merkle_commit(const Api& a, Tree& t) {
   t.layers.clear();
   if (t.evals.empty()) {
     t.layers.emplace_back(a, 8, false);
-    a.ck(stwo_hip_commit_on_layer(a.h, 0, nullptr, nullptr, 0, t.layers[0].p), "commit_on_layer");
+    a.ck(MC::commit_on_layer(a.h, 0, nullptr, nullptr, 0, t.layers[0].p), "commit_on_layer");
     t.root = read_root(a, t.layers[0].p);
     return;
   }
@@ -123,8 +149,8 @@ void merkle_commit(const Api& a, Tree& t) {
     std::vector<const uint32_t*> lc;
     while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(t.evals[order[ci++]].p);
     Col layer(a, (size_t)8 << ls, false);
-    a.ck(stwo_hip_commit_on_layer(a.h, (uint32_t)ls, rev.empty() ? nullptr : rev.back().p, lc.data(),
-                                  (uint32_t)lc.size(), layer.p),
+    a.ck(MC::commit_on_layer(a.h, (uint32_t)ls, rev.empty() ? nullptr : rev.back().p, lc.data(),
+                             (uint32_t)lc.size(), layer.p),
          "commit_on_layer");
     rev.push_back(std::move(layer));
   }
@@ -134,7 +160,8 @@ void merkle_commit(const Api& a, Tree& t) {
 
 // CommitmentSchemeProver::commit of a tree given by coefficients: evaluate_polynomials with the
 // blowup (one call per size), the Merkle tree, and the root into the channel.
-void commit(const Api& a, Tree& t, const PcsCfg& cfg, Channel& ch) {
+template <class MC>
+void commit(const Api& a, Tree& t, const PcsCfg& cfg, typename MC::Ch& ch) {
   std::map<uint32_t, std::vector<size_t>> by;
   for (size_t i = 0; i < t.coeffs.size(); ++i) by[t.logs[i]].push_back(i);
   t.evals.resize(t.coeffs.size());
@@ -151,7 +178,7 @@ void commit(const Api& a, Tree& t, const PcsCfg& cfg, Channel& ch) {
     a.ck(stwo_hip_evaluate_polynomials(a.h, src.data(), dst.data(), (uint32_t)idx.size(), lg, cfg.log_blowup),
          "evaluate_polynomials");
   }
-  merkle_commit(a, t);
+  merkle_commit<MC>(a, t);
   ch.mix_root(t.root);
 }
 
@@ -260,7 +287,8 @@ uint32_t fnv1a(const char* s) {
 
 // prove_ex over committed trees (0 preprocessed, 1 trace, [2 interaction]); appends the
 // composition tree. Returns the sanity-check status (0, or 2 for ConstraintsNotSatisfied).
-int prove_ex(const Api& a, Channel& ch, std::vector<Tree>& trees, const std::vector<Component>& comps,
+template <class MC>
+int prove_ex(const Api& a, typename MC::Ch& ch, std::vector<Tree>& trees, const std::vector<Component>& comps,
              const PcsCfg& cfg, ProofParts& out) {
   // ---- composition (ComponentProvers::compute_composition_polynomial) ----
   const qm31 random_coeff = ch.draw_secure_felt();
@@ -363,7 +391,7 @@ int prove_ex(const Api& a, Channel& ch, std::vector<Tree>& trees, const std::vec
     comp.coeffs = halves;
     comp.logs.assign(8, max_log - 1);
   }
-  commit(a, comp, cfg, ch);
+  commit<MC>(a, comp, cfg, ch);
   trees.push_back(std::move(comp));
 
   // ---- OODS point and prove_values ----
@@ -523,7 +551,7 @@ int prove_ex(const Api& a, Channel& ch, std::vector<Tree>& trees, const std::vec
   // ---- FRI commit ----
   const uint32_t last_log = cfg.log_last_layer + cfg.log_blowup;
   if (Lc - 1 < last_log) throw HipError(3, "FRI: last layer larger than the first folded layer");
-  merkle_commit(a, first.tree);
+  merkle_commit<MC>(a, first.tree);
   ch.mix_root(first.tree.root);
   qm31 alpha = ch.draw_secure_felt();
   FriCol cur = fri_col(a, Lc - 1, true);
@@ -536,7 +564,7 @@ int prove_ex(const Api& a, Channel& ch, std::vector<Tree>& trees, const std::vec
   while (cur.log > last_log) {
     const uint32_t fs = cur.log > last_log + cfg.fold_step ? cfg.fold_step : cur.log - last_log;
     cur.fold_step = fs;
-    merkle_commit(a, cur.tree);
+    merkle_commit<MC>(a, cur.tree);
     ch.mix_root(cur.tree.root);
     alpha = ch.draw_secure_felt();
     FriCol nx = fri_col(a, cur.log - fs, false);
@@ -570,7 +598,7 @@ int prove_ex(const Api& a, Channel& ch, std::vector<Tree>& trees, const std::vec
   }
 
   // ---- proof of work (GrindOps::grind) ----
-  a.ck(stwo_hip_grind(a.h, ch.digest.data(), cfg.pow_bits, &out.pow), "grind");
+  a.ck(MC::grind(a.h, ch, cfg.pow_bits, &out.pow), "grind");
   ch.mix_u64(out.pow);
 
   // ---- queries and decommitments ----
@@ -626,16 +654,15 @@ int prove_ex(const Api& a, Channel& ch, std::vector<Tree>& trees, const std::vec
   return qm_eq(committed, acc) ? 0 : 2;
 }
 
-}  // namespace
-
-ProveResult prove_wide_fib_multi(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace,
-                                 const std::vector<uint32_t>& logs, const std::vector<uint32_t>& ncols,
-                                 const PcsCfg& cfg) {
+template <class MC>
+ProveResult prove_wide_fib_multi_mc(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace,
+                                    const std::vector<uint32_t>& logs, const std::vector<uint32_t>& ncols,
+                                    const PcsCfg& cfg) {
   const Api a{h};
   ProveResult r;
-  Channel ch;
+  typename MC::Ch ch;
   std::vector<Tree> trees(2);
-  commit(a, trees[0], cfg, ch);  // the preprocessed tree has no columns
+  commit<MC>(a, trees[0], cfg, ch);  // the preprocessed tree has no columns
   std::vector<uint32_t> col_logs;
   std::vector<Component> comps;
   size_t start = 0;
@@ -651,23 +678,24 @@ ProveResult prove_wide_fib_multi(stwo_hip_ctx* h, const std::vector<const uint32
   }
   if (start != trace.size()) throw HipError(1, "trace column count does not match the components");
   interpolate_into(a, trace, col_logs, trees[1]);
-  commit(a, trees[1], cfg, ch);
+  commit<MC>(a, trees[1], cfg, ch);
   ProofParts parts;
-  r.status = prove_ex(a, ch, trees, comps, cfg, parts);
+  r.status = prove_ex<MC>(a, ch, trees, comps, cfg, parts);
   r.proof = encode(cfg, parts);
   r.digest = ch.digest;
   return r;
 }
 
-ProveResult prove_poseidon(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
-                           const PcsCfg& cfg) {
+template <class MC>
+ProveResult prove_poseidon_mc(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace, uint32_t log_n,
+                              uint32_t ipr, const PcsCfg& cfg) {
   const Api a{h};
   ProveResult r;
-  Channel ch;
+  typename MC::Ch ch;
   std::vector<Tree> trees(2);
-  commit(a, trees[0], cfg, ch);
+  commit<MC>(a, trees[0], cfg, ch);
   interpolate_into(a, trace, std::vector<uint32_t>(trace.size(), log_n), trees[1]);
-  commit(a, trees[1], cfg, ch);
+  commit<MC>(a, trees[1], cfg, ch);
   Component c;
   c.kind = AIR_POSEIDON;
   c.log_n = log_n;
@@ -694,13 +722,27 @@ ProveResult prove_poseidon(stwo_hip_ctx* h, const std::vector<const uint32_t*>& 
   interpolate_into(a, std::vector<const uint32_t*>(ip.begin(), ip.end()), std::vector<uint32_t>(ip.size(), log_n),
                    trees[2]);
   icols.clear();
-  commit(a, trees[2], cfg, ch);
+  commit<MC>(a, trees[2], cfg, ch);
   r.claimed_sum = c.claimed_sum;
   ProofParts parts;
-  r.status = prove_ex(a, ch, trees, {c}, cfg, parts);
+  r.status = prove_ex<MC>(a, ch, trees, {c}, cfg, parts);
   r.proof = encode(cfg, parts);
   r.digest = ch.digest;
   return r;
+}
+
+}  // namespace
+
+ProveResult prove_wide_fib_multi(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace,
+                                 const std::vector<uint32_t>& logs, const std::vector<uint32_t>& ncols,
+                                 const PcsCfg& cfg, MerkleKind mc) {
+  return mc == MerkleKind::Poseidon252 ? prove_wide_fib_multi_mc<Poseidon252MC>(h, trace, logs, ncols, cfg)
+                                       : prove_wide_fib_multi_mc<Blake2sMC>(h, trace, logs, ncols, cfg);
+}
+ProveResult prove_poseidon(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
+                           const PcsCfg& cfg, MerkleKind mc) {
+  return mc == MerkleKind::Poseidon252 ? prove_poseidon_mc<Poseidon252MC>(h, trace, log_n, ipr, cfg)
+                                       : prove_poseidon_mc<Blake2sMC>(h, trace, log_n, ipr, cfg);
 }
 
 }  // namespace percall

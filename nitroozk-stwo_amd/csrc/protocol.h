@@ -90,7 +90,8 @@ std::vector<qm31> line_interpolate(std::vector<qm31> v, HCoset dom);
 
 // core/verifier.rs verify over the library's proof encoding, for the given components (trace and
 // interaction offsets are assigned here, in order) and the expected configuration: "" when the
-// proof is accepted, else the reason (verifier.cpp).
-std::string verify_proof(const uint8_t* bytes, size_t len, const std::vector<Component>& comps, const PcsCfg& cfg);
+// proof is accepted, else the reason (verifier.cpp); mc names the proof's Merkle channel.
+std::string verify_proof(const uint8_t* bytes, size_t len, const std::vector<Component>& comps, const PcsCfg& cfg,
+                         MerkleKind mc = MerkleKind::Blake2s);
 
 }  // namespace sh

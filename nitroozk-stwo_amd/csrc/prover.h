@@ -81,13 +81,18 @@ ProveResult prove_poseidon(Ctx& ctx, const std::vector<const uint32_t*>& trace, 
 
 typedef struct stwo_hip_ctx stwo_hip_ctx;
 namespace sh {
+// The MerkleChannel of a proof: Blake2sMerkleChannel (core/vcs/blake2_merkle.rs:45-59) or
+// Poseidon252MerkleChannel (core/vcs/poseidon252_merkle.rs:78-85).
+enum class MerkleKind { Blake2s, Poseidon252 };
+
 namespace percall {
 // prove() with every device step through the C ABI's trait entry points (percall.cpp): the plain
-// per-call form, byte-identical to the device-resident prover.
+// per-call form, byte-identical to the device-resident prover (Blake2s), and the prover over the
+// Poseidon252 Merkle channel.
 ProveResult prove_wide_fib_multi(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace,
                                  const std::vector<uint32_t>& logs, const std::vector<uint32_t>& ncols,
-                                 const PcsCfg& cfg);
+                                 const PcsCfg& cfg, MerkleKind mc = MerkleKind::Blake2s);
 ProveResult prove_poseidon(stwo_hip_ctx* h, const std::vector<const uint32_t*>& trace, uint32_t log_n, uint32_t ipr,
-                           const PcsCfg& cfg);
+                           const PcsCfg& cfg, MerkleKind mc = MerkleKind::Blake2s);
 }  // namespace percall
 }  // namespace sh

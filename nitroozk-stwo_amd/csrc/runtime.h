@@ -274,6 +274,30 @@ struct Channel {
   Hash pow_prefix(uint32_t pow_bits) const;
 };
 
+// Poseidon252Channel (core/channel/poseidon252.rs:17-156): the digest is a felt252 below the Stark
+// prime, kept as its 32 little-endian bytes; a draw permutes (digest, n_draws, 3). The part of
+// Channel's interface that prove() uses (mix_u32s is not), except that a draw yields 7 words and the
+// proof of work is checked here.
+struct P252Channel {
+  Hash digest{};
+  uint32_t n_draws = 0;
+  void update_digest(const Hash& d) { digest = d; n_draws = 0; }
+  void mix_root(const Hash& root);
+  void mix_u64(uint64_t v);
+  void mix_felts(const std::vector<qm31>& f);
+  Hash draw_felt252();
+  std::array<uint32_t, 7> draw_u32s();
+  std::array<uint32_t, 8> draw_base_felts();
+  qm31 draw_secure_felt();
+  std::vector<qm31> draw_secure_felts(size_t n);
+  std::array<uint32_t, 8> digest_words() const;
+  bool verify_pow_nonce(uint32_t pow_bits, uint64_t nonce) const;
+};
+// Poseidon252MerkleHasher::hash_node (core/vcs/poseidon252_merkle.rs:18-41) on the host: children
+// (left may be null: a leaf) and n column values, 8 per felt252.
+Hash p252_hash_node(const Hash* left, const Hash* right, const uint32_t* vals, size_t n);
+bool p252_hash_canonical(const Hash& h);
+
 // ---- host circle-domain helpers -------------------------------------------------------------
 inline uint32_t subgroup_gen(uint32_t log) { return 1u << (31 - log); }
 struct HCoset {

@@ -4,7 +4,8 @@
 //   fri_answers, sample batches, line coeffs core/pcs/quotients.rs:60-440; core/constraints.rs:119-134
 //   FriVerifier commit / decommit           core/fri.rs:98-290,424-720
 //   MerkleVerifier::verify                  core/vcs/verifier.rs:85-195
-// It reads the library's canonical proof encoding, replays the Blake2s channel and checks every
+// It reads the library's canonical proof encoding, replays the channel (Blake2s or Poseidon252,
+// the proof's MerkleChannel) and checks every
 // Merkle path, the FRI layers, the proof of work, the out-of-domain sample against the AIRs'
 // constraints, and the quotients at the queried positions. It runs on the host only, so the
 // serving loop can check each proof while the device proves the next.
@@ -118,6 +119,7 @@ Hash hash_node(const Hash* left, const Hash* right, const uint32_t* vals, size_t
   return b.finalize();
 }
 // MerkleVerifier::verify: the root from the queried values and the witnesses, every input consumed.
+template <class MC>
 void merkle_verify(const Hash& root, const std::vector<uint32_t>& col_logs,
                    const std::map<uint32_t, std::vector<size_t>>& qpl, const std::vector<uint32_t>& queried,
                    const MerkleDec& dec, const char* what) {
@@ -160,7 +162,7 @@ void merkle_verify(const Hash& root, const std::vector<uint32_t>& col_logs,
         throw Fail(std::string("Merkle (") + what + "): " + (q ? "TooFewQueriedValues" : "WitnessTooShort"));
       std::copy(src.begin() + at, src.begin() + at + nc, vals.begin());
       at += nc;
-      cur.push_back({node, hash_node(have_last ? &l : nullptr, &r, vals.data(), nc)});
+      cur.push_back({node, MC::node(have_last ? &l : nullptr, &r, vals.data(), nc)});
     }
     last = std::move(cur);
     have_last = true;
@@ -282,8 +284,41 @@ qm31 fold_coset(std::vector<qm31> e, uint32_t initial, uint32_t n, qm31 alpha) {
   return e[0];
 }
 
+// The verifier's side of a MerkleChannel: node hash, channel, proof-of-work check and which 32-byte
+// strings are hashes at all (a felt252 must lie below the Stark prime).
+struct Blake2sMC {
+  using Ch = Channel;
+  static Hash node(const Hash* l, const Hash* r, const uint32_t* v, size_t n) { return hash_node(l, r, v, n); }
+  static bool pow_ok(const Ch& ch, uint32_t bits, uint64_t nonce) { return verify_pow_nonce(ch, bits, nonce); }
+  static bool hash_ok(const Hash&) { return true; }
+};
+struct Poseidon252MC {
+  using Ch = P252Channel;
+  static Hash node(const Hash* l, const Hash* r, const uint32_t* v, size_t n) { return p252_hash_node(l, r, v, n); }
+  static bool pow_ok(const Ch& ch, uint32_t bits, uint64_t nonce) { return ch.verify_pow_nonce(bits, nonce); }
+  static bool hash_ok(const Hash& h) { return p252_hash_canonical(h); }
+};
+
+template <class MC>
+void check_hashes(const Decoded& d) {
+  auto ok = [](const Hash& h) {
+    if (!MC::hash_ok(h)) throw Fail("InvalidStructure: a hash is not a felt252");
+  };
+  for (auto& h : d.commitments) ok(h);
+  for (auto& m : d.decs)
+    for (auto& h : m.hash_witness) ok(h);
+  ok(d.first.commitment);
+  for (auto& h : d.first.dec.hash_witness) ok(h);
+  for (auto& l : d.inner) {
+    ok(l.commitment);
+    for (auto& h : l.dec.hash_witness) ok(h);
+  }
+}
+
+template <class MC>
 std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component> comps, const PcsCfg& want) {
   const Decoded d = decode(bytes, len);
+  check_hashes<MC>(d);
   const PcsCfg& cfg = d.cfg;
   if (cfg.pow_bits != want.pow_bits || cfg.log_blowup != want.log_blowup ||
       cfg.log_last_layer != want.log_last_layer || cfg.n_queries != want.n_queries ||
@@ -304,7 +339,7 @@ std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component>
     }
   }
   if (d.commitments.size() != tlogs.size() + 1) throw Fail("InvalidStructure: commitment count");
-  Channel ch;
+  typename MC::Ch ch;
   ch.mix_root(d.commitments[0]);
   ch.mix_root(d.commitments[1]);
   if (inter) {
@@ -417,7 +452,7 @@ std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component>
       (d.last_poly.size() & (d.last_poly.size() - 1)))
     throw Fail("LastLayerDegreeInvalid");
   ch.mix_felts(d.last_poly);
-  if (!verify_pow_nonce(ch, cfg.pow_bits, d.pow)) throw Fail("ProofOfWork");
+  if (!MC::pow_ok(ch, cfg.pow_bits, d.pow)) throw Fail("ProofOfWork");
   ch.mix_u64(d.pow);
   std::vector<size_t> raw;
   {
@@ -434,7 +469,7 @@ std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component>
     const uint32_t tmax = tlogs[t].empty() ? 0 : *std::max_element(tlogs[t].begin(), tlogs[t].end());
     const auto P = tree_local_positions(qpos, lls, tmax);
     const auto qpl = queries_per_log_size(tlogs[t], P);
-    merkle_verify(d.commitments[t], tlogs[t], qpl, d.queried[t], d.decs[t], "tree");
+    merkle_verify<MC>(d.commitments[t], tlogs[t], qpl, d.queried[t], d.decs[t], "tree");
     // reshape_flat_to_per_column (core/pcs/quotients.rs:360-440): decommit order is by size,
     // largest first, node by node, the size's columns in order
     std::map<uint32_t, std::vector<size_t>> by;
@@ -498,7 +533,7 @@ std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component>
   }
   // FRI decommitment: the first layer
   const Sparse s0 = rebuild(qpos, lls, answers, d.first.witness, 1, "FirstLayer");
-  merkle_verify(d.first.commitment, std::vector<uint32_t>(4, lls), {{lls, s0.positions}}, flat_coords(s0),
+  merkle_verify<MC>(d.first.commitment, std::vector<uint32_t>(4, lls), {{lls, s0.positions}}, flat_coords(s0),
                 d.first.dec, "first FRI layer");
   const HCoset src_half = HCoset::half_odds(lls - 1);
   std::vector<qm31> ev;
@@ -513,7 +548,7 @@ std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component>
   for (size_t i = 0; i < inner.size(); ++i) {
     const Inner& L = inner[i];
     const Sparse s = rebuild(q, qlog, ev, d.inner[i].witness, L.fold_step, "InnerLayer");
-    merkle_verify(d.inner[i].commitment, std::vector<uint32_t>(4, L.dom_log), {{L.dom_log, s.positions}},
+    merkle_verify<MC>(d.inner[i].commitment, std::vector<uint32_t>(4, L.dom_log), {{L.dom_log, s.positions}},
                   flat_coords(s), d.inner[i].dec, "inner FRI layer");
     const HCoset dom = HCoset::make(L.dom_init, L.dom_log);
     ev.clear();
@@ -532,9 +567,11 @@ std::string verify_impl(const uint8_t* bytes, size_t len, std::vector<Component>
 
 }  // namespace
 
-std::string verify_proof(const uint8_t* bytes, size_t len, const std::vector<Component>& comps, const PcsCfg& cfg) {
+std::string verify_proof(const uint8_t* bytes, size_t len, const std::vector<Component>& comps, const PcsCfg& cfg,
+                         MerkleKind mc) {
   try {
-    return verify_impl(bytes, len, comps, cfg);
+    return mc == MerkleKind::Poseidon252 ? verify_impl<Poseidon252MC>(bytes, len, comps, cfg)
+                                         : verify_impl<Blake2sMC>(bytes, len, comps, cfg);
   } catch (const Fail& f) {
     return f.what();
   }

@@ -91,8 +91,18 @@ EXPORTS = [
     "stwo_verify_wide_fibonacci", "stwo_verify_poseidon", "stwo_hip_fold_circle_into_line_grouped",
     "stwo_hip_measure_butterfly_rate", "stwo_hip_kernel_units", "stwo_hip_fft_low_log", "stwo_hip_set_fft_low_log",
     "stwo_hip_commit_on_layer_poseidon252", "stwo_hip_grind_poseidon252", "stwo_poseidon252_hash_many",
-    "stwo_poseidon252_permute",
+    "stwo_poseidon252_permute", "stwo_hip_prove_wide_fibonacci_poseidon252", "stwo_hip_prove_poseidon_poseidon252",
+    "stwo_verify_wide_fibonacci_poseidon252", "stwo_verify_poseidon_poseidon252",
 ]
+
+# The MerkleChannel of a proof: the suffix of its prove / verify entry points.
+MERKLE_CHANNELS = {"blake2s": ("_per_call", ""), "poseidon252": ("_poseidon252", "_poseidon252")}
+
+
+def _merkle_channel(name):
+    if name not in MERKLE_CHANNELS:
+        raise ValueError(f"unknown Merkle channel {name!r} (one of {sorted(MERKLE_CHANNELS)})")
+    return MERKLE_CHANNELS[name]
 
 KERNEL_CLASSES = ["interpolate", "evaluate", "merkle", "composition", "accumulate_numerators", "quotients",
                   "fri_fold", "oods", "logup", "grind", "misc", "fri_tail"]
@@ -197,14 +207,16 @@ def load_library(path=LIB_PATH):
     L.stwo_poseidon252_hash_many.argtypes = [_U32P, c.c_size_t, _U32P]
     L.stwo_poseidon252_permute.argtypes = [_U32P]
     L.stwo_hip_set_fft_low_log.argtypes = [c.c_void_p, c.c_uint32]
-    L.stwo_verify_wide_fibonacci.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, _U32P, _U32P, c.POINTER(PcsConfig),
-                                             c.c_char_p, c.c_size_t]
-    L.stwo_verify_poseidon.argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, c.c_uint32, QM31, c.POINTER(PcsConfig),
-                                       c.c_char_p, c.c_size_t]
-    L.stwo_hip_prove_wide_fibonacci_per_call.argtypes = [c.c_void_p, _VPP, c.c_uint32, _U32P, _U32P,
-                                                         c.POINTER(PcsConfig), c.POINTER(c.c_void_p)]
-    L.stwo_hip_prove_poseidon_per_call.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, c.POINTER(PcsConfig),
-                                                   c.POINTER(c.c_void_p)]
+    for sfx in ("", "_poseidon252"):
+        getattr(L, "stwo_verify_wide_fibonacci" + sfx).argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, _U32P, _U32P,
+                                                                  c.POINTER(PcsConfig), c.c_char_p, c.c_size_t]
+        getattr(L, "stwo_verify_poseidon" + sfx).argtypes = [c.c_char_p, c.c_size_t, c.c_uint32, c.c_uint32, QM31,
+                                                            c.POINTER(PcsConfig), c.c_char_p, c.c_size_t]
+    for sfx in ("_per_call", "_poseidon252"):
+        getattr(L, "stwo_hip_prove_wide_fibonacci" + sfx).argtypes = [c.c_void_p, _VPP, c.c_uint32, _U32P, _U32P,
+                                                                     c.POINTER(PcsConfig), c.POINTER(c.c_void_p)]
+        getattr(L, "stwo_hip_prove_poseidon" + sfx).argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32,
+                                                               c.POINTER(PcsConfig), c.POINTER(c.c_void_p)]
     L.stwo_hip_gen_interaction_trace_poseidon.argtypes = [c.c_void_p, _VPP, c.c_uint32, c.c_uint32, QM31, QM31, _VPP,
                                                           c.POINTER(QM31)]
     _lib = L
@@ -681,21 +693,24 @@ class HipBackend:
                                                           ctypes.byref(config), ctypes.byref(h))
         return self._proof(st, h, allow_unsatisfied)
 
-    def prove_wide_fibonacci_per_call(self, traces, log_n_rows, config, allow_unsatisfied=False):
+    def prove_wide_fibonacci_per_call(self, traces, log_n_rows, config, allow_unsatisfied=False,
+                                      merkle_channel="blake2s"):
         """The same proof through the trait entry points, one C-ABI call per trait-method call (the
-        plain per-call form). traces[i]: the columns of component i."""
+        plain per-call form). traces[i]: the columns of component i. merkle_channel: "blake2s" or
+        "poseidon252" (Poseidon252MerkleChannel: felt252 hashes and channel)."""
+        fn = getattr(self.lib, "stwo_hip_prove_wide_fibonacci" + _merkle_channel(merkle_channel)[0])
         logs = np.array(log_n_rows, np.uint32)
         ncols = np.array([len(t) for t in traces], np.uint32)
         h = ctypes.c_void_p()
-        st = self.lib.stwo_hip_prove_wide_fibonacci_per_call(self.ctx, _concat_ptrs(traces), len(traces),
-                                                             logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
-                                                             ctypes.byref(config), ctypes.byref(h))
+        st = fn(self.ctx, _concat_ptrs(traces), len(traces), logs.ctypes.data_as(_U32P), ncols.ctypes.data_as(_U32P),
+                ctypes.byref(config), ctypes.byref(h))
         return self._proof(st, h, allow_unsatisfied)
 
-    def prove_poseidon_per_call(self, trace, instances_per_row, log_n_rows, config, allow_unsatisfied=False):
+    def prove_poseidon_per_call(self, trace, instances_per_row, log_n_rows, config, allow_unsatisfied=False,
+                                merkle_channel="blake2s"):
+        fn = getattr(self.lib, "stwo_hip_prove_poseidon" + _merkle_channel(merkle_channel)[0])
         h = ctypes.c_void_p()
-        st = self.lib.stwo_hip_prove_poseidon_per_call(self.ctx, _ptrs(trace), instances_per_row, log_n_rows,
-                                                       ctypes.byref(config), ctypes.byref(h))
+        st = fn(self.ctx, _ptrs(trace), instances_per_row, log_n_rows, ctypes.byref(config), ctypes.byref(h))
         return self._proof(st, h, allow_unsatisfied)
 
     def gen_interaction_trace_poseidon(self, trace, instances_per_row, log_n_rows, z, alpha):
@@ -745,25 +760,26 @@ def poseidon252_permute(state):
     return [felt252_from_words(v[8 * i:8 * i + 8]) for i in range(3)]
 
 
-def verify_wide_fibonacci(proof: bytes, log_n_rows, n_cols, config):
+def verify_wide_fibonacci(proof: bytes, log_n_rows, n_cols, config, merkle_channel="blake2s"):
     """verify() of a wide-Fibonacci proof (one or several components) on the host: "" when it is
     accepted, else the verifier's error. log_n_rows / n_cols: per component."""
     L = load_library()
+    fn = getattr(L, "stwo_verify_wide_fibonacci" + _merkle_channel(merkle_channel)[1])
     logs = np.array(log_n_rows, np.uint32).reshape(-1)
     nc = np.array(n_cols, np.uint32).reshape(-1)
     err = ctypes.create_string_buffer(256)
-    st = L.stwo_verify_wide_fibonacci(proof, len(proof), len(logs), logs.ctypes.data_as(_U32P),
-                                      nc.ctypes.data_as(_U32P), ctypes.byref(config), err, 256)
+    st = fn(proof, len(proof), len(logs), logs.ctypes.data_as(_U32P), nc.ctypes.data_as(_U32P), ctypes.byref(config),
+            err, 256)
     if st == 1:
         raise StwoHipError("invalid argument: " + err.value.decode(), st)
     return "" if st == 0 else err.value.decode()
 
 
-def verify_poseidon(proof: bytes, instances_per_row, log_n_rows, claimed_sum, config):
+def verify_poseidon(proof: bytes, instances_per_row, log_n_rows, claimed_sum, config, merkle_channel="blake2s"):
     L = load_library()
+    fn = getattr(L, "stwo_verify_poseidon" + _merkle_channel(merkle_channel)[1])
     err = ctypes.create_string_buffer(256)
-    st = L.stwo_verify_poseidon(proof, len(proof), instances_per_row, log_n_rows, QM31.of(claimed_sum),
-                                ctypes.byref(config), err, 256)
+    st = fn(proof, len(proof), instances_per_row, log_n_rows, QM31.of(claimed_sum), ctypes.byref(config), err, 256)
     if st == 1:
         raise StwoHipError("invalid argument: " + err.value.decode(), st)
     return "" if st == 0 else err.value.decode()

@@ -204,6 +204,17 @@ class Channel:
     def draw_secure_felt(self):
         return tuple(self.draw_base_felts()[:4])
 
+    def draw_secure_felts(self, n):
+        """Base felts of successive draws, flattened, taken four at a time (8 per draw: 2 secure
+        felts per draw, a half-used draw's rest dropped)."""
+        out, buf = [], []
+        while len(out) < n:
+            if len(buf) < 4:
+                buf += self.draw_base_felts()
+            out.append(tuple(buf[:4]))
+            buf = buf[4:]
+        return out
+
     def draw_u32s(self):
         cur = self.draw_felt252()
         return [(cur >> (32 * i)) & 0xFFFFFFFF for i in range(7)]

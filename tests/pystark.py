@@ -34,6 +34,8 @@ little-endian u32 words, counts as u32, fields in StarkProof order.
 """
 import hashlib
 
+import p252
+
 P = (1 << 31) - 1
 GEN_INDEX_LOG = 31
 M31_GEN = (2, 1268011823)
@@ -409,8 +411,28 @@ def grind(channel, bits):
         assert n < (1 << 20), "solution beyond hi = 0 not handled in this restatement"
 
 
+class Blake2sMerkleChannel:
+    """Blake2sMerkleChannel (core/vcs/blake2_merkle.rs:45-59): Blake2s nodes and channel."""
+    Channel = Channel
+    hash_node = staticmethod(hash_node)
+    grind = staticmethod(grind)
+
+
+class Poseidon252MerkleChannel:
+    """Poseidon252MerkleChannel (core/vcs/poseidon252_merkle.rs:78-85): felt252 Poseidon nodes and
+    channel (tests/p252.py); grinding in the (hi, lo < 2^14) order of GrindOps<Poseidon252Channel>.
+    A hash is an int below the Stark prime; the proof carries it as 32 little-endian bytes."""
+    Channel = p252.Channel
+    hash_node = staticmethod(p252.hash_node)
+
+    @staticmethod
+    def grind(channel, bits):
+        return p252.grind(channel.digest, bits)
+
+
 def draw_queries(channel, log_domain, n_queries):
-    """core/queries.rs:11-29: 8 words per draw, each masked to the domain."""
+    """core/queries.rs:11-29: the words of each draw (8 for Blake2s, 7 for Poseidon252), each masked
+    to the domain."""
     out = []
     mask = (1 << log_domain) - 1
     while True:
@@ -434,7 +456,7 @@ class MerkleTree:
     log size from the largest down to 0, node i = H(children 2i, 2i+1 of the layer below, then the
     values at i of the columns of that size, in that order)."""
 
-    def __init__(self, columns):
+    def __init__(self, columns, hash_node=hash_node):
         cols = sorted(columns, key=lambda c: -len(c))  # stable
         if not cols:
             self.layers = [[hash_node(None, [])]]
@@ -545,12 +567,12 @@ def decommit_positions(n_values, positions, fold_step):
 class FriProver:
     """FriProver::commit / decommit (prover/fri.rs:100-360)."""
 
-    def __init__(self, channel, cfg, quotient, L):
+    def __init__(self, channel, cfg, quotient, L, hash_node=hash_node):
         self.cfg = cfg
         self.first = quotient  # QM31 values on CanonicCoset(L), bit-reversed
         self.L = L
         cols = [[v[k] for v in quotient] for k in range(4)]
-        self.first_tree = MerkleTree(cols)
+        self.first_tree = MerkleTree(cols, hash_node)
         channel.mix_root(self.first_tree.root())
         last_log = cfg["log_last_layer"] + cfg["log_blowup"]
         alpha = channel.draw_secure_felt()
@@ -566,13 +588,13 @@ class FriProver:
 
         if L - 1 != last_log:
             while len(vals).bit_length() - 1 > last_log + step:
-                tree = MerkleTree([[v[k] for v in vals] for k in range(4)])
+                tree = MerkleTree([[v[k] for v in vals] for k in range(4)], hash_node)
                 self.inner.append((vals, tree, step))
                 channel.mix_root(tree.root())
                 alpha = channel.draw_secure_felt()
                 vals, xs = fold_n(vals, xs, alpha, step)
             k = len(vals).bit_length() - 1 - last_log
-            tree = MerkleTree([[v[kk] for v in vals] for kk in range(4)])
+            tree = MerkleTree([[v[kk] for v in vals] for kk in range(4)], hash_node)
             self.inner.append((vals, tree, k))
             channel.mix_root(tree.root())
             alpha = channel.draw_secure_felt()
@@ -876,7 +898,8 @@ class Writer:
         self.b += v.to_bytes(8, "little")
 
     def hash(self, h):
-        self.b += h
+        # Blake2s: 32 bytes; felt252: its 32 little-endian bytes
+        self.b += h.to_bytes(32, "little") if isinstance(h, int) else h
 
     def qm(self, v):
         for c in v:
@@ -900,22 +923,23 @@ class Writer:
 
 
 # ---------------------------------------------------------------- prove ---------------------------
-def prove(air, cfg):
+def prove(air, cfg, mc=Blake2sMerkleChannel):
     """prove() over one component with an empty preprocessed tree, as the reference examples build
     it (crates/examples/src/wide_fibonacci/mod.rs:186-261, poseidon/mod.rs:337-419), with
     lifting_log_size None. cfg: pow_bits, log_blowup, log_last_layer, n_queries, fold_step.
+    mc: the MerkleChannel (Blake2sMerkleChannel or Poseidon252MerkleChannel).
     Returns (proof bytes, final channel digest, composition matches at the OODS point)."""
     blow = cfg["log_blowup"]
     n = air.log_n
-    ch = Channel()
-    t0 = MerkleTree([])
+    ch = mc.Channel()
+    t0 = MerkleTree([], mc.hash_node)
     ch.mix_root(t0.root())
     trees = [(t0, [], [])]  # (tree, evaluations, coefficients)
 
     def commit(cols, log):
         coeffs = [interpolate(c, log) for c in cols]
         evals = [evaluate(c, log + blow) for c in coeffs]
-        t = MerkleTree(evals)
+        t = MerkleTree(evals, mc.hash_node)
         ch.mix_root(t.root())
         trees.append((t, evals, coeffs))
 
@@ -943,7 +967,7 @@ def prove(air, cfg):
     half = 1 << (E - 1)
     comp_polys = [c[:half] for c in comp_coeffs] + [c[half:] for c in comp_coeffs]
     commit_cols = [evaluate(c, E - 1 + blow) for c in comp_polys]
-    t_comp = MerkleTree(commit_cols)
+    t_comp = MerkleTree(commit_cols, mc.hash_node)
     ch.mix_root(t_comp.root())
     trees.append((t_comp, commit_cols, comp_polys))
 
@@ -1025,8 +1049,8 @@ def prove(air, cfg):
             acc = q_add(acc, q_mul(num, (di[0], di[1], 0, 0)))
         quot.append(acc)
 
-    fri = FriProver(ch, cfg, quot, lifting)
-    nonce = grind(ch, cfg["pow_bits"])
+    fri = FriProver(ch, cfg, quot, lifting, mc.hash_node)
+    nonce = mc.grind(ch, cfg["pow_bits"])
     ch.mix_u64(nonce)
     queries, fri_layers = fri.decommit(ch)
 

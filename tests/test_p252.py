@@ -40,6 +40,20 @@ def test_channel_draw_counts_and_uniqueness():
     assert c.n_draws == 1 and a != c.draw_u32s()
     d = [c.draw_secure_felt() for _ in range(5)]
     assert len(set(d)) == 5 and all(all(x < (1 << 31) - 1 for x in q) for q in d)
+    # channel/poseidon252.rs:169-180: draw_u32s, then 9 secure felts from 5 draws of 8 base felts
+    c = Q.Channel()
+    c.draw_u32s()
+    nine = c.draw_secure_felts(9)
+    assert c.n_draws == 6
+    # the flattened stream: the first draw's 8 base felts are the first two secure felts
+    c2 = Q.Channel()
+    c2.draw_u32s()
+    b = c2.draw_base_felts()
+    assert nine[0] == tuple(b[:4]) and nine[1] == tuple(b[4:])
+    # channel/poseidon252.rs:203-214: 5 then 4 felts, all distinct
+    c = Q.Channel()
+    f = c.draw_secure_felts(5) + c.draw_secure_felts(4)
+    assert len(set(f)) == 9
 
 
 def test_planted_constant_fails_the_vectors():
