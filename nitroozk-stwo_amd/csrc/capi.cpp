@@ -2,6 +2,7 @@
 #include "../../include/stwo_hip.h"
 
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -598,7 +599,11 @@ int stwo_hip_commit_on_layer_poseidon252(stwo_hip_ctx* h, uint32_t log_size, con
     std::vector<const uint32_t*> v(cols, cols + n_cols);
     const uint32_t* const* d = n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
     {
+      // permutations per node: one for two children alone, else one per absorbed pair plus the last
+      const double absorbed = (prev ? 2.0 : 0.0) + (double)((n_cols + 7) / 8);
+      const double perms = prev && n_cols == 0 ? 1.0 : std::floor(absorbed / 2) + 1;
       KScope ks(c, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * n_cols + 32.0) * (double)(1u << log_size));
+      ks.units(perms * (double)(1u << log_size));
       launch_p252_layer(log_size, prev, d, n_cols, out, c.stream);
     }
     c.sync();
@@ -611,6 +616,7 @@ int stwo_hip_grind_poseidon252(stwo_hip_ctx* h, const uint32_t digest[8], uint32
     if (!p252_canonical(digest)) throw HipError(1, "digest is not a canonical felt252");
     if (pow_bits > 64) throw HipError(3, "pow_bits above 64");
     DevBuf best(&c, 2);
+    KScope ks(c, K_GRIND, 0);
     *nonce = p252_grind(digest, pow_bits, reinterpret_cast<unsigned long long*>(best.p), c.stream);
   });
 }

@@ -3,10 +3,12 @@
 // crates/stwo/src/core/vcs/poseidon252_merkle.rs; crates/stwo/src/core/channel/poseidon252.rs),
 // which calls starknet-crypto's poseidon_hash, poseidon_hash_many and poseidon_permute_comp.
 //
-// An element is eight little-endian 32-bit limbs. Arithmetic is in Montgomery form (R = 2^256):
-// p = 1 (mod 2^32), so the reduction factor of each limb step is -t0 and the products with p's
-// limbs are shifts of the three nonzero ones (1, 17 at limb 6, 2^27 at limb 7). Values enter and
-// leave memory in canonical (normal) form.
+// An element enters and leaves as eight little-endian 32-bit limbs (F), canonical. Inside the
+// permutation it is in Montgomery form (M): nine limbs of 29 bits in 32-bit registers, R = 2^261.
+// Radix 2^29 leaves room in each limb: a product column (at most 9 products of limbs below 2^30)
+// accumulates in one 64-bit register with no carry handling, and sums are limb-wise additions whose
+// carries are propagated once. p = 1 (mod 2^29), so the Montgomery factor of each column is minus
+// its low limb, and p's limbs are zero except limb 0 (1), limb 6 (17 2^18) and limb 8 (2^19).
 //
 // The permutation (state of 3): 4 full rounds (add 3 constants, cube all three), 83 partial rounds
 // (add one constant to the last element, cube it), 4 full rounds; each round ends with the MDS mix
@@ -24,7 +26,6 @@ struct F {
 };
 
 constexpr uint32_t PL[8] = {1u, 0u, 0u, 0u, 0u, 0u, 17u, 0x08000000u};
-constexpr F R2 = {{0x7e000401u, 0xfffffd73u, 0x330fffffu, 0x1u, 0xff6f8000u, 0xffffffffu, 0x5e008810u, 0x7ffd4abu}};
 
 SHD constexpr F zero() { return F{{0, 0, 0, 0, 0, 0, 0, 0}}; }
 SHD constexpr F from_u64(uint64_t v) { return F{{(uint32_t)v, (uint32_t)(v >> 32), 0, 0, 0, 0, 0, 0}}; }
@@ -83,43 +84,12 @@ SHD constexpr F sub(const F& a, const F& b) {
   }
   return r;
 }
-// a b R^-1 mod p (coarsely integrated Montgomery product over 32-bit limbs); a, b < p
-SHD constexpr F mont_mul(const F& a, const F& b) {
-  uint32_t t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < 8; ++i) {
-    uint64_t c = 0;
-    for (int j = 0; j < 8; ++j) {
-      const uint64_t s = (uint64_t)t[j] + (uint64_t)a.w[j] * b.w[i] + c;
-      t[j] = (uint32_t)s;
-      c = s >> 32;
-    }
-    uint64_t s = (uint64_t)t[8] + c;
-    t[8] = (uint32_t)s;
-    t[9] = (uint32_t)(s >> 32);
-    const uint32_t m = 0u - t[0];  // t0 (-p^-1) mod 2^32, with p = 1 (mod 2^32)
-    c = 0;
-    for (int j = 0; j < 8; ++j) {
-      const uint64_t u = (uint64_t)t[j] + (uint64_t)m * PL[j] + c;
-      t[j] = (uint32_t)u;
-      c = u >> 32;
-    }
-    s = (uint64_t)t[8] + c;
-    t[8] = (uint32_t)s;
-    t[9] += (uint32_t)(s >> 32);
-    for (int j = 0; j < 9; ++j) t[j] = t[j + 1];
-    t[9] = 0;
-  }
-  F r = zero();
-  for (int j = 0; j < 8; ++j) r.w[j] = t[j];
-  return geq_p(r) ? sub_p(r) : r;  // below 2p, so t[8] is zero here
+// 2^n mod p (canonical doublings; compile time)
+constexpr F pow2_mod_p(int n) {
+  F x = from_u64(1);
+  for (int i = 0; i < n; ++i) x = add(x, x);
+  return x;
 }
-SHD constexpr F to_mont(const F& a) { return mont_mul(a, R2); }
-SHD constexpr F from_mont(const F& a) { return mont_mul(a, F{{1, 0, 0, 0, 0, 0, 0, 0}}); }
-SHD constexpr F cube(const F& x) { return mont_mul(mont_mul(x, x), x); }
-
-constexpr F ONE = to_mont(F{{1, 0, 0, 0, 0, 0, 0, 0}});
-constexpr F TWO = to_mont(F{{2, 0, 0, 0, 0, 0, 0, 0}});
-constexpr F THREE = to_mont(F{{3, 0, 0, 0, 0, 0, 0, 0}});
 
 constexpr F RC_NORMAL[107] = {
     {{0xac77fe6fu, 0x80b7fd8eu, 0x8109ae47u, 0xe58e2ad9u, 0x562a30b9u, 0x39dd92f9u, 0xea556a23u, 0x06861759u}},
@@ -230,8 +200,81 @@ constexpr F RC_NORMAL[107] = {
     {{0xdc5717a5u, 0xbf1d9ac2u, 0xabd8af54u, 0x7606836eu, 0xa13b7f2cu, 0xf180b1a8u, 0x92db5099u, 0x051274d0u}},
     {{0xbfff0c5bu, 0x861a8db0u, 0x0d9cdbedu, 0xf415e14fu, 0xaa4ca528u, 0x7ad0fb7au, 0xb8eb75e1u, 0x061fc552u}},
 };
+// ---- Montgomery form in radix 2^29 -----------------------------------------------------------
+constexpr uint32_t MASK29 = (1u << 29) - 1;
+struct M {
+  uint32_t w[9];
+};
+
+// F (any value below 2^256) -> nine 29-bit limbs
+SHD constexpr M to_radix29(const F& a) {
+  M r{};
+  for (int i = 0; i < 9; ++i) {
+    const int bit = 29 * i, w = bit / 32, sh = bit % 32;
+    uint64_t v = a.w[w];
+    if (w + 1 < 8) v |= (uint64_t)a.w[w + 1] << 32;
+    r.w[i] = (uint32_t)(v >> sh) & MASK29;
+  }
+  return r;
+}
+// nine 29-bit limbs of a value below 2^256 -> F
+SHD constexpr F from_radix29(const M& a) {
+  F r = zero();
+  for (int i = 0; i < 9; ++i) {
+    const int bit = 29 * i, w = bit / 32, sh = bit % 32;
+    const uint64_t v = (uint64_t)a.w[i] << sh;
+    r.w[w] |= (uint32_t)v;
+    if (w + 1 < 8) r.w[w + 1] |= (uint32_t)(v >> 32);
+  }
+  return r;
+}
+
+// a b 2^-261 (mod p) by product scanning: column k gathers a_i b_j (i + j = k) and the reduction
+// terms m_i p_j in one 64-bit accumulator; for k < 9 the factor m_k = -acc (mod 2^29) clears the
+// column (its carry out is acc >> 29, plus one when the low limb was not zero), above that the
+// columns are the result's limbs. Inputs: limbs below 2^30 (a column then stays below 2^64);
+// output: limbs 0-7 below 2^29, value below a b / 2^261 + p (below 2 p for inputs below 32 p).
+SHD constexpr M mul29(const M& a, const M& b) {
+  uint32_t m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  M r{};
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 17; ++k) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const int j = k - i;
+      if (j >= 0 && j < 9) acc += (uint64_t)a.w[i] * b.w[j];
+    }
+    if (k >= 6 && k - 6 < 9) acc += (uint64_t)m[k - 6] * (17u << 18);
+    if (k >= 8 && k - 8 < 9) acc += (uint64_t)m[k - 8] << 19;
+    if (k < 9) {
+      const uint32_t lo = (uint32_t)acc & MASK29;
+      m[k] = (0u - lo) & MASK29;
+      acc = (acc >> 29) + (lo != 0u ? 1u : 0u);
+    } else {
+      r.w[k - 9] = (uint32_t)acc & MASK29;
+      acc >>= 29;
+    }
+  }
+  r.w[8] = (uint32_t)acc;
+  return r;
+}
+
+constexpr M R2M = to_radix29(pow2_mod_p(522));  // R^2 mod p
+// canonical F -> Montgomery form (below 1.04 p)
+SHD constexpr M to_mont(const F& a) { return mul29(to_radix29(a), R2M); }
+// Montgomery form (limbs below 2^29, value below 2^261) -> canonical F
+SHD constexpr F from_mont(const M& a) {
+  const F r = from_radix29(mul29(a, M{{1, 0, 0, 0, 0, 0, 0, 0, 0}}));  // at most p
+  return geq_p(r) ? sub_p(r) : r;
+}
+
+constexpr M ONE = to_mont(from_u64(1));
+constexpr M TWO = to_mont(from_u64(2));
+constexpr M THREE = to_mont(from_u64(3));
+
 struct RoundConstants {
-  F c[107];
+  M c[107];
 };
 constexpr RoundConstants make_round_constants() {
   RoundConstants t{};
@@ -240,61 +283,131 @@ constexpr RoundConstants make_round_constants() {
 }
 constexpr RoundConstants RC = make_round_constants();
 
-SHD void mix(F* s) {
-  const F t = add(add(s[0], s[1]), s[2]);
-  const F a = add(t, add(s[0], s[0]));
-  const F b = sub(t, add(s[1], s[1]));
-  const F c = sub(t, add(add(s[2], s[2]), s[2]));
-  s[0] = a;
-  s[1] = b;
-  s[2] = c;
+// ---- the permutation's arithmetic: reduced lazily, no branches ----------------------------------
+// limb-wise sum
+SHD M addl(const M& a, const M& b) {
+  M r;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) r.w[i] = a.w[i] + b.w[i];
+  return r;
 }
-SHD void full_round(F* s, int k) {
-  for (int i = 0; i < 3; ++i) s[i] = cube(add(s[i], RC.c[k + i]));
+// carries of limbs below 2^32 - 8: limbs 0-7 below 2^29 after it
+SHD M carry(const M& x) {
+  M r;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t v = x.w[i] + c;
+    r.w[i] = v & MASK29;
+    c = v >> 29;
+  }
+  r.w[8] = x.w[8] + c;
+  return r;
+}
+// x - (q - 1) p for q = floor(x / 2^251) >= 1 (x itself for q = 0): congruent to x and below
+// 2^251 + 2^201 < 2 p, for x with normalized limbs and a value below 2^256. (q - 1) p touches limbs
+// 0, 6 and 8 only; the borrows are -1 or 0.
+SHD M reduce_top(const M& x) {
+  const uint32_t q = x.w[8] >> 19;
+  const uint32_t k = q - (q != 0u ? 1u : 0u);
+  const uint32_t sub[9] = {k, 0u, 0u, 0u, 0u, 0u, (17u << 18) * k, 0u, k << 19};
+  M r;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int32_t v = (int32_t)x.w[i] - (int32_t)sub[i] + c;
+    r.w[i] = (uint32_t)v & MASK29;
+    c = v >> 29;  // arithmetic: -1 or 0
+  }
+  r.w[8] = (uint32_t)((int32_t)x.w[8] - (int32_t)sub[8] + c);
+  return r;
+}
+// The value v plus a multiple of 2^29 moved down from each limb to the one below: the same value
+// with limbs 0-7 at least D (2^29 - 1), so that subtracting D times a normalized element limb by
+// limb leaves every limb non-negative.
+constexpr M borrowed(const M& v, uint32_t D) {
+  M r = v;
+  for (int i = 0; i < 8; ++i) {
+    r.w[i] += D << 29;
+    r.w[i + 1] -= D;
+  }
+  return r;
+}
+constexpr M from_small_multiple_of_p(uint32_t k) {  // k p in radix 2^29, k < 32
+  return M{{k, 0u, 0u, 0u, 0u, 0u, (17u << 18) * k, 0u, k << 19}};
+}
+// 5 p and 7 p, borrowed for subtracting 2 s1 and 3 s2 (their top limbs 5 2^19 and 7 2^19 exceed
+// 2 and 3 times the top limb of elements below 2.01 p, plus the borrow)
+constexpr M K5P = borrowed(from_small_multiple_of_p(5), 2);
+constexpr M K7P = borrowed(from_small_multiple_of_p(7), 3);
+
+// t = s0 + s1 + s2 -> (t + 2 s0, t - 2 s1, t - 3 s2) for normalized inputs below 2.01 p. The
+// unnormalized outputs stay below 9.1 p, 10.1 p and 12.1 p (limbs below 7 2^29); the first two are
+// brought below 2.01 p, the third (which is cubed next) only normalized.
+SHD void mix(M* s) {
+  M a, b, c;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint32_t t = s[0].w[i] + s[1].w[i] + s[2].w[i];
+    a.w[i] = t + 2u * s[0].w[i];
+    b.w[i] = t + K5P.w[i] - 2u * s[1].w[i];
+    c.w[i] = t + K7P.w[i] - 3u * s[2].w[i];
+  }
+  s[0] = reduce_top(carry(a));
+  s[1] = reduce_top(carry(b));
+  s[2] = carry(c);
+}
+SHD M cube(const M& x) { return mul29(mul29(x, x), x); }
+// each element plus its constant (limbs below 2^30, value below 14 p) cubed: below 1.2 p
+SHD void full_round(M* s, int k) {
+  for (int i = 0; i < 3; ++i) s[i] = cube(addl(s[i], RC.c[k + i]));
   mix(s);
 }
-// poseidon_permute_comp on a Montgomery-form state (one copy of the code per program: the device
-// calls it, it is not inlined)
-__host__ __device__ __attribute__((noinline)) inline void permute(F* s) {
+// poseidon_permute_comp on a Montgomery-form state: elements with limbs below 2^30 and values
+// below 3.1 p (Montgomery forms of canonical values, or a permutation's outputs plus an absorbed
+// value). Outputs: s0 and s1 below 2.01 p, s2 below 12.1 p, all normalized (from_mont reduces
+// fully). One copy of the code per program: the device calls it, it is not inlined.
+__host__ __device__ __attribute__((noinline)) inline void permute(M* s) {
+  for (int i = 0; i < 3; ++i) s[i] = carry(s[i]);
   int k = 0;
   for (int r = 0; r < 4; ++r, k += 3) full_round(s, k);
 #ifdef __HIP_DEVICE_COMPILE__
 #pragma unroll 1
 #endif
   for (int r = 0; r < 83; ++r, ++k) {
-    s[2] = cube(add(s[2], RC.c[k]));
+    s[2] = cube(addl(s[2], RC.c[k]));
     mix(s);
   }
   for (int r = 0; r < 4; ++r, k += 3) full_round(s, k);
 }
 // poseidon_hash(x, y) (Montgomery form in and out)
-SHD F hash2(const F& x, const F& y) {
-  F s[3] = {x, y, TWO};
+SHD M hash2(const M& x, const M& y) {
+  M s[3] = {x, y, TWO};
   permute(s);
   return s[0];
 }
-// poseidon_hash_many, absorbing one value at a time (Montgomery form)
+// poseidon_hash_many, absorbing one value at a time (Montgomery form; absorbed values below 1.1 p)
 struct Sponge {
-  F s[3] = {zero(), zero(), zero()};
-  F pending = zero();
+  M s[3] = {M{}, M{}, M{}};
+  M pending = M{};
   bool odd = false;
-  SHD void absorb(const F& v) {
+  SHD void absorb(const M& v) {
     if (!odd) {
       pending = v;
       odd = true;
       return;
     }
-    s[0] = add(s[0], pending);
-    s[1] = add(s[1], v);
+    s[0] = addl(s[0], pending);
+    s[1] = addl(s[1], v);
     permute(s);
     odd = false;
   }
-  SHD F finish() {
+  SHD M finish() {
     if (odd) {
-      s[0] = add(s[0], pending);
-      s[1] = add(s[1], ONE);
+      s[0] = addl(s[0], pending);
+      s[1] = addl(s[1], ONE);
     } else {
-      s[0] = add(s[0], ONE);
+      s[0] = addl(s[0], ONE);
     }
     permute(s);
     return s[0];

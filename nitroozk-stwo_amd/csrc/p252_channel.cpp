@@ -11,6 +11,7 @@ namespace sh {
 namespace {
 
 using f252::F;
+using f252::M;
 
 F felt_of(const Hash& h) {
   F f;
@@ -25,9 +26,9 @@ Hash hash_of(const F& f) {
     for (int j = 0; j < 4; ++j) h[4 * i + j] = (uint8_t)(f.w[i] >> (8 * j));
   return h;
 }
-F mont(const Hash& h) { return f252::to_mont(felt_of(h)); }
-Hash normal(const F& m) { return hash_of(f252::from_mont(m)); }
-F mont_u64(uint64_t v) { return f252::to_mont(f252::from_u64(v)); }
+M mont(const Hash& h) { return f252::to_mont(felt_of(h)); }
+Hash normal(const M& m) { return hash_of(f252::from_mont(m)); }
+M mont_u64(uint64_t v) { return f252::to_mont(f252::from_u64(v)); }
 
 // f << 31 | x for f below 2^221 and x below 2^31 (no reduction: the result stays below 2^252)
 void shl31_or(F& f, uint32_t x) {
@@ -84,7 +85,7 @@ void P252Channel::mix_felts(const std::vector<qm31>& f) {
 }
 
 Hash P252Channel::draw_felt252() {
-  F s[3] = {mont(digest), mont_u64(n_draws), f252::THREE};
+  M s[3] = {mont(digest), mont_u64(n_draws), f252::THREE};
   f252::permute(s);
   ++n_draws;
   return normal(s[0]);

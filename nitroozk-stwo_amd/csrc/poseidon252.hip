@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) p252_layer_kernel(uint32_t n, const uint3
                                                          uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  F h;
+  M h;
   if (prev && n_cols == 0) {
     h = hash2(to_mont(load_felt(prev + 16 * (size_t)i)), to_mont(load_felt(prev + 16 * (size_t)i + 8)));
   } else {
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) p252_layer_kernel(uint32_t n, const uint3
 // trailing zero bits (of its low 128 bits, as u128::trailing_zeros of the big-endian bytes 16..32)
 // records the nonce with an atomic minimum, which is the search order's first.
 constexpr uint32_t P252_GRIND_LOW_BITS = 14;
-__global__ void __launch_bounds__(256) p252_grind_kernel(F prefixed, uint32_t pow_bits, uint64_t hi0,
+__global__ void __launch_bounds__(256) p252_grind_kernel(M prefixed, uint32_t pow_bits, uint64_t hi0,
                                                          unsigned long long* __restrict__ best) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nonce = ((hi0 + (j >> P252_GRIND_LOW_BITS)) << 32) | (j & ((1u << P252_GRIND_LOW_BITS) - 1));
@@ -81,7 +81,7 @@ void launch_p252_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* 
   hipLaunchKernelGGL(p252_layer_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, prev, cols, n_cols, out);
 }
 
-F p252_pow_prefix(const uint32_t digest[8], uint32_t pow_bits) {
+M p252_pow_prefix(const uint32_t digest[8], uint32_t pow_bits) {
   // poseidon_hash_many([POW_PREFIX, digest, n_bits]) (Montgomery form)
   Sponge sp;
   F d;
@@ -94,7 +94,7 @@ F p252_pow_prefix(const uint32_t digest[8], uint32_t pow_bits) {
 
 uint64_t p252_grind(const uint32_t digest[8], uint32_t pow_bits, unsigned long long* d_best, hipStream_t stream) {
   if (pow_bits > 64) throw std::invalid_argument("p252_grind: pow_bits above 64");
-  const F pre = p252_pow_prefix(digest, pow_bits);
+  const M pre = p252_pow_prefix(digest, pow_bits);
   uint64_t hi0 = 0, count = 1u << 16;  // candidates per launch: whole chunks of 2^14
   for (;;) {
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, stream, d_best, ~0ull);
@@ -122,10 +122,11 @@ void p252_hash_many_host(const uint32_t* values, size_t n, uint32_t out[8]) {
 }
 
 void p252_permute_host(uint32_t state[24]) {
-  F s[3];
+  M s[3];
   for (int j = 0; j < 3; ++j) {
-    for (int i = 0; i < 8; ++i) s[j].w[i] = state[8 * j + i];
-    s[j] = to_mont(s[j]);
+    F v;
+    for (int i = 0; i < 8; ++i) v.w[i] = state[8 * j + i];
+    s[j] = to_mont(v);
   }
   permute(s);
   for (int j = 0; j < 3; ++j) {
