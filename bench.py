@@ -481,6 +481,67 @@ def poseidon_extras(args, hb, m, cfg, trace, log_rows):
     return out
 
 
+def poseidon252_extras(args, hb, cfg, trace, log_rows):
+    """The workload over the Poseidon252 Merkle channel (the per-call prover: every tree committed
+    by stwo_hip_commit_on_layer_poseidon252, the felt252 channel on the host, the proof of work by
+    stwo_hip_grind_poseidon252), and the two shapes the README times with that channel: Poseidon2
+    at 1 instance per row and wide-Fibonacci 2^18 x 100. Each region reports ms per proof and the
+    Merkle layers' permutations per second against the measured permutation ceiling."""
+    n = max(3, min(args.steps, 10))
+    ceiling = hb.measure_permutation_rate()
+    pct = lambda v, q: round(sorted(v)[max(0, int(round(q * len(v))) - 1)], 3)
+
+    def region(prove_one):
+        prove_one()  # warm-up
+        device_sync(hb)
+        times = []
+        t0 = time.perf_counter()
+        for _ in range(n):
+            s0 = time.perf_counter()
+            pr = prove_one()
+            times.append((time.perf_counter() - s0) * 1e3)
+        wall = (time.perf_counter() - t0) * 1e3
+        hb.reset_kernel_stats()
+        hb.set_kernel_timing(True)
+        prove_one()
+        hb.set_kernel_timing(False)
+        st, un = hb.kernel_stats(), hb.kernel_units()
+        mk_ms, perms = st.get("merkle", (0.0,))[0], un.get("merkle", 0.0)
+        rate = perms / (mk_ms / 1e3) if mk_ms > 0 else 0.0
+        return {"proofs": n, "ms_per_proof": round(wall / n, 3), "proofs_per_s": round(n * 1e3 / wall, 2),
+                "p50_ms": pct(times, 0.5), "p99_ms": pct(times, 0.99), "proof_status": pr.status,
+                "proof_bytes": len(pr.bytes), "merkle_ms_per_proof": round(mk_ms, 3),
+                "kernel_ms_per_proof": round(sum(v[0] for v in st.values()), 3),
+                "permutations_per_proof": int(perms), "M_permutations_per_s": round(rate / 1e6, 1),
+                "permutation_ceiling_frac": round(rate / ceiling, 4) if ceiling else None}
+
+    out = {"M_permutations_per_s_ceiling": round(ceiling / 1e6, 1),
+           "ceiling_source": "Starknet Poseidon permutations on register-resident states, measured live "
+                             "(stwo_hip_measure_permutation_rate)",
+           "note": "per-call form, trace resident (trace generation excluded); the README's GPU column "
+                   "includes trace generation and twiddles, on the reference's hardware"}
+    if args.workload == "poseidon":
+        out["workload"] = region(lambda: hb.prove_poseidon_per_call(trace, args.ipr, log_rows, cfg,
+                                                                     allow_unsatisfied=True,
+                                                                     merkle_channel="poseidon252"))
+        rows1 = args.log_instances
+        tr1 = hb.gen_trace_poseidon(1, rows1, 1, args.seed)
+        out["poseidon2_one_per_row"] = dict(region(lambda: hb.prove_poseidon_per_call(
+            tr1, 1, rows1, cfg, allow_unsatisfied=True, merkle_channel="poseidon252")), log_n_rows=rows1,
+            reference_readme_gpu_ms={"log": 18, "ms": 348, "source": "README.md:201-220"} if rows1 == 18 else None)
+        hb.free_columns(tr1)
+    else:
+        out["workload"] = region(lambda: hb.prove_wide_fibonacci_per_call([trace], [log_rows], cfg,
+                                                                           merkle_channel="poseidon252"))
+    wcfg = type(cfg).make(5, 1, 0, 3, 1, 0)  # the wide-Fibonacci configuration (5 PoW bits)
+    trw = hb.gen_trace_wide_fibonacci(100, 18, 1, args.seed)
+    out["wide_fib_2e18x100"] = dict(region(lambda: hb.prove_wide_fibonacci_per_call(
+        [trw], [18], wcfg, merkle_channel="poseidon252")), pow_bits=5,
+        reference_readme_gpu_ms={"log": 18, "ms": 133, "source": "README.md:155-176"})
+    hb.free_columns(trw)
+    return {"poseidon252_channel": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -607,6 +668,8 @@ def main():
         extras.update(proof_forms(args, hb, m, cfg, trace, log_rows, times))
     if args.workload == "poseidon" and not args.no_extras:
         extras.update(poseidon_extras(args, hb, m, cfg, trace, log_rows))
+    if args.workload in ("poseidon", "wide_fib") and not args.no_extras:
+        extras.update(poseidon252_extras(args, hb, cfg, trace, log_rows))
 
     cpu = None
     parity = None

@@ -1,6 +1,6 @@
 /*
  * libstwo_hip: MI355X (gfx950) prover backend for Circle STARKs over M31 with the Blake2s
- * Merkle channel. C ABI with plain pointers and sizes: the boundary a reference-side backend
+ * Merkle channel and the Poseidon252 Merkle channel. C ABI with plain pointers and sizes: the boundary a reference-side backend
  * binds, as CudaBackend binds libstwo_cuda through crates/stwo/src/stwo_cuda/bindings.rs.
  *
  * Conventions
@@ -377,10 +377,16 @@ int stwo_hip_measure_int32_issue(stwo_hip_ctx* ctx, double* lane_instr_per_s);
  * the arithmetic ceiling of the interpolate / evaluate kernels, against which their butterflies per
  * second (stwo_hip_kernel_units over the class's event time) are reported. */
 int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* ctx, int inverse, double* butterflies_per_s);
+/* No reference counterpart. Starknet Poseidon permutations (poseidon_permute_comp) per second on
+ * register-resident states with the library's own permutation code: the arithmetic ceiling of
+ * stwo_hip_commit_on_layer_poseidon252, whose permutations per second (stwo_hip_kernel_units of the
+ * merkle class over its event time) are reported against it. */
+int stwo_hip_measure_permutation_rate(stwo_hip_ctx* ctx, double* permutations_per_s);
 int stwo_hip_kernel_stats(stwo_hip_ctx* ctx, uint32_t kernel_class, double* ms, double* bytes, uint64_t* launches);
 /* No reference counterpart. Algorithmic work units of a kernel class since the last reset: circle-FFT
  * butterflies for interpolate (n_cols 2^(log-1) log) and evaluate (n_cols 2^(log-1) k, k the
- * coefficients' log size); 0 for the other classes. */
+ * coefficients' log size); Poseidon permutations for merkle launches of
+ * stwo_hip_commit_on_layer_poseidon252; 0 for the other classes. */
 int stwo_hip_kernel_units(stwo_hip_ctx* ctx, uint32_t kernel_class, double* units);
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* ctx);
 

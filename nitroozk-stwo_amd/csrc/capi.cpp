@@ -1146,6 +1146,17 @@ int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* h, int inverse, double* butter
   });
 }
 
+int stwo_hip_measure_permutation_rate(stwo_hip_ctx* h, double* permutations_per_s) {
+  return guarded(h, [&](Ctx& c) {
+    if (!permutations_per_s) throw HipError(1, "null out");
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    DevBuf scratch(&c, (size_t)cus * 16);
+    c.sync();
+    *permutations_per_s = measure_permutation_rate(scratch.p, c.stream);
+  });
+}
+
 int stwo_hip_set_fft_low_log(stwo_hip_ctx* h, uint32_t low_log) {
   return guarded(h, [&](Ctx& c) {
     const uint32_t lg = low_log & ~FFT_LOW_LOOP;

@@ -230,7 +230,7 @@ SHD constexpr F from_radix29(const M& a) {
 }
 
 // a b 2^-261 (mod p) by product scanning: column k gathers a_i b_j (i + j = k) and the reduction
-// terms m_i p_j in one 64-bit accumulator; for k < 9 the factor m_k = -acc (mod 2^29) clears the
+// terms m_i p_j in 64-bit accumulators; for k < 9 the factor m_k = -acc (mod 2^29) clears the
 // column (its carry out is acc >> 29, plus one when the low limb was not zero), above that the
 // columns are the result's limbs. Inputs: limbs below 2^30 (a column then stays below 2^64);
 // output: limbs 0-7 below 2^29, value below a b / 2^261 + p (below 2 p for inputs below 32 p).
@@ -240,13 +240,22 @@ SHD constexpr M mul29(const M& a, const M& b) {
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < 17; ++k) {
+    // two partial sums per column (products of even and odd i): independent chains of multiply-adds
+    // that do not wait for the previous column's carry
+    uint64_t e = 0, o = 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       const int j = k - i;
-      if (j >= 0 && j < 9) acc += (uint64_t)a.w[i] * b.w[j];
+      if (j >= 0 && j < 9) {
+        if (i & 1)
+          o += (uint64_t)a.w[i] * b.w[j];
+        else
+          e += (uint64_t)a.w[i] * b.w[j];
+      }
     }
-    if (k >= 6 && k - 6 < 9) acc += (uint64_t)m[k - 6] * (17u << 18);
-    if (k >= 8 && k - 8 < 9) acc += (uint64_t)m[k - 8] << 19;
+    if (k >= 6 && k - 6 < 9) o += (uint64_t)m[k - 6] * (17u << 18);
+    if (k >= 8 && k - 8 < 9) e += (uint64_t)m[k - 8] << 19;
+    acc += e + o;
     if (k < 9) {
       const uint32_t lo = (uint32_t)acc & MASK29;
       m[k] = (0u - lo) & MASK29;

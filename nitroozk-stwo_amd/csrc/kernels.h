@@ -250,6 +250,9 @@ void launch_fft_tiny(bool inverse, const uint32_t* const* src, uint32_t* const* 
 double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
 // Forward (or inverse) FFT butterflies per second on register-resident data (butterfly.h).
 double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t stream);
+// Starknet Poseidon permutations per second on register-resident states (poseidon252.hip): the
+// ceiling of the Poseidon252 Merkle layers. scratch: CUs x 16 words.
+double measure_permutation_rate(uint32_t* scratch, hipStream_t stream);
 
 // poseidon252.hip: the Poseidon252 Merkle channel. Hashes are canonical felt252s, 8 little-endian
 // words each. prev: 2^(log_size+1) child hashes or null; cols: n_cols M31 columns of 2^log_size.

@@ -73,12 +73,43 @@ __global__ void __launch_bounds__(256) p252_grind_kernel(M prefixed, uint32_t po
 
 __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// The permutation-rate ceiling: every lane permutes a register-resident state `iters` times (no
+// memory traffic), with 16 workgroups of 256 lanes per CU, enough waves to fill every SIMD.
+__global__ void __launch_bounds__(256) p252_perm_rate_kernel(uint32_t iters, uint32_t seed, uint32_t* out) {
+  M s[3] = {ONE, TWO, THREE};
+  s[0].w[0] ^= (seed * (threadIdx.x + 1u)) & 0xFFFFu;  // distinct per lane; the limb stays below 2^29
+  for (uint32_t it = 0; it < iters; ++it) permute(s);
+  const uint32_t x = s[0].w[0] ^ s[1].w[1] ^ s[2].w[2];
+  if (x == 0x12345678u) out[blockIdx.x] = x;  // keeps the states live
+}
+
 }  // namespace
 
 void launch_p252_layer(uint32_t log_size, const uint32_t* prev, const uint32_t* const* cols, uint32_t n_cols,
                        uint32_t* out, hipStream_t stream) {
   const uint32_t n = 1u << log_size;
   hipLaunchKernelGGL(p252_layer_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, prev, cols, n_cols, out);
+}
+
+double measure_permutation_rate(uint32_t* scratch, hipStream_t stream) {
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const uint32_t blocks = (uint32_t)cus * 16, threads = 256, iters = 2;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(p252_perm_rate_kernel, dim3(blocks), dim3(threads), 0, stream, 1u, 3u, scratch);  // warm-up
+  (void)hipEventRecord(e0, stream);
+  hipLaunchKernelGGL(p252_perm_rate_kernel, dim3(blocks), dim3(threads), 0, stream, iters, 5u, scratch);
+  (void)hipEventRecord(e1, stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double perms = (double)blocks * threads * iters;
+  return ms > 0 ? perms / (ms * 1e-3) : 0.0;
 }
 
 M p252_pow_prefix(const uint32_t digest[8], uint32_t pow_bits) {

@@ -92,7 +92,7 @@ EXPORTS = [
     "stwo_hip_measure_butterfly_rate", "stwo_hip_kernel_units", "stwo_hip_fft_low_log", "stwo_hip_set_fft_low_log",
     "stwo_hip_commit_on_layer_poseidon252", "stwo_hip_grind_poseidon252", "stwo_poseidon252_hash_many",
     "stwo_poseidon252_permute", "stwo_hip_prove_wide_fibonacci_poseidon252", "stwo_hip_prove_poseidon_poseidon252",
-    "stwo_verify_wide_fibonacci_poseidon252", "stwo_verify_poseidon_poseidon252",
+    "stwo_verify_wide_fibonacci_poseidon252", "stwo_verify_poseidon_poseidon252", "stwo_hip_measure_permutation_rate",
 ]
 
 # The MerkleChannel of a proof: the suffix of its prove / verify entry points.
@@ -199,6 +199,7 @@ def load_library(path=LIB_PATH):
                                                     c.POINTER(QM31)]
     L.stwo_hip_measure_int32_issue.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
     L.stwo_hip_measure_butterfly_rate.argtypes = [c.c_void_p, c.c_int, c.POINTER(c.c_double)]
+    L.stwo_hip_measure_permutation_rate.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
     L.stwo_hip_kernel_units.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double)]
     L.stwo_hip_fft_low_log.argtypes = [c.c_void_p, c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, _U32P]
     L.stwo_hip_commit_on_layer_poseidon252.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32,
@@ -627,6 +628,13 @@ class HipBackend:
         ceiling of the transform kernels' arithmetic on this device."""
         v = ctypes.c_double()
         self._check(self.lib.stwo_hip_measure_butterfly_rate(self.ctx, int(bool(inverse)), ctypes.byref(v)))
+        return v.value
+
+    def measure_permutation_rate(self):
+        """Starknet Poseidon permutations per second with register-resident states: the ceiling of
+        the Poseidon252 Merkle layers on this device."""
+        v = ctypes.c_double()
+        self._check(self.lib.stwo_hip_measure_permutation_rate(self.ctx, ctypes.byref(v)))
         return v.value
 
     def reset_kernel_stats(self):

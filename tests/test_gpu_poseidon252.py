@@ -94,3 +94,21 @@ def test_grind_beyond_one_chunk(hb):
     # spot-check the order: the restatement agrees on the chunk, and no nonce of it below lo passes
     if hi == 0:
         assert all(not c.verify_pow_nonce(16, x) for x in range(max(0, lo - 64), lo))
+
+
+def test_permutation_rate_and_merkle_units(hb):
+    """The permutation ceiling is plausible (one permutation is about 60 thousand VALU instructions,
+    the device issues tens of trillions per second), and the merkle class counts one permutation per
+    two absorbed felt252s plus one, or one for children alone."""
+    rate = hb.measure_permutation_rate()
+    say(f"permutations per second: {rate:.3e}")
+    assert 5e7 < rate < 5e9
+    hb.reset_kernel_stats()
+    hb.set_kernel_timing(True)
+    rng = np.random.default_rng(3)
+    cols = [hb.from_cpu(rng.integers(0, M31, 1 << 6, dtype=np.uint32)) for _ in range(19)]
+    leaves = hb.commit_on_layer_poseidon252(6, None, cols)  # 3 felts: 2 permutations per node
+    mid = hb.commit_on_layer_poseidon252(5, leaves, cols[:5])  # 2 children + 1 felt: 2 per node
+    hb.commit_on_layer_poseidon252(4, mid, [])  # children only: 1 per node
+    hb.set_kernel_timing(False)
+    assert hb.kernel_units()["merkle"] == 2 * 64 + 2 * 32 + 16
