@@ -13,7 +13,9 @@
 // nothing here launches a kernel or touches the context's internals. It is the "plain per-call"
 // form of the prover: each call synchronises, columns are allocated per call, and the channel
 // waits on every root. The device-resident prover (prover.cpp) is the fused form of the same
-// protocol; both produce the same bytes.
+// protocol; both produce the same bytes. Like the reference's prove(), it is generic over the
+// MerkleChannel: Blake2s, or Poseidon252 (felt252 hashes through commit_on_layer_poseidon252 and
+// grind_poseidon252, the felt252 channel of p252_channel.cpp).
 #include "../../include/stwo_hip.h"
 
 #include <algorithm>
