@@ -32,11 +32,19 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {("poseidon2", 18, 8, "evaluate"): 1413.9e6}
 
 
-def roofline_entry(dom, by_per_step, ms_per_step, traffic, units_per_step, unit_peak):
+BUTTERFLIES = ("T butterflies/s", 1e12, "butterflies_per_step",
+               "M31 butterflies on register-resident data, measured live (stwo_hip_measure_butterfly_rate)")
+PERMUTATIONS = ("M permutations/s", 1e6, "permutations_per_step",
+                "Starknet Poseidon permutations on register-resident states, measured live "
+                "(stwo_hip_measure_permutation_rate)")
+
+
+def roofline_entry(dom, by_per_step, ms_per_step, traffic, units_per_step, unit_peak, kind=BUTTERFLIES):
     """The roofline object of a bench line for the dominant kernel class: its HBM figures, and for
     the transform classes its butterflies per second against the butterfly ceiling measured live on
     the same device (stwo_hip_measure_butterfly_rate: the transform kernels' own butterfly code on
-    register-resident values). `bound` names the resource the class is closer to saturating."""
+    register-resident values), for the Poseidon252 Merkle layers its permutations per second
+    against the permutation ceiling. `bound` names the resource the class is closer to saturating."""
     hbm_ach = by_per_step / (ms_per_step / 1e3) / 1e9
     hbm = {"achieved": round(hbm_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 4),
            "traffic": traffic, "traffic_unit": "HBM bytes per step over the class's launches" if traffic else None,
@@ -44,10 +52,9 @@ def roofline_entry(dom, by_per_step, ms_per_step, traffic, units_per_step, unit_
     valu = None
     if units_per_step and unit_peak:
         ach = units_per_step / (ms_per_step / 1e3)
-        valu = {"achieved": round(ach / 1e12, 3), "peak": round(unit_peak / 1e12, 3), "unit": "T butterflies/s",
-                "frac": round(ach / unit_peak, 4), "butterflies_per_step": units_per_step,
-                "peak_source": "M31 butterflies on register-resident data, measured live "
-                               "(stwo_hip_measure_butterfly_rate)"}
+        unit, scale, key, source = kind
+        valu = {"achieved": round(ach / scale, 3), "peak": round(unit_peak / scale, 3), "unit": unit,
+                "frac": round(ach / unit_peak, 4), key: units_per_step, "peak_source": source}
     if valu and valu["frac"] > hbm["frac"]:
         r = {"bound": "valu", "kernel": dom}
         r.update({k: valu[k] for k in ("achieved", "peak", "unit", "frac")})
@@ -567,6 +574,9 @@ def main():
                     help="skip the plain per-call and batched (multi-context) regions")
     ap.add_argument("--no-extras", action="store_true",
                     help="Poseidon2: skip the end-to-end (trace generation included) and 1-per-row regions")
+    ap.add_argument("--merkle-channel", choices=["blake2s", "poseidon252"], default="blake2s",
+                    help="poseidon / wide_fib: poseidon252 times the per-call prover over the Poseidon252 Merkle "
+                         "channel instead (no oracle baseline, no forms or extras)")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -612,12 +622,23 @@ def main():
         workload = {"workload": "wide_fibonacci", "log_n_rows": log_rows, "trace_columns": args.n_cols}
         oracle_kw = dict(air="wide_fib", log_n_rows=log_rows, n_cols=args.n_cols)
     workload.update({"log_blowup": 1, "n_queries": 3, "pow_bits": args.pow_bits, "log_last_layer_degree_bound": 0,
-                     "channel": "blake2s", "seed": args.seed})
+                     "channel": args.merkle_channel, "seed": args.seed})
+    p252 = args.merkle_channel == "poseidon252"
+    if p252:
+        # the per-call prover over Poseidon252MerkleChannel; the oracle has no such channel
+        if args.workload == "poseidon":
+            prove = lambda: hb.prove_poseidon_per_call(trace, args.ipr, log_rows, cfg, allow_unsatisfied=True,
+                                                       merkle_channel="poseidon252")
+        else:
+            prove = lambda: hb.prove_wide_fibonacci_per_call([trace], [log_rows], cfg, merkle_channel="poseidon252")
+        metric += ", Poseidon252 Merkle channel, per-call form"
+        args.no_cpu_baseline = args.no_forms = args.no_extras = True
 
     device_sync(hb)  # brings up torch's device context before anything is timed
     # the VALU ceilings of this device (untimed): 32-bit add issue, and the transform butterflies
     valu_peak = hb.measure_int32_issue()
     bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
+    perm_peak = hb.measure_permutation_rate() if p252 else None
     for _ in range(args.warmup):
         prove()
 
@@ -659,6 +680,10 @@ def main():
     for k, u in units.items():
         if k in kernels and stats[k][0] > 0:
             rate = u / (stats[k][0] / 1e3)
+            if k == "merkle":
+                kernels[k]["M_permutations_per_s"] = round(rate / 1e6, 1)
+                kernels[k]["permutation_ceiling_frac"] = round(rate / perm_peak, 4) if perm_peak else None
+                continue
             kernels[k]["T_butterflies_per_s"] = round(rate / 1e12, 3)
             kernels[k]["butterfly_ceiling_frac"] = round(rate / bfly_peak[k], 4) if bfly_peak.get(k) else None
     stage = {k: round(v, 2) for k, v in last.stage_ms.items()}
@@ -720,13 +745,18 @@ def main():
         "byte_identical_to_oracle": parity,
         "stage_ms": stage,
         "roofline": roofline_entry(dom, by / args.steps, ms / args.steps, traffic, units.get(dom, 0.0) / args.steps,
-                                   bfly_peak.get(dom)),
+                                   perm_peak if dom == "merkle" else bfly_peak.get(dom),
+                                   PERMUTATIONS if dom == "merkle" else BUTTERFLIES),
         "valu_ceilings": {"int32_add_issue_T_lane_instr_per_s": round(valu_peak / 1e12, 2),
                           "forward_butterflies_T_per_s": round(bfly_peak["evaluate"] / 1e12, 3),
-                          "inverse_butterflies_T_per_s": round(bfly_peak["interpolate"] / 1e12, 3)},
+                          "inverse_butterflies_T_per_s": round(bfly_peak["interpolate"] / 1e12, 3),
+                          "M_permutations_per_s": round(perm_peak / 1e6, 1) if perm_peak else None},
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
+    if p252:
+        out["cpu_baseline_note"] = ("no CPU baseline over this channel: the oracle proves over Blake2s only, and the "
+                                    "Python restatement that pins these proofs is far too slow at this size")
     out.update(extras)
     print(json.dumps(out), flush=True)
     hb.close()
