@@ -171,6 +171,23 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_
 int stwo_hip_commit_on_layer_poseidon252(stwo_hip_ctx* ctx, uint32_t log_size, const uint32_t* prev,
                                          const uint32_t* const* cols, uint32_t n_cols, uint32_t* out);
 
+/* MerkleOpsLifted<H> (prover/vcs_lifted/ops.rs:8-18; CpuBackend prover/backend/cpu/merkle_lifted.rs:
+ * 12-87; the CUDA backend's blake2s_build_leaves_fused / blake2s_lift_states / ... and
+ * poseidon252_lifted_build_next_layer, stwo_cuda/bindings.rs:223,562). hasher 0: Blake2sMerkleHasher
+ * (core/vcs_lifted/blake2_merkle.rs), 1: Poseidon252MerkleHasher (core/vcs_lifted/poseidon252_merkle.rs);
+ * hashes are 8 words as for commit_on_layer / commit_on_layer_poseidon252.
+ * build_leaves: cols[c] has 2^log_sizes[c] words, log sizes non-decreasing, each in
+ * [1, lifting_log_size] (status 1 otherwise). Leaf i hashes, in column order, every column's value
+ * at row ((i >> (lifting_log_size - l + 1)) << 1) | (i & 1) (the lifted polynomial on the
+ * bit-reversed domain): Blake2s of the values as u32 LE, or poseidon_hash_many of them packed 8 per
+ * felt252. out: 2^lifting_log_size x 8 words; with no columns, one hash (the empty hasher's).
+ * build_next_layer: out (2^log_size x 8 words) = hash_children(prev[2i], prev[2i+1]). */
+int stwo_hip_build_leaves_lifted(stwo_hip_ctx* ctx, uint32_t hasher, const uint32_t* const* cols,
+                                 const uint32_t* log_sizes, uint32_t n_cols, uint32_t lifting_log_size,
+                                 uint32_t* out);
+int stwo_hip_build_next_layer_lifted(stwo_hip_ctx* ctx, uint32_t hasher, const uint32_t* prev, uint32_t log_size,
+                                     uint32_t* out);
+
 /* ---------------------------------------------------------------- FriOps --------------- */
 /* fold_circle_into_line (prover/fri.rs:46-66; bindings.rs:196-212): dst (4 coordinate columns of
  * 2^(log_src-1)) = dst * alpha^2 + fold(src), src on CanonicCoset(log_src). Uses the cached twiddles. */

@@ -610,6 +610,52 @@ int stwo_hip_commit_on_layer_poseidon252(stwo_hip_ctx* h, uint32_t log_size, con
     c.reset_arena();
   });
 }
+int stwo_hip_build_leaves_lifted(stwo_hip_ctx* h, uint32_t hasher, const uint32_t* const* cols,
+                                 const uint32_t* log_sizes, uint32_t n_cols, uint32_t lifting_log_size,
+                                 uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (hasher > LIFTED_POSEIDON252) throw HipError(1, "unknown lifted hasher");
+    if (!out || (n_cols && (!cols || !log_sizes))) throw HipError(1, "null argument");
+    if (lifting_log_size > 30) throw HipError(3, "lifting_log_size out of range");
+    std::vector<LiftedCol> t(n_cols);
+    double bytes = 32.0 * (double)(n_cols ? 1u << lifting_log_size : 1u);
+    for (uint32_t i = 0; i < n_cols; ++i) {
+      if (log_sizes[i] < 1) throw HipError(1, "a column must be of length >= 2");
+      if (i && log_sizes[i] < log_sizes[i - 1])
+        throw HipError(1, "columns must come in non-decreasing order of length");
+      if (log_sizes[i] > lifting_log_size) throw HipError(1, "a column is longer than the lifting domain");
+      t[i] = LiftedCol{cols[i], lifting_log_size - log_sizes[i], 0u};
+      bytes += 4.0 * (double)(1u << log_sizes[i]);
+    }
+    const LiftedCol* d = n_cols ? (const LiftedCol*)c.upload_vec(t) : nullptr;
+    {
+      KScope ks(c, K_MERKLE, bytes);
+      if (hasher == LIFTED_POSEIDON252)
+        ks.units((std::floor((double)((n_cols + 7) / 8) / 2) + 1) * (double)(n_cols ? 1u << lifting_log_size : 1u));
+      launch_lifted_leaves(hasher, lifting_log_size, d, n_cols, out, c.stream);
+    }
+    c.sync();
+    c.reset_arena();
+  });
+}
+int stwo_hip_build_next_layer_lifted(stwo_hip_ctx* h, uint32_t hasher, const uint32_t* prev, uint32_t log_size,
+                                     uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (hasher > LIFTED_POSEIDON252) throw HipError(1, "unknown lifted hasher");
+    if (!out || !prev) throw HipError(1, "null argument");
+    if (log_size > 29) throw HipError(3, "log_size out of range");
+    {
+      KScope ks(c, K_MERKLE, 96.0 * (double)(1u << log_size));
+      if (hasher == LIFTED_POSEIDON252) {
+        ks.units((double)(1u << log_size));
+        launch_p252_layer(log_size, prev, nullptr, 0, out, c.stream);
+      } else {
+        launch_merkle_layer_small(log_size, prev, nullptr, 0, out, c.stream);
+      }
+    }
+    c.sync();
+  });
+}
 int stwo_hip_grind_poseidon252(stwo_hip_ctx* h, const uint32_t digest[8], uint32_t pow_bits, uint64_t* nonce) {
   return guarded(h, [&](Ctx& c) {
     if (!digest || !nonce) throw HipError(1, "null argument");

@@ -250,6 +250,17 @@ void launch_fft_tiny(bool inverse, const uint32_t* const* src, uint32_t* const* 
 double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
 // Forward (or inverse) FFT butterflies per second on register-resident data (butterfly.h).
 double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t stream);
+// Lifted Merkle leaves (merkle_lifted.hip): column c's value at leaf i is its row
+// ((i >> (shift + 1)) << 1) | (i & 1), shift = lifting log size - its log size.
+constexpr uint32_t LIFTED_BLAKE2S = 0, LIFTED_POSEIDON252 = 1;
+struct LiftedCol {
+  const uint32_t* ptr;
+  uint32_t shift, pad;
+};
+// 2^lifting_log_size leaves of 8 words (one leaf when n_cols is 0)
+void launch_lifted_leaves(uint32_t hasher, uint32_t lifting_log_size, const LiftedCol* cols, uint32_t n_cols,
+                          uint32_t* out, hipStream_t stream);
+
 // Starknet Poseidon permutations per second on register-resident states (poseidon252.hip): the
 // ceiling of the Poseidon252 Merkle layers. scratch: CUs x 16 words.
 double measure_permutation_rate(uint32_t* scratch, hipStream_t stream);

@@ -93,7 +93,9 @@ EXPORTS = [
     "stwo_hip_commit_on_layer_poseidon252", "stwo_hip_grind_poseidon252", "stwo_poseidon252_hash_many",
     "stwo_poseidon252_permute", "stwo_hip_prove_wide_fibonacci_poseidon252", "stwo_hip_prove_poseidon_poseidon252",
     "stwo_verify_wide_fibonacci_poseidon252", "stwo_verify_poseidon_poseidon252", "stwo_hip_measure_permutation_rate",
+    "stwo_hip_build_leaves_lifted", "stwo_hip_build_next_layer_lifted",
 ]
+LIFTED_HASHERS = {"blake2s": 0, "poseidon252": 1}
 
 # The MerkleChannel of a proof: the suffix of its prove / verify entry points.
 MERKLE_CHANNELS = {"blake2s": ("_per_call", ""), "poseidon252": ("_poseidon252", "_poseidon252")}
@@ -200,6 +202,9 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_measure_int32_issue.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
     L.stwo_hip_measure_butterfly_rate.argtypes = [c.c_void_p, c.c_int, c.POINTER(c.c_double)]
     L.stwo_hip_measure_permutation_rate.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
+    L.stwo_hip_build_leaves_lifted.argtypes = [c.c_void_p, c.c_uint32, _VPP, _U32P, c.c_uint32, c.c_uint32,
+                                               c.c_void_p]
+    L.stwo_hip_build_next_layer_lifted.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, c.c_uint32, c.c_void_p]
     L.stwo_hip_kernel_units.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double)]
     L.stwo_hip_fft_low_log.argtypes = [c.c_void_p, c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, _U32P]
     L.stwo_hip_commit_on_layer_poseidon252.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32,
@@ -505,6 +510,24 @@ class HipBackend:
         out = out if out is not None else self.column(8 << log_size)
         self._check(self.lib.stwo_hip_commit_on_layer_poseidon252(
             self.ctx, log_size, prev.ptr if prev is not None else None, _ptrs(cols), len(cols), out.ptr))
+        return out
+
+    # ---- MerkleOpsLifted ----
+    def build_leaves_lifted(self, cols, log_sizes, lifting_log_size, hasher="blake2s"):
+        """MerkleOpsLifted::build_leaves: columns in non-decreasing order of log size, lifted to
+        2^lifting_log_size leaves (one leaf for no columns); hasher "blake2s" or "poseidon252"."""
+        logs = np.array(log_sizes, np.uint32).reshape(-1)
+        out = self.column(8 << lifting_log_size if len(cols) else 8)
+        self._check(self.lib.stwo_hip_build_leaves_lifted(self.ctx, LIFTED_HASHERS[hasher], _ptrs(cols),
+                                                          logs.ctypes.data_as(_U32P), len(cols), lifting_log_size,
+                                                          out.ptr))
+        return out
+
+    def build_next_layer_lifted(self, prev, log_size, hasher="blake2s"):
+        """MerkleOpsLifted::build_next_layer: 2^log_size parents of the 2^(log_size + 1) hashes in prev."""
+        out = self.column(8 << log_size)
+        self._check(self.lib.stwo_hip_build_next_layer_lifted(self.ctx, LIFTED_HASHERS[hasher], prev.ptr, log_size,
+                                                              out.ptr))
         return out
 
     # ---- FriOps ----
