@@ -361,6 +361,61 @@ int stwo_hip_gen_interaction_trace_poseidon(stwo_hip_ctx* ctx, const uint32_t* c
                                             uint32_t log_n_rows, stwo_qm31 z, stwo_qm31 alpha, uint32_t* const* out,
                                             stwo_qm31* claimed_sum);
 
+/* ---------------------------------------------------------------- device channel ----- */
+/* Blake2sChannel (core/channel/blake2s.rs:19-140) with its state in device memory: a buffer of
+ * STWO_HIP_CHANNEL_WORDS words the caller allocates (digest in words 0..7, n_draws in word 8, the
+ * rest the library's). No reference counterpart as an FFI (the CUDA backend keeps the channel on the
+ * host); it is the device channel API that the batched form needs: every call below except
+ * stwo_hip_channel_read is one launch on the context stream with no host synchronisation, so the
+ * FRI commit loop (commit a layer, mix its root, draw the folding coefficient, fold) runs without
+ * host round trips and can be captured in a graph. Results equal the host channel's.
+ *  - init: digest (32 bytes) and n_draws;
+ *  - mix_root: digest = H(digest || root) (Blake2sMerkleChannel::mix_root, 8 device words);
+ *  - mix_u32s: digest = H(digest || words) for n device words (mix_felts: 4 words a, b, c, d per
+ *    felt; mix_u64: low word, high word), n_draws = 0;
+ *  - draw_secure_felts: n felts (4 device words each), as draw_secure_felts(n);
+ *  - draw_queries: n_queries raw positions, each draw_u32s word masked to 2^log_domain
+ *    (core/queries.rs:11-29; sorting and deduplication are the caller's);
+ *  - grind: the smallest nonce of GrindOps<Blake2sChannel>::grind for the device digest into
+ *    nonce[0..1] (low, high); under graph capture only the first 2^(pow_bits+5) candidates (at least
+ *    2^12) are searched, and nonce reads 0xffffffff 0xffffffff in the e^-32-likely event of a miss;
+ *    the caller mixes the nonce (mix_u32s of the two words) as prove() does;
+ *  - read: the digest and n_draws (synchronous). */
+#define STWO_HIP_CHANNEL_WORDS 32
+int stwo_hip_channel_init(stwo_hip_ctx* ctx, uint32_t* state, const uint8_t digest[32], uint32_t n_draws);
+int stwo_hip_channel_mix_root(stwo_hip_ctx* ctx, uint32_t* state, const uint32_t* root);
+int stwo_hip_channel_mix_u32s(stwo_hip_ctx* ctx, uint32_t* state, const uint32_t* words, uint32_t n);
+int stwo_hip_channel_draw_secure_felts(stwo_hip_ctx* ctx, uint32_t* state, uint32_t n, uint32_t* out);
+int stwo_hip_channel_draw_queries(stwo_hip_ctx* ctx, uint32_t* state, uint32_t log_domain, uint32_t n_queries,
+                                  uint32_t* out);
+int stwo_hip_channel_grind(stwo_hip_ctx* ctx, uint32_t* state, uint32_t pow_bits, uint32_t* nonce);
+int stwo_hip_channel_read(stwo_hip_ctx* ctx, const uint32_t* state, uint8_t digest[32], uint32_t* n_draws);
+/* The folds with the coefficient in device memory (4 words, as drawn by
+ * stwo_hip_channel_draw_secure_felts): fold_circle_into_line (dst = dst alpha^2 + fold(src); with
+ * dst_is_zero the destination's contents are ignored, as for FriProver's zero first line layer,
+ * prover/fri.rs:148-152) and fold_line with fold step 1. */
+int stwo_hip_fold_circle_into_line_dev(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4],
+                                       uint32_t* const dst[4], const uint32_t* alpha, uint32_t dst_is_zero);
+int stwo_hip_fold_line_dev(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
+                           const uint32_t* alpha);
+
+/* ---------------------------------------------------------------- graphs -------------- */
+/* No reference counterpart. Capture and replay of a fixed sequence of calls on one context (HIP
+ * stream capture, thread-local mode): stwo_hip_graph_begin synchronises the context and starts
+ * capturing its stream; until stwo_hip_graph_end, only these entry points run, launching without
+ * synchronising: the device channel calls above (read excepted), the two _dev folds,
+ * stwo_hip_fold_circle_into_line, stwo_hip_fold_line with fold step 1 (both need their twiddle sizes
+ * built by an earlier call), and stwo_hip_commit_on_layer with at most 16 columns or columns at a
+ * stride of one column. Any other entry point returns 3 without acting, and the capture stays open
+ * until stwo_hip_graph_end. stwo_hip_graph_launch replays the graph on the context stream with the
+ * same device buffers (their current contents); stwo_hip_synchronize waits for it. A captured
+ * sequence that starts with stwo_hip_channel_init starts every replay from that channel state. */
+typedef struct stwo_hip_graph stwo_hip_graph;
+int stwo_hip_graph_begin(stwo_hip_ctx* ctx);
+int stwo_hip_graph_end(stwo_hip_ctx* ctx, stwo_hip_graph** out);
+int stwo_hip_graph_launch(stwo_hip_ctx* ctx, const stwo_hip_graph* graph);
+int stwo_hip_graph_free(stwo_hip_ctx* ctx, stwo_hip_graph* graph);
+
 /* ---------------------------------------------------------------- FFT plans ---------- */
 /* No reference counterpart (the reference's transforms take no launch parameters). A circle FFT
  * of k >= 11 stages runs stages [0, S) in one kernel whose workgroup owns 2^S consecutive values

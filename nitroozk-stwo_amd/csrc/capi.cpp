@@ -35,14 +35,18 @@ static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
 namespace {
 
 template <class F>
-int guarded(stwo_hip_ctx* h, F&& f, bool may_recycle = true) {
+int guarded(stwo_hip_ctx* h, F&& f, bool may_recycle = true, bool capture_ok = false) {
   if (!h || !h->ctx) return 1;
   try {
     // the calling thread may differ from the one that created the context: select its device
-    HIP_CHECK(hipSetDevice(h->ctx->device));
+    int cur = -1;
+    HIP_CHECK(hipGetDevice(&cur));
+    if (cur != h->ctx->device) HIP_CHECK(hipSetDevice(h->ctx->device));
+    if (h->ctx->capturing && !capture_ok)
+      throw HipError(3, "not capturable: this entry point synchronises, allocates or stages host data");
     // between entry points no host code holds a table address, so a full table store is emptied
     // here (the no-allocation entry point skips this: it may free a spilled table)
-    if (may_recycle) h->ctx->recycle_tables();
+    if (may_recycle && !h->ctx->capturing) h->ctx->recycle_tables();
     f(*h->ctx);
     return 0;
   } catch (const HipError& e) {
@@ -397,15 +401,29 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* h, uint32_t log_size, const uint32_t*
     // columns laid out at a fixed stride of one column need no address table
     bool strided = n_cols > 1;
     for (uint32_t k = 1; strided && k < n_cols; ++k) strided = v[k] == v[0] + ((size_t)k << log_size);
-    const uint32_t* const* d = !strided && n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
+    // under graph capture, at most 16 columns travel as kernel arguments (no staged table)
+    const bool small = c.capturing && !strided;
+    if (small && n_cols > 16) throw HipError(3, "not capturable: more than 16 columns not at one stride");
+    const uint32_t* const* d = !strided && !small && n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
     {
       KScope ks(c, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * n_cols + 32.0) * (double)(1u << log_size));
       if (strided) launch_merkle_layer_strided(log_size, prev, v[0], (size_t)1 << log_size, n_cols, out, c.stream);
+      else if (small) launch_merkle_layer_small(log_size, prev, v.data(), n_cols, out, c.stream);
       else launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
     }
+    if (c.capturing) return;
     c.sync();
     c.reset_arena();
-  });
+  }, true, true);
+}
+
+// The twiddle trees cover log_domain; under graph capture they must exist already (building them
+// allocates and uploads).
+static void need_twiddles(Ctx& c, uint32_t log_domain) {
+  if (!c.capturing) return c.ensure_twiddles(log_domain);
+  if (c.tw_log < log_domain)
+    throw HipError(3, "not capturable: the twiddles of this size are not built yet (one call of the size "
+                      "outside the capture builds them)");
 }
 
 int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4],
@@ -413,14 +431,43 @@ int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint
   return guarded(h, [&](Ctx& c) {
     if (log_src < 3 || log_src > 30) throw HipError(3, "log_src must be in [3, 30]");
     check_canonical(alpha);
-    c.ensure_twiddles(log_src);
+    need_twiddles(c, log_src);
     {
       KScope ks(c, K_FRI_FOLD, 16.0 * (double)(1u << log_src) + 32.0 * (double)(1u << (log_src - 1)));
       launch_fold_circle_into_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
                                    q(alpha), c.itw.p, c.tree_len(), c.stream);
     }
-    c.sync();
-  });
+    if (!c.capturing) c.sync();
+  }, true, true);
+}
+int stwo_hip_fold_circle_into_line_dev(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4],
+                                       uint32_t* const dst[4], const uint32_t* alpha, uint32_t dst_is_zero) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_src < 3 || log_src > 30) throw HipError(3, "log_src must be in [3, 30]");
+    if (!alpha) throw HipError(1, "null alpha");
+    need_twiddles(c, log_src);
+    {
+      KScope ks(c, K_FRI_FOLD, 16.0 * (double)(1u << log_src) + 32.0 * (double)(1u << (log_src - 1)));
+      launch_fold_circle_into_line_dev(log_src, {{src[0], src[1], src[2], src[3]}},
+                                       {{dst[0], dst[1], dst[2], dst[3]}}, (const qm31*)alpha, dst_is_zero != 0, c.itw.p,
+                                       c.tree_len(), c.stream);
+    }
+    if (!c.capturing) c.sync();
+  }, true, true);
+}
+int stwo_hip_fold_line_dev(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
+                           const uint32_t* alpha) {
+  return guarded(h, [&](Ctx& c) {
+    if (log_src < 1 || log_src > 29) throw HipError(3, "log_src must be in [1, 29]");
+    if (!alpha) throw HipError(1, "null alpha");
+    need_twiddles(c, log_src + 1);
+    {
+      KScope ks(c, K_FRI_FOLD, 24.0 * (double)(1u << log_src));
+      launch_fold_line_dev(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
+                           (const qm31*)alpha, 0, c.itw.p, c.tree_len(), c.stream);
+    }
+    if (!c.capturing) c.sync();
+  }, true, true);
 }
 int stwo_hip_fold_circle_into_line_grouped(stwo_hip_ctx* h, uint32_t log_src, uint32_t n_groups,
                                            const uint32_t* const* src, uint32_t* const* dst, stwo_qm31 alpha) {
@@ -456,13 +503,15 @@ int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const 
     if (log_src < 1 || log_src > 29) throw HipError(3, "log_src must be in [1, 29]");
     if (fold_step < 1 || fold_step > log_src) throw HipError(1, "fold_step must be in [1, log_src]");
     check_canonical(alpha);
-    c.ensure_twiddles(log_src + 1);
+    if (c.capturing && fold_step != 1) throw HipError(3, "not capturable: fold_step above 1 allocates");
+    need_twiddles(c, log_src + 1);
     // fold_step folds, the coefficient squared before each after the first (prover/backend/cpu/fri.rs:15-30);
     // the intermediate layers live in one temporary buffer: layers after even steps at its start
     // (at most 4 x 2^(log_src-1) words), after odd steps from word 2^(log_src+1) on (at most
     // 4 x 2^(log_src-2) words)
     const size_t tmp_words = fold_step > 1 ? (size_t)3 << log_src : 0;
-    DevBuf tmp(&c, tmp_words ? tmp_words : 1);
+    DevBuf tmp;
+    if (tmp_words) tmp = DevBuf(&c, tmp_words);
     CCols4 from = {{src[0], src[1], src[2], src[3]}};
     qm31 a = q(alpha);
     for (uint32_t st = 0; st < fold_step; ++st) {
@@ -481,8 +530,8 @@ int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const 
       from = {{to.c[0], to.c[1], to.c[2], to.c[3]}};
       a = qm_mul(a, a);
     }
-    c.sync();
-  });
+    if (!c.capturing) c.sync();
+  }, true, true);
 }
 
 int stwo_hip_accumulate_numerators(stwo_hip_ctx* h, uint32_t log_size, const uint32_t* const* cols, uint32_t n_cols,
@@ -1167,6 +1216,152 @@ int stwo_hip_eval_at_point_by_folding(stwo_hip_ctx* h, const uint32_t* evals, ui
     for (int k = 0; k < 4; ++k) c.d2h(&v[k], cur.c[k], 4);
     const qm31 r = qm_mul_m31({v[0], v[1], v[2], v[3]}, m31_inv(1u << log_size));
     *out = q(r);
+  });
+}
+
+// ---- the channel on the device and graph capture ----
+static uint32_t* need_state(uint32_t* st) {
+  if (!st) throw HipError(1, "null channel state");
+  return st;
+}
+int stwo_hip_channel_init(stwo_hip_ctx* h, uint32_t* state, const uint8_t digest[32], uint32_t n_draws) {
+  return guarded(h, [&](Ctx& c) {
+    if (!digest) throw HipError(1, "null digest");
+    ChanDigest d;
+    for (int i = 0; i < 8; ++i)
+      d.w[i] = (uint32_t)digest[4 * i] | (uint32_t)digest[4 * i + 1] << 8 | (uint32_t)digest[4 * i + 2] << 16 |
+               (uint32_t)digest[4 * i + 3] << 24;
+    launch_chan_init(need_state(state), d, n_draws, c.stream);
+  }, true, true);
+}
+int stwo_hip_channel_mix_root(stwo_hip_ctx* h, uint32_t* state, const uint32_t* root) {
+  return guarded(h, [&](Ctx& c) {
+    if (!root) throw HipError(1, "null root");
+    launch_chan_mix_words(need_state(state), root, 8, c.stream);
+  }, true, true);
+}
+int stwo_hip_channel_mix_u32s(stwo_hip_ctx* h, uint32_t* state, const uint32_t* words, uint32_t n) {
+  return guarded(h, [&](Ctx& c) {
+    if (n && !words) throw HipError(1, "null words");
+    launch_chan_mix_words(need_state(state), words, n, c.stream);
+  }, true, true);
+}
+int stwo_hip_channel_draw_secure_felts(stwo_hip_ctx* h, uint32_t* state, uint32_t n, uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (n && !out) throw HipError(1, "null out");
+    launch_chan_draw_felts(need_state(state), n, out, c.stream);
+  }, true, true);
+}
+int stwo_hip_channel_draw_queries(stwo_hip_ctx* h, uint32_t* state, uint32_t log_domain, uint32_t n_queries,
+                                  uint32_t* out) {
+  return guarded(h, [&](Ctx& c) {
+    if (n_queries && !out) throw HipError(1, "null out");
+    if (log_domain > 31) throw HipError(1, "log_domain above 31");
+    launch_chan_draw_queries(need_state(state), log_domain, n_queries, out, c.stream);
+  }, true, true);
+}
+int stwo_hip_channel_grind(stwo_hip_ctx* h, uint32_t* state, uint32_t pow_bits, uint32_t* nonce) {
+  return guarded(h, [&](Ctx& c) {
+    if (!nonce) throw HipError(1, "null nonce");
+    if (pow_bits > 32) throw HipError(3, "pow_bits > 32 is not supported");
+    uint32_t* st = need_state(state);
+    // the first launch covers 32 times the expected nonce (a miss has probability about e^-32)
+    const uint32_t bits = pow_bits + 5 < 12 ? 12 : (pow_bits + 5 > 24 ? 24 : pow_bits + 5);
+    uint64_t count = (uint64_t)1 << bits;
+    {
+      KScope ks(c, K_GRIND, 0.0);
+      launch_chan_grind(st, pow_bits, (uint32_t)count, nonce, c.stream);
+    }
+    if (c.capturing) return;
+    unsigned long long found = ~0ull;
+    c.d2h(&found, st + 10, 8);
+    // beyond the first range: further launches on the device prefix, doubling, as device_grind
+    const uint64_t limit = (uint64_t)0x7fffffff << 20;
+    for (uint64_t start = count; found == ~0ull; start += count) {
+      if (start >= limit) throw HipError(4, "Grind failed to find a solution.");
+      count = count < (1u << 24) ? count * 2 : count;
+      launch_grind_dev(st + 12, pow_bits, start, (uint32_t)count, reinterpret_cast<unsigned long long*>(st + 10),
+                       c.stream);
+      c.d2h(&found, st + 10, 8);
+    }
+    const uint32_t w[2] = {(uint32_t)found, (uint32_t)(found >> 32)};
+    HIP_CHECK(hipMemcpyAsync(nonce, w, 8, hipMemcpyHostToDevice, c.stream));
+    c.sync();
+  }, true, true);
+}
+int stwo_hip_channel_read(stwo_hip_ctx* h, const uint32_t* state, uint8_t digest[32], uint32_t* n_draws) {
+  return guarded(h, [&](Ctx& c) {
+    if (!state || !digest || !n_draws) throw HipError(1, "null argument");
+    uint32_t w[9];
+    c.d2h(w, state, sizeof(w));
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 4; ++j) digest[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
+    *n_draws = w[8];
+  });
+}
+
+struct stwo_hip_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+};
+int stwo_hip_graph_begin(stwo_hip_ctx* h) {
+  return guarded(h, [&](Ctx& c) {
+    if (c.capturing) throw HipError(1, "a capture is already open on this context");
+    c.sync();  // nothing of earlier calls is left in flight, on either stream
+    c.reset_arena();
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    c.capturing = true;
+  });
+}
+int stwo_hip_graph_end(stwo_hip_ctx* h, stwo_hip_graph** out) {
+  if (!h || !h->ctx) return 1;
+  Ctx& c = *h->ctx;
+  if (!c.capturing) {
+    h->err = "no capture is open on this context";
+    return 1;
+  }
+  // the capture ends whatever happens next
+  c.capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c.stream, &g);
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    h->err = std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e);
+    return 2;
+  }
+  if (!out) {
+    (void)hipGraphDestroy(g);
+    h->err = "null out";
+    return 1;
+  }
+  auto* gr = new stwo_hip_graph();
+  gr->graph = g;
+  gr->device = c.device;
+  const hipError_t ie = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    delete gr;
+    h->err = std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ie);
+    return 2;
+  }
+  *out = gr;
+  return 0;
+}
+int stwo_hip_graph_launch(stwo_hip_ctx* h, const stwo_hip_graph* g) {
+  return guarded(h, [&](Ctx& c) {
+    if (!g || !g->exec) throw HipError(1, "null graph");
+    if (g->device != c.device) throw HipError(1, "the graph was captured on another device");
+    HIP_CHECK(hipGraphLaunch(g->exec, c.stream));
+  });
+}
+int stwo_hip_graph_free(stwo_hip_ctx* h, stwo_hip_graph* g) {
+  return guarded(h, [&](Ctx& c) {
+    if (!g) return;
+    c.sync();  // a launch of it may still run
+    if (g->exec) HIP_CHECK(hipGraphExecDestroy(g->exec));
+    if (g->graph) HIP_CHECK(hipGraphDestroy(g->graph));
+    delete g;
   });
 }
 

@@ -250,6 +250,19 @@ void launch_fft_tiny(bool inverse, const uint32_t* const* src, uint32_t* const* 
 double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
 // Forward (or inverse) FFT butterflies per second on register-resident data (butterfly.h).
 double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t stream);
+// The Blake2s channel on the device (channel_dev.hip): state of CHANNEL_WORDS words (digest 0..7,
+// n_draws 8, the grind's u64 result slot 10..11, the proof-of-work prefix 12..19). Single-lane
+// launches; launch_chan_grind covers the first `count` candidates of the nonce order.
+constexpr uint32_t CHANNEL_WORDS = 32;
+struct ChanDigest {
+  uint32_t w[8];
+};
+void launch_chan_init(uint32_t* st, const ChanDigest& d, uint32_t n_draws, hipStream_t stream);
+void launch_chan_mix_words(uint32_t* st, const uint32_t* words, uint32_t n, hipStream_t stream);
+void launch_chan_draw_felts(uint32_t* st, uint32_t n, uint32_t* out, hipStream_t stream);
+void launch_chan_draw_queries(uint32_t* st, uint32_t log_domain, uint32_t n, uint32_t* out, hipStream_t stream);
+void launch_chan_grind(uint32_t* st, uint32_t pow_bits, uint32_t count, uint32_t* nonce, hipStream_t stream);
+
 // Lifted Merkle leaves (merkle_lifted.hip): column c's value at leaf i is its row
 // ((i >> (shift + 1)) << 1) | (i & 1), shift = lifting log size - its log size.
 constexpr uint32_t LIFTED_BLAKE2S = 0, LIFTED_POSEIDON252 = 1;

@@ -78,6 +78,10 @@ struct KernelStat {
 struct Ctx {
   int device = 0;
   bool timing = false;
+  // The stream is being captured into a graph (stwo_hip_graph_begin .. _end): only entry points
+  // that launch on the stream with no synchronisation, allocation or staged upload may run, and
+  // kernel timing records no events.
+  bool capturing = false;
   struct Pending { int cls; double bytes, units; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -210,7 +214,7 @@ struct KScope {
   hipStream_t st;
   hipEvent_t a = nullptr;
   KScope(Ctx& ctx, int k, double b, hipStream_t s = nullptr) : c(ctx), cls(k), bytes(b), st(s ? s : ctx.stream) {
-    if (c.timing) { a = c.get_event(); (void)hipEventRecord(a, st); }
+    if (c.timing && !c.capturing) { a = c.get_event(); (void)hipEventRecord(a, st); }
   }
   // The launch group's algorithmic work units (KernelStat::units).
   KScope& units(double u) { n_units = u; return *this; }

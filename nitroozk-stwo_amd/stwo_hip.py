@@ -94,7 +94,12 @@ EXPORTS = [
     "stwo_poseidon252_permute", "stwo_hip_prove_wide_fibonacci_poseidon252", "stwo_hip_prove_poseidon_poseidon252",
     "stwo_verify_wide_fibonacci_poseidon252", "stwo_verify_poseidon_poseidon252", "stwo_hip_measure_permutation_rate",
     "stwo_hip_build_leaves_lifted", "stwo_hip_build_next_layer_lifted",
+    "stwo_hip_channel_init", "stwo_hip_channel_mix_root", "stwo_hip_channel_mix_u32s",
+    "stwo_hip_channel_draw_secure_felts", "stwo_hip_channel_draw_queries", "stwo_hip_channel_grind",
+    "stwo_hip_channel_read", "stwo_hip_fold_circle_into_line_dev", "stwo_hip_fold_line_dev",
+    "stwo_hip_graph_begin", "stwo_hip_graph_end", "stwo_hip_graph_launch", "stwo_hip_graph_free",
 ]
+CHANNEL_WORDS = 32
 LIFTED_HASHERS = {"blake2s": 0, "poseidon252": 1}
 
 # The MerkleChannel of a proof: the suffix of its prove / verify entry points.
@@ -205,6 +210,19 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_build_leaves_lifted.argtypes = [c.c_void_p, c.c_uint32, _VPP, _U32P, c.c_uint32, c.c_uint32,
                                                c.c_void_p]
     L.stwo_hip_build_next_layer_lifted.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, c.c_uint32, c.c_void_p]
+    L.stwo_hip_channel_init.argtypes = [c.c_void_p, c.c_void_p, c.c_char_p, c.c_uint32]
+    L.stwo_hip_channel_mix_root.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+    L.stwo_hip_channel_mix_u32s.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_uint32]
+    L.stwo_hip_channel_draw_secure_felts.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, c.c_void_p]
+    L.stwo_hip_channel_draw_queries.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, c.c_uint32, c.c_void_p]
+    L.stwo_hip_channel_grind.argtypes = [c.c_void_p, c.c_void_p, c.c_uint32, c.c_void_p]
+    L.stwo_hip_channel_read.argtypes = [c.c_void_p, c.c_void_p, c.c_char_p, c.POINTER(c.c_uint32)]
+    L.stwo_hip_fold_circle_into_line_dev.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, c.c_void_p, c.c_uint32]
+    L.stwo_hip_fold_line_dev.argtypes = [c.c_void_p, c.c_uint32, _VPP, _VPP, c.c_void_p]
+    L.stwo_hip_graph_begin.argtypes = [c.c_void_p]
+    L.stwo_hip_graph_end.argtypes = [c.c_void_p, c.POINTER(c.c_void_p)]
+    L.stwo_hip_graph_launch.argtypes = [c.c_void_p, c.c_void_p]
+    L.stwo_hip_graph_free.argtypes = [c.c_void_p, c.c_void_p]
     L.stwo_hip_kernel_units.argtypes = [c.c_void_p, c.c_uint32, c.POINTER(c.c_double)]
     L.stwo_hip_fft_low_log.argtypes = [c.c_void_p, c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, _U32P]
     L.stwo_hip_commit_on_layer_poseidon252.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, _VPP, c.c_uint32,
@@ -529,6 +547,66 @@ class HipBackend:
         self._check(self.lib.stwo_hip_build_next_layer_lifted(self.ctx, LIFTED_HASHERS[hasher], prev.ptr, log_size,
                                                               out.ptr))
         return out
+
+    # ---- the channel on the device, and graphs ----
+    def channel_new(self, digest=bytes(32), n_draws=0):
+        """A device channel state (CHANNEL_WORDS words), initialised (stream-ordered)."""
+        st = self.column(CHANNEL_WORDS, zeroed=True)
+        self.channel_init(st, digest, n_draws)
+        return st
+
+    def channel_init(self, st, digest=bytes(32), n_draws=0):
+        self._check(self.lib.stwo_hip_channel_init(self.ctx, st.ptr, bytes(digest), n_draws))
+
+    def channel_mix_root(self, st, root, offset=0):
+        """root: a device column; its words offset..offset + 8 are the root."""
+        self._check(self.lib.stwo_hip_channel_mix_root(self.ctx, st.ptr, root.ptr + 4 * offset))
+
+    def channel_mix_u32s(self, st, words, n=None):
+        n = words.n if n is None else n
+        self._check(self.lib.stwo_hip_channel_mix_u32s(self.ctx, st.ptr, words.ptr, n))
+
+    def channel_draw_secure_felts(self, st, n, out=None):
+        out = out if out is not None else self.column(4 * n)
+        self._check(self.lib.stwo_hip_channel_draw_secure_felts(self.ctx, st.ptr, n, out.ptr))
+        return out
+
+    def channel_draw_queries(self, st, log_domain, n_queries, out=None):
+        out = out if out is not None else self.column(n_queries)
+        self._check(self.lib.stwo_hip_channel_draw_queries(self.ctx, st.ptr, log_domain, n_queries, out.ptr))
+        return out
+
+    def channel_grind(self, st, pow_bits, nonce=None):
+        nonce = nonce if nonce is not None else self.column(2)
+        self._check(self.lib.stwo_hip_channel_grind(self.ctx, st.ptr, pow_bits, nonce.ptr))
+        return nonce
+
+    def channel_read(self, st):
+        d = ctypes.create_string_buffer(32)
+        n = ctypes.c_uint32()
+        self._check(self.lib.stwo_hip_channel_read(self.ctx, st.ptr, d, ctypes.byref(n)))
+        return d.raw, n.value
+
+    def fold_circle_into_line_dev(self, dst4, src4, alpha_col, log_src, dst_is_zero=False):
+        self._check(self.lib.stwo_hip_fold_circle_into_line_dev(self.ctx, log_src, _ptrs(src4), _ptrs(dst4),
+                                                                alpha_col.ptr, int(bool(dst_is_zero))))
+
+    def fold_line_dev(self, dst4, src4, alpha_col, log_src):
+        self._check(self.lib.stwo_hip_fold_line_dev(self.ctx, log_src, _ptrs(src4), _ptrs(dst4), alpha_col.ptr))
+
+    def graph_begin(self):
+        self._check(self.lib.stwo_hip_graph_begin(self.ctx))
+
+    def graph_end(self):
+        g = ctypes.c_void_p()
+        self._check(self.lib.stwo_hip_graph_end(self.ctx, ctypes.byref(g)))
+        return g
+
+    def graph_launch(self, g):
+        self._check(self.lib.stwo_hip_graph_launch(self.ctx, g))
+
+    def graph_free(self, g):
+        self._check(self.lib.stwo_hip_graph_free(self.ctx, g))
 
     # ---- FriOps ----
     def fold_circle_into_line(self, dst4, src4, alpha, log_src):
