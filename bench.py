@@ -412,7 +412,7 @@ def proof_forms(args, hb, m, cfg, trace, log_rows, fused_times):
              "p50_ms": pct(times, 0.5), "p99_ms": pct(times, 0.99), "bytes_equal_fused": pr.bytes == ref.bytes}
     # batched: K contexts, each with its own copy of the trace, proving concurrently
     K = max(1, args.contexts)
-    per = max(2, args.steps // K)
+    per = max(40, args.steps // K)  # enough proofs per context for a p99 (a proof takes about 2 ms)
     backends, traces = [], []
     for _ in range(K):
         b = m.HipBackend(0)
@@ -547,6 +547,81 @@ def poseidon252_extras(args, hb, cfg, trace, log_rows):
         reference_readme_gpu_ms={"log": 18, "ms": 133, "source": "README.md:155-176"})
     hb.free_columns(trw)
     return {"poseidon252_channel": out}
+
+
+def graph_extras(hb, first_log=17, last_log=1, reps=5):
+    """The FRI commit loop of the headline shape (first layer 2^17, down to 2^1) driven through the
+    C ABI's device-side calls only: each layer's Merkle levels (commit_on_layer), its root mixed and
+    the folding coefficient drawn by the device channel, the fold with that coefficient from device
+    memory, then queries and the proof of work. Timed eager (one synchronising C-ABI call each) and
+    as one captured graph replayed (stwo_hip_graph_*); the replay's transcript must equal the eager
+    one's."""
+    import numpy as np
+    first = hb.columns(4, 1 << first_log)
+    rng = np.random.default_rng(11)
+    for c in first:
+        hb._check(hb.lib.stwo_hip_copy_h2d(hb.ctx, c.ptr, rng.integers(0, (1 << 31) - 1, 1 << first_log,
+                                                                        dtype=np.uint32).ctypes.data, 1 << first_log))
+    layers = {k: hb.columns(4, 1 << k) for k in range(last_log, first_log)}
+    trees = {k: [hb.column(8 << ls) for ls in range(k + 1)] for k in range(last_log + 1, first_log + 1)}
+    alphas = {k: hb.column(4) for k in trees}
+    st, queries, nonce = hb.column(32, zeroed=True), hb.column(3), hb.column(2)
+    calls = [0]
+
+    def run():
+        calls[0] = 0
+
+        def c(f, *a, **kw):
+            calls[0] += 1
+            return f(*a, **kw)
+        c(hb.channel_init, st, bytes(32), 0)
+        for k in range(first_log, last_log, -1):
+            src = first if k == first_log else layers[k]
+            c(hb.commit_on_layer, k, None, list(src), out=trees[k][k])
+            for ls in range(k - 1, -1, -1):
+                c(hb.commit_on_layer, ls, trees[k][ls + 1], [], out=trees[k][ls])
+            c(hb.channel_mix_root, st, trees[k][0])
+            c(hb.channel_draw_secure_felts, st, 1, out=alphas[k])
+            if k == first_log:
+                c(hb.fold_circle_into_line_dev, list(layers[k - 1]), list(first), alphas[k], k, dst_is_zero=True)
+            else:
+                c(hb.fold_line_dev, list(layers[k - 1]), list(layers[k]), alphas[k], k)
+        c(hb.channel_mix_u32s, st, layers[last_log][0], 1 << last_log)
+        c(hb.channel_draw_queries, st, first_log, 3, out=queries)
+        c(hb.channel_grind, st, 10, nonce=nonce)
+
+    def transcript():
+        return (hb.channel_read(st), queries.to_cpu().tolist(), nonce.to_cpu().tolist(),
+                [alphas[k].to_cpu().tolist() for k in sorted(alphas)])
+
+    run()
+    hb.synchronize()
+    ref = transcript()
+    eager = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        hb.synchronize()
+        eager.append((time.perf_counter() - t0) * 1e3)
+    hb.graph_begin()
+    try:
+        run()
+    finally:
+        g = hb.graph_end()
+    replay, same = [], True
+    try:
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            hb.graph_launch(g)
+            hb.synchronize()
+            replay.append((time.perf_counter() - t0) * 1e3)
+            same = same and transcript() == ref
+    finally:
+        hb.graph_free(g)
+    e, r = statistics.median(eager), statistics.median(replay)
+    return {"graph_fri_commit_loop": {"first_log": first_log, "last_log": last_log, "calls": calls[0],
+                                      "eager_ms": round(e, 3), "graph_replay_ms": round(r, 3),
+                                      "eager_over_replay": round(e / r, 2), "replays_equal_eager": same}}
 
 
 def main():
@@ -695,6 +770,7 @@ def main():
         extras.update(poseidon_extras(args, hb, m, cfg, trace, log_rows))
     if args.workload in ("poseidon", "wide_fib") and not args.no_extras:
         extras.update(poseidon252_extras(args, hb, cfg, trace, log_rows))
+        extras.update(graph_extras(hb))
 
     cpu = None
     parity = None
