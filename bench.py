@@ -204,6 +204,21 @@ def mixed(args, m):
     device_sync(hb)
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     cells = sum(nc << lg for lg, nc in zip(logs, ncols))
+    # the same proofs with kernel events: per-class time, and the roofline of the dominant class
+    bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
+    hb.reset_kernel_stats()
+    hb.set_kernel_timing(True)
+    for _ in range(args.steps):
+        prove()
+    hb.set_kernel_timing(False)
+    stats, units = hb.kernel_stats(), hb.kernel_units()
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[2] // args.steps,
+                   "GB_per_s": round(v[1] / (v[0] / 1e3) / 1e9, 1) if v[0] > 0 and v[1] > 0 else None}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
+    for k, u in units.items():
+        if k in kernels and stats[k][0] > 0 and bfly_peak.get(k):
+            kernels[k]["butterfly_ceiling_frac"] = round(u / (stats[k][0] / 1e3) / bfly_peak[k], 4)
+    dom = max((k for k in stats if k != "grind"), key=lambda k: stats[k][0])
     out = {
         "metric": "prove ms (%d mixed wide-Fibonacci components)" % args.components, "value": round(ms, 3), "unit": "ms",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
@@ -213,6 +228,9 @@ def mixed(args, m):
                    "trace_cells": cells, "log_blowup": 1, "n_queries": 3, "pow_bits": args.pow_bits},
         "p50_ms": round(statistics.median(times), 3), "proof_status": pr.status, "proof_bytes": len(pr.bytes),
         "trace_cells_per_s": round(cells / (ms / 1e3), 1),
+        "roofline": roofline_entry(dom, stats[dom][1] / args.steps, stats[dom][0] / args.steps, None,
+                                   units.get(dom, 0.0) / args.steps, bfly_peak.get(dom)),
+        "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
     hb.close()

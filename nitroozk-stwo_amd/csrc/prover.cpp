@@ -294,6 +294,9 @@ uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits, unsigned 
   return found;
 }
 
+// Levels of at least 2^MERKLE_WIDE_LOG nodes are hashed one launch per level.
+constexpr uint32_t MERKLE_WIDE_LOG = 17;
+
 void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
   t.layers.clear();
   if (t.evals.empty()) {
@@ -315,6 +318,17 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
   size_t ci = 0;
   int ls = (int)max_log;
   while (ls >= 0) {
+    if (!rev.empty() && !has_cols[ls] && ls >= (int)MERKLE_WIDE_LOG) {
+      // a wide level without injected columns: one lane per node, one launch per level (the fused
+      // subtree kernel hashes a node with a quad of lanes and waits between levels inside a
+      // workgroup, which pays on narrow levels only; on 2^22 nodes it ran at a quarter of this)
+      DevBuf layer(&ctx, (size_t)8 << ls);
+      KScope ks(ctx, K_MERKLE, 96.0 * (double)(1u << ls));
+      launch_merkle_layer_small((uint32_t)ls, rev.back().p, nullptr, 0, layer.p, ctx.stream);
+      rev.push_back(std::move(layer));
+      --ls;
+      continue;
+    }
     if (!rev.empty() && !has_cols[ls]) {
       // run of levels without injected columns: fused, up to 9 levels per launch
       uint32_t levels = 0;
