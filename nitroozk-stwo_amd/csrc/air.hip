@@ -53,6 +53,64 @@ void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const ui
                      pows, denom_inv, acc, (uint32_t)accumulate);
 }
 
+// Several small wide-Fibonacci components in one launch (the batched composition of a
+// multi-component proof, ComponentProvers::compute_composition_polynomial's batch hook,
+// crates/stwo/src/prover/air/component_prover.rs:118-141): components are grouped by evaluation
+// size; a block covers 64 rows of one group and its WF_WAVES waves take the group's components in
+// turn, their partial sums meeting in LDS. The sum over a group's components times the group's
+// denominator equals the per-component accumulation, so the accumulator ends the same.
+constexpr uint32_t WF_WAVES = 4;
+__global__ void __launch_bounds__(256) wide_fib_comp_grouped_kernel(const WfGroup* __restrict__ groups,
+                                                                    uint32_t n_groups,
+                                                                    const WfComp* __restrict__ comps) {
+  __shared__ uint32_t part[WF_WAVES][4][64];
+  const uint32_t lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint32_t g = 0;  // the block's group: the last whose first block is at most this one (uniform)
+  for (uint32_t k = 1; k < n_groups; ++k)
+    if (groups[k].block_off <= blockIdx.x) g = k;
+  const WfGroup& G = groups[g];
+  const uint32_t row = (blockIdx.x - G.block_off) * 64 + lane;
+  const bool active = row < (1u << G.eval_log);
+  qacc acc = qacc_zero();
+  if (active) {
+    for (uint32_t j = w; j < G.n_comps; j += WF_WAVES) {
+      const WfComp& C = comps[G.comp_off + j];
+      uint32_t a = C.cols[0][row], b = C.cols[1][row];
+      for (uint32_t k = 2; k < C.n_cols; ++k) {
+        const uint32_t c = C.cols[k][row];
+        const uint32_t s = m31_reduce64((uint64_t)a * a + (uint64_t)b * b);
+        qacc_mad(acc, C.pows[k - 2], m31_sub(c, s));
+        if (k % 3 == 1) qacc_fold(acc);
+        a = b;
+        b = c;
+      }
+      qacc_fold(acc);
+    }
+  }
+  const qm31 r = qacc_reduce(acc);
+  part[w][0][lane] = r.a;
+  part[w][1][lane] = r.b;
+  part[w][2][lane] = r.c;
+  part[w][3][lane] = r.d;
+  __syncthreads();
+  if (w == 0 && active) {
+    qm31 tot = qm_zero();
+    for (uint32_t v = 0; v < WF_WAVES; ++v)
+      tot = qm_add(tot, qm31{part[v][0][lane], part[v][1][lane], part[v][2][lane], part[v][3][lane]});
+    tot = qm_mul_m31(tot, G.dinv[row >> G.trace_log]);
+    if (G.accumulate) tot = qm_add(qm31{G.acc[0][row], G.acc[1][row], G.acc[2][row], G.acc[3][row]}, tot);
+    G.acc[0][row] = tot.a;
+    G.acc[1][row] = tot.b;
+    G.acc[2][row] = tot.c;
+    G.acc[3][row] = tot.d;
+  }
+}
+void launch_wide_fib_composition_grouped(const WfGroup* groups, uint32_t n_groups, const WfComp* comps,
+                                         uint32_t n_blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(wide_fib_comp_grouped_kernel, dim3(n_blocks), dim3(64 * WF_WAVES), 0, stream, groups, n_groups,
+                     comps);
+}
+
 // ---- Poseidon2 -------------------------------------------------------------------------------
 constexpr uint32_t PSD_COLS_PER_REP = 158;
 constexpr uint32_t PSD_ROUND_CONST = 1234;

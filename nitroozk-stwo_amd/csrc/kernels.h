@@ -199,6 +199,20 @@ void launch_decommit_gather(const DecItem* items, uint32_t n_items, const uint32
 
 // Composition (constraint quotients on the evaluation domain), added into acc (accumulate) or
 // written to it.
+// Grouped wide-Fibonacci composition: groups of components of one evaluation size, 64 rows per block
+// (group g's blocks start at block_off), components comp_off .. comp_off + n_comps of `comps`.
+struct WfGroup {
+  const uint32_t* dinv;
+  uint32_t* acc[4];
+  uint32_t block_off, eval_log, trace_log, accumulate, comp_off, n_comps;
+};
+struct WfComp {
+  const uint32_t* const* cols;
+  const qm31* pows;
+  uint32_t n_cols, pad;
+};
+void launch_wide_fib_composition_grouped(const WfGroup* groups, uint32_t n_groups, const WfComp* comps,
+                                         uint32_t n_blocks, hipStream_t stream);
 void launch_wide_fib_composition(uint32_t eval_log, uint32_t trace_log, const uint32_t* const* cols, uint32_t n_cols,
                                  const qm31* pows, const uint32_t* denom_inv, Cols4 acc, bool accumulate,
                                  hipStream_t stream);

@@ -294,6 +294,8 @@ uint64_t device_grind(Ctx& ctx, const Hash& prefix, uint32_t pow_bits, unsigned 
   return found;
 }
 
+// Wide-Fibonacci components on evaluation domains below 2^WF_GROUP_LOG share one composition launch.
+constexpr uint32_t WF_GROUP_LOG = 16;
 // Levels of at least 2^MERKLE_WIDE_LOG nodes are hashed one launch per level.
 constexpr uint32_t MERKLE_WIDE_LOG = 17;
 
@@ -593,16 +595,75 @@ int prove_ex(Ctx& ctx, Channel& ch, std::vector<DevTree>& trees, const std::vect
   }
   HMARK("x_pows");
   auto& accs = prep->accs;
+  // each component's constraint coefficients: its slice of the powers, reversed, in component order
+  std::vector<const qm31*> comp_pows(comps.size());
   for (size_t ci = 0; ci < comps.size(); ++ci) {
-    const Component& c = comps[ci];
-    CompLaunch& L = prep->items[ci];
-    size_t nc = c.n_constraints();
+    const size_t nc = comps[ci].n_constraints();
     std::vector<qm31> cp(powers.end() - nc, powers.end());
     powers.resize(powers.size() - nc);
     std::reverse(cp.begin(), cp.end());
+    comp_pows[ci] = ctx.stage_vec(cp);
+  }
+  HMARK("x_tables");
+  // Small wide-Fibonacci components (evaluation domains below 2^WF_GROUP_LOG) go in one grouped
+  // launch, by evaluation size, when every component of that size is one: a separate launch of a
+  // 2^11-row component occupies a few CUs for tens of microseconds.
+  std::vector<char> grouped(comps.size(), 0);
+  {
+    std::map<uint32_t, std::vector<size_t>> small;
+    std::set<uint32_t> other;
+    for (size_t ci = 0; ci < comps.size(); ++ci) {
+      const uint32_t E = prep->items[ci].E;
+      if (comps[ci].kind == AIR_WIDE_FIB && E < WF_GROUP_LOG) small[E].push_back(ci);
+      else other.insert(E);
+    }
+    for (uint32_t E : other) small.erase(E);
+    size_t n_small = 0;
+    for (auto& kv : small) n_small += kv.second.size();
+    if (n_small >= 2) {
+      std::vector<WfGroup> gs;
+      std::vector<WfComp> cs;
+      uint32_t blocks = 0;
+      double bytes = 0;
+      bool join = false;
+      for (auto& [E, idx] : small) {
+        const CompLaunch& L0 = prep->items[idx[0]];
+        WfGroup G{};
+        G.dinv = L0.d_dinv;
+        for (int k = 0; k < 4; ++k) G.acc[k] = L0.a4.c[k];
+        G.block_off = blocks;
+        G.eval_log = E;
+        G.trace_log = comps[idx[0]].log_n;
+        G.accumulate = L0.accumulate ? 1u : 0u;
+        G.comp_off = (uint32_t)cs.size();
+        G.n_comps = (uint32_t)idx.size();
+        for (size_t ci : idx) {
+          const CompLaunch& L = prep->items[ci];
+          // one evaluation size: the same trace size, denominators (equal tables) and accumulator
+          if (comps[ci].log_n != G.trace_log || L.a4.c[0] != G.acc[0])
+            throw HipError(4, "grouped composition: components of one size disagree");
+          cs.push_back(WfComp{L.d_t, comp_pows[ci], comps[ci].n_cols, 0u});
+          grouped[ci] = 1;
+          bytes += L.cbytes;
+          join |= L.join_side;
+        }
+        blocks += ((1u << E) + 63) / 64;
+        gs.push_back(G);
+      }
+      if (join) ctx.join();
+      const WfGroup* dg = ctx.stage_vec(gs);
+      const WfComp* dc = ctx.stage_vec(cs);
+      ctx.commit_staged();
+      KScope ks(ctx, K_COMPOSITION, bytes);
+      launch_wide_fib_composition_grouped(dg, (uint32_t)gs.size(), dc, blocks, ctx.stream);
+    }
+  }
+  for (size_t ci = 0; ci < comps.size(); ++ci) {
+    if (grouped[ci]) continue;
+    const Component& c = comps[ci];
+    CompLaunch& L = prep->items[ci];
     const uint32_t E = L.E;
-    const qm31* d_pows = ctx.stage_vec(cp);
-    HMARK("x_tables");
+    const qm31* d_pows = comp_pows[ci];
     if (L.join_side) ctx.join();
     if (c.kind == AIR_WIDE_FIB) {
       ctx.commit_staged();
