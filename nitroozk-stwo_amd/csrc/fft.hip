@@ -113,6 +113,51 @@ __global__ void __launch_bounds__(256) fft_pass_kernel(const uint32_t* const* __
   }
 }
 
+// ---- high stages in registers ------------------------------------------------------------------
+// Stages [e, e + S) of a transform whose low e >= 11 stages are done: a lane owns the 2^S values
+// at hi 2^(e+S) + t 2^e + lo (t < 2^S) and runs all S stages on them in registers. The 64 lanes of
+// a wave take 64 consecutive lo, so each of a lane's loads and stores is one 256-byte segment of the
+// wave, and every twiddle (index (hi 2^(S-1-j)) + (t >> (j + 1)) at stage e + j) is the same for the
+// whole wave: a scalar load. No LDS and no barrier.
+template <bool INV, int S>
+__global__ void __launch_bounds__(256) fft_pass_reg_kernel(const uint32_t* const* __restrict__ src,
+                                                           uint32_t* const* __restrict__ dst, uint32_t log_n,
+                                                           uint32_t in_mask, uint32_t e,
+                                                           const uint32_t* __restrict__ tree, uint32_t tree_len,
+                                                           uint32_t scale, uint32_t prio) {
+  if (prio) CRITICAL_PATH_WAVE();
+  const uint32_t col = blockIdx.y;
+  const GLOBAL_AS uint32_t* __restrict__ in = gmem(src[col]);
+  GLOBAL_AS uint32_t* __restrict__ out = gmem(dst[col]);
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lo = g & ((1u << e) - 1);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(g >> e);  // e >= 6: one value per wave
+  const uint32_t base = (hi << (e + S)) | lo;
+  uint32_t v[1 << S];
+#pragma unroll
+  for (uint32_t t = 0; t < (1u << S); ++t) v[t] = in[(base | (t << e)) & in_mask];
+#pragma unroll
+  for (int jj = 0; jj < S; ++jj) {
+    const int j = INV ? jj : S - 1 - jj;  // the inverse ascends the stages, the forward descends
+    const uint32_t* __restrict__ row = tree + (tree_len - (1u << (log_n - e - j)));
+#pragma unroll
+    for (uint32_t u = 0; u < (1u << (S - 1 - j)); ++u) {
+      const uint32_t tw = row[(hi << (S - 1 - j)) | u];
+#pragma unroll
+      for (uint32_t l = 0; l < (1u << j); ++l) {
+        const uint32_t q = (u << (j + 1)) | l;
+        bfly<INV>(v[q], v[q | (1u << j)], tw);
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < (1u << S); ++t) {
+    const uint32_t x = v[t];
+    out[base | (t << e)] = scale ? mul_tw(x, scale) : x;
+  }
+}
+constexpr uint32_t PASS_REG_MAX = 6;  // stages per register pass (2^S values per lane)
+
 // ---- low stages of a 2^S-element block in one pass (11 <= S <= 15) ---------------------------
 // A workgroup of 2^(S-5) threads owns one block (tile) of 2^S consecutive elements of one column
 // and runs stages [0, S) on it as three rounds (bits 0-4, 5-9, 10..S-1); in a round each thread
@@ -549,6 +594,24 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
   const uint32_t k = inverse ? log_n : log_in;  // number of stages actually run
   const uint32_t inv_n = inverse ? 2 * m31_inv(1u << log_n) : 0;  // doubled, as the twiddles
   auto run_pass = [&](const FftPass& ps, const uint32_t* const* from, uint32_t mask, uint32_t scale) {
+    if (ps.c == ~0u) {  // a register pass (after the low stages: e >= 11, 2^(log_n - S) >= 256 lanes)
+      const dim3 grid(1u << (log_n - ps.S - 8), n_cols), block(256);
+#define PASS_REG(SS)                                                                                            \
+  case SS:                                                                                                      \
+    if (inverse)                                                                                                \
+      hipLaunchKernelGGL((fft_pass_reg_kernel<true, SS>), grid, block, 0, stream, from, dst, log_n, mask, ps.e, \
+                         tree, tree_len, scale, prio);                                                          \
+    else                                                                                                        \
+      hipLaunchKernelGGL((fft_pass_reg_kernel<false, SS>), grid, block, 0, stream, from, dst, log_n, mask,      \
+                         ps.e, tree, tree_len, scale, prio);                                                    \
+    break;
+      switch (ps.S) {
+        PASS_REG(1) PASS_REG(2) PASS_REG(3) PASS_REG(4) PASS_REG(5) PASS_REG(6)
+        default: throw std::invalid_argument("launch_fft: register pass of more than 6 stages");
+      }
+#undef PASS_REG
+      return;
+    }
     const uint32_t tiles = 1u << (log_n - ps.S - ps.c), tile_n = 1u << (ps.S + ps.c);
     const uint32_t threads = tile_n / 2 < 64 ? 64 : (tile_n / 2 > 256 ? 256 : tile_n / 2);
     if (inverse)
@@ -567,15 +630,16 @@ void launch_fft(bool inverse, const uint32_t* const* src, uint32_t* const* dst, 
     if (low_log != 0 && (low_log < 11 || low_log > smax)) throw std::invalid_argument("launch_fft: low_log out of range");
     const uint32_t S = low_log ? low_log : smax;
     if (loop && S != 15) throw std::invalid_argument("launch_fft: the looping low kernel needs low_log 15");
+    // the stages above the low tile: register passes of at most PASS_REG_MAX stages, as even as
+    // the count allows (8 stages: 4 + 4, not 6 + 2)
     std::vector<FftPass> high;
-    for (uint32_t e = S; e < k;) {
-      uint32_t st = k - e < 8 ? k - e : 8, c = 12 - st;
-      if (c > e) c = e;
-      // few columns (the composition's 4 or 8): narrower tiles, so that the pass still spreads
-      // over at least 2048 workgroups; 2^6 contiguous words keep every wave's access one segment
-      while (c > 6 && ((size_t)n_cols << (log_n - st - c)) < 2048) --c;
-      high.push_back({e, st, c});
-      e += st;
+    {
+      const uint32_t rest = k - S, n_pass = (rest + PASS_REG_MAX - 1) / PASS_REG_MAX;
+      for (uint32_t i = 0, e = S; i < n_pass; ++i) {
+        const uint32_t st = (rest - (e - S) + (n_pass - i) - 1) / (n_pass - i);
+        high.push_back({e, st, ~0u});
+        e += st;
+      }
     }
     auto low = [&](bool inv, const uint32_t* const* from, uint32_t mask, uint32_t scale) {
       if (inv) (loop ? launch_low_loop<true> : launch_low<true>)(S, n_cols, from, dst, log_n, mask, tree, tree_len, scale, prio, stream);
