@@ -239,7 +239,7 @@ __device__ __forceinline__ void poseidon_rounds_constraints(uint32_t* s, const C
 // last instance minus the previous row's last column, plus the claimed-sum shift).
 constexpr uint32_t PSD_MAX_WAVES = 8;
 template <class Cols>
-__global__ void __launch_bounds__(512) poseidon_comp_kernel(uint32_t eval_log, uint32_t trace_log, Cols trace,
+__global__ void __launch_bounds__(512, 4) poseidon_comp_kernel(uint32_t eval_log, uint32_t trace_log, Cols trace,
                                                             Cols inter, const PoseidonParams* __restrict__ params,
                                                             const qm31* __restrict__ pows,
                                                             const uint32_t* __restrict__ denom_inv, Cols4 acc_out,
