@@ -401,8 +401,9 @@ int stwo_hip_fold_line_dev(stwo_hip_ctx* ctx, uint32_t log_src, const uint32_t* 
 
 /* ---------------------------------------------------------------- graphs -------------- */
 /* No reference counterpart. Capture and replay of a fixed sequence of calls on one context (HIP
- * stream capture, thread-local mode): stwo_hip_graph_begin synchronises the context and starts
- * capturing its stream; until stwo_hip_graph_end, only these entry points run, launching without
+ * stream capture, thread-local mode: begin, the captured calls and end on one host thread):
+ * stwo_hip_graph_begin synchronises the context and starts capturing its stream; until
+ * stwo_hip_graph_end, only these entry points run, launching without
  * synchronising: the device channel calls above (read excepted), the two _dev folds,
  * stwo_hip_fold_circle_into_line, stwo_hip_fold_line with fold step 1 (both need their twiddle sizes
  * built by an earlier call), and stwo_hip_commit_on_layer with at most 16 columns or columns at a

@@ -1323,6 +1323,8 @@ int stwo_hip_graph_end(stwo_hip_ctx* h, stwo_hip_graph** out) {
   }
   // the capture ends whatever happens next
   c.capturing = false;
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur != c.device) (void)hipSetDevice(c.device);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c.stream, &g);
   if (e != hipSuccess || !g) {
