@@ -231,18 +231,19 @@ SHD constexpr F from_radix29(const M& a) {
 
 // a b 2^-261 (mod p) by product scanning: column k gathers a_i b_j (i + j = k) and the reduction
 // terms m_i p_j in 64-bit accumulators; for k < 9 the factor m_k = -acc (mod 2^29) clears the
-// column (its carry out is acc >> 29, plus one when the low limb was not zero), above that the
-// columns are the result's limbs. Inputs: limbs below 2^30 (a column then stays below 2^64);
-// output: limbs 0-7 below 2^29, value below a b / 2^261 + p (below 2 p for inputs below 32 p).
+// column (p's limb 0 is 1: acc + m_k is a multiple of 2^29, and its quotient is the carry out),
+// above that the columns are the result's limbs. Inputs: limbs below 2^30 (a column then stays
+// below 2^64); output: limbs 0-7 below 2^29, value below a b / 2^261 + p (below 2 p for inputs
+// below 32 p).
 SHD constexpr M mul29(const M& a, const M& b) {
   uint32_t m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   M r{};
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < 17; ++k) {
-    // two partial sums per column (products of even and odd i): independent chains of multiply-adds
-    // that do not wait for the previous column's carry
-    uint64_t e = 0, o = 0;
+    // two partial sums per column (products of even and odd i, the even one starting from the
+    // carry in): chains of multiply-adds of which only one waits for the previous column
+    uint64_t e = acc, o = 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       const int j = k - i;
@@ -255,11 +256,10 @@ SHD constexpr M mul29(const M& a, const M& b) {
     }
     if (k >= 6 && k - 6 < 9) o += (uint64_t)m[k - 6] * (17u << 18);
     if (k >= 8 && k - 8 < 9) e += (uint64_t)m[k - 8] << 19;
-    acc += e + o;
+    acc = e + o;
     if (k < 9) {
-      const uint32_t lo = (uint32_t)acc & MASK29;
-      m[k] = (0u - lo) & MASK29;
-      acc = (acc >> 29) + (lo != 0u ? 1u : 0u);
+      m[k] = (0u - (uint32_t)acc) & MASK29;
+      acc = (acc + m[k]) >> 29;
     } else {
       r.w[k - 9] = (uint32_t)acc & MASK29;
       acc >>= 29;
