@@ -406,7 +406,8 @@ def serve(args, m):
 
 def proof_forms(args, hb, m, cfg, trace, log_rows, fused_times):
     """The metric's second half: the plain per-call form (every device step a separate C-ABI trait
-    call that synchronises, as the reference's generic prove() drives a backend) against the
+    call, stream-ordered, with the host waiting wherever it reads device data, as the reference's
+    generic prove() drives a backend) against the
     batched form (the fused, device-resident prover on --contexts contexts proving concurrently
     from as many host threads). Both prove the same trace; proofs/s and p50/p99 per form."""
     import threading
@@ -571,7 +572,7 @@ def graph_extras(hb, first_log=17, last_log=1, reps=5):
     """The FRI commit loop of the headline shape (first layer 2^17, down to 2^1) driven through the
     C ABI's device-side calls only: each layer's Merkle levels (commit_on_layer), its root mixed and
     the folding coefficient drawn by the device channel, the fold with that coefficient from device
-    memory, then queries and the proof of work. Timed eager (one synchronising C-ABI call each) and
+    memory, then queries and the proof of work. Timed eager (one stream-ordered C-ABI call each) and
     as one captured graph replayed (stwo_hip_graph_*); the replay's transcript must equal the eager
     one's."""
     import numpy as np

@@ -212,14 +212,24 @@ int stwo_hip_copy_d2h(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_
 int stwo_hip_batch_at_multi(stwo_hip_ctx* h, const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows,
                             uint32_t n_rows, uint32_t* out_host) {
   return guarded(h, [&](Ctx& c) {
-    std::vector<const uint32_t*> bases(cols, cols + n_cols);
-    std::vector<uint32_t> bi, of;
-    for (uint32_t r = 0; r < n_rows; ++r)
-      for (uint32_t k = 0; k < n_cols; ++k) { bi.push_back(k); of.push_back(rows[r]); }
-    DevBuf o(&c, bi.size());
-    launch_gather_words((const uint32_t* const*)c.upload_vec(bases), c.upload_vec(bi), c.upload_vec(of),
-                        (uint32_t)bi.size(), o.p, c.stream);
-    c.d2h(out_host, o.p, bi.size() * 4);
+    const size_t n = (size_t)n_cols * n_rows;
+    if (!n) return;
+    if (!cols || !rows || !out_host) throw HipError(1, "null argument");
+    uint32_t* o = c.gather_out(n);
+    if (n_cols <= GATHER_ARG_COLS && n_rows <= GATHER_ARG_ROWS) {
+      // a decommitment's reads: a few rows of a layer's hashes or of a few columns
+      GatherArgs g{};
+      for (uint32_t k = 0; k < n_cols; ++k) g.cols[k] = cols[k];
+      for (uint32_t r = 0; r < n_rows; ++r) g.rows[r] = rows[r];
+      launch_gather_args(g, n_cols, n_rows, o, c.stream);
+    } else {
+      // the column table repeats between the queried and witness reads of a layer: kept in the
+      // table store; the rows go through the staging arena
+      const std::vector<const uint32_t*> bases(cols, cols + n_cols);
+      const std::vector<uint32_t> rv(rows, rows + n_rows);
+      launch_gather_rows(c.table_vec(bases), n_cols, c.upload_vec(rv), n_rows, o, c.stream);
+    }
+    c.d2h(out_host, o, n * 4);
     c.reset_arena();
   });
 }
@@ -256,8 +266,7 @@ int stwo_hip_interpolate_columns(stwo_hip_ctx* h, const uint32_t* const* evals, 
       ks.units(fft_butterflies(n_cols, log_size, log_size));
       run_fft(c, true, ds, dd, n_cols, log_size, log_size, c.itw2.p, c.tree_len(), canonic_half_init(log_size), low);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs, uint32_t* const* evals,
@@ -276,8 +285,7 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx* h, const uint32_t* const* coeffs
       ks.units(fft_butterflies(n_cols, log_n, log_size));
       run_fft(c, false, ds, dd, n_cols, log_size, log_n, c.tw2.p, c.tree_len(), canonic_half_init(log_n), low);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_batch_eval_at_point(stwo_hip_ctx* h, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
@@ -411,9 +419,7 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* h, uint32_t log_size, const uint32_t*
       else if (small) launch_merkle_layer_small(log_size, prev, v.data(), n_cols, out, c.stream);
       else launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
     }
-    if (c.capturing) return;
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   }, true, true);
 }
 
@@ -437,7 +443,7 @@ int stwo_hip_fold_circle_into_line(stwo_hip_ctx* h, uint32_t log_src, const uint
       launch_fold_circle_into_line(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
                                    q(alpha), c.itw.p, c.tree_len(), c.stream);
     }
-    if (!c.capturing) c.sync();
+    c.end_stream_ordered();
   }, true, true);
 }
 int stwo_hip_fold_circle_into_line_dev(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4],
@@ -452,7 +458,7 @@ int stwo_hip_fold_circle_into_line_dev(stwo_hip_ctx* h, uint32_t log_src, const 
                                        {{dst[0], dst[1], dst[2], dst[3]}}, (const qm31*)alpha, dst_is_zero != 0, c.itw.p,
                                        c.tree_len(), c.stream);
     }
-    if (!c.capturing) c.sync();
+    c.end_stream_ordered();
   }, true, true);
 }
 int stwo_hip_fold_line_dev(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
@@ -466,7 +472,7 @@ int stwo_hip_fold_line_dev(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* co
       launch_fold_line_dev(log_src, {{src[0], src[1], src[2], src[3]}}, {{dst[0], dst[1], dst[2], dst[3]}},
                            (const qm31*)alpha, 0, c.itw.p, c.tree_len(), c.stream);
     }
-    if (!c.capturing) c.sync();
+    c.end_stream_ordered();
   }, true, true);
 }
 int stwo_hip_fold_circle_into_line_grouped(stwo_hip_ctx* h, uint32_t log_src, uint32_t n_groups,
@@ -493,8 +499,7 @@ int stwo_hip_fold_circle_into_line_grouped(stwo_hip_ctx* h, uint32_t log_src, ui
                                      {{dst[4 * g], dst[4 * g + 1], dst[4 * g + 2], dst[4 * g + 3]}}, q(alpha), c.itw.p,
                                      c.tree_len(), c.stream);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const src[4], uint32_t* const dst[4],
@@ -530,7 +535,7 @@ int stwo_hip_fold_line(stwo_hip_ctx* h, uint32_t log_src, const uint32_t* const 
       from = {{to.c[0], to.c[1], to.c[2], to.c[3]}};
       a = qm_mul(a, a);
     }
-    if (!c.capturing) c.sync();
+    c.end_stream_ordered();
   }, true, true);
 }
 
@@ -571,8 +576,7 @@ int stwo_hip_accumulate_numerators(stwo_hip_ctx* h, uint32_t log_size, const uin
     const qm31* d_bs = c.stage_vec(Bs);
     uint32_t* const* d_out = (uint32_t* const*)c.upload_vec(ov);
     launch_accumulate_numerators(log_size, d_ents, d_starts, d_bs, n_batches, d_out, aligned, c.stream);
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_compute_quotients_and_combine(stwo_hip_ctx* h, const stwo_accumulated_numerators* accs, uint32_t n_accs,
@@ -597,8 +601,7 @@ int stwo_hip_compute_quotients_and_combine(stwo_hip_ctx* h, const stwo_accumulat
     }
     launch_quotients(lifting_log_size, c.domain_xy(lifting_log_size), c.upload_vec(pts), n_accs, c.upload_vec(parts),
                      {{out[0], out[1], out[2], out[3]}}, c.stream);
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 
@@ -606,7 +609,7 @@ int stwo_hip_accumulate(stwo_hip_ctx* h, uint32_t log_size, uint32_t* const dst[
   return guarded(h, [&](Ctx& c) {
     launch_accumulate_lifted(log_size, {{dst[0], dst[1], dst[2], dst[3]}}, log_size, {{src[0], src[1], src[2], src[3]}},
                              c.stream);
-    c.sync();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_lift_and_accumulate(stwo_hip_ctx* h, const uint32_t* const* cols4, const uint32_t* log_sizes,
@@ -622,7 +625,7 @@ int stwo_hip_lift_and_accumulate(stwo_hip_ctx* h, const uint32_t* const* cols4, 
     for (uint32_t i = 0; i + 1 < n_cols; ++i)
       launch_accumulate_lifted(L, o, log_sizes[i], {{cols4[4 * i], cols4[4 * i + 1], cols4[4 * i + 2], cols4[4 * i + 3]}},
                                c.stream);
-    c.sync();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_generate_secure_powers(stwo_hip_ctx* h, stwo_qm31 felt, uint32_t n, stwo_qm31* out) {
@@ -655,8 +658,7 @@ int stwo_hip_commit_on_layer_poseidon252(stwo_hip_ctx* h, uint32_t log_size, con
       ks.units(perms * (double)(1u << log_size));
       launch_p252_layer(log_size, prev, d, n_cols, out, c.stream);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_build_leaves_lifted(stwo_hip_ctx* h, uint32_t hasher, const uint32_t* const* cols,
@@ -683,8 +685,7 @@ int stwo_hip_build_leaves_lifted(stwo_hip_ctx* h, uint32_t hasher, const uint32_
         ks.units((std::floor((double)((n_cols + 7) / 8) / 2) + 1) * (double)(n_cols ? 1u << lifting_log_size : 1u));
       launch_lifted_leaves(hasher, lifting_log_size, d, n_cols, out, c.stream);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_build_next_layer_lifted(stwo_hip_ctx* h, uint32_t hasher, const uint32_t* prev, uint32_t log_size,
@@ -702,7 +703,7 @@ int stwo_hip_build_next_layer_lifted(stwo_hip_ctx* h, uint32_t hasher, const uin
         launch_merkle_layer_small(log_size, prev, nullptr, 0, out, c.stream);
       }
     }
-    c.sync();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_grind_poseidon252(stwo_hip_ctx* h, const uint32_t digest[8], uint32_t pow_bits, uint64_t* nonce) {
@@ -799,8 +800,7 @@ int stwo_hip_evaluate_constraint_quotients_on_domain(stwo_hip_ctx* h, uint32_t* 
     } else {
       throw HipError(5, "no device kernel for this component");
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 
@@ -1077,7 +1077,7 @@ int stwo_hip_bit_reverse_column(stwo_hip_ctx* h, uint32_t* col, uint32_t log_siz
     if (!col) throw HipError(1, "null column");
     if (log_size > 31) throw HipError(3, "log_size out of range");
     launch_bit_reverse(col, log_size, c.stream);
-    c.sync();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_column_at(stwo_hip_ctx* h, const uint32_t* col, size_t index, uint32_t* out) {
@@ -1112,7 +1112,7 @@ int stwo_hip_extend(stwo_hip_ctx* h, const uint32_t* coeffs, uint32_t log_size, 
     const size_t n = (size_t)1 << log_size, m = (size_t)1 << new_log_size;
     if (out != coeffs) launch_copy_words(out, coeffs, n, c.stream);
     if (m > n) launch_fill_zero(out + n, m - n, c.stream);
-    c.sync();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_twiddles_create(stwo_hip_ctx* h, uint32_t coset_initial_index, uint32_t coset_log_size,
@@ -1159,8 +1159,7 @@ int stwo_hip_interpolate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, 
       run_fft(c, true, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_size,
               log_size, t->itw2.p, 1u << t->log_size, half_coset_initial_index, low);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_evaluate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uint32_t half_coset_initial_index,
@@ -1180,8 +1179,7 @@ int stwo_hip_evaluate_on_domain(stwo_hip_ctx* h, const stwo_hip_twiddles* t, uin
       run_fft(c, false, (const uint32_t* const*)c.upload_vec(s), (uint32_t* const*)c.upload_vec(d), n_cols, log_coeffs,
               log_size, t->tw2.p, 1u << t->log_size, half_coset_initial_index, low);
     }
-    c.sync();
-    c.reset_arena();
+    c.end_stream_ordered();
   });
 }
 int stwo_hip_eval_at_point_by_folding(stwo_hip_ctx* h, const uint32_t* evals, uint32_t log_size,

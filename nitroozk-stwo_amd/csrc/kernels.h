@@ -170,6 +170,17 @@ void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, ui
 
 void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx, const uint32_t* offsets, uint32_t n,
                          uint32_t* out, hipStream_t stream);
+// Column::batch_at_multi: out[r * n_cols + c] = cols[c][rows[r]]. A gather of at most
+// GATHER_ARG_COLS columns and GATHER_ARG_ROWS rows takes both lists as kernel arguments (no
+// transfer before the launch); a larger one reads them from device tables.
+constexpr uint32_t GATHER_ARG_COLS = 16, GATHER_ARG_ROWS = 48;
+struct GatherArgs {
+  const uint32_t* cols[GATHER_ARG_COLS];
+  uint32_t rows[GATHER_ARG_ROWS];
+};
+void launch_gather_args(const GatherArgs& g, uint32_t n_cols, uint32_t n_rows, uint32_t* out, hipStream_t stream);
+void launch_gather_rows(const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows, uint32_t n_rows,
+                        uint32_t* out, hipStream_t stream);
 
 // Decommitment reads for a set of query positions, in one launch. A Merkle tree (or FRI layer)
 // decommitment only touches, at each level, the block of 2^w nodes that holds the query's path

@@ -690,6 +690,31 @@ void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx,
                      out);
 }
 
+__global__ void gather_args_kernel(GatherArgs g, uint32_t n_cols, uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = g.cols[i % n_cols][g.rows[i / n_cols]];
+}
+__global__ void gather_rows_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                   const uint32_t* __restrict__ rows, uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cols[i % n_cols][rows[i / n_cols]];
+}
+void launch_gather_args(const GatherArgs& g, uint32_t n_cols, uint32_t n_rows, uint32_t* out, hipStream_t stream) {
+  if (n_cols > GATHER_ARG_COLS || n_rows > GATHER_ARG_ROWS)
+    throw std::invalid_argument("launch_gather_args: more columns or rows than the argument block holds");
+  const uint32_t n = n_cols * n_rows;
+  if (!n) return;
+  hipLaunchKernelGGL(gather_args_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, g, n_cols, n, out);
+}
+void launch_gather_rows(const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows, uint32_t n_rows,
+                        uint32_t* out, hipStream_t stream) {
+  const uint64_t n = (uint64_t)n_cols * n_rows;
+  if (!n) return;
+  if (n > 0xFFFFFFFFull - 255) throw std::invalid_argument("launch_gather_rows: gather too large");
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, cols, n_cols, rows,
+                     (uint32_t)n, out);
+}
+
 struct DecQueries { uint32_t first, n, p[DEC_MAX_QUERIES]; };
 
 __device__ __forceinline__ uint32_t dec_pos(uint32_t mode, uint32_t a, uint32_t p) {

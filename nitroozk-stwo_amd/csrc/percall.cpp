@@ -11,8 +11,10 @@
 // This is the host side a Rust HipBackend would run above the C ABI, restated in C++ because the
 // image has no Rust toolchain. Every device step is a stwo_hip_* call from include/stwo_hip.h;
 // nothing here launches a kernel or touches the context's internals. It is the "plain per-call"
-// form of the prover: each call synchronises, columns are allocated per call, and the channel
-// waits on every root. The device-resident prover (prover.cpp) is the fused form of the same
+// form of the prover: calls are stream-ordered on one stream, as a backend's trait calls are on
+// the reference's CUDA stream, and the host waits wherever it reads device data (every Merkle root
+// for the channel, the sampled values, the decommitment's values, the nonce); columns are allocated
+// per call. The device-resident prover (prover.cpp) is the fused form of the same
 // protocol; both produce the same bytes. Like the reference's prove(), it is generic over the
 // MerkleChannel: Blake2s, or Poseidon252 (felt252 hashes through commit_on_layer_poseidon252 and
 // grind_poseidon252, the felt252 channel of p252_channel.cpp).

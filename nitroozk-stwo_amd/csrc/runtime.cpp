@@ -85,6 +85,7 @@ Ctx::~Ctx() {
   itw2.reset();
   domain_points.clear();
   fri_tickets.reset();
+  gather_buf.reset();
   for (auto& [bytes, v] : col_cache)
     for (void* p : v) (void)hipFreeAsync(p, stream);
   for (auto& [p, bytes] : col_live) (void)hipFreeAsync(p, stream);
@@ -288,6 +289,11 @@ void Ctx::collect_timing() {
   }
   pending.clear();
 }
+void Ctx::end_stream_ordered() {
+  if (capturing) return;
+  if (arena_off > arena_cap / 2 || pinned_off > pinned_cap / 2) sync();
+  reset_arena();
+}
 void Ctx::reset_arena() {
   // staging still read by queued copies (no sync since the last upload) is kept until a later call
   // (and, while forked work may still read the arena, until a full sync)
@@ -365,6 +371,14 @@ uint32_t* Ctx::fri_ticket_words(size_t n) {
     fri_tickets = DevBuf(this, n < 64 ? 64 : n, true);
   }
   return fri_tickets.p;
+}
+
+uint32_t* Ctx::gather_out(size_t n) {
+  if (gather_buf.n < n) {
+    PersistentAllocScope keep(*this);
+    gather_buf = DevBuf(this, n < ((size_t)1 << 14) ? ((size_t)1 << 14) : n);
+  }
+  return gather_buf.p;
 }
 
 const uint32_t* Ctx::domain_xy(uint32_t log) {

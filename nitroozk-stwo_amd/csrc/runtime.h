@@ -116,6 +116,9 @@ struct Ctx {
   // launch leaves its counter at zero again.
   DevBuf fri_tickets;
   uint32_t* fri_ticket_words(size_t n);
+  // Device output of Column::batch_at_multi gathers, reused call after call (grown on demand).
+  DevBuf gather_buf;
+  uint32_t* gather_out(size_t n);
   // small-upload arena (device) and pinned staging (host)
   uint8_t* pinned = nullptr;
   size_t pinned_cap = 0, pinned_off = 0;
@@ -160,6 +163,10 @@ struct Ctx {
   void sync();       // both streams
   void sync_main();  // the main stream only
   void reset_arena();  // rewinds the staging arena; a no-op while uploads are queued
+  // The end of a stream-ordered entry point: nothing waits for the device, except that staging
+  // areas more than half full are drained (a sync) before the arena is rewound. A no-op while
+  // capturing.
+  void end_stream_ordered();
   void ensure_twiddles(uint32_t log_domain);
   // Circle-FFT launch plans (fft_plan.cpp): the low-stage tile (2^S values per workgroup) for a
   // transform shape, as launch_fft's low_log; 0 for transforms of fewer than 11 stages. background:

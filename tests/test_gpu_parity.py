@@ -190,6 +190,21 @@ def test_batch_at_multi(hb):
     assert list(got) == exp
 
 
+def test_batch_at_multi_both_paths(hb):
+    """Gathers of at most 16 columns and 48 rows pass their lists as kernel arguments, larger ones
+    as device tables; both write into one reused output buffer, grown on demand."""
+    rng = np.random.default_rng(4)
+    host = rand_cols(rng, 40, 10)
+    dev = [hb.from_cpu(c) for c in host]
+    shapes = [(16, 48), (17, 48), (16, 49), (1, 1), (40, 700), (3, 2), (40, 5), (40, 5), (16, 48)]
+    for n_cols, n_rows in shapes:
+        pick = sorted(rng.choice(40, n_cols, replace=False).tolist())
+        rows = [int(r) for r in rng.integers(0, 1 << 10, n_rows)]
+        got = hb.batch_at_multi([dev[c] for c in pick], rows)
+        exp = [int(host[c][r]) for r in rows for c in pick]
+        assert [int(v) for v in got] == exp, (n_cols, n_rows)
+
+
 def test_trace_generation_matches_oracle(hb):
     say("trace gen")
     for mode in (0, 1):
