@@ -37,14 +37,18 @@ BUTTERFLIES = ("T butterflies/s", 1e12, "butterflies_per_step",
 PERMUTATIONS = ("M permutations/s", 1e6, "permutations_per_step",
                 "Starknet Poseidon permutations on register-resident states, measured live "
                 "(stwo_hip_measure_permutation_rate)")
+COMPRESSIONS = ("M compressions/s", 1e6, "compressions_per_step",
+                "Blake2s compressions on register-resident states and blocks with the Merkle kernels' own "
+                "code, measured live (stwo_hip_measure_compression_rate)")
 
 
 def roofline_entry(dom, by_per_step, ms_per_step, traffic, units_per_step, unit_peak, kind=BUTTERFLIES):
     """The roofline object of a bench line for the dominant kernel class: its HBM figures, and for
     the transform classes its butterflies per second against the butterfly ceiling measured live on
     the same device (stwo_hip_measure_butterfly_rate: the transform kernels' own butterfly code on
-    register-resident values), for the Poseidon252 Merkle layers its permutations per second
-    against the permutation ceiling. `bound` names the resource the class is closer to saturating."""
+    register-resident values), for the Merkle layers their Blake2s compressions (or Poseidon
+    permutations) per second against the compression (or permutation) ceiling. `bound` names the
+    resource the class is closer to saturating."""
     hbm_ach = by_per_step / (ms_per_step / 1e3) / 1e9
     hbm = {"achieved": round(hbm_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 4),
            "traffic": traffic, "traffic_unit": "HBM bytes per step over the class's launches" if traffic else None,
@@ -131,6 +135,7 @@ def kernel_batch(args, m):
     for _ in range(max(0, args.warmup - 1)):
         step()
     bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
+    comp_peak = hb.measure_compression_rate()
     hb.reset_kernel_stats()
     hb.set_kernel_timing(True)
     device_sync(hb)
@@ -153,8 +158,13 @@ def kernel_batch(args, m):
         tot_ms += ms
         stages[label] = {"ms": round(ms, 3), "GB": round(by / 1e9, 3), "GB_per_s": round(by / (ms / 1e3) / 1e9, 1) if ms else None}
         if cls in units and ms:
-            stages[label]["T_butterflies_per_s"] = round(units[cls] / args.steps / (ms / 1e3) / 1e12, 3)
-            stages[label]["butterfly_ceiling_frac"] = round(units[cls] / args.steps / (ms / 1e3) / bfly_peak[cls], 4)
+            rate = units[cls] / args.steps / (ms / 1e3)
+            if cls == "merkle":
+                stages[label]["M_compressions_per_s"] = round(rate / 1e6, 1)
+                stages[label]["compression_ceiling_frac"] = round(rate / comp_peak, 4) if comp_peak else None
+            else:
+                stages[label]["T_butterflies_per_s"] = round(rate / 1e12, 3)
+                stages[label]["butterfly_ceiling_frac"] = round(rate / bfly_peak[cls], 4)
     dom_label = max(stages, key=lambda s: stages[s]["ms"])
     dom = {v: k for k, v in names.items()}[dom_label]
     dms, dby, _ = stats[dom]
@@ -168,7 +178,9 @@ def kernel_batch(args, m):
                    "fold_columns": n_cols // 4},
         "stages": stages, "parity_spot_check": parity,
         "roofline": roofline_entry(dom, dby / args.steps, dms / args.steps, None, units.get(dom, 0.0) / args.steps,
-                                   bfly_peak.get(dom)),
+                                   comp_peak if dom == "merkle" else bfly_peak.get(dom),
+                                   COMPRESSIONS if dom == "merkle" else BUTTERFLIES),
+        "blake2s_M_compressions_per_s_ceiling": round(comp_peak / 1e6, 1) if comp_peak else None,
     }
     print(json.dumps(out), flush=True)
     hb.close()
@@ -206,6 +218,7 @@ def mixed(args, m):
     cells = sum(nc << lg for lg, nc in zip(logs, ncols))
     # the same proofs with kernel events: per-class time, and the roofline of the dominant class
     bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
+    comp_peak = hb.measure_compression_rate()
     hb.reset_kernel_stats()
     hb.set_kernel_timing(True)
     for _ in range(args.steps):
@@ -218,6 +231,9 @@ def mixed(args, m):
     for k, u in units.items():
         if k in kernels and stats[k][0] > 0 and bfly_peak.get(k):
             kernels[k]["butterfly_ceiling_frac"] = round(u / (stats[k][0] / 1e3) / bfly_peak[k], 4)
+        if k == "merkle" and k in kernels and stats[k][0] > 0:
+            kernels[k]["M_compressions_per_s"] = round(u / (stats[k][0] / 1e3) / 1e6, 1)
+            kernels[k]["compression_ceiling_frac"] = round(u / (stats[k][0] / 1e3) / comp_peak, 4) if comp_peak else None
     dom = max((k for k in stats if k != "grind"), key=lambda k: stats[k][0])
     out = {
         "metric": "prove ms (%d mixed wide-Fibonacci components)" % args.components, "value": round(ms, 3), "unit": "ms",
@@ -229,7 +245,9 @@ def mixed(args, m):
         "p50_ms": round(statistics.median(times), 3), "proof_status": pr.status, "proof_bytes": len(pr.bytes),
         "trace_cells_per_s": round(cells / (ms / 1e3), 1),
         "roofline": roofline_entry(dom, stats[dom][1] / args.steps, stats[dom][0] / args.steps, None,
-                                   units.get(dom, 0.0) / args.steps, bfly_peak.get(dom)),
+                                   units.get(dom, 0.0) / args.steps,
+                                   comp_peak if dom == "merkle" else bfly_peak.get(dom),
+                                   COMPRESSIONS if dom == "merkle" else BUTTERFLIES),
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
@@ -733,6 +751,7 @@ def main():
     valu_peak = hb.measure_int32_issue()
     bfly_peak = {"evaluate": hb.measure_butterfly_rate(False), "interpolate": hb.measure_butterfly_rate(True)}
     perm_peak = hb.measure_permutation_rate() if p252 else None
+    comp_peak = None if p252 else hb.measure_compression_rate()
     for _ in range(args.warmup):
         prove()
 
@@ -774,9 +793,13 @@ def main():
     for k, u in units.items():
         if k in kernels and stats[k][0] > 0:
             rate = u / (stats[k][0] / 1e3)
-            if k == "merkle":
+            if k == "merkle" and p252:
                 kernels[k]["M_permutations_per_s"] = round(rate / 1e6, 1)
                 kernels[k]["permutation_ceiling_frac"] = round(rate / perm_peak, 4) if perm_peak else None
+                continue
+            if k == "merkle":
+                kernels[k]["M_compressions_per_s"] = round(rate / 1e6, 1)
+                kernels[k]["compression_ceiling_frac"] = round(rate / comp_peak, 4) if comp_peak else None
                 continue
             kernels[k]["T_butterflies_per_s"] = round(rate / 1e12, 3)
             kernels[k]["butterfly_ceiling_frac"] = round(rate / bfly_peak[k], 4) if bfly_peak.get(k) else None
@@ -840,12 +863,13 @@ def main():
         "byte_identical_to_oracle": parity,
         "stage_ms": stage,
         "roofline": roofline_entry(dom, by / args.steps, ms / args.steps, traffic, units.get(dom, 0.0) / args.steps,
-                                   perm_peak if dom == "merkle" else bfly_peak.get(dom),
-                                   PERMUTATIONS if dom == "merkle" else BUTTERFLIES),
+                                   (perm_peak if p252 else comp_peak) if dom == "merkle" else bfly_peak.get(dom),
+                                   (PERMUTATIONS if p252 else COMPRESSIONS) if dom == "merkle" else BUTTERFLIES),
         "valu_ceilings": {"int32_add_issue_T_lane_instr_per_s": round(valu_peak / 1e12, 2),
                           "forward_butterflies_T_per_s": round(bfly_peak["evaluate"] / 1e12, 3),
                           "inverse_butterflies_T_per_s": round(bfly_peak["interpolate"] / 1e12, 3),
-                          "M_permutations_per_s": round(perm_peak / 1e6, 1) if perm_peak else None},
+                          "M_permutations_per_s": round(perm_peak / 1e6, 1) if perm_peak else None,
+                          "blake2s_M_compressions_per_s": round(comp_peak / 1e6, 1) if comp_peak else None},
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -455,11 +455,18 @@ int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* ctx, int inverse, double* butt
  * stwo_hip_commit_on_layer_poseidon252, whose permutations per second (stwo_hip_kernel_units of the
  * merkle class over its event time) are reported against it. */
 int stwo_hip_measure_permutation_rate(stwo_hip_ctx* ctx, double* permutations_per_s);
+/* No reference counterpart. Blake2s compressions per second on register-resident states and message
+ * blocks with the Merkle kernels' own one-lane compression code: the arithmetic ceiling of
+ * stwo_hip_commit_on_layer and of the device prover's Blake2s Merkle layers, whose compressions per
+ * second (stwo_hip_kernel_units of the merkle class over its event time) are reported against it. */
+int stwo_hip_measure_compression_rate(stwo_hip_ctx* ctx, double* compressions_per_s);
 int stwo_hip_kernel_stats(stwo_hip_ctx* ctx, uint32_t kernel_class, double* ms, double* bytes, uint64_t* launches);
 /* No reference counterpart. Algorithmic work units of a kernel class since the last reset: circle-FFT
  * butterflies for interpolate (n_cols 2^(log-1) log) and evaluate (n_cols 2^(log-1) k, k the
- * coefficients' log size); Poseidon permutations for merkle launches of
- * stwo_hip_commit_on_layer_poseidon252; 0 for the other classes. */
+ * coefficients' log size); for merkle launches, Blake2s compressions (stwo_hip_commit_on_layer, the
+ * lifted Blake2s layers and the device prover's trees: one per 64-byte block of a node's message,
+ * one for an empty message) or Poseidon permutations (stwo_hip_commit_on_layer_poseidon252); 0 for
+ * the other classes. */
 int stwo_hip_kernel_units(stwo_hip_ctx* ctx, uint32_t kernel_class, double* units);
 int stwo_hip_reset_kernel_stats(stwo_hip_ctx* ctx);
 

@@ -415,6 +415,7 @@ int stwo_hip_commit_on_layer(stwo_hip_ctx* h, uint32_t log_size, const uint32_t*
     const uint32_t* const* d = !strided && !small && n_cols ? (const uint32_t* const*)c.upload_vec(v) : nullptr;
     {
       KScope ks(c, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * n_cols + 32.0) * (double)(1u << log_size));
+      ks.units(blake2s_node_compressions(prev != nullptr, n_cols) * (double)(1u << log_size));
       if (strided) launch_merkle_layer_strided(log_size, prev, v[0], (size_t)1 << log_size, n_cols, out, c.stream);
       else if (small) launch_merkle_layer_small(log_size, prev, v.data(), n_cols, out, c.stream);
       else launch_merkle_layer(log_size, prev, d, n_cols, out, c.stream);
@@ -683,6 +684,8 @@ int stwo_hip_build_leaves_lifted(stwo_hip_ctx* h, uint32_t hasher, const uint32_
       KScope ks(c, K_MERKLE, bytes);
       if (hasher == LIFTED_POSEIDON252)
         ks.units((std::floor((double)((n_cols + 7) / 8) / 2) + 1) * (double)(n_cols ? 1u << lifting_log_size : 1u));
+      else
+        ks.units(blake2s_node_compressions(false, n_cols) * (double)(n_cols ? 1u << lifting_log_size : 1u));
       launch_lifted_leaves(hasher, lifting_log_size, d, n_cols, out, c.stream);
     }
     c.end_stream_ordered();
@@ -700,6 +703,7 @@ int stwo_hip_build_next_layer_lifted(stwo_hip_ctx* h, uint32_t hasher, const uin
         ks.units((double)(1u << log_size));
         launch_p252_layer(log_size, prev, nullptr, 0, out, c.stream);
       } else {
+        ks.units((double)(1u << log_size));
         launch_merkle_layer_small(log_size, prev, nullptr, 0, out, c.stream);
       }
     }
@@ -1384,6 +1388,17 @@ int stwo_hip_measure_butterfly_rate(stwo_hip_ctx* h, int inverse, double* butter
     DevBuf scratch(&c, (size_t)cus * 16);
     c.sync();
     *butterflies_per_s = measure_butterfly_rate(inverse != 0, scratch.p, c.stream);
+  });
+}
+
+int stwo_hip_measure_compression_rate(stwo_hip_ctx* h, double* compressions_per_s) {
+  return guarded(h, [&](Ctx& c) {
+    if (!compressions_per_s) throw HipError(1, "null out");
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    DevBuf scratch(&c, (size_t)cus * 16);
+    c.sync();
+    *compressions_per_s = measure_compression_rate(scratch.p, c.stream);
   });
 }
 

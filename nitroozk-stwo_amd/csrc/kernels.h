@@ -275,6 +275,9 @@ void launch_fft_tiny(bool inverse, const uint32_t* const* src, uint32_t* const* 
 double measure_int32_issue(uint32_t* scratch, hipStream_t stream);
 // Forward (or inverse) FFT butterflies per second on register-resident data (butterfly.h).
 double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t stream);
+// Blake2s compressions per second with register-resident states and messages, the Merkle kernels'
+// one-lane compression code: the ceiling of the Blake2s Merkle layers. scratch: CUs x 16 words.
+double measure_compression_rate(uint32_t* scratch, hipStream_t stream);
 // The Blake2s channel on the device (channel_dev.hip): state of CHANNEL_WORDS words (digest 0..7,
 // n_draws 8, the grind's u64 result slot 10..11, the proof-of-work prefix 12..19). Single-lane
 // launches; launch_chan_grind covers the first `count` candidates of the nonce order.

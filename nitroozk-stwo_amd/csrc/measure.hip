@@ -10,7 +10,13 @@
 // twiddles held in registers: the rate of the transform's arithmetic with no memory traffic, the
 // ceiling of the FFT kernels' butterflies per second.
 //
-// Both launch 16 workgroups of 256 threads per CU, enough waves to fill every SIMD.
+// compression_kernel: every lane holds a Blake2s chaining state and a 64-byte message block in
+// registers and compresses with the Merkle kernels' own one-lane code (blake2s.h), each result
+// folded into the next block: Blake2s compressions per second with no memory traffic, the ceiling
+// of the Merkle layers' hashing.
+//
+// All launch 16 workgroups of 256 threads per CU, enough waves to fill every SIMD.
+#include "blake2s.h"
 #include "butterfly.h"
 #include "field.h"
 #include "kernels.h"
@@ -111,6 +117,43 @@ double measure_butterfly_rate(bool inverse, uint32_t* scratch, hipStream_t strea
   (void)hipEventDestroy(e1);
   const double butterflies = (double)blocks * threads * iters * 80.0;
   return ms > 0 ? butterflies / (ms * 1e-3) : 0.0;
+}
+
+__global__ void __launch_bounds__(256) compression_kernel(uint32_t iters, uint32_t seed, uint32_t* out) {
+  uint32_t h[8], m[16];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) h[k] = seed * (threadIdx.x + 1) + 0x9E3779B9u * k;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) m[k] = (seed ^ blockIdx.x) + 0x85EBCA6Bu * k + threadIdx.x;
+  for (uint32_t it = 0; it < iters; ++it) {
+    blake2s_compress(h, m, 64, true);
+    m[it & 15] ^= h[it & 7];  // the next block depends on this one's result
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s ^= h[k];
+  if (s == 0x12345678u) out[blockIdx.x] = s;  // keeps the state live
+}
+
+double measure_compression_rate(uint32_t* scratch, hipStream_t stream) {
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const uint32_t blocks = (uint32_t)cus * 16, threads = 256, iters = 128;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(compression_kernel, dim3(blocks), dim3(threads), 0, stream, 4u, 3u, scratch);  // warm-up
+  (void)hipEventRecord(e0, stream);
+  hipLaunchKernelGGL(compression_kernel, dim3(blocks), dim3(threads), 0, stream, iters, 5u, scratch);
+  (void)hipEventRecord(e1, stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double compressions = (double)blocks * threads * iters;
+  return ms > 0 ? compressions / (ms * 1e-3) : 0.0;
 }
 
 }  // namespace sh

@@ -326,6 +326,7 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
       // workgroup, which pays on narrow levels only; on 2^22 nodes it ran at a quarter of this)
       DevBuf layer(&ctx, (size_t)8 << ls);
       KScope ks(ctx, K_MERKLE, 96.0 * (double)(1u << ls));
+      ks.units((double)(1u << ls));
       launch_merkle_layer_small((uint32_t)ls, rev.back().p, nullptr, 0, layer.p, ctx.stream);
       rev.push_back(std::move(layer));
       --ls;
@@ -343,6 +344,7 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
       }
       const uint32_t* child = rev[rev.size() - levels - 1].p;
       KScope ks(ctx, K_MERKLE, 96.0 * (double)(1u << (log_child - 1)) * 2.0);
+      ks.units((double)((1u << log_child) - (1u << (log_child - levels))));  // one per node of the levels
       launch_merkle_subtree(log_child, child, levels, outs.data(), ctx.stream);
       ls -= (int)levels;
       continue;
@@ -351,15 +353,19 @@ void merkle_commit(Ctx& ctx, DevTree& t, bool read_root) {
     while (ci < order.size() && t.eval_logs[order[ci]] == (uint32_t)ls) lc.push_back(t.evals[order[ci++]]);
     DevBuf layer(&ctx, (size_t)8 << ls);
     const uint32_t* prev = rev.empty() ? nullptr : rev.back().p;
+    const double comps = blake2s_node_compressions(prev != nullptr, lc.size()) * (double)(1u << ls);
     if (lc.size() <= 16) {
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
+      ks.units(comps);
       launch_merkle_layer_small((uint32_t)ls, prev, lc.data(), (uint32_t)lc.size(), layer.p, ctx.stream);
     } else if (is_strided(lc, (size_t)1 << ls)) {
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
+      ks.units(comps);
       launch_merkle_layer_strided((uint32_t)ls, prev, lc[0], (size_t)1 << ls, (uint32_t)lc.size(), layer.p, ctx.stream);
     } else {
       const uint32_t* const* cp = (const uint32_t* const*)ctx.table_vec(lc);
       KScope ks(ctx, K_MERKLE, ((prev ? 64.0 : 0.0) + 4.0 * lc.size() + 32.0) * (double)(1u << ls));
+      ks.units(comps);
       launch_merkle_layer((uint32_t)ls, prev, cp, (uint32_t)lc.size(), layer.p, ctx.stream);
     }
     rev.push_back(std::move(layer));

@@ -233,6 +233,12 @@ struct KScope {
     }
   }
 };
+// Blake2s compressions of one Merkle node over its two children (when prev) and n_cols column words:
+// one per 64-byte block of the message, one for an empty message.
+inline double blake2s_node_compressions(bool prev, size_t n_cols) {
+  const size_t bytes = (prev ? 64 : 0) + 4 * n_cols;
+  return bytes == 0 ? 1.0 : (double)((bytes + 63) / 64);
+}
 // Butterflies of a circle FFT over n_cols columns of 2^log_n values that runs `stages` stages
 // (interpolation: all log_n; evaluation from 2^k coefficients: k, the top stages only copy).
 inline double fft_butterflies(size_t n_cols, uint32_t log_n, uint32_t stages) {

@@ -93,6 +93,7 @@ EXPORTS = [
     "stwo_hip_commit_on_layer_poseidon252", "stwo_hip_grind_poseidon252", "stwo_poseidon252_hash_many",
     "stwo_poseidon252_permute", "stwo_hip_prove_wide_fibonacci_poseidon252", "stwo_hip_prove_poseidon_poseidon252",
     "stwo_verify_wide_fibonacci_poseidon252", "stwo_verify_poseidon_poseidon252", "stwo_hip_measure_permutation_rate",
+    "stwo_hip_measure_compression_rate",
     "stwo_hip_build_leaves_lifted", "stwo_hip_build_next_layer_lifted",
     "stwo_hip_channel_init", "stwo_hip_channel_mix_root", "stwo_hip_channel_mix_u32s",
     "stwo_hip_channel_draw_secure_felts", "stwo_hip_channel_draw_queries", "stwo_hip_channel_grind",
@@ -207,6 +208,7 @@ def load_library(path=LIB_PATH):
     L.stwo_hip_measure_int32_issue.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
     L.stwo_hip_measure_butterfly_rate.argtypes = [c.c_void_p, c.c_int, c.POINTER(c.c_double)]
     L.stwo_hip_measure_permutation_rate.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
+    L.stwo_hip_measure_compression_rate.argtypes = [c.c_void_p, c.POINTER(c.c_double)]
     L.stwo_hip_build_leaves_lifted.argtypes = [c.c_void_p, c.c_uint32, _VPP, _U32P, c.c_uint32, c.c_uint32,
                                                c.c_void_p]
     L.stwo_hip_build_next_layer_lifted.argtypes = [c.c_void_p, c.c_uint32, c.c_void_p, c.c_uint32, c.c_void_p]
@@ -731,6 +733,13 @@ class HipBackend:
         self._check(self.lib.stwo_hip_measure_butterfly_rate(self.ctx, int(bool(inverse)), ctypes.byref(v)))
         return v.value
 
+    def measure_compression_rate(self):
+        """Blake2s compressions per second with register-resident states and blocks: the ceiling of
+        the Blake2s Merkle layers' hashing on this device."""
+        v = ctypes.c_double()
+        self._check(self.lib.stwo_hip_measure_compression_rate(self.ctx, ctypes.byref(v)))
+        return v.value
+
     def measure_permutation_rate(self):
         """Starknet Poseidon permutations per second with register-resident states: the ceiling of
         the Poseidon252 Merkle layers on this device."""
@@ -743,7 +752,7 @@ class HipBackend:
 
     def kernel_units(self):
         """{class: algorithmic work units} for the classes that count them (butterflies for
-        interpolate and evaluate)."""
+        interpolate and evaluate; Blake2s compressions or Poseidon permutations for merkle)."""
         out = {}
         for i, name in enumerate(KERNEL_CLASSES):
             u = ctypes.c_double()

@@ -306,3 +306,22 @@ def test_poseidon_unsatisfied_proof_is_an_error_by_default(hb):
     assert e.value.status == 6
     pr = hb.prove_poseidon(trace, 1, 5, cfg_of(WF), allow_unsatisfied=True)
     assert pr.status == 2 and len(pr.bytes) > 0
+
+
+def test_compression_rate_and_merkle_units(hb):
+    """The Blake2s compression ceiling is plausible (a compression is about a thousand VALU
+    instructions, the device issues tens of trillions per second), and the merkle class counts one
+    compression per 64-byte block of a node's message, one for an empty message."""
+    rate = hb.measure_compression_rate()
+    say(f"compressions per second: {rate:.3e}")
+    assert 1e9 < rate < 1e12
+    hb.reset_kernel_stats()
+    hb.set_kernel_timing(True)
+    rng = np.random.default_rng(5)
+    cols = [hb.from_cpu(rng.integers(0, P, 1 << 6, dtype=np.uint32)) for _ in range(19)]
+    leaves = hb.commit_on_layer(6, None, cols)  # 76 bytes: 2 compressions per node
+    mid = hb.commit_on_layer(5, leaves, cols[:5])  # 64 + 20 bytes: 2 per node
+    hb.commit_on_layer(4, mid, [])  # children only: 1 per node
+    hb.commit_on_layer(0, None, [])  # the empty message: 1
+    hb.set_kernel_timing(False)
+    assert hb.kernel_units()["merkle"] == 2 * 64 + 2 * 32 + 16 + 1
