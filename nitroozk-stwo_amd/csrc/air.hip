@@ -144,10 +144,14 @@ __device__ __forceinline__ void internal_matrix(uint32_t* s) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) s[i] = m31_add(m31_rotl(s[i], i + 1), sm);
 }
+// canonical state words and coefficients: products below 2^62, three between folds
 __device__ __forceinline__ qm31 combine16(const PoseidonParams& p, const uint32_t* v) {
   qacc acc = qacc_zero();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) qacc_mac(acc, p.alpha_pow[i], v[i]);
+  for (int i = 0; i < 16; ++i) {
+    qacc_mad(acc, p.alpha_pow[i], v[i]);
+    if (i % 3 == 2) qacc_fold(acc);
+  }
   return qm_sub(qacc_reduce(acc), p.z);
 }
 __device__ __forceinline__ uint32_t offset_prev_row(uint32_t row, uint32_t trace_log, uint32_t eval_log) {

@@ -248,13 +248,27 @@ __device__ __forceinline__ void mac4(qacc* acc, qm31 c, u32x4 v) {
   qacc_mac(acc[2], c, v.z);
   qacc_mac(acc[3], c, v.w);
 }
+// The same without folding: canonical coefficients and values make each product below 2^62, so
+// three of them may follow a fold (fold4), which leaves every coordinate below 2^34.
+__device__ __forceinline__ void mad4(qacc* acc, qm31 c, u32x4 v) {
+  qacc_mad(acc[0], c, v.x);
+  qacc_mad(acc[1], c, v.y);
+  qacc_mad(acc[2], c, v.z);
+  qacc_mad(acc[3], c, v.w);
+}
+__device__ __forceinline__ void fold4(qacc* acc) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) qacc_fold(acc[r]);
+}
 __global__ void __launch_bounds__(512) accumulate_numerators_wide_kernel(const NumEntry* __restrict__ entries,
                                                                          const uint32_t* __restrict__ batch_start,
                                                                          const qm31* __restrict__ batch_b,
                                                                          uint32_t n_batches,
                                                                          uint32_t* const* __restrict__ out) {
   __shared__ uint32_t part[NUM_WAVES][4][256];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index is uniform across the wave: readfirstlane lets the entries (column address and
+  // coefficient) load through the scalar cache into scalar registers
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t row0 = blockIdx.x * 256 + lane * 4;
   for (uint32_t b = 0; b < n_batches; ++b) {
     qacc acc[4] = {qacc_zero(), qacc_zero(), qacc_zero(), qacc_zero()};
@@ -272,7 +286,10 @@ __global__ void __launch_bounds__(512) accumulate_numerators_wide_kernel(const N
 #pragma unroll
       for (uint32_t k = 0; k < NUM_UNROLL; ++k) v[k] = *reinterpret_cast<const GLOBAL_AS u32x4*>(gmem(col[k]) + row0);
 #pragma unroll
-      for (uint32_t k = 0; k < NUM_UNROLL; ++k) mac4(acc, c[k], v[k]);
+      for (uint32_t k = 0; k < NUM_UNROLL; ++k) {
+        mad4(acc, c[k], v[k]);
+        if (k % 3 == 2 || k == NUM_UNROLL - 1) fold4(acc);
+      }
     }
     for (; e < e_end; e += NUM_WAVES) {
       const NumEntry en = entries[e];
