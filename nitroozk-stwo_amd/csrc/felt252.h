@@ -314,8 +314,9 @@ SHD M carry(const M& x) {
   return r;
 }
 // x - (q - 1) p for q = floor(x / 2^251) >= 1 (x itself for q = 0): congruent to x and below
-// 2^251 + 2^201 < 2 p, for x with normalized limbs and a value below 2^256. (q - 1) p touches limbs
-// 0, 6 and 8 only; the borrows are -1 or 0.
+// 2^251 + p < 2 p, for x with normalized limbs and q at most 120 (a value below 2^257.9: the limb-6
+// term 17 2^18 (q - 1) stays below 2^29). (q - 1) p touches limbs 0, 6 and 8 only; the borrows are
+// -1 or 0.
 SHD M reduce_top(const M& x) {
   const uint32_t q = x.w[8] >> 19;
   const uint32_t k = q - (q != 0u ? 1u : 0u);
@@ -346,9 +347,11 @@ constexpr M from_small_multiple_of_p(uint32_t k) {  // k p in radix 2^29, k < 32
   return M{{k, 0u, 0u, 0u, 0u, 0u, (17u << 18) * k, 0u, k << 19}};
 }
 // 5 p and 7 p, borrowed for subtracting 2 s1 and 3 s2 (their top limbs 5 2^19 and 7 2^19 exceed
-// 2 and 3 times the top limb of elements below 2.01 p, plus the borrow)
+// 2 and 3 times the top limb of elements below 2.01 p, plus the borrow); 21 p for 2 s1 with s1
+// below 10.3 p (the partial rounds that follow a round without reduction)
 constexpr M K5P = borrowed(from_small_multiple_of_p(5), 2);
 constexpr M K7P = borrowed(from_small_multiple_of_p(7), 3);
+constexpr M K21P = borrowed(from_small_multiple_of_p(21), 2);
 
 // t = s0 + s1 + s2 -> (t + 2 s0, t - 2 s1, t - 3 s2) for normalized inputs below 2.01 p. The
 // unnormalized outputs stay below 9.1 p, 10.1 p and 12.1 p (limbs below 7 2^29); the first two are
@@ -366,6 +369,30 @@ SHD void mix(M* s) {
   s[1] = reduce_top(carry(b));
   s[2] = carry(c);
 }
+// The partial rounds' mix, reducing s0 and s1 every other round (s2, cubed next, is only
+// normalized). REDUCE = false takes s0, s1 below 2.01 p and leaves them normalized below 9.3 p and
+// 10.3 p (c below 12.3 p); REDUCE = true takes those, subtracts 2 s1 from 21 p, and brings a (below
+// 39.2 p) and b (below 41.7 p) below 2 p (c below 27.7 p, its cube below 1.06 p). Limbs stay below
+// 2^32 - 8 either way: every input is normalized.
+template <bool REDUCE>
+SHD void mix_partial(M* s) {
+  M a, b, c;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint32_t t = s[0].w[i] + s[1].w[i] + s[2].w[i];
+    a.w[i] = t + 2u * s[0].w[i];
+    b.w[i] = t + (REDUCE ? K21P.w[i] : K5P.w[i]) - 2u * s[1].w[i];
+    c.w[i] = t + K7P.w[i] - 3u * s[2].w[i];
+  }
+  if (REDUCE) {
+    s[0] = reduce_top(carry(a));
+    s[1] = reduce_top(carry(b));
+  } else {
+    s[0] = carry(a);
+    s[1] = carry(b);
+  }
+  s[2] = carry(c);
+}
 SHD M cube(const M& x) { return mul29(mul29(x, x), x); }
 // each element plus its constant (limbs below 2^30, value below 14 p) cubed: below 1.2 p
 SHD void full_round(M* s, int k) {
@@ -375,7 +402,10 @@ SHD void full_round(M* s, int k) {
 // poseidon_permute_comp on a Montgomery-form state: elements with limbs below 2^30 and values
 // below 3.1 p (Montgomery forms of canonical values, or a permutation's outputs plus an absorbed
 // value). Outputs: s0 and s1 below 2.01 p, s2 below 12.1 p, all normalized (from_mont reduces
-// fully). One copy of the code per program: the device calls it, it is not inlined.
+// fully). The 83 partial rounds run as 41 pairs (without, then with the reduction of s0 and s1)
+// and one round without it, whose outputs (below 12.3 p plus a constant: under the full rounds'
+// 14 p) go to the last full rounds. One copy of the code per program: the device calls it, it is
+// not inlined.
 __host__ __device__ __attribute__((noinline)) inline void permute(M* s) {
   for (int i = 0; i < 3; ++i) s[i] = carry(s[i]);
   int k = 0;
@@ -383,10 +413,15 @@ __host__ __device__ __attribute__((noinline)) inline void permute(M* s) {
 #ifdef __HIP_DEVICE_COMPILE__
 #pragma unroll 1
 #endif
-  for (int r = 0; r < 83; ++r, ++k) {
+  for (int r = 0; r < 41; ++r, k += 2) {
     s[2] = cube(addl(s[2], RC.c[k]));
-    mix(s);
+    mix_partial<false>(s);
+    s[2] = cube(addl(s[2], RC.c[k + 1]));
+    mix_partial<true>(s);
   }
+  s[2] = cube(addl(s[2], RC.c[k]));
+  mix_partial<false>(s);
+  ++k;
   for (int r = 0; r < 4; ++r, k += 3) full_round(s, k);
 }
 // poseidon_hash(x, y) (Montgomery form in and out)
