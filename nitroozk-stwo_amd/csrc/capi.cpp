@@ -209,20 +209,29 @@ int stwo_hip_copy_h2d(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_
 int stwo_hip_copy_d2h(stwo_hip_ctx* h, uint32_t* dst, const uint32_t* src, size_t n) {
   return guarded(h, [&](Ctx& c) { c.d2h(dst, src, n * 4); });
 }
+static_assert(GATHER_ARG_COLS * GATHER_ARG_ROWS <= Ctx::gather_host_words,
+              "an argument-block gather must fit the coherent host buffer");
 int stwo_hip_batch_at_multi(stwo_hip_ctx* h, const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows,
                             uint32_t n_rows, uint32_t* out_host) {
   return guarded(h, [&](Ctx& c) {
     const size_t n = (size_t)n_cols * n_rows;
     if (!n) return;
     if (!cols || !rows || !out_host) throw HipError(1, "null argument");
-    uint32_t* o = c.gather_out(n);
     if (n_cols <= GATHER_ARG_COLS && n_rows <= GATHER_ARG_ROWS) {
-      // a decommitment's reads: a few rows of a layer's hashes or of a few columns
+      // a decommitment's reads: a few rows of a layer's hashes or of a few columns, written by the
+      // kernel straight into the context's coherent host buffer (GATHER_ARG_COLS x GATHER_ARG_ROWS
+      // words fit in it)
       GatherArgs g{};
       for (uint32_t k = 0; k < n_cols; ++k) g.cols[k] = cols[k];
       for (uint32_t r = 0; r < n_rows; ++r) g.rows[r] = rows[r];
-      launch_gather_args(g, n_cols, n_rows, o, c.stream);
-    } else {
+      launch_gather_args(g, n_cols, n_rows, c.gather_host_dev, true, c.stream);
+      c.sync_main();
+      std::memcpy(out_host, c.gather_host, n * 4);
+      c.reset_arena();
+      return;
+    }
+    uint32_t* o = c.gather_out(n);
+    {
       // the column table repeats between the queried and witness reads of a layer: kept in the
       // table store; the rows go through the staging arena
       const std::vector<const uint32_t*> bases(cols, cols + n_cols);

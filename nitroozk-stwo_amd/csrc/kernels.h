@@ -178,7 +178,10 @@ struct GatherArgs {
   const uint32_t* cols[GATHER_ARG_COLS];
   uint32_t rows[GATHER_ARG_ROWS];
 };
-void launch_gather_args(const GatherArgs& g, uint32_t n_cols, uint32_t n_rows, uint32_t* out, hipStream_t stream);
+// out_is_host: out is the device address of coherent pinned host memory (the kernel fences at
+// system scope before it ends)
+void launch_gather_args(const GatherArgs& g, uint32_t n_cols, uint32_t n_rows, uint32_t* out, bool out_is_host,
+                        hipStream_t stream);
 void launch_gather_rows(const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows, uint32_t n_rows,
                         uint32_t* out, hipStream_t stream);
 

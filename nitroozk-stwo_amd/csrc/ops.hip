@@ -707,21 +707,29 @@ void launch_gather_words(const uint32_t* const* bases, const uint32_t* base_idx,
                      out);
 }
 
+// HOST: out is coherent host memory, read by the host once the stream has synchronised; the
+// system-scope fence orders the store before the kernel's completion
+template <bool HOST>
 __global__ void gather_args_kernel(GatherArgs g, uint32_t n_cols, uint32_t n, uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = g.cols[i % n_cols][g.rows[i / n_cols]];
+  if (HOST) __threadfence_system();
 }
 __global__ void gather_rows_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                    const uint32_t* __restrict__ rows, uint32_t n, uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = cols[i % n_cols][rows[i / n_cols]];
 }
-void launch_gather_args(const GatherArgs& g, uint32_t n_cols, uint32_t n_rows, uint32_t* out, hipStream_t stream) {
+void launch_gather_args(const GatherArgs& g, uint32_t n_cols, uint32_t n_rows, uint32_t* out, bool out_is_host,
+                        hipStream_t stream) {
   if (n_cols > GATHER_ARG_COLS || n_rows > GATHER_ARG_ROWS)
     throw std::invalid_argument("launch_gather_args: more columns or rows than the argument block holds");
   const uint32_t n = n_cols * n_rows;
   if (!n) return;
-  hipLaunchKernelGGL(gather_args_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, g, n_cols, n, out);
+  if (out_is_host)
+    hipLaunchKernelGGL(gather_args_kernel<true>, dim3((n + 255) / 256), dim3(256), 0, stream, g, n_cols, n, out);
+  else
+    hipLaunchKernelGGL(gather_args_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, stream, g, n_cols, n, out);
 }
 void launch_gather_rows(const uint32_t* const* cols, uint32_t n_cols, const uint32_t* rows, uint32_t n_rows,
                         uint32_t* out, hipStream_t stream) {

@@ -71,6 +71,8 @@ Ctx::Ctx(int dev) : device(dev) {
   pinned_cap = 64u << 20;
   HIP_CHECK(hipHostMalloc((void**)&pinned, pinned_cap, hipHostMallocDefault));
   HIP_CHECK(hipHostMalloc((void**)&d2h_pinned, d2h_cap, hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc((void**)&gather_host, gather_host_words * 4, hipHostMallocCoherent | hipHostMallocMapped));
+  HIP_CHECK(hipHostGetDevicePointer((void**)&gather_host_dev, gather_host, 0));
   arena_cap = 64u << 20;
   HIP_CHECK(hipMalloc((void**)&dev_arena, arena_cap));
 }
@@ -96,6 +98,7 @@ Ctx::~Ctx() {
   if (pinned) (void)hipHostFree(pinned);
   if (rb_pinned) (void)hipHostFree(rb_pinned);
   if (d2h_pinned) (void)hipHostFree(d2h_pinned);
+  if (gather_host) (void)hipHostFree(gather_host);
   if (table_dev) (void)hipFree(table_dev);
   if (table_pinned) (void)hipHostFree(table_pinned);
   (void)hipEventDestroy(ev_fork);

@@ -182,6 +182,11 @@ struct Ctx {
   const uint32_t* domain_xy(uint32_t log);
   uint32_t tree_len() const { return 1u << (tw_log - 1); }
   void d2h(void* dst, const void* src, size_t bytes);  // synchronous
+  // Coherent pinned host memory that small gathers write directly from the device (zero copy; the
+  // kernel ends with a system-scope fence, the host reads after the stream synchronises).
+  uint32_t* gather_host = nullptr;
+  uint32_t* gather_host_dev = nullptr;  // the device's address of the same memory
+  static constexpr size_t gather_host_words = 16384;
   uint8_t* d2h_pinned = nullptr;  // read-backs of up to d2h_cap bytes land here first
   static constexpr size_t d2h_cap = 4u << 20;
   // Pinned host memory for asynchronous read-backs on the main stream (kept and grown as needed;
