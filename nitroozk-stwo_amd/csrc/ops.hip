@@ -564,19 +564,42 @@ void launch_barycentric_eval(uint32_t log_size, const uint32_t* const* evals, ui
 // coefficients with a fold after every 2 products, the block sums its lanes,
 // and for columns longer than 2^16 the block sum is multiplied by the factors of its index bits
 // >= 16.
+// A block of 256 threads writes 256 consecutive entries: the weights of their low 8 index bits are
+// built in LDS by doubling (step b multiplies the first 2^b entries by f_b), the product of the
+// factors of the block's bits 8.. by a 3-level tree in the same steps, and each entry is one
+// product of the two (QM31 products are exact: the order of the factors does not matter).
 __global__ void __launch_bounds__(256) oods_weights_kernel(const qm31* __restrict__ factors,
                                                            const uint32_t* __restrict__ group_log,
                                                            qm31* __restrict__ W) {
+  __shared__ qm31 lo[256];
+  __shared__ qm31 hi[8];
   const uint32_t g = blockIdx.y;
   const uint32_t k = group_log[g];
   const uint32_t T = k < OODS_TABLE_LOG ? k : OODS_TABLE_LOG;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (1u << T)) return;
+  const uint32_t base = blockIdx.x * 256;  // blockDim.x is 256
+  if (base >= (1u << T)) return;  // uniform per block
   const qm31* f = factors + (size_t)g * 32;
-  qm31 w = qm_one();
-  for (uint32_t b = 0; b < T; ++b)
-    if ((i >> b) & 1) w = qm_mul(w, f[b]);
-  W[((size_t)g << OODS_TABLE_LOG) + i] = w;
+  const uint32_t t = threadIdx.x;
+  const uint32_t tl = T < 8 ? T : 8;  // index bits below the block's
+  if (t == 0) lo[0] = qm_one();
+  if (t >= 128 && t < 136) {
+    const uint32_t b = 8 + (t - 128);
+    hi[t - 128] = (b < T && ((blockIdx.x >> (b - 8)) & 1)) ? f[b] : qm_one();
+  }
+  __syncthreads();
+  for (uint32_t b = 0; b < 8; ++b) {
+    qm31 v;
+    const bool dbl = b < tl && t < (1u << b);
+    const bool tree = b < 3 && t >= 128 && t < 128 + (4u >> b);
+    if (dbl) v = qm_mul(lo[t], f[b]);
+    if (tree) v = qm_mul(hi[2 * (t - 128)], hi[2 * (t - 128) + 1]);
+    __syncthreads();  // every read of the step is done
+    if (dbl) lo[t + (1u << b)] = v;
+    if (tree) hi[t - 128] = v;
+    __syncthreads();
+  }
+  const uint32_t i = base + t;
+  if (i < (1u << T)) W[((size_t)g << OODS_TABLE_LOG) + i] = T > 8 ? qm_mul(lo[t], hi[0]) : lo[t];
 }
 
 // A block takes 2^OODS_BLOCK_LOG coefficients of up to OODS_BATCH columns of one group (tasks
