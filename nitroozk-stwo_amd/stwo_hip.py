@@ -6,7 +6,9 @@ Device memory is owned by DeviceColumn objects (the analogue of BaseFieldVec,
 crates/stwo/src/stwo_cuda/base_field_vec.rs:7-140). Every call goes to the HIP library; there is
 no CPU fallback, and constructing a backend fails loudly when the library or the GPU is missing.
 """
+import collections.abc
 import ctypes
+import operator
 import os
 
 import numpy as np
@@ -249,16 +251,71 @@ def load_library(path=LIB_PATH):
     return L
 
 
-class ColumnList(tuple):
-    """An immutable sequence of device columns that keeps the array of their addresses, so that a
-    library call taking the columns (a whole trace, every proof) does not rebuild it element by
-    element in Python."""
+class ColumnList(collections.abc.Sequence):
+    """Columns allocated together (HipBackend.columns), as an immutable sequence that keeps the
+    array of their addresses: a library call taking them all (a whole trace, every proof) passes the
+    array as it is, and a column's DeviceColumn object is made only when the column is indexed, so
+    a trace of a thousand columns costs no Python object per column unless one is used. A column
+    freed through its object leaves a null address in the array."""
 
-    def __new__(cls, cols):
-        self = super().__new__(cls, cols)
-        self._arr = (ctypes.c_void_p * max(1, len(self)))(
-            *[col.ptr if isinstance(col, DeviceColumn) else col for col in self])
-        return self
+    def __init__(self, backend, arr, n_cols, n_words):
+        self._backend, self._arr, self._n, self._words = backend, arr, n_cols, n_words
+        self._cols = [None] * n_cols
+        self._made = []  # indices that have a DeviceColumn
+
+    def _addrs(self):
+        return np.frombuffer(self._arr, dtype=np.uint64, count=self._n)
+
+    def __len__(self):
+        return self._n
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(self._n))]
+        i = operator.index(i)
+        if i < 0:
+            i += self._n
+        if not 0 <= i < self._n:
+            raise IndexError("column index out of range")
+        col = self._cols[i]
+        if col is None:
+            addr = self._arr[i]
+            col = DeviceColumn(self._backend, self._words, ptr=addr if addr else 0)
+            if not addr:
+                col.ptr = None  # freed already
+            col._slot = (self, i)
+            self._cols[i] = col
+            self._made.append(i)
+        return col
+
+    def _live(self):
+        return self._n == 0 or bool(np.all(self._addrs()))
+
+    def _freed(self):
+        ctypes.memset(ctypes.addressof(self._arr), 0, self._n * ctypes.sizeof(ctypes.c_void_p))
+        for i in self._made:
+            self._cols[i].ptr = None
+
+    def __del__(self):
+        # columns never indexed have no DeviceColumn to free them (those that do free their own)
+        try:
+            b = self._backend
+            if not b.ctx or self._n == 0:
+                return
+            a = self._addrs().copy()
+            a[self._made] = 0
+            live = np.ascontiguousarray(a[a != 0])
+            if live.size:
+                b.lib.stwo_hip_free_columns(b.ctx, live.ctypes.data_as(ctypes.POINTER(ctypes.c_void_p)), live.size)
+        except Exception:
+            pass
+
+
+def _min_words(cols):
+    """The shortest column's length, without making a DeviceColumn per column of a ColumnList."""
+    if isinstance(cols, ColumnList):
+        return cols._words
+    return min((c.n for c in cols), default=0)
 
 
 def _ptrs(cols):
@@ -306,6 +363,9 @@ class DeviceColumn:
     def free(self):
         if self.ptr and self.parent is None:
             self.backend.lib.stwo_hip_free(self.backend.ctx, self.ptr)
+            slot = getattr(self, "_slot", None)
+            if slot is not None:  # a column of a ColumnList: its address leaves the list's array
+                slot[0]._arr[slot[1]] = None
         self.ptr = None
 
     # Column::at / set / split_at_mid (crates/stwo/src/prover/backend/mod.rs:59-62,90-91)
@@ -409,18 +469,24 @@ class HipBackend:
         """n_cols columns of n_words each, allocated in one library call."""
         arr = (ctypes.c_void_p * max(1, n_cols))()
         self._check(self.lib.stwo_hip_alloc_columns(self.ctx, n_cols, n_words, int(zeroed), arr))
-        return ColumnList(DeviceColumn(self, n_words, ptr=arr[k]) for k in range(n_cols))
+        return ColumnList(self, arr, n_cols, n_words)
 
     def free_columns(self, cols):
         """Frees columns in one library call."""
-        if isinstance(cols, ColumnList) and all(col.ptr for col in cols):
-            live, arr = cols, cols._arr  # the cached address array
-        else:
-            live = [col for col in cols if col.ptr]
-            arr = _ptrs(live)
+        if isinstance(cols, ColumnList):
+            if cols._live():
+                st = self.lib.stwo_hip_free_columns(self.ctx, cols._arr, len(cols))  # the kept array
+            else:  # some columns were freed one by one
+                a = cols._addrs()
+                live = [int(x) for x in a[a != 0]]
+                st = self.lib.stwo_hip_free_columns(self.ctx, _ptrs(live), len(live)) if live else 0
+            cols._freed()
+            self._check(st)
+            return
+        live = [col for col in cols if col.ptr]
         if not live:
             return
-        st = self.lib.stwo_hip_free_columns(self.ctx, arr, len(live))
+        st = self.lib.stwo_hip_free_columns(self.ctx, _ptrs(live), len(live))
         for col in live:
             col.ptr = None
         self._check(st)
@@ -773,14 +839,20 @@ class HipBackend:
         return out
 
     # ---- trace generation and whole proofs ----
-    def gen_trace_wide_fibonacci(self, n_cols, log_n_rows, mode=1, seed=0):
-        cols = self.columns(n_cols, 1 << log_n_rows)
+    def gen_trace_wide_fibonacci(self, n_cols, log_n_rows, mode=1, seed=0, out=None):
+        """The trace on the device, into `out` (n_cols columns of 2^log_n_rows words) when given."""
+        cols = out if out is not None else self.columns(n_cols, 1 << log_n_rows)
+        if len(cols) != n_cols or _min_words(cols) < (1 << log_n_rows):
+            raise ValueError("out must hold n_cols columns of 2^log_n_rows words")
         self._check(self.lib.stwo_hip_gen_trace_wide_fibonacci(self.ctx, _ptrs(cols), n_cols, log_n_rows, mode,
                                                                seed))
         return cols
 
-    def gen_trace_poseidon(self, instances_per_row, log_n_rows, mode=1, seed=0):
-        cols = self.columns(158 * instances_per_row, 1 << log_n_rows)
+    def gen_trace_poseidon(self, instances_per_row, log_n_rows, mode=1, seed=0, out=None):
+        """The trace on the device, into `out` (158 columns per instance of a row) when given."""
+        cols = out if out is not None else self.columns(158 * instances_per_row, 1 << log_n_rows)
+        if len(cols) != 158 * instances_per_row or _min_words(cols) < (1 << log_n_rows):
+            raise ValueError("out must hold 158 columns of 2^log_n_rows words per instance of a row")
         self._check(self.lib.stwo_hip_gen_trace_poseidon(self.ctx, _ptrs(cols), instances_per_row, log_n_rows, mode,
                                                          seed))
         return cols
