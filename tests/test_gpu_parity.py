@@ -325,3 +325,34 @@ def test_compression_rate_and_merkle_units(hb):
     hb.commit_on_layer(0, None, [])  # the empty message: 1
     hb.set_kernel_timing(False)
     assert hb.kernel_units()["merkle"] == 2 * 64 + 2 * 32 + 16 + 1
+
+
+def test_stream_ordered_chain(hb):
+    """Trait entries return without waiting for the device: a chain of them (interpolate, evaluate
+    with blowup, the Merkle tree of the evaluations level by level, a fold) with no read in between
+    gives the oracle's results when read at the end."""
+    say("stream-ordered chain")
+    rng = np.random.default_rng(21)
+    log_n, n_cols = 12, 8
+    cols = rand_cols(rng, n_cols, log_n)
+    coeffs = hb.interpolate_columns([hb.from_cpu(c) for c in cols], log_n)
+    ev = hb.evaluate_polynomials(coeffs, log_n, 1)
+    layers = [hb.commit_on_layer(log_n + 1, None, ev)]
+    for ls in range(log_n, -1, -1):
+        layers.append(hb.commit_on_layer(ls, layers[-1], []))
+    alpha = qm(rng)
+    dst = [hb.column(1 << log_n, zeroed=True) for _ in range(4)]
+    hb.fold_circle_into_line(dst, ev[:4], alpha, log_n + 1)
+    # the reads, in reverse order of the launches
+    exp_ev = [O.evaluate(O.interpolate(c), log_n + 1) for c in cols]
+    fold = O.fold_circle_into_line(exp_ev[:4], [np.zeros(1 << log_n, np.uint32)] * 4, alpha)
+    for d, e in zip(dst, fold):
+        assert np.array_equal(d.to_cpu(), e)
+    prev = O.commit_on_layer(log_n + 1, None, exp_ev)
+    leaf = prev
+    for ls in range(log_n, -1, -1):
+        prev = O.commit_on_layer(ls, prev, [])
+    assert layers[-1].to_cpu().view(np.uint8).tobytes() == prev.tobytes()
+    assert layers[0].to_cpu().view(np.uint8).tobytes() == leaf.tobytes()
+    for e, x in zip(ev, exp_ev):
+        assert np.array_equal(e.to_cpu(), x)
